@@ -1,0 +1,65 @@
+/*
+ * Engine extensions of the MI355X PIMD engine.  None of these has a
+ * counterpart in the reference; they are additive entry points for drivers
+ * that want the fused, graph-captured MD step, run-time knobs, or access to
+ * internal exchange quantities for testing.  All follow the reference's
+ * conventions: int status 0 = success, -1 = failure.
+ */
+#ifndef MD_HIP_X_H
+#define MD_HIP_X_H
+
+#include "md_pimd_x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Chain length M integrated by md_update_nhc_VV3_1_x / _2_x (even,
+ * 2 <= M <= MD_NHC_LENGTH_X; default MD_NHC_LENGTH_X).  Applies to objects
+ * uploaded after the call. */
+int md_set_nhc_length_x(int M);
+int md_get_nhc_length_x(void);
+
+/* Waits for all queued device work of the engine. */
+int md_hip_synchronize_x(void);
+
+/* Real-space Ewald cutoff in units of 1/alpha (default 6: erfc(6) ~ 2e-17). */
+int md_set_ewald_real_cutoff_x(md_num_t_x x_cut);
+
+/* Internal exchange quantities after md_pimd_fill_VB_x (tests, diagnostics):
+ * backward potentials Vt^(0..N) and the connection-probability matrix
+ * G[(l-1)*N + (j-1)] = P(bead P of l links to bead 1 of j). */
+int md_pimd_get_backward_potentials_x(md_pimd_t_x *pimd, md_num_t_x *Vt);
+int md_pimd_get_connection_matrix_x(md_pimd_t_x *pimd, md_num_t_x *G);
+
+/* One MD step of the reference driver loop (velocity Verlet with NHC, force
+ * reset, trap, pair, cycle energies, V_B and exchange forces), fused on the
+ * device.  The exchange branch runs on its own stream, concurrently with the
+ * pair forces. */
+typedef struct {
+  md_trap_force_t_x trap; /* md_harmonic_trap_force_x or NULL */
+  md_pair_force_t_x pair; /* a supported pair callback or NULL */
+  md_num_t_x dt;
+  int pbc;       /* wrap positions into the box after the drift */
+  int image;     /* minimum-image springs (periodic systems) */
+  int use_graph; /* replay a captured hipGraph of the step */
+} md_pimd_step_config_x;
+
+/* Forces at the current positions (step stages B3-B6). */
+int md_pimd_compute_forces_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg);
+/* nsteps full steps (B1-B7); returns when the work is queued. */
+int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps);
+
+/* Per-stage device time of nsteps eager steps, averaged per step (ms):
+ *   [0] NHC first half + wrap   [1] cycle energies   [2] V_B recursions
+ *   [3] reset/trap/ring springs [4] pair real space  [5] pair reciprocal
+ *   [6] exchange end-bead forces [7] NHC second half [8] whole step (wall)
+ * Each stage is bracketed by events on the stream it runs on. */
+#define MD_PIMD_NSTAGES_X 9
+int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, double *stage_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

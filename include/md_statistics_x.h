@@ -1,0 +1,50 @@
+/*
+ * Block-averaged statistics and energy callbacks (drop-in for the
+ * reference's md_statistics_x.h:7-91, nocl build, hot-path subset).
+ */
+#ifndef MD_STATISTICS_X_H
+#define MD_STATISTICS_X_H
+
+#include "md_particle_x.h"
+#include "md_simulation_x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MD_MAX_STATS_COUNT 10000
+
+typedef struct {
+  int N;            /* number of blocks (the last one is open) */
+  int count;        /* samples in the open block */
+  md_num_t_x e;     /* finalised mean of block means */
+  md_num_t_x *es;   /* per-block accumulators / means */
+  md_num_t_x T;
+  md_num_t_x *Ts;
+  int points;
+  md_num_t_x *fx;
+  md_num_t_x **fxs;
+} md_stats_t_x;
+
+md_stats_t_x *md_stats_sync_host_x(md_stats_t_x *stats);
+
+typedef md_num_t_x (*md_pair_energy_t_x)(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+typedef md_num_t_x (*md_trap_energy_t_x)(md_num_t_x *p1, md_num_t_x *L, md_num_t_x m);
+
+md_stats_t_x *md_alloc_stats_x(int points);
+md_stats_t_x *md_update_stats_x(md_stats_t_x *stats);
+md_stats_t_x *md_finalize_stats_x(md_stats_t_x *stats);
+
+md_num_t_x md_harmonic_trap_energy_x(md_num_t_x *p1, md_num_t_x *L, md_num_t_x m);
+md_num_t_x md_gaussian_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+md_num_t_x md_periodic_gaussian_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+md_num_t_x md_coulomb_3d_ewald_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+
+int md_calc_kinetic_x(md_simulation_t_x *sim, md_stats_t_x *stats);
+int md_calc_nhc_kinetic_x(md_simulation_t_x *sim, md_stats_t_x *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

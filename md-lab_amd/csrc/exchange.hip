@@ -1,0 +1,829 @@
+// Bosonic / fictitious-particle exchange on the GPU: cycle energies E_n^(k)
+// (md_pimd_x.c:1969-2210), the forward V_B recursion (:814-953), the
+// backward potentials, connection probabilities and exchange forces
+// (:2369-2721), and the estimator recursions (:1111-1295, 3959-4556,
+// 4652-4705).
+//
+// Formulation used throughout.  With particles 1..N and
+//   c(n, m) = (n-m-1) ln xi - beta E_n^(n-m) - ln n       (0 <= m < n <= N)
+// (only n - m = 1 is present when xi = 0), the reference's recursions are
+//   forward   L(n) = LSE_{m<n} [c(n,m) + L(m)],  L(0) = 0,  V_B^(n) = -L(n)/beta
+//   backward  R(m) = LSE_{n>m} [c(n,m) + R(n)],  R(N) = 0,  Vt^(m)  = -R(m)/beta
+//   G(l, j)   = exp(c(l, j-1) + L(j-1) + R(l) - L(N))  for j <= l,
+//               1 - exp(L(N) - L(l) - R(l))            for j = l+1.
+// These are algebraically identical to md_pimd_xminE_x / md_pimd_fill_VB_x /
+// md_pimd_fast_xminE_x / md_pimd_fast_fill_force_VB_x; the shift of each
+// log-sum-exp is the running maximum instead of a separate minimum pass.
+//
+// Cycle energies come from double-double prefix sums of the open-chain
+// energies C_i = Eint_i + s(x_{i,P}, x_{i+1,1}):
+//   E_n^(k) = sum_{i=n-k+1}^{n-1} C_i + Eint_n + s(x_{n,P}, x_{n-k+1,1}),
+// the closed form of the reference's recurrence (md_pimd_x.c:2150-2206), so
+// every (n, k) is independent and the fill is one parallel pass.  Two copies
+// are kept: the reference's row layout (E_n^(.) contiguous) for the forward
+// pass and the host mirror, and a column layout (cycles sharing their first
+// particle contiguous) for the backward pass and the exchange-force columns.
+//
+// Recursion kernel (one 1024-thread workgroup per direction, both directions
+// run concurrently as the two blocks of one launch).  Positions are
+// processed in blocks of 64: all 16 waves reduce the off-diagonal part of
+// the block's 64 rows (terms from finished blocks, two-pass LSE per row),
+// then one wave runs the 64-step diagonal with the right-looking update
+//   (M_q, S_q) <- (M_q, S_q) (+) S_r exp(c(q, r) + M_r)
+// that needs one exp per lane per step and no log on the critical path;
+// values are broadcast between steps with scalar readlanes.
+#include <cmath>
+
+#include "xmd_device.h"
+#include "xmd_internal.h"
+
+namespace xmd {
+
+constexpr double NEG = -1.0e300;  // "no term" marker (exp(NEG - x) == 0)
+constexpr int RB = 64;             // diagonal block
+constexpr int RT = 1024;           // recursion workgroup
+constexpr int MAXN_REC = 4096;     // LDS-resident recursion limit
+
+__device__ __forceinline__ size_t tri_row(int n) { return (size_t)(n - 1) * n / 2; }  // row n (1-based)
+__device__ __forceinline__ size_t col_off(int m, int N) { return (size_t)m * N - (size_t)m * (m - 1) / 2; }
+
+struct Geo {
+  double L0, L1, L2;
+  int image;
+};
+
+__device__ __forceinline__ double comp(double a, double L, int image) { return image ? dv_min_image(a, L) : a; }
+
+// s(a, b) = 1/2 m_a w^2 d^2 with d the (minimum-image) distance, as in
+// md_pimd_x.c:2120-2124.
+__device__ __forceinline__ double spring(const double *X, int nb, int sa, int sb, double m, double om, Geo g) {
+  double dx = comp(X[sa] - X[sb], g.L0, g.image);
+  double dy = comp(X[nb + sa] - X[nb + sb], g.L1, g.image);
+  double dz = comp(X[2 * nb + sa] - X[2 * nb + sb], g.L2, g.image);
+  double d = sqrt(dx * dx + dy * dy + dz * dz);
+  return 0.5 * m * om * om * d * d;
+}
+
+// 1/2 m_a w^2 (x_a - x_b).(x_a - x_c), md_pimd_x.c:4376-4420.
+__device__ __forceinline__ double e2(const double *X, const double *M, int nb, int sa, int sb, int sc, double om,
+                                     Geo g) {
+  double r = 0;
+  double m = M[sa];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double L = d == 0 ? g.L0 : (d == 1 ? g.L1 : g.L2);
+    double u = comp(X[d * nb + sa] - X[d * nb + sb], L, g.image);
+    double v = comp(X[d * nb + sa] - X[d * nb + sc], L, g.image);
+    r += 0.5 * m * om * om * u * v;
+  }
+  return r;
+}
+
+// ------------------------------------------------------------ E_n^(k)
+
+// Thread per particle l (0-based): Eint_l, E_l^(1), link C_l.
+__global__ void k_chain(const double *__restrict__ X, const double *__restrict__ M, int N, int P, double om, Geo g,
+                        double *__restrict__ Eint, double *__restrict__ Clink, double *__restrict__ ENk,
+                        double *__restrict__ ET) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= N) return;
+  const int nb = N * P;
+  double e = 0;
+  for (int j = 0; j < P - 1; ++j) e += spring(X, nb, j * N + l, (j + 1) * N + l, M[j * N + l], om, g);
+  Eint[l] = e;
+  const int sP = (P - 1) * N + l;
+  double e1 = e + spring(X, nb, sP, l, M[sP], om, g);
+  ENk[tri_row(l + 1)] = e1;
+  ET[col_off(l, N)] = e1;
+  Clink[l] = (l < N - 1) ? e + spring(X, nb, sP, l + 1, M[sP], om, g) : 0.0;
+}
+
+// Exclusive double-double prefix sum of in[0..n-1] into pre[2*i] (hi, lo).
+__global__ __launch_bounds__(1024) void k_scan_dd(const double *__restrict__ in, int n, double *__restrict__ pre) {
+  __shared__ double sh[1024], sl[1024];
+  const int t = threadIdx.x;
+  const int seg = (n + 1023) / 1024;
+  const int a = min(n, t * seg), b = min(n, a + seg);
+  dd tot{0, 0};
+  for (int i = a; i < b; ++i) tot = dd_add(tot, dd{in[i], 0});
+  sh[t] = tot.hi;
+  sl[t] = tot.lo;
+  __syncthreads();
+  for (int w = 1; w < 1024; w <<= 1) {
+    dd v{sh[t], sl[t]};
+    if (t >= w) v = dd_add(v, dd{sh[t - w], sl[t - w]});
+    __syncthreads();
+    sh[t] = v.hi;
+    sl[t] = v.lo;
+    __syncthreads();
+  }
+  dd run = t > 0 ? dd{sh[t - 1], sl[t - 1]} : dd{0, 0};
+  for (int i = a; i < b; ++i) {
+    pre[2 * i] = run.hi;
+    pre[2 * i + 1] = run.lo;
+    run = dd_add(run, dd{in[i], 0});
+  }
+}
+
+// Row n' (0-based) of the cycle-energy triangle, k >= 2, both layouts.
+__global__ __launch_bounds__(256) void k_enk_fill(const double *__restrict__ X, const double *__restrict__ M, int N,
+                                                  int P, double om, Geo g, const double *__restrict__ Eint,
+                                                  const double *__restrict__ pre, double *__restrict__ ENk,
+                                                  double *__restrict__ ET) {
+  const int r = blockIdx.x;  // 0-based last particle of the cycle
+  const int nb = N * P;
+  const int sP = (P - 1) * N + r;
+  const double mP = M[sP], ein = Eint[r];
+  const dd ar{pre[2 * r], pre[2 * r + 1]};
+  for (int k = 2 + threadIdx.x; k <= r + 1; k += blockDim.x) {
+    const int m0 = r - k + 1;  // 0-based first particle
+    double chain = dd_sub_to_double(ar, dd{pre[2 * m0], pre[2 * m0 + 1]});
+    double E = chain + ein + spring(X, nb, sP, m0, mP, om, g);
+    ENk[tri_row(r + 1) + k - 1] = E;
+    ET[col_off(m0, N) + k - 1] = E;
+  }
+}
+
+// Host-provided ENk (row layout) -> column layout.
+__global__ void k_enk_transpose(const double *__restrict__ ENk, double *__restrict__ ET, int N) {
+  const int r = blockIdx.x;
+  for (int k = 1 + threadIdx.x; k <= r + 1; k += blockDim.x) {
+    int m0 = r - k + 1;
+    ET[col_off(m0, N) + k - 1] = ENk[tri_row(r + 1) + k - 1];
+  }
+}
+
+// ------------------------------------------------------------ recursion
+
+struct RecArgs {
+  const double *ENk;  // row layout
+  const double *ET;   // column layout
+  const double *lnn;  // ln n, n = 0..N (ln 0 unused)
+  double *Lf;         // forward output, N+1
+  double *Rb;         // backward output, N+1
+  double *VBN;        // V_B^(n) = -L(n)/beta
+  int N;
+  double beta, lnxi;
+  int xi0;            // xi == 0: only k = 1 terms
+};
+
+// Position p = 0..N along the solve order; p' < p are solved earlier.
+//   forward:  value L(p),      coef(p, p') = c(n = p,      m = p')
+//   backward: value R(N - p),  coef(p, p') = c(n = N - p', m = N - p)
+// In both directions k = p - p'.
+template <bool FWD>
+__device__ __forceinline__ double rec_coef(const RecArgs &a, int p, int pp) {
+  const int k = p - pp;
+  if (a.xi0 && k != 1) return NEG;
+  double E, ln_n;
+  if (FWD) {
+    E = a.ENk[tri_row(p) + k - 1];
+    ln_n = a.lnn[p];
+  } else {
+    E = a.ET[col_off(a.N - p, a.N) + k - 1];
+    ln_n = a.lnn[a.N - pp];
+  }
+  return (k - 1) * a.lnxi - a.beta * E - ln_n;
+}
+
+template <bool FWD>
+__device__ void recursion_body(const RecArgs &a, double *val, double *dg, double *mo, double *so) {
+  const int N = a.N;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) val[0] = 0.0;
+  __syncthreads();
+  for (int p0 = 1; p0 <= N; p0 += RB) {
+    const int nrows = min(RB, N - p0 + 1);
+    // (A) off-diagonal: rows p0 + r against solved positions 0..p0-1
+    for (int r = wv; r < nrows; r += RT / 64) {
+      const int p = p0 + r;
+      double mx = NEG;
+      for (int pp = lane; pp < p0; pp += 64) {
+        double c = rec_coef<FWD>(a, p, pp);
+        if (c > NEG) mx = fmax(mx, c + val[pp]);
+      }
+      mx = wave_max(mx);
+      double s = 0;
+      if (mx > NEG) {
+        for (int pp = lane; pp < p0; pp += 64) {
+          double c = rec_coef<FWD>(a, p, pp);
+          if (c > NEG) {
+            double t = c + val[pp] - mx;
+            if (t > -745.0) s += exp(t);
+          }
+        }
+      }
+      s = wave_sum(s);
+      if (lane == 0) {
+        mo[r] = mx;
+        so[r] = s;
+      }
+    }
+    // diagonal coefficients, lower triangle q > r stored at q(q-1)/2 + r
+    for (int w = tid; w < RB * (RB - 1) / 2; w += RT) {
+      int q = (int)((1.0 + sqrt(1.0 + 8.0 * w)) / 2.0);
+      while (q * (q - 1) / 2 > w) --q;
+      while ((q + 1) * q / 2 <= w) ++q;
+      int r = w - q * (q - 1) / 2;
+      dg[w] = (q < nrows) ? rec_coef<FWD>(a, p0 + q, p0 + r) : NEG;
+    }
+    __syncthreads();
+    // (B) diagonal block, one wave
+    if (wv == 0) {
+      double M = lane < nrows ? mo[lane] : NEG;
+      double S = lane < nrows ? so[lane] : 0.0;
+      for (int r = 0; r < nrows; ++r) {
+        const double Mr = readlane_d(M, r), Sr = readlane_d(S, r);
+        if (lane > r && lane < nrows) {
+          const double c = dg[lane * (lane - 1) / 2 + r];
+          if (c > NEG) {
+            const double T = c + Mr;
+            if (T > M) {
+              S = S * exp(M - T) + Sr;
+              M = T;
+            } else {
+              S += Sr * exp(T - M);
+            }
+          }
+        }
+      }
+      if (lane < nrows) val[p0 + lane] = M + log(S);
+    }
+    __syncthreads();
+  }
+  for (int p = tid; p <= N; p += RT) {
+    if (FWD) {
+      a.Lf[p] = val[p];
+      a.VBN[p] = -val[p] / a.beta;
+    } else {
+      a.Rb[N - p] = val[p];
+    }
+  }
+}
+
+__global__ __launch_bounds__(RT) void k_recursion(RecArgs a, int dirmask) {
+  extern __shared__ double lds[];
+  double *dg = lds;                          // RB*(RB-1)/2
+  double *mo = dg + RB * (RB - 1) / 2;       // RB
+  double *so = mo + RB;                      // RB
+  double *val = so + RB;                     // N+1
+  // block 0: forward (if requested), block 1: backward (if requested)
+  int dir = blockIdx.x;
+  if (gridDim.x == 1) dir = (dirmask & 1) ? 0 : 1;
+  if (dir == 0)
+    recursion_body<true>(a, val, dg, mo, so);
+  else
+    recursion_body<false>(a, val, dg, mo, so);
+}
+
+__global__ void k_lnn(double *lnn, int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= N) lnn[i] = i > 0 ? log((double)i) : 0.0;
+}
+
+__global__ void k_vbn_to_L(const double *VBN, double *Lf, int N, double beta) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= N) Lf[i] = -beta * VBN[i];
+}
+
+// --------------------------------------------------------- exchange force
+
+// Block per particle l (1-based l = blockIdx.x + 1): bead P against every
+// bead 1 (row l of G) and bead 1 against every bead P (column l of G).
+__global__ __launch_bounds__(256) void k_exch_force(const double *__restrict__ X, const double *__restrict__ M,
+                                                    double *__restrict__ F, int N, int P, double om, Geo g,
+                                                    RecArgs a) {
+  __shared__ double red[8][256];
+  const int l = blockIdx.x + 1;
+  const int nb = N * P;
+  const int sP = (P - 1) * N + (l - 1), s1 = l - 1;
+  const double LN = a.Lf[N];
+  const double Rl = a.Rb[l], Llm1 = a.Lf[l - 1];
+  const double xP[3] = {X[sP], X[nb + sP], X[2 * nb + sP]};
+  const double x1[3] = {X[s1], X[nb + s1], X[2 * nb + s1]};
+  const double Ls[3] = {g.L0, g.L1, g.L2};
+  double gs = 0, gc[3] = {0, 0, 0}, hs = 0, hc[3] = {0, 0, 0};
+  // row l: partners j = 1..l+1 (bead 1 of particle j)
+  for (int j = 1 + threadIdx.x; j <= l + 1; j += blockDim.x) {
+    double G;
+    if (j <= l) {
+      const int k = l - j + 1;
+      if (a.xi0 && k != 1) continue;
+      const double c = (k - 1) * a.lnxi - a.beta * a.ENk[tri_row(l) + k - 1] - a.lnn[l];
+      G = exp(c + a.Lf[j - 1] + Rl - LN);
+    } else {
+      if (l == N) continue;
+      G = 1 - exp(LN - a.Lf[l] - Rl);
+    }
+    const int sj = j - 1;
+    gs += G;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gc[d] += G * comp(xP[d] - X[d * nb + sj], Ls[d], g.image);
+  }
+  // column l: partners j' = l-1 .. N (bead P of particle j')
+  for (int jp = max(1, l - 1) + threadIdx.x; jp <= N; jp += blockDim.x) {
+    double G;
+    if (jp >= l) {
+      const int k = jp - l + 1;
+      if (a.xi0 && k != 1) continue;
+      const double c = (k - 1) * a.lnxi - a.beta * a.ET[col_off(l - 1, N) + k - 1] - a.lnn[jp];
+      G = exp(c + Llm1 + a.Rb[jp] - LN);
+    } else {
+      G = 1 - exp(LN - a.Lf[l - 1] - a.Rb[l - 1]);
+    }
+    const int sj = (P - 1) * N + jp - 1;
+    hs += G;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) hc[d] += G * comp(x1[d] - X[d * nb + sj], Ls[d], g.image);
+  }
+  double v[8] = {gs, gc[0], gc[1], gc[2], hs, hc[0], hc[1], hc[2]};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int sPm = (P - 2) * N + (l - 1), s2 = N + (l - 1);
+    const double mP = M[sP], m1 = M[s1];
+    for (int d = 0; d < 3; ++d) {
+      double aP = comp(xP[d] - X[d * nb + sPm], Ls[d], g.image);
+      double gradP = (mP * om * om) * (red[0][0] * aP) + (mP * om * om) * red[1 + d][0];
+      F[d * nb + sP] += -gradP / mP;
+      double a1 = comp(x1[d] - X[d * nb + s2], Ls[d], g.image);
+      double grad1 = (m1 * om * om) * (red[4][0] * a1) + (m1 * om * om) * red[5 + d][0];
+      F[d * nb + s1] += -grad1 / m1;
+    }
+  }
+}
+
+// Optional full connection matrix (tests): G[(l-1)*N + (j-1)].
+__global__ void k_connection_matrix(RecArgs a, double *__restrict__ G) {
+  const int N = a.N;
+  const int l = blockIdx.x + 1;
+  const double LN = a.Lf[N];
+  for (int j = 1 + threadIdx.x; j <= N; j += blockDim.x) {
+    double v = 0;
+    if (j == l + 1) {
+      v = 1 - exp(LN - a.Lf[l] - a.Rb[l]);
+    } else if (j <= l) {
+      const int k = l - j + 1;
+      if (!(a.xi0 && k != 1)) {
+        const double c = (k - 1) * a.lnxi - a.beta * a.ENk[tri_row(l) + k - 1] - a.lnn[l];
+        v = exp(c + a.Lf[j - 1] + a.Rb[l] - LN);
+      }
+    }
+    G[(size_t)(l - 1) * N + (j - 1)] = v;
+  }
+}
+
+// ------------------------------------------------------------ ENk2
+
+// Link terms: nocl orientation b_i = e2((i,1),(i+1,P),(i+1,1)); forward
+// orientation a_i = e2((i+1,1),(i,P),(i,1))  (0-based particles).
+__global__ void k_chain2(const double *__restrict__ X, const double *__restrict__ M, int N, int P, double om, Geo g,
+                         int forward, double *__restrict__ link) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int nb = N * P;
+  if (i == N - 1) {
+    link[i] = 0;
+    return;
+  }
+  const int last = (P - 1) * N;
+  link[i] = forward ? e2(X, M, nb, i + 1, last + i, i, om, g) : e2(X, M, nb, i, last + i + 1, i + 1, om, g);
+}
+
+__global__ __launch_bounds__(256) void k_enk2_fill(const double *__restrict__ X, const double *__restrict__ M, int N,
+                                                   int P, double om, Geo g, int forward,
+                                                   const double *__restrict__ pre, double *__restrict__ ENk2) {
+  const int r = blockIdx.x;
+  const int nb = N * P, last = (P - 1) * N;
+  const dd ar{pre[2 * r], pre[2 * r + 1]};
+  for (int k = 1 + threadIdx.x; k <= r + 1; k += blockDim.x) {
+    const int m0 = r - k + 1;
+    double v;
+    if (k == 1) {
+      v = e2(X, M, nb, r, last + r, r, om, g);
+    } else {
+      double links = dd_sub_to_double(ar, dd{pre[2 * m0], pre[2 * m0 + 1]});
+      double close = forward ? e2(X, M, nb, m0, last + r, r, om, g) : e2(X, M, nb, r, last + m0, m0, om, g);
+      v = links + close;
+    }
+    ENk2[tri_row(r + 1) + k - 1] = v;
+  }
+}
+
+// ------------------------------------------------------------ estimators
+
+// res(n) = sum_{m<n} w(n,m) (res(m) - B(n,m)) / sum_{m<n} w(n,m) with
+// w(n,m) = exp(c(n,m) + L(m) - L(n)), B = ENk or ENk2
+// (md_pimd_x.c:1273-1288, 4294-4319).  Linear recursion: blocked like the
+// forward pass, the diagonal is a 64-step forward substitution.
+__global__ __launch_bounds__(RT) void k_estimator(RecArgs a, const double *__restrict__ Bm, double *__restrict__ out) {
+  extern __shared__ double lds[];
+  const int N = a.N;
+  double *wd = lds;                       // RB*(RB-1)/2 weights
+  double *bd = wd + RB * (RB - 1) / 2;    // RB*(RB-1)/2 matrix values
+  double *ao = bd + RB * (RB - 1) / 2;    // RB
+  double *wo = ao + RB;                   // RB
+  double *res = wo + RB;                  // N+1
+  double *Ls = res + N + 1;               // N+1
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int p = tid; p <= N; p += RT) Ls[p] = a.Lf[p];
+  if (tid == 0) res[0] = 0.0;
+  __syncthreads();
+  for (int p0 = 1; p0 <= N; p0 += RB) {
+    const int nrows = min(RB, N - p0 + 1);
+    for (int r = wv; r < nrows; r += RT / 64) {
+      const int n = p0 + r;
+      double acc = 0, sw = 0;
+      for (int m = lane; m < p0; m += 64) {
+        double c = rec_coef<true>(a, n, m);
+        if (c > NEG) {
+          double w = exp(c + Ls[m] - Ls[n]);
+          acc += w * (res[m] - Bm[tri_row(n) + (n - m) - 1]);
+          sw += w;
+        }
+      }
+      acc = wave_sum(acc);
+      sw = wave_sum(sw);
+      if (lane == 0) {
+        ao[r] = acc;
+        wo[r] = sw;
+      }
+    }
+    for (int w = tid; w < RB * (RB - 1) / 2; w += RT) {
+      int q = (int)((1.0 + sqrt(1.0 + 8.0 * w)) / 2.0);
+      while (q * (q - 1) / 2 > w) --q;
+      while ((q + 1) * q / 2 <= w) ++q;
+      int r = w - q * (q - 1) / 2;
+      double wt = 0, bv = 0;
+      if (q < nrows) {
+        const int n = p0 + q, m = p0 + r;
+        double c = rec_coef<true>(a, n, m);
+        if (c > NEG) wt = exp(c + Ls[m] - Ls[n]);
+        bv = Bm[tri_row(n) + (n - m) - 1];
+      }
+      wd[w] = wt;
+      bd[w] = bv;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      double acc = lane < nrows ? ao[lane] : 0.0;
+      double sw = lane < nrows ? wo[lane] : 0.0;
+      for (int r = 0; r < nrows; ++r) {
+        const double rr = readlane_d(acc, r) / readlane_d(sw, r);
+        if (lane > r && lane < nrows) {
+          const int w = lane * (lane - 1) / 2 + r;
+          acc += wd[w] * (rr - bd[w]);
+          sw += wd[w];
+        }
+      }
+      if (lane < nrows) res[p0 + lane] = acc / sw;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *out = res[N];
+}
+
+// -1/2 sum_{j>=2} (x_{l,j} - x_{l,1}) . f_{l,j} m  partials (md_pimd_x.c:4332-4345)
+__global__ __launch_bounds__(256) void k_vir_term(const double *__restrict__ X, const double *__restrict__ F,
+                                                  const double *__restrict__ M, int N, int P, Geo g,
+                                                  double *__restrict__ part) {
+  __shared__ double red[256];
+  const int nb = N * P;
+  double s = 0;
+  for (int i = N + blockIdx.x * 256 + threadIdx.x; i < nb; i += gridDim.x * 256) {
+    const int l = i % N;
+    const double m = M[i];
+    s += -comp(X[i] - X[l], g.L0, g.image) * F[i] * m;
+    s += -comp(X[nb + i] - X[nb + l], g.L1, g.image) * F[nb + i] * m;
+    s += -comp(X[2 * nb + i] - X[2 * nb + l], g.L2, g.image) * F[2 * nb + i] * m;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Sign-tracked forward recursion at |xi'| (md_pimd_x.c:4675-4701): one
+// workgroup, one n at a time (a per-sample estimator, not per step).
+__global__ __launch_bounds__(256) void k_vi_sign(RecArgs a, double vi2, double *__restrict__ out) {
+  __shared__ double red[256];
+  extern __shared__ double lds[];
+  const int N = a.N;
+  double *V2 = lds, *sg = lds + N + 1;
+  const double av = fabs(vi2), sv = vi2 < 0 ? -1.0 : 1.0;
+  const double lv = av > 0 ? log(av) : 0.0;
+  if (threadIdx.x == 0) {
+    V2[0] = 0;
+    sg[0] = 1;
+  }
+  __syncthreads();
+  for (int n = 1; n <= N; ++n) {
+    double mn = 1e10;
+    if (vi2 == 0.0) {
+      mn = a.beta * (a.ENk[tri_row(n)] + V2[n - 1]);
+    } else {
+      for (int k = 1 + threadIdx.x; k <= n; k += 256) mn = fmin(mn, -(k - 1) * lv + a.beta * (a.ENk[tri_row(n) + k - 1] + V2[n - k]));
+      red[threadIdx.x] = mn;
+      __syncthreads();
+      for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+      }
+      mn = red[0];
+      __syncthreads();
+    }
+    double s = 0;
+    for (int k = 1 + threadIdx.x; k <= n; k += 256) {
+      if (vi2 == 0.0 && k != 1) continue;
+      double sk = ((k - 1) % 2 == 1) ? sv : 1.0;
+      double arg = (vi2 == 0.0 ? 0.0 : (k - 1) * lv) - a.beta * (a.ENk[tri_row(n) + k - 1] + V2[n - k]) + mn;
+      s += sg[n - k] * sk * exp(arg);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double t = red[0];
+      sg[n] = t >= 0 ? 1 : -1;
+      V2[n] = (mn - log(fabs(t)) + log((double)n)) / a.beta;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sg[N] * exp(-a.beta * V2[N] - a.Lf[N]);
+}
+
+// ------------------------------------------------------------ launchers
+
+static Geo geo_of(DevPimd *p, int image) {
+  md_simulation_t_x *sim = p->host->sim;
+  Geo g;
+  g.L0 = box_L(sim, 0);
+  g.L1 = box_L(sim, 1);
+  g.L2 = box_L(sim, 2);
+  g.image = image;
+  return g;
+}
+
+static RecArgs rec_args(DevPimd *p) {
+  RecArgs a;
+  a.ENk = p->ENk;
+  a.ET = p->ET;
+  a.lnn = p->work;  // first N+1 entries of the scratch hold ln n
+  a.Lf = p->Lf;
+  a.Rb = p->Rb;
+  a.VBN = p->VBN;
+  a.N = p->N;
+  a.beta = p->host->beta;
+  a.xi0 = p->host->vi == 0.0;
+  a.lnxi = a.xi0 ? 0.0 : log(p->host->vi);
+  return a;
+}
+
+int launch_enk(DevPimd *p, int image, hipStream_t st) {
+  DevSim *d = p->ds;
+  const int N = p->N, P = p->P;
+  Geo g = geo_of(p, image);
+  double om = p->host->omegaP;
+  double *clink = p->work + (N + 1);
+  hipLaunchKernelGGL(k_chain, dim3((N + 255) / 256), dim3(256), 0, st, d->X, d->M, N, P, om, g, p->Eint, clink,
+                     p->ENk, p->ET);
+  XMD_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_dd, dim3(1), dim3(1024), 0, st, clink, N, p->Cpre);
+  XMD_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_enk_fill, dim3(N), dim3(256), 0, st, d->X, d->M, N, P, om, g, p->Eint, p->Cpre, p->ENk, p->ET);
+  XMD_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_lnn, dim3((N + 256) / 256), dim3(256), 0, st, p->work, N);
+  XMD_CHECK(hipGetLastError());
+  p->enk_dev = true;
+  p->last_image = image;
+  p->enk_version++;
+  return 0;
+}
+
+static size_t rec_lds(int N) { return (RB * (RB - 1) / 2 + 2 * RB + (N + 1)) * sizeof(double); }
+
+int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
+  if (p->N > MAXN_REC) {
+    fprintf(stderr, "xmd: exchange recursion supports N <= %d\n", MAXN_REC);
+    return -1;
+  }
+  RecArgs a = rec_args(p);
+  int nblk = (fwd ? 1 : 0) + (bwd ? 1 : 0);
+  if (nblk == 0) return 0;
+  int mask = (fwd ? 1 : 0) | (bwd ? 2 : 0);
+  hipLaunchKernelGGL(k_recursion, dim3(nblk), dim3(RT), rec_lds(p->N), st, a, mask);
+  XMD_CHECK(hipGetLastError());
+  if (fwd) p->vbn_dev = true;
+  if (bwd) {
+    p->back_dev = true;
+    p->back_version = p->enk_version;
+  }
+  return 0;
+}
+
+int launch_exchange_force(DevPimd *p, int image, hipStream_t st) {
+  DevSim *d = p->ds;
+  Geo g = geo_of(p, image);
+  RecArgs a = rec_args(p);
+  hipLaunchKernelGGL(k_exch_force, dim3(p->N), dim3(256), 0, st, d->X, d->M, d->F, p->N, p->P, p->host->omegaP, g, a);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ENk filled on the host (driver-provided) -> device, both layouts.
+static int ensure_enk(DevPimd *p) {
+  if (p->enk_dev) return 0;
+  size_t tri = (size_t)p->N * (p->N + 1) / 2;
+  hipStream_t st = rt().s0;
+  XMD_CHECK(hipMemcpyAsync(p->ENk, p->host->ENk[0], tri * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_enk_transpose, dim3(p->N), dim3(256), 0, st, p->ENk, p->ET, p->N);
+  XMD_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_lnn, dim3((p->N + 256) / 256), dim3(256), 0, st, p->work, p->N);
+  XMD_CHECK(hipGetLastError());
+  XMD_CHECK(hipStreamSynchronize(st));
+  p->enk_dev = true;
+  p->enk_version++;
+  return 0;
+}
+
+static int ensure_vbn(DevPimd *p) {
+  if (p->vbn_dev) return 0;
+  hipStream_t st = rt().s0;
+  XMD_CHECK(hipMemcpyAsync(p->VBN, p->host->VBN, (p->N + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_vbn_to_L, dim3((p->N + 256) / 256), dim3(256), 0, st, p->VBN, p->Lf, p->N, p->host->beta);
+  XMD_CHECK(hipGetLastError());
+  XMD_CHECK(hipStreamSynchronize(st));
+  p->vbn_dev = true;
+  return 0;
+}
+
+static size_t est_lds(int N) { return (size_t)(RB * (RB - 1) + 2 * RB + 2 * (N + 1)) * sizeof(double); }
+
+static int run_estimator(DevPimd *p, const double *Bm, double *out) {
+  if (est_lds(p->N) > 65536) {
+    fprintf(stderr, "xmd: estimator recursion supports N <= 2015\n");
+    return -1;
+  }
+  RecArgs a = rec_args(p);
+  double *d_out = p->work + 4 * (p->N + 1);
+  hipLaunchKernelGGL(k_estimator, dim3(1), dim3(RT), est_lds(p->N), rt().s0, a, Bm, d_out);
+  XMD_CHECK(hipGetLastError());
+  XMD_CHECK(hipMemcpyAsync(out, d_out, sizeof(double), hipMemcpyDeviceToHost, rt().s0));
+  XMD_CHECK(hipStreamSynchronize(rt().s0));
+  return 0;
+}
+
+static int fill_enk2(DevPimd *p, int image, int forward) {
+  DevSim *d = p->ds;
+  const int N = p->N, P = p->P;
+  Geo g = geo_of(p, image);
+  double om = p->host->omegaP;
+  hipStream_t st = rt().s0;
+  double *link = p->work + 2 * (N + 1);
+  hipLaunchKernelGGL(k_chain2, dim3((N + 255) / 256), dim3(256), 0, st, d->X, d->M, N, P, om, g, forward, link);
+  XMD_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_dd, dim3(1), dim3(1024), 0, st, link, N, p->C2pre);
+  XMD_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_enk2_fill, dim3(N), dim3(256), 0, st, d->X, d->M, N, P, om, g, forward, p->C2pre, p->ENk2);
+  XMD_CHECK(hipGetLastError());
+  p->enk2_dev = true;
+  return 0;
+}
+
+int connection_matrix(DevPimd *p, double *host_G) {
+  size_t nn = (size_t)p->N * p->N;
+  if (!p->Gm) XMD_CHECK(hipMalloc(&p->Gm, nn * sizeof(double)));
+  RecArgs a = rec_args(p);
+  hipLaunchKernelGGL(k_connection_matrix, dim3(p->N), dim3(256), 0, rt().s0, a, p->Gm);
+  XMD_CHECK(hipGetLastError());
+  XMD_CHECK(hipMemcpyAsync(host_G, p->Gm, nn * sizeof(double), hipMemcpyDeviceToHost, rt().s0));
+  XMD_CHECK(hipStreamSynchronize(rt().s0));
+  return 0;
+}
+
+int backward_potentials(DevPimd *p, double *host_Vt) {
+  std::vector<double> R(p->N + 1);
+  XMD_CHECK(hipMemcpy(R.data(), p->Rb, (p->N + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i <= p->N; ++i) host_Vt[i] = -R[i] / p->host->beta;
+  return 0;
+}
+
+}  // namespace xmd
+
+using namespace xmd;
+
+extern "C" {
+
+int md_pimd_fast_fill_ENk_x(md_pimd_t_x *pimd, int image) {
+  if (ensure_pimd_device(pimd)) return -1;
+  return launch_enk(pimd_state(pimd, false), image, rt().s0);
+}
+
+// The cycle-walk algorithm (md_pimd_x.c:593-665) defines the same E_n^(k);
+// the engine fills it with the closed form above.
+int md_pimd_fill_ENk_x(md_pimd_t_x *pimd, int image) { return md_pimd_fast_fill_ENk_x(pimd, image); }
+
+int md_pimd_fill_VB_x(md_pimd_t_x *pimd) {
+  if (ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  if (ensure_enk(p)) return -1;
+  // forward and backward passes are independent: one launch, two workgroups
+  return launch_recursion(p, true, true, rt().s0);
+}
+
+int md_pimd_fast_fill_force_VB_x(md_pimd_t_x *pimd, int image) {
+  if (ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  if (ensure_enk(p) || ensure_vbn(p)) return -1;
+  if (!p->back_dev || p->back_version != p->enk_version)
+    if (launch_recursion(p, false, true, rt().s0)) return -1;
+  md_simulation_t_x *sim = pimd->sim;
+  double L[3] = {box_L(sim, 0), box_L(sim, 1), box_L(sim, 2)};
+  if (launch_ring_springs(p, image, L, rt().s0)) return -1;
+  return launch_exchange_force(p, image, rt().s0);
+}
+
+// The forward-mode gradient of md_pimd_x.c:997-1109 gives the same forces
+// as the backward-pass formulation.
+int md_pimd_fill_force_VB_x(md_pimd_t_x *pimd, int image) { return md_pimd_fast_fill_force_VB_x(pimd, image); }
+
+int md_pimd_calc_VBN_energy_x(md_pimd_t_x *pimd, md_stats_t_x *stats) {
+  if (ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  if (ensure_enk(p) || ensure_vbn(p)) return -1;
+  double r = 0;
+  if (run_estimator(p, p->ENk, &r)) return -1;
+  stats->es[stats->N - 1] += r;
+  stats->es[stats->N - 1] += MD_DIMENSION_X * pimd->N * pimd->P / (2 * pimd->beta);
+  return 0;
+}
+
+int md_pimd_fast_fill_ENk2_x(md_pimd_t_x *pimd, int image) {
+  if (ensure_pimd_device(pimd)) return -1;
+  return fill_enk2(pimd_state(pimd, false), image, 0);
+}
+
+int md_pimd_fill_ENk2_x(md_pimd_t_x *pimd, int image) {
+  if (ensure_pimd_device(pimd)) return -1;
+  return fill_enk2(pimd_state(pimd, false), image, 1);
+}
+
+int md_pimd_calc_vi_virial_energy_x(md_pimd_t_x *pimd, md_stats_t_x *stats, int image) {
+  if (ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  if (ensure_enk(p) || ensure_vbn(p)) return -1;
+  if (!p->enk2_dev) {
+    size_t tri = (size_t)p->N * (p->N + 1) / 2;
+    XMD_CHECK(hipMemcpy(p->ENk2, pimd->ENk2[0], tri * sizeof(double), hipMemcpyHostToDevice));
+    p->enk2_dev = true;
+  }
+  double r = 0;
+  if (run_estimator(p, p->ENk2, &r)) return -1;
+  DevSim *d = p->ds;
+  int nblk = (int)std::min<long>((p->N * (long)(p->P - 1) + 255) / 256, 1024);
+  if (nblk < 1) nblk = 1;
+  hipLaunchKernelGGL(k_vir_term, dim3(nblk), dim3(256), 0, rt().s0, d->X, d->F, d->M, p->N, p->P, geo_of(p, image),
+                     d->red);
+  XMD_CHECK(hipGetLastError());
+  double v = 0;
+  if (device_sum(d->red, nblk, &v)) return -1;
+  stats->es[stats->N - 1] += r;
+  stats->es[stats->N - 1] += MD_DIMENSION_X * pimd->N / (2 * pimd->beta);
+  stats->es[stats->N - 1] += v / 2.0;
+  return 0;
+}
+
+int md_pimd_calc_vi_sign_x(md_pimd_t_x *pimd, md_stats_t_x *e_s, md_stats_t_x *s_s, md_num_t_x vi2) {
+  if (ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  if (ensure_enk(p) || ensure_vbn(p)) return -1;
+  RecArgs a = rec_args(p);
+  double *d_out = p->work + 4 * (p->N + 1);
+  if (2 * (p->N + 1) * sizeof(double) > 65536) {
+    fprintf(stderr, "xmd: sign reweighting supports N <= 4094\n");
+    return -1;
+  }
+  hipLaunchKernelGGL(k_vi_sign, dim3(1), dim3(256), 2 * (p->N + 1) * sizeof(double), rt().s0, a, vi2, d_out);
+  XMD_CHECK(hipGetLastError());
+  double s = 0;
+  XMD_CHECK(hipMemcpyAsync(&s, d_out, sizeof(double), hipMemcpyDeviceToHost, rt().s0));
+  XMD_CHECK(hipStreamSynchronize(rt().s0));
+  e_s->es[0] *= s;
+  s_s->es[0] *= s;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+// The fused MD step and its hipGraph capture, plus run-time knobs.
+//
+// Stream layout of one step (reference driver order, SURVEY call stack B):
+//   s0: NHC half 1 (+ wrap) --fork--> reset F, trap, ring springs, pair real,
+//       pair reciprocal --wait join--> exchange end-bead forces, NHC half 2
+//   s1: (after fork) cycle energies -> forward + backward recursions --join-->
+// The exchange branch only reads positions, so it overlaps the pair forces;
+// the end-bead exchange forces add into F after both branches are done.
+#include <cmath>
+
+#include "md_hip_x.h"
+#include "xmd_internal.h"
+
+namespace xmd {
+
+struct StepPlan {
+  DevPimd *p = nullptr;
+  int trap = 0;
+  int pair_kind = -1;
+  double dt = 0;
+  int pbc = 0, image = 0;
+  double L[3] = {0, 0, 0};
+};
+
+static int plan_of(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, StepPlan &sp) {
+  if (ensure_pimd_device(pimd)) return -1;
+  sp.p = pimd_state(pimd, false);
+  if (cfg->trap && cfg->trap != md_harmonic_trap_force_x) {
+    fprintf(stderr, "xmd: trap callback has no device kernel\n");
+    return -1;
+  }
+  sp.trap = cfg->trap != nullptr;
+  sp.pair_kind = -1;
+  if (cfg->pair) {
+    sp.pair_kind = pair_kind_of(cfg->pair);
+    if (sp.pair_kind < 0) {
+      fprintf(stderr, "xmd: pair-force callback has no device kernel\n");
+      return -1;
+    }
+  }
+  sp.dt = cfg->dt;
+  sp.pbc = cfg->pbc;
+  sp.image = cfg->image;
+  for (int d = 0; d < 3; ++d) sp.L[d] = box_L(pimd->sim, d);
+  return 0;
+}
+
+// ev: optional per-stage events (2 per stage: start, stop), may be null.
+static int enqueue_forces(StepPlan &sp, hipEvent_t *ev) {
+  Runtime &R = rt();
+  DevPimd *p = sp.p;
+  DevSim *d = p->ds;
+  auto rec = [&](int k, hipStream_t st) -> int {
+    if (ev) XMD_CHECK(hipEventRecord(ev[k], st));
+    return 0;
+  };
+  XMD_CHECK(hipEventRecord(R.fork, R.s0));
+  XMD_CHECK(hipStreamWaitEvent(R.s1, R.fork, 0));
+  // exchange branch
+  if (rec(2, R.s1) || launch_enk(p, sp.image, R.s1) || rec(3, R.s1)) return -1;
+  if (rec(4, R.s1) || launch_recursion(p, true, true, R.s1) || rec(5, R.s1)) return -1;
+  XMD_CHECK(hipEventRecord(R.join, R.s1));
+  // bead-local and pair forces
+  if (rec(6, R.s0) || launch_reset_force(d, R.s0)) return -1;
+  if (sp.trap && launch_trap_force(d, md_trap_frequency_x, md_trap_center_x, (double)p->P, R.s0)) return -1;
+  if (launch_ring_springs(p, sp.image, sp.L, R.s0) || rec(7, R.s0)) return -1;
+  if (sp.pair_kind >= 0) {
+    if (rec(8, R.s0) || launch_pair_real(p, sp.pair_kind, R.s0) || rec(9, R.s0)) return -1;
+    if (pair_has_recip(sp.pair_kind)) {
+      if (rec(10, R.s0) || launch_pair_recip(p, R.s0) || rec(11, R.s0)) return -1;
+    } else if (rec(10, R.s0) || rec(11, R.s0)) {
+      return -1;
+    }
+  } else if (rec(8, R.s0) || rec(9, R.s0) || rec(10, R.s0) || rec(11, R.s0)) {
+    return -1;
+  }
+  XMD_CHECK(hipStreamWaitEvent(R.s0, R.join, 0));
+  if (rec(12, R.s0) || launch_exchange_force(p, sp.image, R.s0) || rec(13, R.s0)) return -1;
+  return 0;
+}
+
+static int enqueue_step(StepPlan &sp, hipEvent_t *ev) {
+  Runtime &R = rt();
+  DevSim *d = sp.p->ds;
+  if (ev) XMD_CHECK(hipEventRecord(ev[0], R.s0));
+  if (launch_nhc(d, 1, sp.dt, R.s0)) return -1;
+  if (sp.pbc && launch_pbc(d, sp.L, R.s0)) return -1;
+  if (ev) XMD_CHECK(hipEventRecord(ev[1], R.s0));
+  if (enqueue_forces(sp, ev)) return -1;
+  if (ev) XMD_CHECK(hipEventRecord(ev[14], R.s0));
+  if (launch_nhc(d, 2, sp.dt, R.s0)) return -1;
+  if (ev) XMD_CHECK(hipEventRecord(ev[15], R.s0));
+  return 0;
+}
+
+}  // namespace xmd
+
+using namespace xmd;
+
+extern "C" {
+
+int md_set_nhc_length_x(int M) {
+  if (M < 2 || M > MD_NHC_LENGTH_X || M % 2) return -1;
+  g_nhc_len = M;
+  return 0;
+}
+
+int md_get_nhc_length_x(void) { return g_nhc_len; }
+
+int md_set_ewald_real_cutoff_x(md_num_t_x x_cut) {
+  if (!(x_cut > 0)) return -1;
+  g_ewald_xcut = x_cut;
+  return 0;
+}
+
+int md_hip_synchronize_x(void) {
+  if (ensure_runtime()) return -1;
+  XMD_CHECK(hipStreamSynchronize(rt().s1));
+  XMD_CHECK(hipStreamSynchronize(rt().s0));
+  return 0;
+}
+
+int md_pimd_get_backward_potentials_x(md_pimd_t_x *pimd, md_num_t_x *Vt) {
+  DevPimd *p = pimd_state(pimd, false);
+  if (!p || !p->back_dev) return -1;
+  XMD_CHECK(hipStreamSynchronize(rt().s0));
+  return backward_potentials(p, Vt);
+}
+
+int md_pimd_get_connection_matrix_x(md_pimd_t_x *pimd, md_num_t_x *G) {
+  DevPimd *p = pimd_state(pimd, false);
+  if (!p || !p->back_dev || !p->vbn_dev) return -1;
+  return connection_matrix(p, G);
+}
+
+int md_pimd_compute_forces_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg) {
+  StepPlan sp;
+  if (plan_of(pimd, cfg, sp)) return -1;
+  return enqueue_forces(sp, nullptr);
+}
+
+int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps) {
+  StepPlan sp;
+  if (plan_of(pimd, cfg, sp)) return -1;
+  if (nsteps <= 0) return 0;
+  if (!cfg->use_graph || nsteps == 1) {
+    for (int s = 0; s < nsteps; ++s)
+      if (enqueue_step(sp, nullptr)) return -1;
+    return 0;
+  }
+  // The first step runs eagerly: it allocates every lazily created buffer,
+  // so the captured graph contains kernel nodes only.
+  if (enqueue_step(sp, nullptr)) return -1;
+  Runtime &R = rt();
+  XMD_CHECK(hipStreamSynchronize(R.s0));
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
+  int rc = enqueue_step(sp, nullptr);
+  hipError_t ec = hipStreamEndCapture(R.s0, &graph);
+  if (rc || ec != hipSuccess) {
+    fprintf(stderr, "xmd: step capture failed\n");
+    return -1;
+  }
+  XMD_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  for (int s = 1; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
+  XMD_CHECK(hipStreamSynchronize(R.s0));
+  XMD_CHECK(hipGraphExecDestroy(exec));
+  XMD_CHECK(hipGraphDestroy(graph));
+  return 0;
+}
+
+int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, double *stage_ms) {
+  StepPlan sp;
+  if (plan_of(pimd, cfg, sp)) return -1;
+  const int NE = 16;
+  hipEvent_t ev[NE];
+  for (int k = 0; k < NE; ++k) XMD_CHECK(hipEventCreate(&ev[k]));
+  for (int k = 0; k < MD_PIMD_NSTAGES_X; ++k) stage_ms[k] = 0;
+  // stage k <- events (a, b)
+  const int pairs[8][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {10, 11}, {12, 13}, {14, 15}};
+  for (int s = 0; s < nsteps; ++s) {
+    if (enqueue_step(sp, ev)) return -1;
+    XMD_CHECK(hipStreamSynchronize(rt().s0));
+    for (int k = 0; k < 8; ++k) {
+      float ms = 0;
+      XMD_CHECK(hipEventElapsedTime(&ms, ev[pairs[k][0]], ev[pairs[k][1]]));
+      stage_ms[k] += ms;
+    }
+    float ms = 0;
+    XMD_CHECK(hipEventElapsedTime(&ms, ev[0], ev[15]));
+    stage_ms[8] += ms;
+  }
+  for (int k = 0; k < MD_PIMD_NSTAGES_X; ++k) stage_ms[k] /= nsteps;
+  for (int k = 0; k < NE; ++k) XMD_CHECK(hipEventDestroy(ev[k]));
+  return 0;
+}
+
+}  // extern "C"

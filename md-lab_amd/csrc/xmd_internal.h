@@ -1,0 +1,153 @@
+// Internal declarations of the MI355X PIMD engine (not installed).
+//
+// Device state model
+// ------------------
+// The public structs keep the reference's host layout (AoS beads,
+// particle-major).  Each md_simulation_t_x / md_pimd_t_x / md_stats_t_x gets a
+// side record here that owns its device mirror.  The device copy is
+// authoritative once a GPU entry point has touched the object and until
+// md_simulation_sync_host_x(sim, 0) hands authority back to the host, which
+// is the same contract as the reference's OpenCL build
+// (md_util_x.c:481-551).
+//
+// Device layout (structure of arrays, FP64):
+//   bead (l, j) (1-based) lives at slot s = (j-1)*N + (l-1)  ("slice-major"),
+//   coordinate d of slot s at X[d*nb + s]; same for V and F.  One bead slice
+//   is a contiguous run of N slots, which is what the pair kernels tile over.
+//   A simulation without a PIMD object uses N = nb, P = 1 (identity order).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+extern "C" {
+#include "md_pimd_x.h"
+}
+
+#define XMD_CHECK(call)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      fprintf(stderr, "xmd: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return -1;                                                                           \
+    }                                                                                      \
+  } while (0)
+
+#define XMD_CHECK_NULL(call)                                                               \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      fprintf(stderr, "xmd: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return NULL;                                                                         \
+    }                                                                                      \
+  } while (0)
+
+namespace xmd {
+
+// ------------------------------------------------------------------ runtime
+struct Runtime {
+  bool ready = false;
+  int device = 0;
+  hipStream_t s0 = nullptr;   // main stream (all API work is ordered on it)
+  hipStream_t s1 = nullptr;   // exchange branch of the fused step
+  hipEvent_t fork = nullptr, join = nullptr;
+  int num_cus = 0;
+};
+Runtime &rt();
+int ensure_runtime();
+
+// Run-time chain length of the NHC integrator (even, <= MD_NHC_LENGTH_X).
+extern int g_nhc_len;
+
+// -------------------------------------------------------------- device sim
+struct DevSim {
+  md_simulation_t_x *host = nullptr;
+  int nb = 0;            // beads
+  int N = 0, P = 1;      // layout (particles per slice, slices)
+  bool dev_valid = false;
+  double *X = nullptr, *V = nullptr, *F = nullptr, *M = nullptr;  // SoA, 3*nb / nb
+  void *stage = nullptr;                                          // AoS staging (nb*80 B)
+  // thermostat chains
+  int Nf = 0;
+  bool massive = false;   // one chain per degree of freedom, in device DOF order
+  int Mlen = 8;
+  double *th = nullptr, *vth = nullptr, *Q = nullptr;  // [k*Nf + chain]
+  int *gf = nullptr;      // general mode: chain target DOF count (nhcs[i].f)
+  int *gf0 = nullptr, *gf2 = nullptr;  // general mode: DOF range per chain (reference order)
+  void *nstage = nullptr; // AoS chain staging (Nf*200 B)
+  double T = 0;
+  bool uniform_mass = true;
+  double *red = nullptr;  // reduction scratch (partials + result)
+  int red_len = 0;
+};
+
+struct DevPimd {
+  md_pimd_t_x *host = nullptr;
+  DevSim *ds = nullptr;
+  int N = 0, P = 0;
+  double *ENk = nullptr;    // reference row layout: E_n^(k) at (n-1)n/2 + k-1
+  double *ET = nullptr;     // column layout: cycle of particles m+1..m+k at off(m) + k-1
+  double *ENk2 = nullptr;
+  double *Eint = nullptr;   // N
+  double *Cpre = nullptr;   // chain prefix sums, double-double (2*(N+1))
+  double *C2pre = nullptr;  // ENk2 link prefix sums, double-double (2*(N+1))
+  double *VBN = nullptr;    // N+1
+  double *Lf = nullptr;     // forward  L(n) = -beta V_B^(n), N+1
+  double *Rb = nullptr;     // backward R(m) = -beta Vt^(m),  N+1
+  double *Gm = nullptr;     // optional connection matrix (debug/tests), N*N
+  double *work = nullptr;   // estimator scratch
+  bool enk_dev = false, enk2_dev = false, vbn_dev = false, back_dev = false;
+  bool enk_host_stale = false;
+  unsigned enk_version = 0, back_version = 0;
+  int last_image = 0;
+};
+
+DevSim *sim_state(md_simulation_t_x *sim, bool create);
+DevPimd *pimd_state(md_pimd_t_x *pimd, bool create);
+void forget_sim(md_simulation_t_x *sim);
+int set_sim_layout(md_simulation_t_x *sim, int N, int P);
+int ensure_device(md_simulation_t_x *sim);          // upload if host is authoritative
+int ensure_pimd_device(md_pimd_t_x *pimd);
+int download_sim(DevSim *d);                         // device -> host mirror
+int download_pimd(DevPimd *p);
+
+// reductions: sum of n doubles on device into *out (host), deterministic
+int device_sum(const double *src, int n, double *out);
+
+inline double box_L(const md_simulation_t_x *sim, int d) {
+  if (sim->box && sim->box->type == MD_ND_RECT_BOX_X && sim->box->box)
+    return ((nd_rect_t_x *)sim->box->box)->L[d];
+  return 0.0;
+}
+
+// ------------------------------------------------------------ kernel entry
+// Launchers (enqueue on `st`; return 0 / -1).
+int launch_reset_force(DevSim *d, hipStream_t st);
+int launch_trap_force(DevSim *d, double w, const double *c, double invP, hipStream_t st);
+int launch_pbc(DevSim *d, const double *L, hipStream_t st);
+int launch_nhc(DevSim *d, int half, double h, hipStream_t st);
+int launch_ring_springs(DevPimd *p, int image, const double *L, hipStream_t st);
+
+enum PairKind { PK_GAUSS = 0, PK_PGAUSS = 1, PK_EWALD = 2, PK_EWALD_R = 3, PK_EWALD_NI = 4 };
+int pair_kind_of(md_pair_force_t_x pf);
+int pair_energy_kind_of(md_pair_energy_t_x pe);
+int launch_pair_force(DevPimd *p, int kind, hipStream_t st);
+int launch_pair_real(DevPimd *p, int kind, hipStream_t st);
+int launch_pair_recip(DevPimd *p, hipStream_t st);
+bool pair_has_recip(int kind);
+int pair_energy(DevPimd *p, int kind, double *out);
+
+int launch_enk(DevPimd *p, int image, hipStream_t st);
+int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st);
+int launch_exchange_force(DevPimd *p, int image, hipStream_t st);
+int connection_matrix(DevPimd *p, double *host_G);
+int backward_potentials(DevPimd *p, double *host_Vt);
+int bead_sum(DevSim *d, int mode, const double *trap_c, double w, const double *L, double *out);
+extern double g_ewald_xcut;
+
+
+}  // namespace xmd

@@ -1,0 +1,328 @@
+/*
+ * Golden-vector generator.  Links the reference's own nocl library
+ * (oracle/_ref/libxmd_ref.so, compiled from the reference sources by
+ * oracle/Makefile) and writes inputs and outputs of every per-step stage for
+ * small seeded systems into a binary fixture file:
+ *
+ *   record := char name[64] (NUL padded) | int64 count | double data[count]
+ *
+ * Run in the build container only (the reference tree is not shipped):
+ *   make -C oracle ref && oracle/_ref/make_golden tests/golden/golden.bin
+ */
+#include "md_pimd_x.h"
+#include "md_simulation_x.h"
+#include "md_statistics_x.h"
+#include "md_util_x.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static FILE *out;
+
+static void dump(const char *name, const double *d, int64_t n) {
+  char buf[64];
+  memset(buf, 0, sizeof buf);
+  strncpy(buf, name, 63);
+  fwrite(buf, 1, 64, out);
+  fwrite(&n, sizeof n, 1, out);
+  fwrite(d, sizeof(double), (size_t)n, out);
+}
+
+static void dump1(const char *name, double v) { dump(name, &v, 1); }
+
+static void dumpf(const char *prefix, const char *what, const double *d, int64_t n) {
+  char name[64];
+  snprintf(name, sizeof name, "%s/%s", prefix, what);
+  dump(name, d, n);
+}
+
+/* particles -> flat arrays in reference bead order */
+static void dump_beads(const char *prefix, const char *tag, md_simulation_t_x *sim) {
+  int nb = sim->N;
+  double *a = malloc(sizeof(double) * 3 * nb);
+  char w[48];
+  const char *fld[3] = {"x", "v", "f"};
+  for (int k = 0; k < 3; ++k) {
+    for (int i = 0; i < nb; ++i)
+      for (int d = 0; d < 3; ++d) {
+        const md_particle_t_x *p = &sim->particles[i];
+        a[3 * i + d] = k == 0 ? p->x[d] : (k == 1 ? p->v[d] : p->f[d]);
+      }
+    snprintf(w, sizeof w, "%s%s", fld[k], tag);
+    dumpf(prefix, w, a, 3 * nb);
+  }
+  for (int i = 0; i < nb; ++i) a[i] = sim->particles[i].m;
+  snprintf(w, sizeof w, "m%s", tag);
+  dumpf(prefix, w, a, nb);
+  free(a);
+}
+
+static void dump_forces(const char *prefix, const char *what, md_simulation_t_x *sim) {
+  int nb = sim->N;
+  double *a = malloc(sizeof(double) * 3 * nb);
+  for (int i = 0; i < nb; ++i)
+    for (int d = 0; d < 3; ++d) a[3 * i + d] = sim->particles[i].f[d];
+  dumpf(prefix, what, a, 3 * nb);
+  free(a);
+}
+
+static void dump_nhc(const char *prefix, const char *what, md_simulation_t_x *sim) {
+  int rec = 1 + 3 * MD_NHC_LENGTH_X;
+  double *a = malloc(sizeof(double) * rec * sim->Nf);
+  for (int i = 0; i < sim->Nf; ++i) {
+    double *r = &a[rec * i];
+    r[0] = sim->nhcs[i].f;
+    for (int j = 0; j < MD_NHC_LENGTH_X; ++j) {
+      r[1 + j] = sim->nhcs[i].theta[j];
+      r[1 + MD_NHC_LENGTH_X + j] = sim->nhcs[i].vtheta[j];
+      r[1 + 2 * MD_NHC_LENGTH_X + j] = sim->nhcs[i].Q[j];
+    }
+  }
+  dumpf(prefix, what, a, (int64_t)rec * sim->Nf);
+  free(a);
+}
+
+static void dump_tri(const char *prefix, const char *what, md_num_t_x **T, int N) {
+  int64_t n = (int64_t)N * (N + 1) / 2;
+  dumpf(prefix, what, T[0], n);
+}
+
+static double stat_of(md_stats_t_x *s) { return s->es[s->N - 1]; }
+
+typedef struct {
+  const char *name;
+  int N, P;
+  double T, vi;
+  int kind;          /* 0 trap+gaussian, 1 UEG ewald */
+  int image;
+  double L;          /* box edge (UEG) */
+  int nsum;          /* Ewald cube half-width */
+  int fgroup;        /* DOFs per thermostat chain */
+  int steps;
+  double dt;
+  unsigned seed;
+} gcase;
+
+static md_pair_force_t_x pair_force_of(const gcase *c) {
+  return c->kind == 0 ? md_gaussian_force_x : md_coulomb_3d_ewald_force_x;
+}
+
+static md_pair_energy_t_x pair_energy_of(const gcase *c) {
+  return c->kind == 0 ? md_gaussian_energy_x : md_coulomb_3d_ewald_energy_x;
+}
+
+static void set_params(const gcase *c) {
+  md_num_t_x zero[3] = {0, 0, 0};
+  md_set_harmonic_trap_frequency_x(1.0);
+  md_set_harmonic_trap_center_x(zero);
+  md_set_gaussian_force_strength_x(3.0);
+  md_set_guassian_force_range_x(0.5);
+  md_set_coulomb_force_strength_x(1.0);
+  md_set_coulomb_truc_x(5.0 / c->L);
+  md_set_coulomb_n_sum_x(c->nsum);
+}
+
+static void full_force(const gcase *c, md_pimd_t_x *pimd) {
+  md_reset_force_x(pimd->sim);
+  if (c->kind == 0) md_pimd_calc_trap_force_x(pimd, md_harmonic_trap_force_x);
+  md_pimd_calc_pair_force_x(pimd, pair_force_of(c));
+  md_pimd_fast_fill_ENk_x(pimd, c->image);
+  md_pimd_fill_VB_x(pimd);
+  md_pimd_fast_fill_force_VB_x(pimd, c->image);
+}
+
+static void run_case(const gcase *c) {
+  const char *pf = c->name;
+  int N = c->N, P = c->P, nb = N * P;
+  set_params(c);
+  md_set_seed_x(c->seed);
+  md_simulation_t_x *sim = md_alloc_simulation_x(nb, c->fgroup, 0, MD_ND_RECT_BOX_X);
+  md_num_t_x L3[3] = {c->L, c->L, c->L};
+  md_init_nd_rect_box_x(sim->box, L3);
+  md_pimd_t_x *pimd = md_alloc_pimd_x(sim, N, P, c->vi, c->T);
+  md_init_particle_mass_x(sim, 1.0, 0, nb);
+  if (c->kind == 0) {
+    /* positions i.i.d. Normal(0,1): particle l, bead j, then dimension */
+    for (int l = 1; l <= N; ++l)
+      for (int j = 1; j <= P; ++j)
+        for (int d = 0; d < 3; ++d) sim->particles[MD_PIMD_INDEX_X(l, j, P)].x[d] = md_random_gaussian_x();
+  } else {
+    md_num_t_x ctr[3] = {c->L / 2, c->L / 2, c->L / 2};
+    md_pimd_init_particle_uniform_pos_x(pimd, ctr, L3, 0.4);
+    md_periodic_boundary_x(sim);
+  }
+  md_init_maxwell_vel_x(sim, c->T);
+  md_init_nhc_x(sim, md_get_quick_Q_x(c->T, pimd->omegaP), 0, sim->Nf, 0);
+  for (int i = 0; i < nb; ++i)
+    for (int d = 0; d < 3; ++d) sim->particles[i].f[d] = 0;
+
+  double meta[10] = {N, P, c->T, c->vi, c->kind, c->image, c->L, c->nsum, c->fgroup, c->dt};
+  dumpf(pf, "meta", meta, 10);
+  dumpf(pf, "beta_omegaP", (double[]){pimd->beta, pimd->omegaP}, 2);
+  dump_beads(pf, "0", sim);
+  dump_nhc(pf, "nhc0", sim);
+
+  /* stage forces, each from zero */
+  if (c->kind == 0) {
+    md_reset_force_x(sim);
+    md_pimd_calc_trap_force_x(pimd, md_harmonic_trap_force_x);
+    dump_forces(pf, "f_trap", sim);
+  }
+  md_reset_force_x(sim);
+  md_pimd_calc_pair_force_x(pimd, pair_force_of(c));
+  dump_forces(pf, "f_pair", sim);
+  if (c->kind == 1) {
+    md_reset_force_x(sim);
+    md_pimd_calc_pair_force_x(pimd, md_coulomb_3d_ewald_force_R_x);
+    dump_forces(pf, "f_pair_R", sim);
+    md_reset_force_x(sim);
+    md_pimd_calc_pair_force_x(pimd, md_coulomb_3d_ewald_force_NI_x);
+    dump_forces(pf, "f_pair_NI", sim);
+    dumpf(pf, "madelung", (double[]){md_calc_3d_ueg_madelung_x(L3)}, 1);
+  }
+  md_pimd_fill_ENk_x(pimd, c->image);
+  dump_tri(pf, "ENk_slow", pimd->ENk, N);
+  md_pimd_fast_fill_ENk_x(pimd, c->image);
+  dump_tri(pf, "ENk", pimd->ENk, N);
+  dumpf(pf, "Eint", pimd->Eint, N);
+  md_pimd_fill_VB_x(pimd);
+  dumpf(pf, "VBN", pimd->VBN, N + 1);
+  md_reset_force_x(sim);
+  md_pimd_fast_fill_force_VB_x(pimd, c->image);
+  dump_forces(pf, "f_exch", sim);
+  if (N <= 16) {
+    md_reset_force_x(sim);
+    md_pimd_fill_force_VB_x(pimd, c->image);
+    dump_forces(pf, "f_exch_slow", sim);
+  }
+
+  /* estimators on the full force */
+  full_force(c, pimd);
+  dump_forces(pf, "f_total", sim);
+  md_stats_t_x *st = md_alloc_stats_x(1);
+  md_pimd_calc_VBN_energy_x(pimd, st);
+  dumpf(pf, "e_VBN", (double[]){stat_of(st)}, 1);
+  md_pimd_fast_fill_ENk2_x(pimd, c->image);
+  dump_tri(pf, "ENk2", pimd->ENk2, N);
+  st->es[0] = 0;
+  md_pimd_calc_vi_virial_energy_x(pimd, st, c->image);
+  dumpf(pf, "e_vir", (double[]){stat_of(st)}, 1);
+  md_pimd_fill_ENk2_x(pimd, c->image);
+  dump_tri(pf, "ENk2_slow", pimd->ENk2, N);
+  st->es[0] = 0;
+  md_pimd_calc_vi_virial_energy_x(pimd, st, c->image);
+  dumpf(pf, "e_vir_slow", (double[]){stat_of(st)}, 1);
+  if (c->kind == 0) {
+    st->es[0] = 0;
+    md_pimd_calc_trap_energy_x(pimd, md_harmonic_trap_energy_x, st);
+    dumpf(pf, "e_trap", (double[]){stat_of(st)}, 1);
+  }
+  st->es[0] = 0;
+  md_pimd_calc_pair_energy_x(pimd, pair_energy_of(c), st);
+  dumpf(pf, "e_pair", (double[]){stat_of(st)}, 1);
+  st->es[0] = 0;
+  md_calc_kinetic_x(sim, st);
+  dumpf(pf, "e_kin", (double[]){stat_of(st)}, 1);
+  st->es[0] = 0;
+  md_calc_nhc_kinetic_x(sim, st);
+  dumpf(pf, "e_nhc", (double[]){stat_of(st)}, 1);
+  st->es[0] = 0;
+  md_pimd_calc_virial_energy_x(pimd, st);
+  dumpf(pf, "e_virial_plain", (double[]){stat_of(st)}, 1);
+  st->es[0] = 0;
+  md_pimd_calc_centroid_virial_energy_x(pimd, st, 3);
+  dumpf(pf, "e_virial_centroid", (double[]){stat_of(st)}, 1);
+  {
+    double vis[4] = {-1.0, -0.5, 0.0, 0.5}, sgn[4];
+    md_stats_t_x *es = md_alloc_stats_x(1), *ss = md_alloc_stats_x(1);
+    for (int q = 0; q < 4; ++q) {
+      es->es[0] = 1;
+      ss->es[0] = 1;
+      md_pimd_calc_vi_sign_x(pimd, es, ss, vis[q]);
+      sgn[q] = ss->es[0];
+    }
+    dumpf(pf, "vi_sign_args", vis, 4);
+    dumpf(pf, "vi_sign", sgn, 4);
+  }
+
+  /* trajectory of full MD steps (velocity Verlet with NHC) */
+  double *ev = malloc(sizeof(double) * (c->steps > 0 ? c->steps : 1));
+  for (int s = 0; s < c->steps; ++s) {
+    md_update_nhc_VV3_1_x(sim, c->dt);
+    if (c->kind == 1) md_periodic_boundary_x(sim);
+    full_force(c, pimd);
+    md_update_nhc_VV3_2_x(sim, c->dt);
+    md_pimd_fast_fill_ENk2_x(pimd, c->image);
+    st->es[0] = 0;
+    md_pimd_calc_vi_virial_energy_x(pimd, st, c->image);
+    ev[s] = stat_of(st);
+  }
+  if (c->steps > 0) {
+    dump_beads(pf, "_end", sim);
+    dump_nhc(pf, "nhc_end", sim);
+    dumpf(pf, "e_vir_series", ev, c->steps);
+  }
+  free(ev);
+}
+
+static void rng_case(void) {
+  md_set_seed_x(1234);
+  double u[8], g[9];
+  for (int i = 0; i < 8; ++i) u[i] = md_random_uniform_x(-1.0, 2.0);
+  for (int i = 0; i < 9; ++i) g[i] = md_random_gaussian_x();
+  dump("rng/uniform", u, 8);
+  dump("rng/gaussian", g, 9);
+  /* Box-Muller cache is now primed with one value: consume it. */
+  md_random_gaussian_x();
+  md_set_seed_x(77);
+  md_simulation_t_x *sim = md_alloc_simulation_x(12, 3, 0, MD_ND_RECT_BOX_X);
+  md_init_particle_mass_x(sim, 1.0, 0, 6);
+  md_init_particle_mass_x(sim, 2.5, 6, 12);
+  md_init_maxwell_vel_x(sim, 0.7);
+  double v[36];
+  for (int i = 0; i < 12; ++i)
+    for (int d = 0; d < 3; ++d) v[3 * i + d] = sim->particles[i].v[d];
+  dump("rng/maxwell_v", v, 36);
+  md_set_seed_x(5);
+  md_simulation_t_x *s2 = md_alloc_simulation_x(15, 1, 0, MD_ND_RECT_BOX_X);
+  md_pimd_t_x *p2 = md_alloc_pimd_x(s2, 5, 3, 1.0, 1.0);
+  md_num_t_x ctr[3] = {0.5, 1.0, -0.5}, L[3] = {2, 3, 4};
+  md_pimd_init_particle_uniform_pos_x(p2, ctr, L, 0.1);
+  double x[45];
+  for (int i = 0; i < 15; ++i)
+    for (int d = 0; d < 3; ++d) x[3 * i + d] = s2->particles[i].x[d];
+  dump("rng/uniform_pos", x, 45);
+}
+
+int main(int argc, char **argv) {
+  const char *path = argc > 1 ? argv[1] : "golden.bin";
+  out = fopen(path, "wb");
+  if (!out) return 1;
+  md_init_config_x cfg = {1, 0, 0, 1, ""};
+  md_init_setup_x(NULL, &cfg);
+  md_set_pimd_mode_x(1);
+  rng_case();
+  double rs = 2.0;
+  double L8 = pow(4 * M_PI * 8 / 3.0, 1.0 / 3.0) * rs;
+  double L16 = pow(4 * M_PI * 16 / 3.0, 1.0 / 3.0) * rs;
+  /* UEG temperature: theta = 0.5 of T_F = k_F^2 / 2, k_F = (6 pi^2 n)^(1/3) */
+  double kF8 = pow(6 * M_PI * M_PI * 8 / (L8 * L8 * L8), 1.0 / 3.0);
+  double kF16 = pow(6 * M_PI * M_PI * 16 / (L16 * L16 * L16), 1.0 / 3.0);
+  gcase cases[] = {
+      /* name         N   P   T      vi   kind img L    nsum fgrp steps dt    seed */
+      {"trap16",      16, 32, 1.0,   1.0, 0, 0, 10.0, 2, 1, 20, 0.01, 1},
+      {"trap16_g3",   16, 32, 1.0,   1.0, 0, 0, 10.0, 2, 3, 5,  0.01, 11},
+      {"trap8_xi0",   8,  8,  0.5,   0.0, 0, 0, 10.0, 2, 1, 5,  0.01, 2},
+      {"trap8_xi05",  8,  8,  0.5,   0.5, 0, 0, 10.0, 2, 1, 5,  0.01, 3},
+      {"trap24_xi025",24, 6,  0.5,   0.25,0, 0, 10.0, 2, 7, 3,  0.01, 4},
+      {"ueg8_n2",     8,  4,  0.25 * kF8 * kF8, 1.0, 1, 1, L8, 2, 1, 5, 0.05, 5},
+      {"ueg8_n4_xi0", 8,  4,  0.25 * kF8 * kF8, 0.0, 1, 1, L8, 4, 1, 3, 0.05, 6},
+      {"ueg16_n8",    16, 2,  0.25 * kF16 * kF16, 1.0, 1, 1, L16, 8, 1, 0, 0.05, 7},
+  };
+  for (size_t i = 0; i < sizeof cases / sizeof cases[0]; ++i) run_case(&cases[i]);
+  fclose(out);
+  return 0;
+}

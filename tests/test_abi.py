@@ -1,0 +1,173 @@
+"""The C-ABI library: it loads, exports every symbol the public headers
+declare, keeps the reference struct layouts, and its host-side seeded set-up
+reproduces the reference's initial states (no GPU needed)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyxmd
+from orc import ROOT, nhc_from
+
+HEADERS = ["md_particle_x.h", "md_simulation_x.h", "md_statistics_x.h", "md_util_x.h", "md_pimd_x.h", "md_hip_x.h"]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(md_\w+)\s*\(", txt, flags=re.M):
+            if "typedef" in m.group(0):
+                continue
+            names.add(m.group(1))
+    return names
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return pyxmd.Lib()
+
+
+def test_exports_every_declared_symbol(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib.path], capture_output=True, text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l or " B " in l or " D " in l}
+    names = declared_functions()
+    assert len(names) > 90
+    missing = sorted(n for n in names if n not in exported)
+    assert not missing, missing
+    for g in ["md_trap_frequency_x", "md_gaussian_strength_x", "md_coulomb_n_sum_x", "md_pimd_mode_x"]:
+        assert g in exported
+
+
+def test_struct_layouts():
+    assert C.sizeof(pyxmd.Particle) == 80
+    assert C.sizeof(pyxmd.Nhc) == 200
+    assert C.sizeof(pyxmd.Simulation) == 88
+    # offsets the reference's drivers dereference directly
+    assert pyxmd.Pimd.VBN.offset == 48 and pyxmd.Pimd.beta.offset == 72
+    assert C.sizeof(pyxmd.Pimd) == 224
+    assert C.sizeof(pyxmd.Stats) == 64
+
+
+def test_layouts_match_compiled_reference_headers(tmp_path):
+    """sizeof/offsetof of the public structs, compiled against our headers and
+    against the reference's, must agree (skipped where the reference tree is absent)."""
+    ref = "/root/reference"
+    if not os.path.isdir(ref):
+        pytest.skip("reference tree not present")
+    src = tmp_path / "lay.c"
+    src.write_text("""
+#include <stdio.h>
+#include <stddef.h>
+#include "md_pimd_x.h"
+int main(void){
+ printf("%zu %zu %zu %zu %zu\\n", sizeof(md_particle_t_x), sizeof(md_nhc_t_x), sizeof(md_simulation_t_x), sizeof(md_pimd_t_x), sizeof(md_stats_t_x));
+ printf("%zu %zu %zu %zu %zu %zu\\n", offsetof(md_pimd_t_x, VBN), offsetof(md_pimd_t_x, beta), offsetof(md_pimd_t_x, Eint), offsetof(md_pimd_t_x, worm_pos), offsetof(md_simulation_t_x, nhcs), offsetof(md_simulation_t_x, T));
+ return 0;}
+""")
+    outs = []
+    for inc in (os.path.join(ROOT, "include"), ref):
+        exe = tmp_path / "lay"
+        subprocess.run(["gcc", str(src), "-I", inc, "-o", str(exe)], check=True)
+        outs.append(subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout)
+    assert outs[0] == outs[1]
+
+
+def test_host_rng_matches_reference(lib, golden):
+    lib.md_set_seed_x(1234)
+    u = [lib.md_random_uniform_x(-1.0, 2.0) for _ in range(8)]
+    g = [lib.md_random_gaussian_x() for _ in range(9)]
+    np.testing.assert_array_equal(u, golden["rng/uniform"])
+    np.testing.assert_array_equal(g, golden["rng/gaussian"])
+    lib.md_random_gaussian_x()
+    lib.md_set_seed_x(77)
+    sim = lib.md_alloc_simulation_x(12, 3, 0, 0)
+    lib.md_init_particle_mass_x(sim, 1.0, 0, 6)
+    lib.md_init_particle_mass_x(sim, 2.5, 6, 12)
+    assert lib.md_init_maxwell_vel_x(sim, 0.7) == 0
+    np.testing.assert_array_equal(pyxmd.beads(sim)["v"].reshape(-1), golden["rng/maxwell_v"])
+
+
+@pytest.mark.parametrize("name", ["trap16", "trap16_g3", "ueg8_n2", "trap24_xi025"])
+def test_host_setup_matches_reference(lib, golden, name):
+    """Seeded positions, velocities and chains built through the C ABI equal the
+    reference's (the set-up is host work; nothing touches the GPU)."""
+    g = {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(name + "/")}
+    N, P, T, vi, kind, image, L, nsum, fgroup, dt = g["meta"]
+    N, P, fgroup = int(N), int(P), int(fgroup)
+    lib.md_set_pimd_mode_x(1)
+    lib.md_set_seed_x({"trap16": 1, "trap16_g3": 11, "ueg8_n2": 5, "trap24_xi025": 4}[name])
+    sim = lib.md_alloc_simulation_x(N * P, fgroup, 0, 0)
+    lib.md_init_nd_rect_box_x(sim.contents.box, pyxmd.dptr(np.array([L, L, L])))
+    pimd = lib.md_alloc_pimd_x(sim, N, P, vi, T)
+    lib.md_init_particle_mass_x(sim, 1.0, 0, N * P)
+    if int(kind) == 0:
+        b = pyxmd.beads(sim)
+        for l in range(N):
+            for j in range(P):
+                for d in range(3):
+                    b["x"][l * P + j, d] = lib.md_random_gaussian_x()
+        pyxmd.set_beads(sim, b)
+    else:
+        lib.md_pimd_init_particle_uniform_pos_x(pimd, pyxmd.dptr(np.array([L / 2] * 3)), pyxmd.dptr(np.array([L] * 3)), 0.4)
+        b = pyxmd.beads(sim)
+        x = b["x"]
+        x[x < 0] += L
+        x[x > L] -= L
+        b["x"] = x
+        pyxmd.set_beads(sim, b)
+    lib.md_init_maxwell_vel_x(sim, T)
+    p = pimd.contents
+    assert p.beta == g["beta_omegaP"][0] and p.omegaP == g["beta_omegaP"][1]
+    lib.md_init_nhc_x(sim, lib.md_get_quick_Q_x(T, p.omegaP), 0, sim.contents.Nf, 0.0)
+    b = pyxmd.beads(sim)
+    np.testing.assert_array_equal(b["x"].reshape(-1), g["x0"])
+    np.testing.assert_array_equal(b["v"].reshape(-1), g["v0"])
+    ch = pyxmd.nhcs(sim)
+    ref = nhc_from(g["nhc0"])
+    np.testing.assert_array_equal(ch["f"], ref["f"])
+    np.testing.assert_array_equal(ch["Q"], ref["Q"])
+    np.testing.assert_array_equal(ch["vtheta"], ref["vtheta"])
+
+
+def test_madelung_host_constant(lib, golden):
+    L = golden["ueg8_n2/meta"][6]
+    lib.md_set_coulomb_force_strength_x(1.0)
+    lib.md_set_coulomb_truc_x(5.0 / L)
+    lib.md_set_coulomb_n_sum_x(2)
+    got = lib.md_calc_3d_ueg_madelung_x(pyxmd.dptr(np.array([L, L, L])))
+    assert got == pytest.approx(golden["ueg8_n2/madelung"][0], rel=1e-14)
+
+
+def test_stats_block_averages(lib):
+    st = lib.md_alloc_stats_x(1)
+    vals = np.linspace(0, 1, 25000)
+    for v in vals:
+        st.contents.es[st.contents.N - 1] += v
+        st = lib.md_update_stats_x(st)
+    lib.md_finalize_stats_x(st)
+    blocks = [vals[:10000].mean(), vals[10000:20000].mean(), vals[20000:].mean()]
+    assert st.contents.N == 3
+    assert st.contents.e == pytest.approx(np.mean(blocks), rel=1e-12)
+
+
+def test_no_device_fails_loudly(lib):
+    """Without a HIP device every per-step entry point refuses (no CPU path)."""
+    import torch  # noqa: F401  (device presence is probed through HIP itself)
+    if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not _has_gpu():
+        cfg = pyxmd.InitConfig(1, 1, 0, 1, b"")
+        assert lib.md_init_setup_x(None, C.byref(cfg)) == -1
+    else:
+        pytest.skip("a GPU is present")
+
+
+def _has_gpu():
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False

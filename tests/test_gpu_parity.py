@@ -1,0 +1,181 @@
+"""GPU parity: every per-step stage of the HIP engine, called through the C
+ABI, against the golden vectors of the compiled reference (same seeded
+inputs) and against the CPU oracle.
+
+Tolerances (FP64): forces max|dF| / RMS(F_ref) <= 1e-11 per
+stage; V_B, cycle energies and estimators relative 1e-12; short trajectories
+1e-10 (chaotic amplification of round-off over the steps).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyxmd
+from orc import PAIR_KIND, Oracle, beads_from, dptr, make_params, nhc_from, ptr, read_golden, rel_err, tri_size
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["trap16", "trap16_g3", "trap8_xi0", "trap8_xi05", "trap24_xi025", "ueg8_n2", "ueg8_n4_xi0", "ueg16_n8"]
+FTOL = 1e-11
+
+
+@pytest.fixture(scope="module")
+def lib():
+    lib = pyxmd.Lib()
+    lib.md_set_nhc_length_x(8)
+    return lib
+
+
+def meta(golden, name):
+    g = {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(name + "/")}
+    N, P, T, vi, kind, image, L, nsum, fgroup, dt = g["meta"]
+    return g, dict(N=int(N), P=int(P), T=T, vi=vi, kind="trap" if kind == 0 else "ueg", image=int(image), L=L,
+                   nsum=int(nsum), fgroup=int(fgroup), dt=dt)
+
+
+def system(lib, golden, name):
+    g, m = meta(golden, name)
+    s = pyxmd.System(lib, m["kind"], m["N"], m["P"], m["T"], m["vi"], L=m["L"], nsum=m["nsum"],
+                     fgroup=m["fgroup"], positions=g["x0"])
+    pyxmd.load_state(s, g["x0"], g["v0"], nhc_from(g["nhc0"]))
+    return s, g, m
+
+
+def forces_of(s):
+    return s.host_beads()["f"].reshape(-1)
+
+
+def stat(lib, fn, *args):
+    st = lib.md_alloc_stats_x(1)
+    a = list(args)
+    a.insert(1, st)
+    assert fn(*a) == 0
+    return st.contents.es[st.contents.N - 1]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_pair_and_trap_forces(lib, golden, name):
+    s, g, m = system(lib, golden, name)
+    if m["kind"] == "trap":
+        assert lib.md_reset_force_x(s.sim) == 0
+        assert lib.md_pimd_calc_trap_force_x(s.pimd, lib.cb["md_harmonic_trap_force_x"]) == 0
+        assert rel_err(forces_of(s), g["f_trap"]) < 1e-15
+    assert lib.md_reset_force_x(s.sim) == 0
+    assert lib.md_pimd_calc_pair_force_x(s.pimd, s.pair_cb()) == 0
+    err = rel_err(forces_of(s), g["f_pair"])
+    print(f"{name}: pair force rel err {err:.2e}")
+    assert err < FTOL
+    if m["kind"] == "ueg":
+        for split in ("R", "NI"):
+            assert lib.md_reset_force_x(s.sim) == 0
+            assert lib.md_pimd_calc_pair_force_x(s.pimd, lib.cb["md_coulomb_3d_ewald_force_" + split + "_x"]) == 0
+            assert rel_err(forces_of(s), g["f_pair_" + split]) < FTOL
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_exchange(lib, golden, name):
+    s, g, m = system(lib, golden, name)
+    N = m["N"]
+    assert lib.md_pimd_fast_fill_ENk_x(s.pimd, m["image"]) == 0
+    assert lib.md_pimd_fill_VB_x(s.pimd) == 0
+    assert lib.md_reset_force_x(s.sim) == 0
+    assert lib.md_pimd_fast_fill_force_VB_x(s.pimd, m["image"]) == 0
+    f = forces_of(s)
+    s.host_pimd()
+    np.testing.assert_allclose(pyxmd.tri(s.pimd), g["ENk"], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(pyxmd.vbn(s.pimd), g["VBN"], rtol=1e-12, atol=1e-12)
+    err = rel_err(f, g["f_exch"])
+    print(f"{name}: exchange force rel err {err:.2e}")
+    assert err < FTOL
+    # backward potentials and G against the oracle
+    o = Oracle()
+    b = beads_from(g["x0"], m=g["m0"])
+    Vt = np.zeros(N + 1)
+    G = np.zeros(N * N)
+    o.fast_force_VB(N, m["P"], s.beta, m["vi"], s.omegaP, m["image"], dptr(np.array([m["L"]] * 3)),
+                    dptr(g["ENk"]), dptr(g["VBN"]), ptr(b), dptr(Vt), dptr(G))
+    Vt_g = np.zeros(N + 1)
+    G_g = np.zeros(N * N)
+    assert lib.md_pimd_get_backward_potentials_x(s.pimd, dptr(Vt_g)) == 0
+    assert lib.md_pimd_get_connection_matrix_x(s.pimd, dptr(G_g)) == 0
+    np.testing.assert_allclose(Vt_g, Vt, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(G_g, G, rtol=1e-10, atol=1e-13)
+    if "f_exch_slow" in g:
+        assert lib.md_reset_force_x(s.sim) == 0
+        assert lib.md_pimd_fill_force_VB_x(s.pimd, m["image"]) == 0
+        assert rel_err(forces_of(s), g["f_exch_slow"]) < FTOL
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_estimators(lib, golden, name):
+    s, g, m = system(lib, golden, name)
+    s.full_force()
+    assert rel_err(forces_of(s), g["f_total"]) < FTOL
+    approx = lambda v, ref: v == pytest.approx(ref, rel=1e-12, abs=1e-12)
+    assert approx(stat(lib, lib.md_pimd_calc_VBN_energy_x, s.pimd), g["e_VBN"][0])
+    assert lib.md_pimd_fast_fill_ENk2_x(s.pimd, m["image"]) == 0
+    s.host_pimd()
+    np.testing.assert_allclose(pyxmd.tri(s.pimd, "ENk2"), g["ENk2"], rtol=1e-12, atol=1e-11)
+    assert approx(stat(lib, lib.md_pimd_calc_vi_virial_energy_x, s.pimd, m["image"]), g["e_vir"][0])
+    assert lib.md_pimd_fill_ENk2_x(s.pimd, m["image"]) == 0
+    s.host_pimd()
+    np.testing.assert_allclose(pyxmd.tri(s.pimd, "ENk2"), g["ENk2_slow"], rtol=1e-12, atol=1e-11)
+    assert approx(stat(lib, lib.md_pimd_calc_vi_virial_energy_x, s.pimd, m["image"]), g["e_vir_slow"][0])
+    if m["kind"] == "trap":
+        assert approx(stat(lib, lib.md_pimd_calc_trap_energy_x, s.pimd, lib.cb["md_harmonic_trap_energy_x"]),
+                      g["e_trap"][0])
+    assert approx(stat(lib, lib.md_pimd_calc_pair_energy_x, s.pimd, s.pair_energy_cb()), g["e_pair"][0])
+    assert approx(stat(lib, lib.md_calc_kinetic_x, s.sim), g["e_kin"][0])
+    assert approx(stat(lib, lib.md_calc_nhc_kinetic_x, s.sim), g["e_nhc"][0])
+    assert approx(stat(lib, lib.md_pimd_calc_virial_energy_x, s.pimd), g["e_virial_plain"][0])
+    st = lib.md_alloc_stats_x(1)
+    assert lib.md_pimd_calc_centroid_virial_energy_x(s.pimd, st, 3) == 0
+    assert approx(st.contents.es[0], g["e_virial_centroid"][0])
+    for a, sg in zip(g["vi_sign_args"], g["vi_sign"]):
+        es, ss = lib.md_alloc_stats_x(1), lib.md_alloc_stats_x(1)
+        es.contents.es[0] = 1
+        ss.contents.es[0] = 1
+        assert lib.md_pimd_calc_vi_sign_x(s.pimd, es, ss, a) == 0
+        assert ss.contents.es[0] == pytest.approx(sg, rel=1e-10, abs=1e-300)
+
+
+def trajectory_check(s, g, ev=None):
+    b = s.host_beads()
+    assert rel_err(b["x"].reshape(-1), g["x_end"]) < 1e-10
+    assert rel_err(b["v"].reshape(-1), g["v_end"]) < 1e-9
+    assert rel_err(b["f"].reshape(-1), g["f_end"]) < 1e-9
+    ch = pyxmd.nhcs(s.sim)
+    ref = nhc_from(g["nhc_end"])
+    assert rel_err(ch["vtheta"], ref["vtheta"]) < 1e-9
+    assert rel_err(ch["theta"], ref["theta"]) < 1e-9
+    if ev is not None:
+        np.testing.assert_allclose(ev, g["e_vir_series"], rtol=1e-9)
+
+
+@pytest.mark.parametrize("name", [c for c in CASES if c != "ueg16_n8"])
+def test_trajectory_api_calls(lib, golden, name):
+    """The reference driver loop, one C call per stage."""
+    s, g, m = system(lib, golden, name)
+    s.full_force()
+    ev = []
+    for _ in range(len(g["e_vir_series"])):
+        s.step(m["dt"])
+        assert lib.md_pimd_fast_fill_ENk2_x(s.pimd, m["image"]) == 0
+        st = lib.md_alloc_stats_x(1)
+        assert lib.md_pimd_calc_vi_virial_energy_x(s.pimd, st, m["image"]) == 0
+        ev.append(st.contents.es[0])
+    trajectory_check(s, g, ev)
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("name", [c for c in CASES if c != "ueg16_n8"])
+def test_trajectory_fused_step(lib, golden, name, graph):
+    """The fused step (two streams, optionally replayed as a hipGraph) gives
+    the reference trajectory."""
+    s, g, m = system(lib, golden, name)
+    cfg = s.step_config(m["dt"], use_graph=graph)
+    assert lib.md_pimd_compute_forces_x(s.pimd, C.byref(cfg)) == 0
+    assert lib.md_pimd_run_steps_x(s.pimd, C.byref(cfg), len(g["e_vir_series"])) == 0
+    assert lib.md_hip_synchronize_x() == 0
+    trajectory_check(s, g)

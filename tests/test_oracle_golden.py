@@ -1,0 +1,185 @@
+"""The CPU oracle against golden vectors written by the compiled reference.
+
+Every per-step stage of the reference (nocl build) is reproduced by the oracle
+on the same seeded inputs.  Golden vectors: tests/golden/golden.bin, written by
+tests/golden/make_golden.c linked against the reference library.
+"""
+import numpy as np
+import pytest
+
+from orc import (PAIR_KIND, beads_from, dptr, make_params, nhc_from, ptr, rel_err, tri_size)
+
+CASES = ["trap16", "trap16_g3", "trap8_xi0", "trap8_xi05", "trap24_xi025", "ueg8_n2",
+         "ueg8_n4_xi0", "ueg16_n8"]
+
+
+def case(golden, name):
+    g = {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(name + "/")}
+    N, P, T, vi, kind, image, L, nsum, fgroup, dt = g["meta"]
+    c = dict(N=int(N), P=int(P), T=T, vi=vi, kind=int(kind), image=int(image), L=L,
+             nsum=int(nsum), fgroup=int(fgroup), dt=dt, beta=g["beta_omegaP"][0],
+             omegaP=g["beta_omegaP"][1])
+    c["Lv"] = np.array([L, L, L])
+    c["params"] = make_params(alpha=5.0 / L, n=int(nsum))
+    c["pair"] = "gaussian" if c["kind"] == 0 else "ewald"
+    return c, g
+
+
+def beads0(g):
+    return beads_from(g["x0"], g["v0"], np.zeros_like(g["x0"]), g["m0"])
+
+
+def forces(b):
+    return b["f"].reshape(-1).copy()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_pair_and_trap_forces(golden, oracle, name):
+    c, g = case(golden, name)
+    b = beads0(g)
+    oracle.pair_force(PAIR_KIND[c["pair"]], ptr_p(c), c["N"], c["P"], dptr(c["Lv"]), ptr(b))
+    assert rel_err(forces(b), g["f_pair"]) < 1e-13
+    if c["kind"] == 0:
+        b = beads0(g)
+        oracle.trap_force(ptr_p(c), c["N"], c["P"], ptr(b))
+        assert rel_err(forces(b), g["f_trap"]) < 1e-15
+    else:
+        for split in ("R", "NI"):
+            b = beads0(g)
+            oracle.pair_force(PAIR_KIND["ewald_" + split], ptr_p(c), c["N"], c["P"], dptr(c["Lv"]), ptr(b))
+            assert rel_err(forces(b), g["f_pair_" + split]) < 1e-13
+        assert abs(oracle.madelung(ptr_p(c), dptr(c["Lv"])) - g["madelung"][0]) < 1e-14
+
+
+def ptr_p(c):
+    import ctypes
+    return ctypes.byref(c["params"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_exchange(golden, oracle, name):
+    c, g = case(golden, name)
+    N, P = c["N"], c["P"]
+    b = beads0(g)
+    ENk = np.zeros(tri_size(N))
+    Eint = np.zeros(N)
+    oracle.fast_fill_ENk(N, P, c["omegaP"], c["image"], dptr(c["Lv"]), ptr(b), dptr(ENk), dptr(Eint))
+    np.testing.assert_allclose(ENk, g["ENk"], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(Eint, g["Eint"], rtol=1e-14, atol=0)
+    ENks = np.zeros(tri_size(N))
+    oracle.fill_ENk_slow(N, P, c["omegaP"], c["image"], dptr(c["Lv"]), ptr(b), dptr(ENks))
+    np.testing.assert_allclose(ENks, g["ENk_slow"], rtol=1e-14, atol=0)
+    VBN = np.zeros(N + 1)
+    oracle.fill_VB(N, c["beta"], c["vi"], dptr(g["ENk"]), dptr(VBN))
+    np.testing.assert_allclose(VBN, g["VBN"], rtol=1e-14, atol=1e-14)
+    Vt = np.zeros(N + 1)
+    G = np.zeros(N * N)
+    oracle.fast_force_VB(N, P, c["beta"], c["vi"], c["omegaP"], c["image"], dptr(c["Lv"]),
+                         dptr(g["ENk"]), dptr(g["VBN"]), ptr(b), dptr(Vt), dptr(G))
+    assert rel_err(forces(b), g["f_exch"]) < 1e-13
+    if "f_exch_slow" in g:
+        b = beads0(g)
+        oracle.slow_force_VB(N, P, c["beta"], c["vi"], c["omegaP"], c["image"], dptr(c["Lv"]),
+                             dptr(g["ENk"]), dptr(g["VBN"]), ptr(b))
+        assert rel_err(forces(b), g["f_exch_slow"]) < 1e-13
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_estimators(golden, oracle, name):
+    c, g = case(golden, name)
+    N, P = c["N"], c["P"]
+    b = beads_from(g["x0"], g["v0"], g["f_total"], g["m0"])
+    e = oracle.VBN_energy(N, P, c["beta"], c["vi"], dptr(g["ENk"]), dptr(g["VBN"]))
+    assert e == pytest.approx(g["e_VBN"][0], rel=1e-13, abs=1e-13)
+    E2 = np.zeros(tri_size(N))
+    oracle.fast_fill_ENk2(N, P, c["omegaP"], c["image"], dptr(c["Lv"]), ptr(b), dptr(E2))
+    np.testing.assert_allclose(E2, g["ENk2"], rtol=1e-13, atol=1e-12)
+    e = oracle.vi_virial_energy(N, P, c["beta"], c["vi"], c["image"], dptr(c["Lv"]), dptr(g["ENk"]),
+                                dptr(g["ENk2"]), dptr(g["VBN"]), ptr(b))
+    assert e == pytest.approx(g["e_vir"][0], rel=1e-13, abs=1e-13)
+    E2s = np.zeros(tri_size(N))
+    oracle.fill_ENk2_slow(N, P, c["omegaP"], c["image"], dptr(c["Lv"]), ptr(b), dptr(E2s))
+    np.testing.assert_allclose(E2s, g["ENk2_slow"], rtol=1e-13, atol=1e-12)
+    e = oracle.vi_virial_energy(N, P, c["beta"], c["vi"], c["image"], dptr(c["Lv"]), dptr(g["ENk"]),
+                                dptr(g["ENk2_slow"]), dptr(g["VBN"]), ptr(b))
+    assert e == pytest.approx(g["e_vir_slow"][0], rel=1e-13, abs=1e-13)
+    if c["kind"] == 0:
+        e = oracle.trap_energy(ptr_p(c), N, P, ptr(b))
+        assert e == pytest.approx(g["e_trap"][0], rel=1e-14)
+    e = oracle.pair_energy(PAIR_KIND[c["pair"]], ptr_p(c), N, P, dptr(c["Lv"]), ptr(b))
+    assert e == pytest.approx(g["e_pair"][0], rel=1e-13, abs=1e-13)
+    assert oracle.kinetic(N * P, ptr(b)) == pytest.approx(g["e_kin"][0], rel=1e-14)
+    ch = nhc_from(g["nhc0"])
+    assert oracle.nhc_kinetic(len(ch), ptr(ch), c["T"], 8) == pytest.approx(g["e_nhc"][0], rel=1e-14)
+    assert oracle.virial_energy(N, P, dptr(c["Lv"]), ptr(b)) == pytest.approx(g["e_virial_plain"][0], rel=1e-13)
+    assert oracle.centroid_virial_energy(N, P, c["beta"], 3, ptr(b)) == pytest.approx(
+        g["e_virial_centroid"][0], rel=1e-13)
+    for a, s in zip(g["vi_sign_args"], g["vi_sign"]):
+        got = oracle.vi_sign(N, c["beta"], a, dptr(g["ENk"]), dptr(g["VBN"]))
+        assert got == pytest.approx(s, rel=1e-11, abs=1e-300)
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if n != "ueg16_n8"])
+def test_trajectory(golden, oracle, name):
+    """Full MD steps (NHC VV3 halves, PBC, all forces) vs the reference trajectory."""
+    c, g = case(golden, name)
+    N, P = c["N"], c["P"]
+    steps = len(g.get("e_vir_series", []))
+    if steps == 0:
+        pytest.skip("no trajectory in fixture")
+    b = beads0(g)
+    ch = nhc_from(g["nhc0"])
+    Lv = c["Lv"]
+    p = ptr_p(c)
+
+    def full_force():
+        b["f"] = 0
+        if c["kind"] == 0:
+            oracle.trap_force(p, N, P, ptr(b))
+        oracle.pair_force(PAIR_KIND[c["pair"]], p, N, P, dptr(Lv), ptr(b))
+        ENk = np.zeros(tri_size(N))
+        Eint = np.zeros(N)
+        oracle.fast_fill_ENk(N, P, c["omegaP"], c["image"], dptr(Lv), ptr(b), dptr(ENk), dptr(Eint))
+        VBN = np.zeros(N + 1)
+        oracle.fill_VB(N, c["beta"], c["vi"], dptr(ENk), dptr(VBN))
+        oracle.fast_force_VB(N, P, c["beta"], c["vi"], c["omegaP"], c["image"], dptr(Lv), dptr(ENk),
+                             dptr(VBN), ptr(b), None, None)
+        return ENk, VBN
+
+    full_force()
+    ev = []
+    for s in range(steps):
+        oracle.nhc_vv3_1(N * P, len(ch), ptr(ch), ptr(b), c["T"], c["dt"], 8)
+        if c["kind"] == 1:
+            oracle.periodic_boundary(N * P, dptr(Lv), ptr(b))
+        ENk, VBN = full_force()
+        oracle.nhc_vv3_2(N * P, len(ch), ptr(ch), ptr(b), c["T"], c["dt"], 8)
+        E2 = np.zeros(tri_size(N))
+        oracle.fast_fill_ENk2(N, P, c["omegaP"], c["image"], dptr(Lv), ptr(b), dptr(E2))
+        ev.append(oracle.vi_virial_energy(N, P, c["beta"], c["vi"], c["image"], dptr(Lv), dptr(ENk),
+                                          dptr(E2), dptr(VBN), ptr(b)))
+    assert rel_err(b["x"].reshape(-1), g["x_end"]) < 1e-11
+    assert rel_err(b["v"].reshape(-1), g["v_end"]) < 1e-10
+    assert rel_err(b["f"].reshape(-1), g["f_end"]) < 1e-10
+    ref = nhc_from(g["nhc_end"])
+    assert rel_err(ch["vtheta"], ref["vtheta"]) < 1e-10
+    assert rel_err(ch["theta"], ref["theta"]) < 1e-10
+    np.testing.assert_allclose(ev, g["e_vir_series"], rtol=1e-9)
+
+
+def test_rng(golden, oracle):
+    oracle.set_seed(1234)
+    u = [oracle.random_uniform(-1.0, 2.0) for _ in range(8)]
+    gs = [oracle.random_gaussian() for _ in range(9)]
+    np.testing.assert_array_equal(u, golden["rng/uniform"])
+    np.testing.assert_array_equal(gs, golden["rng/gaussian"])
+    oracle.random_gaussian()
+    oracle.set_seed(77)
+    m = np.r_[np.ones(6), np.full(6, 2.5)]
+    b = beads_from(np.zeros(36), m=m)
+    oracle.init_maxwell_vel(12, ptr(b), 0.7)
+    np.testing.assert_array_equal(b["v"].reshape(-1), golden["rng/maxwell_v"])
+    oracle.set_seed(5)
+    b = beads_from(np.zeros(45))
+    oracle.init_uniform_pos(5, 3, dptr(np.array([0.5, 1.0, -0.5])), dptr(np.array([2.0, 3.0, 4.0])), 0.1, ptr(b))
+    np.testing.assert_array_equal(b["x"].reshape(-1), golden["rng/uniform_pos"])
