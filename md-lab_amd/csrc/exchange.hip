@@ -10,7 +10,7 @@
 //   forward   L(n) = LSE_{m<n} [c(n,m) + L(m)],  L(0) = 0,  V_B^(n) = -L(n)/beta
 //   backward  R(m) = LSE_{n>m} [c(n,m) + R(n)],  R(N) = 0,  Vt^(m)  = -R(m)/beta
 //   G(l, j)   = exp(c(l, j-1) + L(j-1) + R(l) - L(N))  for j <= l,
-//               1 - exp(L(N) - L(l) - R(l))            for j = l+1.
+//               1 - exp(L(l) + R(l) - L(N))            for j = l+1.
 // These are algebraically identical to md_pimd_xminE_x / md_pimd_fill_VB_x /
 // md_pimd_fast_xminE_x / md_pimd_fast_fill_force_VB_x; the shift of each
 // log-sum-exp is the running maximum instead of a separate minimum pass.
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_exch_force(const double *__restrict__ X
       G = exp(c + a.Lf[j - 1] + Rl - LN);
     } else {
       if (l == N) continue;
-      G = 1 - exp(LN - a.Lf[l] - Rl);
+      G = 1 - exp(a.Lf[l] + Rl - LN);
     }
     const int sj = j - 1;
     gs += G;
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void k_exch_force(const double *__restrict__ X
       const double c = (k - 1) * a.lnxi - a.beta * a.ET[col_off(l - 1, N) + k - 1] - a.lnn[jp];
       G = exp(c + Llm1 + a.Rb[jp] - LN);
     } else {
-      G = 1 - exp(LN - a.Lf[l - 1] - a.Rb[l - 1]);
+      G = 1 - exp(a.Lf[l - 1] + a.Rb[l - 1] - LN);
     }
     const int sj = (P - 1) * N + jp - 1;
     hs += G;
@@ -368,7 +368,7 @@ __global__ void k_connection_matrix(RecArgs a, double *__restrict__ G) {
   for (int j = 1 + threadIdx.x; j <= N; j += blockDim.x) {
     double v = 0;
     if (j == l + 1) {
-      v = 1 - exp(LN - a.Lf[l] - a.Rb[l]);
+      v = 1 - exp(a.Lf[l] + a.Rb[l] - LN);
     } else if (j <= l) {
       const int k = l - j + 1;
       if (!(a.xi0 && k != 1)) {

@@ -42,14 +42,26 @@ def system(lib, golden, name):
     return s, g, m
 
 
+def exch_tol(g):
+    """1e-11, or twice the reference's own disagreement between its two exchange
+    algorithms (fast backward pass vs forward-mode gradient) on this input when
+    that is larger: on ill-conditioned inputs (random beads, beta*E ~ 1e4) the
+    1 - exp(-x) entries of G cancel and the reference itself is only
+    reproducible to that level."""
+    if "f_exch_slow" in g:
+        return max(FTOL, 2 * rel_err(g["f_exch"], g["f_exch_slow"]))
+    return FTOL
+
+
 def forces_of(s):
     return s.host_beads()["f"].reshape(-1)
 
 
-def stat(lib, fn, *args):
+def stat(lib, fn, *args, pos=1):
+    """Call an estimator entry point with a fresh stats object at argument `pos`."""
     st = lib.md_alloc_stats_x(1)
     a = list(args)
-    a.insert(1, st)
+    a.insert(pos, st)
     assert fn(*a) == 0
     return st.contents.es[st.contents.N - 1]
 
@@ -86,8 +98,8 @@ def test_exchange(lib, golden, name):
     np.testing.assert_allclose(pyxmd.tri(s.pimd), g["ENk"], rtol=1e-13, atol=0)
     np.testing.assert_allclose(pyxmd.vbn(s.pimd), g["VBN"], rtol=1e-12, atol=1e-12)
     err = rel_err(f, g["f_exch"])
-    print(f"{name}: exchange force rel err {err:.2e}")
-    assert err < FTOL
+    print(f"{name}: exchange force rel err {err:.2e} (tolerance {exch_tol(g):.1e})")
+    assert err < exch_tol(g)
     # backward potentials and G against the oracle
     o = Oracle()
     b = beads_from(g["x0"], m=g["m0"])
@@ -100,18 +112,19 @@ def test_exchange(lib, golden, name):
     assert lib.md_pimd_get_backward_potentials_x(s.pimd, dptr(Vt_g)) == 0
     assert lib.md_pimd_get_connection_matrix_x(s.pimd, dptr(G_g)) == 0
     np.testing.assert_allclose(Vt_g, Vt, rtol=1e-12, atol=1e-12)
-    np.testing.assert_allclose(G_g, G, rtol=1e-10, atol=1e-13)
+    # entries 1 - exp(~0) are round-off sized in both
+    np.testing.assert_allclose(G_g, G, rtol=1e-10, atol=1e-12)
     if "f_exch_slow" in g:
         assert lib.md_reset_force_x(s.sim) == 0
         assert lib.md_pimd_fill_force_VB_x(s.pimd, m["image"]) == 0
-        assert rel_err(forces_of(s), g["f_exch_slow"]) < FTOL
+        assert rel_err(forces_of(s), g["f_exch_slow"]) < exch_tol(g)
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_estimators(lib, golden, name):
     s, g, m = system(lib, golden, name)
     s.full_force()
-    assert rel_err(forces_of(s), g["f_total"]) < FTOL
+    assert rel_err(forces_of(s), g["f_total"]) < exch_tol(g)
     approx = lambda v, ref: v == pytest.approx(ref, rel=1e-12, abs=1e-12)
     assert approx(stat(lib, lib.md_pimd_calc_VBN_energy_x, s.pimd), g["e_VBN"][0])
     assert lib.md_pimd_fast_fill_ENk2_x(s.pimd, m["image"]) == 0
@@ -123,9 +136,9 @@ def test_estimators(lib, golden, name):
     np.testing.assert_allclose(pyxmd.tri(s.pimd, "ENk2"), g["ENk2_slow"], rtol=1e-12, atol=1e-11)
     assert approx(stat(lib, lib.md_pimd_calc_vi_virial_energy_x, s.pimd, m["image"]), g["e_vir_slow"][0])
     if m["kind"] == "trap":
-        assert approx(stat(lib, lib.md_pimd_calc_trap_energy_x, s.pimd, lib.cb["md_harmonic_trap_energy_x"]),
+        assert approx(stat(lib, lib.md_pimd_calc_trap_energy_x, s.pimd, lib.cb["md_harmonic_trap_energy_x"], pos=2),
                       g["e_trap"][0])
-    assert approx(stat(lib, lib.md_pimd_calc_pair_energy_x, s.pimd, s.pair_energy_cb()), g["e_pair"][0])
+    assert approx(stat(lib, lib.md_pimd_calc_pair_energy_x, s.pimd, s.pair_energy_cb(), pos=2), g["e_pair"][0])
     assert approx(stat(lib, lib.md_calc_kinetic_x, s.sim), g["e_kin"][0])
     assert approx(stat(lib, lib.md_calc_nhc_kinetic_x, s.sim), g["e_nhc"][0])
     assert approx(stat(lib, lib.md_pimd_calc_virial_energy_x, s.pimd), g["e_virial_plain"][0])
