@@ -104,9 +104,9 @@ def main():
     sec = t1 - t0
     value = N * P * args.steps / sec
 
-    # per-stage device times (HIP events on each stage's own stream)
+    # per-stage device times, every stage alone on one stream (HIP events)
     stages = np.zeros(9)
-    assert lib.md_pimd_profile_steps_x(s.pimd, C.byref(cfg), 5, pyxmd.dptr(stages)) == 0
+    assert lib.md_pimd_profile_steps_x(s.pimd, C.byref(cfg), 5, 1, pyxmd.dptr(stages)) == 0
     names = ["nhc1_wrap", "cycle_energies", "vb_recursions", "bead_forces", "ewald_real", "ewald_recip",
              "exchange_forces", "nhc2", "step"]
     real_ms = stages[4]

@@ -50,13 +50,16 @@ int md_pimd_compute_forces_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg
 /* nsteps full steps (B1-B7); returns when the work is queued. */
 int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps);
 
-/* Per-stage device time of nsteps eager steps, averaged per step (ms):
+/* Per-stage device time of nsteps eager steps, averaged per step (ms).  With
+ * serial != 0 every stage runs alone on one stream (true kernel times);
+ * otherwise the two branches overlap as in md_pimd_run_steps_x:
  *   [0] NHC first half + wrap   [1] cycle energies   [2] V_B recursions
  *   [3] reset/trap/ring springs [4] pair real space  [5] pair reciprocal
  *   [6] exchange end-bead forces [7] NHC second half [8] whole step (wall)
  * Each stage is bracketed by events on the stream it runs on. */
 #define MD_PIMD_NSTAGES_X 9
-int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, double *stage_ms);
+int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, int serial,
+                            double *stage_ms);
 
 #ifdef __cplusplus
 }

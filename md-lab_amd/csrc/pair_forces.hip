@@ -28,11 +28,10 @@
 #include <cmath>
 #include <vector>
 
+#include "xmd_device.h"
 #include "xmd_internal.h"
 
 namespace xmd {
-
-constexpr int TILE = 512;
 
 __device__ __forceinline__ int pf_img_count(double x, double L) {
   int c = (int)(fabs(x) / L);
@@ -116,19 +115,172 @@ __device__ __forceinline__ void ewald_real(double dx, double dy, double dz, cons
   }
 }
 
-// MODE: PK_GAUSS, PK_PGAUSS, PK_EWALD (real part), PK_EWALD_R (real part,
-// minimum image skipped), PK_EWALD_NI (minimum image only).  ENERGY: write
-// per-block energy partials instead of forces.
-template <int MODE, bool ENERGY>
-__global__ __launch_bounds__(256) void k_pair(const double *__restrict__ X, const double *__restrict__ M,
-                                              double *__restrict__ F, int N, int nb, PairParams pp,
-                                              double *__restrict__ epart) {
-  __shared__ double sx[TILE], sy[TILE], sz[TILE], sm[TILE];
-  __shared__ double red[4][4][64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int slice = blockIdx.y;
+// (1 + 2x) exp(x^2) erfc(x) on 0 <= x <= 6.5 as a degree-20 Chebyshev series
+// in s(y), y = (x - 3)/(x + 3) (the transform of Shepherd & Laframboise,
+// Math. Comp. 36 (1981) 249); coefficients fitted here at 40 digits,
+// max relative error 4.4e-16 in FP64 evaluation.
+__device__ __constant__ double kErfcxCheb[21] = {
+    1.2007410605728304,     0.065198450961816964,   -0.092409466117412367,  0.034405477918768981,
+    -0.0076296298533925291, 0.0010403531055794199,  -6.3272734583294028e-05, -4.6011890721509966e-06,
+    1.0119739574626279e-06, 9.9431600651180912e-09, -1.3295432865350941e-08, -9.4590207011356631e-12,
+    1.9114449190078913e-10, 3.6405902241118161e-12, -2.8882712550806537e-12, -1.780186683919635e-13,
+    3.9543737314097065e-14, 5.4923737447575301e-15, -3.4794118944385234e-16, -1.0306316042149898e-16,
+    -1.3876025693966222e-17};
+constexpr double kErfcxY1 = 0.36842105263157893;  // y at x = 6.5
+
+// erfc(x) given e = exp(-x^2), 0 <= x <= 6.5.  One reciprocal serves both
+// the transform y = (x-3)/(x+3) and the final division by (1 + 2x).
+__device__ __forceinline__ double erfc_cheb(double x, double e) {
+  const double d1 = x + 3.0, d2 = 1.0 + 2.0 * x;
+  const double a = d1 * d2;
+  double r = __builtin_amdgcn_rcp(a);
+  r = fma(fma(-a, r, 1.0), r, r);
+  r = fma(fma(-a, r, 1.0), r, r);
+  const double y = (x - 3.0) * (d2 * r);
+  const double sc = 2.0 / (kErfcxY1 + 1.0), sh = (kErfcxY1 - 1.0) / (kErfcxY1 + 1.0);
+  const double t = sc * y - sh, t2 = 2.0 * t;
+  double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+  for (int k = 20; k >= 1; --k) {
+    const double b0 = fma(t2, b1, kErfcxCheb[k] - b2);
+    b2 = b1;
+    b1 = b0;
+  }
+  const double c = fma(t, b1, kErfcxCheb[0] - b2);
+  return e * (c * (d1 * r));
+}
+
+__device__ __forceinline__ double erfc_from_exp(double x, double e) {
+  if (x > 6.5) return erfc(x);
+  return erfc_cheb(x, e);
+}
+
+// One real-space image r (= p1 - image of p2) at distance R: force factor
+// (q 2 alpha exp(-alpha^2 R^2)/(sqrt(pi) R^2) + q erfc(alpha R)/R^3) as at
+// md_simulation_x.c:288-289, energy q erfc(alpha R)/R (md_statistics_x.c:393).
+// INRANGE: alpha R <= 6.5 is guaranteed by the caller (images inside the
+// cutoff 6/alpha).
+template <bool FORCE, bool ENERGY, bool INRANGE = false>
+__device__ __forceinline__ void ewald_image(double rx, double ry, double rz, double R2, const PairParams &pp,
+                                            double &gx, double &gy, double &gz, double &e) {
+  const double invR = rsqrt(R2);
+  const double R = R2 * invR;
+  const double x = pp.alpha * R;
+  const double ex = exp(-x * x);
+  const double ec = INRANGE ? erfc_cheb(x, ex) : erfc_from_exp(x, ex);
+  if (FORCE) {
+    const double i2 = invR * invR;
+    const double w = i2 * ((pp.q * 2 * pp.alpha / sqrt(M_PI)) * ex + pp.q * ec * invR);
+    gx += rx * w;
+    gy += ry * w;
+    gz += rz * w;
+  }
+  if (ENERGY) e += pp.q * ec * invR;
+}
+
+// Images of one pair within r_c when r_c < sqrt(2) L: every such image takes
+// per dimension one of the three nearest periodic copies a (minimum image),
+// b = a -+ L, c = a +- L, and at most one dimension takes c.  Candidates are
+// numbered 0..19 (0-7: {a,b}^3, 8-11/12-15/16-19: c in x/y/z); each lane
+// builds the bit mask of its candidates inside r_c and the wave walks the
+// set bits, so a step costs the largest per-lane image count rather than
+// all candidates any lane needs.  skip_min drops (a,a,a) (the R split),
+// only_min keeps only it (the NI split).
+template <bool FORCE, bool ENERGY>
+__device__ __forceinline__ void ewald_real_near(double dx, double dy, double dz, const PairParams &pp, int skip_min,
+                                                int only_min, double &gx, double &gy, double &gz, double &e) {
+  const double ax = pf_min_image(dx, pp.L[0]), ay = pf_min_image(dy, pp.L[1]), az = pf_min_image(dz, pp.L[2]);
+  if (only_min) {
+    ewald_image<FORCE, ENERGY>(ax, ay, az, ax * ax + ay * ay + az * az, pp, gx, gy, gz, e);
+    return;
+  }
+  const double bx = ax - copysign(pp.L[0], ax), by = ay - copysign(pp.L[1], ay), bz = az - copysign(pp.L[2], az);
+  const double cx = ax + copysign(pp.L[0], ax), cy = ay + copysign(pp.L[1], ay), cz = az + copysign(pp.L[2], az);
+  const double ax2 = ax * ax, ay2 = ay * ay, az2 = az * az;
+  const double bx2 = bx * bx, by2 = by * by, bz2 = bz * bz;
+  const double cx2 = cx * cx, cy2 = cy * cy, cz2 = cz * cz;
+  const double rc2 = pp.rc2;
+  unsigned mask = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double R2 = ((q & 1) ? bx2 : ax2) + ((q & 2) ? by2 : ay2) + ((q & 4) ? bz2 : az2);
+    mask |= (R2 <= rc2 ? 1u : 0u) << q;
+  }
+  if (skip_min) mask &= ~1u;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const double yx = (u & 1) ? by2 : ay2, zx = (u & 2) ? bz2 : az2;  // c in x
+    const double xy = (u & 1) ? bx2 : ax2, zy = (u & 2) ? bz2 : az2;  // c in y
+    const double xz = (u & 1) ? bx2 : ax2, yz = (u & 2) ? by2 : ay2;  // c in z
+    mask |= (cx2 + yx + zx <= rc2 ? 1u : 0u) << (8 + u);
+    mask |= (xy + cy2 + zy <= rc2 ? 1u : 0u) << (12 + u);
+    mask |= (xz + yz + cz2 <= rc2 ? 1u : 0u) << (16 + u);
+  }
+  while (mask) {
+    const int q = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const int cdim = q < 8 ? -1 : (q - 8) >> 2;
+    const int u = q < 8 ? q : ((q - 8) & 3);
+    const int xb = u & 1;
+    const int yb = cdim == 0 ? (u & 1) : ((u >> 1) & 1);
+    const int zb = cdim < 0 ? ((u >> 2) & 1) : ((u >> 1) & 1);
+    const double rx = cdim == 0 ? cx : (xb ? bx : ax);
+    const double ry = cdim == 1 ? cy : (yb ? by : ay);
+    const double rz = cdim == 2 ? cz : (zb ? bz : az);
+    ewald_image<FORCE, ENERGY, true>(rx, ry, rz, rx * rx + ry * ry + rz * rz, pp, gx, gy, gz, e);
+  }
+}
+
+// Force (raw, not divided by mass) on bead i from bead j and/or pair energy.
+template <int MODE, bool ENERGY, bool NEAR>
+__device__ __forceinline__ void pair_term(double dx, double dy, double dz, const PairParams &pp, double &gx,
+                                          double &gy, double &gz, double &e) {
+  if (MODE == PK_GAUSS || MODE == PK_PGAUSS) {
+    if (MODE == PK_PGAUSS) {
+      dx = pf_min_image(dx, pp.L[0]);
+      dy = pf_min_image(dy, pp.L[1]);
+      dz = pf_min_image(dz, pp.L[2]);
+    }
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const double w = (pp.g / (M_PI * pp.s2)) * exp(-d * d / pp.s2);
+    if (ENERGY) {
+      e += w;
+    } else {
+      gx += w * (2 * dx) / pp.s2;
+      gy += w * (2 * dy) / pp.s2;
+      gz += w * (2 * dz) / pp.s2;
+    }
+  } else if (NEAR) {
+    ewald_real_near<!ENERGY, ENERGY>(dx, dy, dz, pp, MODE == PK_EWALD_R, MODE == PK_EWALD_NI, gx, gy, gz, e);
+  } else {
+    ewald_real<!ENERGY, ENERGY>(dx, dy, dz, pp, MODE == PK_EWALD_R, MODE == PK_EWALD_NI, 1.0, gx, gy, gz, e);
+  }
+}
+
+// Symmetric tile-pair kernel.  One 64-lane workgroup per (slice, I, J),
+// I <= J (64-bead tiles of one slice).  Lane t owns bead i = 64 I + t; the
+// J tile sits in LDS together with its reaction accumulators.  At rotation
+// step s lane t meets partner (t + s) mod 64, so within a step every lane
+// updates a different reaction slot and no atomics are needed.  Each pair
+// is evaluated once (Newton's third law), divided by the mass of the
+// higher-index bead as in md_pimd_x.c:1669-1673.
+//
+// Output: per-bead partial sums in Fpart[(d*nb + bead)*T + slot], where
+// slot is the other tile of the pair, so every slot is written exactly once
+// and k_reduce_tiles sums them in a fixed order (deterministic forces).
+template <int MODE, bool ENERGY, bool NEAR>
+__global__ __launch_bounds__(64) void k_pair_tiles(const double *__restrict__ X, const double *__restrict__ M,
+                                                   double *__restrict__ Fpart, double *__restrict__ Epart,
+                                                   const int2 *__restrict__ tiles, int N, int nb, int T, int ntp,
+                                                   PairParams pp) {
+  __shared__ double jx[64], jy[64], jz[64], jm[64], rx[64], ry[64], rz[64];
+  const int item = blockIdx.x;
+  const int slice = item / ntp;
+  const int2 IJ = tiles[item - slice * ntp];
+  const int I = IJ.x, J = IJ.y;
+  const int t = threadIdx.x;
   const int base = slice * N;
-  const int i = blockIdx.x * 64 + lane;
+  const int i = I * 64 + t, jl = J * 64 + t;
   const bool act = i < N;
   double xi = 0, yi = 0, zi = 0, mi = 1;
   if (act) {
@@ -137,65 +289,64 @@ __global__ __launch_bounds__(256) void k_pair(const double *__restrict__ X, cons
     zi = X[2 * nb + base + i];
     mi = M[base + i];
   }
+  const bool jact = jl < N;
+  jx[t] = jact ? X[base + jl] : 0.0;
+  jy[t] = jact ? X[nb + base + jl] : 0.0;
+  jz[t] = jact ? X[2 * nb + base + jl] : 0.0;
+  jm[t] = jact ? M[base + jl] : 1.0;
+  rx[t] = 0;
+  ry[t] = 0;
+  rz[t] = 0;
+  __syncthreads();
+  const bool diag = I == J;
+  const int s_lo = diag ? 1 : 0, s_hi = diag ? 32 : 63;
   double fx = 0, fy = 0, fz = 0, e = 0;
-  for (int t0 = 0; t0 < N; t0 += TILE) {
-    const int cnt = min(TILE, N - t0);
-    __syncthreads();
-    for (int q = threadIdx.x; q < cnt; q += 256) {
-      sx[q] = X[base + t0 + q];
-      sy[q] = X[nb + base + t0 + q];
-      sz[q] = X[2 * nb + base + t0 + q];
-      sm[q] = M[base + t0 + q];
-    }
-    __syncthreads();
-    if (act) {
-      for (int q = wv; q < cnt; q += 4) {
-        const int j = t0 + q;
-        if (j == i) continue;
-        const double invm = 1.0 / (j > i ? sm[q] : mi);
-        double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
-        if (MODE == PK_GAUSS || MODE == PK_PGAUSS) {
-          if (MODE == PK_PGAUSS) {
-            dx = pf_min_image(dx, pp.L[0]);
-            dy = pf_min_image(dy, pp.L[1]);
-            dz = pf_min_image(dz, pp.L[2]);
-          }
-          double d = sqrt(dx * dx + dy * dy + dz * dz);
-          double w = (pp.g / (M_PI * pp.s2)) * exp(-d * d / pp.s2);
-          if (ENERGY) {
-            e += w;
-          } else {
-            fx += w * (2 * dx) / pp.s2 * invm;
-            fy += w * (2 * dy) / pp.s2 * invm;
-            fz += w * (2 * dz) / pp.s2 * invm;
-          }
-        } else {
-          ewald_real<!ENERGY, ENERGY>(dx, dy, dz, pp, MODE == PK_EWALD_R, MODE == PK_EWALD_NI, invm, fx, fy, fz, e);
-        }
+  for (int s = s_lo; s <= s_hi; ++s) {
+    const int j = (t + s) & 63;
+    const int jg = J * 64 + j;
+    const bool ok = act && jg < N && !(diag && s == 32 && t >= 32);
+    if (ok) {
+      double gx = 0, gy = 0, gz = 0;
+      pair_term<MODE, ENERGY, NEAR>(xi - jx[j], yi - jy[j], zi - jz[j], pp, gx, gy, gz, e);
+      if (!ENERGY) {
+        const double invm = 1.0 / (jg > i ? jm[j] : mi);
+        fx += gx * invm;
+        fy += gy * invm;
+        fz += gz * invm;
+        rx[j] -= gx * invm;
+        ry[j] -= gy * invm;
+        rz[j] -= gz * invm;
       }
     }
   }
-  red[0][wv][lane] = fx;
-  red[1][wv][lane] = fy;
-  red[2][wv][lane] = fz;
-  red[3][wv][lane] = e;
   __syncthreads();
   if (ENERGY) {
-    if (threadIdx.x < 64) {
-      double s = red[3][0][lane] + red[3][1][lane] + red[3][2][lane] + red[3][3][lane];
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-      if (lane == 0) epart[blockIdx.y * gridDim.x + blockIdx.x] = s;
-    }
+    e = wave_sum(e);
+    if (t == 0) Epart[item] = e;
     return;
   }
-  if (wv == 0 && act) {
-    double gx = red[0][0][lane] + red[0][1][lane] + red[0][2][lane] + red[0][3][lane];
-    double gy = red[1][0][lane] + red[1][1][lane] + red[1][2][lane] + red[1][3][lane];
-    double gz = red[2][0][lane] + red[2][1][lane] + red[2][2][lane] + red[2][3][lane];
-    F[base + i] += gx / pp.P;
-    F[nb + base + i] += gy / pp.P;
-    F[2 * nb + base + i] += gz / pp.P;
+  auto put = [&](int bead, int slot, double vx, double vy, double vz) {
+    const size_t s0 = (size_t)(base + bead) * T + slot;
+    Fpart[s0] = vx;
+    Fpart[(size_t)nb * T + s0] = vy;
+    Fpart[2 * (size_t)nb * T + s0] = vz;
+  };
+  if (diag) {
+    if (act) put(i, I, fx + rx[t], fy + ry[t], fz + rz[t]);
+  } else {
+    if (act) put(i, J, fx, fy, fz);
+    if (jact) put(jl, I, rx[t], ry[t], rz[t]);
   }
+}
+
+// F[d*nb + s] += sum_slot Fpart / P, slots in a fixed order.
+__global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restrict__ F, int n3, int T, double P) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n3) return;
+  const double *src = Fpart + (size_t)k * T;
+  double s = 0;
+  for (int q = 0; q < T; ++q) s += src[q];
+  F[k] += s / P;
 }
 
 // ------------------------------------------------------ reciprocal space
@@ -460,11 +611,69 @@ static PairParams make_params(DevPimd *p) {
   return pp;
 }
 
+// Tile-pair schedule and partial-sum scratch (grown on demand, kept).
+struct TileBufs {
+  int N = -1, T = 0, ntp = 0;
+  int2 *tiles = nullptr;
+  double *Fpart = nullptr, *Epart = nullptr;
+  size_t fcap = 0, ecap = 0;
+};
+static TileBufs g_tb;
+
+static int ensure_tiles(int N, int P) {
+  TileBufs &B = g_tb;
+  if (B.N != N) {
+    int T = (N + 63) / 64;
+    std::vector<int2> tl;
+    for (int I = 0; I < T; ++I)
+      for (int J = I; J < T; ++J) tl.push_back(make_int2(I, J));
+    if (B.tiles) (void)hipFree(B.tiles);
+    XMD_CHECK(hipMalloc(&B.tiles, tl.size() * sizeof(int2)));
+    XMD_CHECK(hipMemcpy(B.tiles, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice));
+    B.N = N;
+    B.T = T;
+    B.ntp = (int)tl.size();
+  }
+  size_t fneed = (size_t)3 * N * P * B.T, eneed = (size_t)P * B.ntp;
+  if (fneed > B.fcap) {
+    if (B.Fpart) (void)hipFree(B.Fpart);
+    XMD_CHECK(hipMalloc(&B.Fpart, fneed * sizeof(double)));
+    B.fcap = fneed;
+  }
+  if (eneed > B.ecap) {
+    if (B.Epart) (void)hipFree(B.Epart);
+    XMD_CHECK(hipMalloc(&B.Epart, eneed * sizeof(double)));
+    B.ecap = eneed;
+  }
+  return 0;
+}
+
+// The near-image enumeration covers every image within r_c when
+// r_c < sqrt(2) L and the reference's image cube (|n| <= n_sum around the raw
+// separation) holds all of them, i.e. n_sum >= 3 for wrapped positions.
+static int near_ok(const PairParams &pp) {
+  double Lmin = fmin(pp.L[0], fmin(pp.L[1], pp.L[2]));
+  // (the image kernel's erfc series also needs alpha r_c <= 6.5)
+  return (pp.nsum >= 3 && pp.rc2 < 2.0 * Lmin * Lmin && pp.alpha * sqrt(pp.rc2) <= 6.5) ? 1 : 0;
+}
+
 template <int MODE, bool ENERGY>
-static int launch_pair_mode(DevPimd *p, const PairParams &pp, hipStream_t st, double *epart) {
+static int launch_pair_mode(DevPimd *p, const PairParams &pp, hipStream_t st, double *esum) {
   DevSim *d = p->ds;
-  dim3 grid((p->N + 63) / 64, p->P);
-  hipLaunchKernelGGL((k_pair<MODE, ENERGY>), grid, dim3(256), 0, st, d->X, d->M, d->F, p->N, d->nb, pp, epart);
+  if (ensure_tiles(p->N, p->P)) return -1;
+  TileBufs &B = g_tb;
+  const bool near = (MODE == PK_EWALD_NI) || near_ok(pp);
+  const int items = p->P * B.ntp;
+  if (near)
+    hipLaunchKernelGGL((k_pair_tiles<MODE, ENERGY, true>), dim3(items), dim3(64), 0, st, d->X, d->M, B.Fpart,
+                       B.Epart, B.tiles, p->N, d->nb, B.T, B.ntp, pp);
+  else
+    hipLaunchKernelGGL((k_pair_tiles<MODE, ENERGY, false>), dim3(items), dim3(64), 0, st, d->X, d->M, B.Fpart,
+                       B.Epart, B.tiles, p->N, d->nb, B.T, B.ntp, pp);
+  XMD_CHECK(hipGetLastError());
+  if (ENERGY) return device_sum(B.Epart, items, esum);
+  const int n3 = 3 * d->nb;
+  hipLaunchKernelGGL(k_reduce_tiles, dim3((n3 + 255) / 256), dim3(256), 0, st, B.Fpart, d->F, n3, B.T, pp.P);
   XMD_CHECK(hipGetLastError());
   return 0;
 }
@@ -534,23 +743,16 @@ int pair_energy(DevPimd *p, int kind, double *out) {
   PairParams pp = make_params(p);
   DevSim *d = p->ds;
   hipStream_t st = rt().s0;
-  int nblk = ((p->N + 63) / 64) * p->P;
-  if (nblk > d->red_len) {
-    fprintf(stderr, "xmd: reduction scratch too small\n");
-    return -1;
-  }
-  double e_pairs = 0;
+  double e = 0;  // every unordered pair once
   if (kind == PK_GAUSS) {
-    if (launch_pair_mode<PK_GAUSS, true>(p, pp, st, d->red)) return -1;
+    if (launch_pair_mode<PK_GAUSS, true>(p, pp, st, &e)) return -1;
   } else if (kind == PK_PGAUSS) {
-    if (launch_pair_mode<PK_PGAUSS, true>(p, pp, st, d->red)) return -1;
+    if (launch_pair_mode<PK_PGAUSS, true>(p, pp, st, &e)) return -1;
   } else if (kind == PK_EWALD) {
-    if (launch_pair_mode<PK_EWALD, true>(p, pp, st, d->red)) return -1;
+    if (launch_pair_mode<PK_EWALD, true>(p, pp, st, &e)) return -1;
   } else {
     return -1;
   }
-  if (device_sum(d->red, nblk, &e_pairs)) return -1;
-  double e = e_pairs / 2;  // ordered pairs counted twice
   if (kind == PK_EWALD) {
     if (launch_recip(p, pp, st, false)) return -1;
     KSpace &K = g_ks;

@@ -1,10 +1,11 @@
 // The fused MD step and its hipGraph capture, plus run-time knobs.
 //
-// Stream layout of one step (reference driver order, SURVEY call stack B):
-//   s0: NHC half 1 (+ wrap) --fork--> reset F, trap, ring springs, pair real,
-//       pair reciprocal --wait join--> exchange end-bead forces, NHC half 2
-//   s1: (after fork) cycle energies -> forward + backward recursions --join-->
-// The exchange branch only reads positions, so it overlaps the pair forces;
+// Stream layout of one step (the reference driver's order of stages):
+//   s0: NHC half 1 (+ wrap), cycle energies --fork--> reset F, trap, ring
+//       springs, pair real, pair reciprocal --wait join--> exchange end-bead
+//       forces, NHC half 2
+//   s1: (after fork) forward + backward recursions --join-->
+// The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces add into F after both branches are done.
 #include <cmath>
 
@@ -46,20 +47,26 @@ static int plan_of(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, StepPlan
 }
 
 // ev: optional per-stage events (2 per stage: start, stop), may be null.
-static int enqueue_forces(StepPlan &sp, hipEvent_t *ev) {
+// serial: run every stage on s0 (true per-stage device times for profiling).
+static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial = false) {
   Runtime &R = rt();
   DevPimd *p = sp.p;
   DevSim *d = p->ds;
+  hipStream_t sx = serial ? R.s0 : R.s1;
   auto rec = [&](int k, hipStream_t st) -> int {
     if (ev) XMD_CHECK(hipEventRecord(ev[k], st));
     return 0;
   };
-  XMD_CHECK(hipEventRecord(R.fork, R.s0));
-  XMD_CHECK(hipStreamWaitEvent(R.s1, R.fork, 0));
-  // exchange branch
-  if (rec(2, R.s1) || launch_enk(p, sp.image, R.s1) || rec(3, R.s1)) return -1;
-  if (rec(4, R.s1) || launch_recursion(p, true, true, R.s1) || rec(5, R.s1)) return -1;
-  XMD_CHECK(hipEventRecord(R.join, R.s1));
+  // Cycle energies (short, position-only) first; then the two recursions are
+  // forked onto s1 before the pair kernels are queued, so their two
+  // workgroups are resident before the pair grid fills the device.
+  if (rec(2, R.s0) || launch_enk(p, sp.image, R.s0) || rec(3, R.s0)) return -1;
+  if (!serial) {
+    XMD_CHECK(hipEventRecord(R.fork, R.s0));
+    XMD_CHECK(hipStreamWaitEvent(R.s1, R.fork, 0));
+  }
+  if (rec(4, sx) || launch_recursion(p, true, true, sx) || rec(5, sx)) return -1;
+  if (!serial) XMD_CHECK(hipEventRecord(R.join, R.s1));
   // bead-local and pair forces
   if (rec(6, R.s0) || launch_reset_force(d, R.s0)) return -1;
   if (sp.trap && launch_trap_force(d, md_trap_frequency_x, md_trap_center_x, (double)p->P, R.s0)) return -1;
@@ -74,19 +81,19 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev) {
   } else if (rec(8, R.s0) || rec(9, R.s0) || rec(10, R.s0) || rec(11, R.s0)) {
     return -1;
   }
-  XMD_CHECK(hipStreamWaitEvent(R.s0, R.join, 0));
+  if (!serial) XMD_CHECK(hipStreamWaitEvent(R.s0, R.join, 0));
   if (rec(12, R.s0) || launch_exchange_force(p, sp.image, R.s0) || rec(13, R.s0)) return -1;
   return 0;
 }
 
-static int enqueue_step(StepPlan &sp, hipEvent_t *ev) {
+static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial = false) {
   Runtime &R = rt();
   DevSim *d = sp.p->ds;
   if (ev) XMD_CHECK(hipEventRecord(ev[0], R.s0));
   if (launch_nhc(d, 1, sp.dt, R.s0)) return -1;
   if (sp.pbc && launch_pbc(d, sp.L, R.s0)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[1], R.s0));
-  if (enqueue_forces(sp, ev)) return -1;
+  if (enqueue_forces(sp, ev, serial)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[14], R.s0));
   if (launch_nhc(d, 2, sp.dt, R.s0)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[15], R.s0));
@@ -170,7 +177,8 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   return 0;
 }
 
-int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, double *stage_ms) {
+int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, int serial,
+                            double *stage_ms) {
   StepPlan sp;
   if (plan_of(pimd, cfg, sp)) return -1;
   const int NE = 16;
@@ -180,7 +188,7 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
   // stage k <- events (a, b)
   const int pairs[8][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {10, 11}, {12, 13}, {14, 15}};
   for (int s = 0; s < nsteps; ++s) {
-    if (enqueue_step(sp, ev)) return -1;
+    if (enqueue_step(sp, ev, serial != 0)) return -1;
     XMD_CHECK(hipStreamSynchronize(rt().s0));
     for (int k = 0; k < 8; ++k) {
       float ms = 0;

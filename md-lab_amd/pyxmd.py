@@ -136,7 +136,7 @@ _SIGS = {
     "md_pimd_get_connection_matrix_x": (C.c_int, [PIMD_P, DP]),
     "md_pimd_compute_forces_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig)]),
     "md_pimd_run_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), C.c_int]),
-    "md_pimd_profile_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), C.c_int, DP]),
+    "md_pimd_profile_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), C.c_int, C.c_int, DP]),
 }
 
 CALLBACKS = ["md_harmonic_trap_force_x", "md_gaussian_force_x", "md_periodic_gaussian_force_x",
