@@ -263,6 +263,7 @@ static int alloc_pimd(DevPimd *p) {
   XMD_CHECK(hipMalloc(&p->Lf, (N + 1) * sizeof(double)));
   XMD_CHECK(hipMalloc(&p->Rb, (N + 1) * sizeof(double)));
   XMD_CHECK(hipMalloc(&p->work, 8 * (N + 1) * sizeof(double) + 64 * 1024 * sizeof(double)));
+  XMD_CHECK(hipMalloc(&p->rec, 6 * (N + 1) * sizeof(double)));
   return 0;
 }
 

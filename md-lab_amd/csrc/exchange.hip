@@ -40,9 +40,9 @@
 namespace xmd {
 
 constexpr double NEG = -1.0e300;  // "no term" marker (exp(NEG - x) == 0)
-constexpr int RB = 64;             // diagonal block
-constexpr int RT = 1024;           // recursion workgroup
-constexpr int MAXN_REC = 4096;     // LDS-resident recursion limit
+constexpr int RB = 64;             // recursion block (rows solved per diagonal launch)
+constexpr int RT = 1024;           // estimator workgroup
+constexpr double LSE_SKIP = 40.0;  // log-sum-exp terms below max - 40 are < 4.3e-18 relative
 
 __device__ __forceinline__ size_t tri_row(int n) { return (size_t)(n - 1) * n / 2; }  // row n (1-based)
 __device__ __forceinline__ size_t col_off(int m, int N) { return (size_t)m * N - (size_t)m * (m - 1) / 2; }
@@ -162,6 +162,8 @@ struct RecArgs {
   double *Lf;         // forward output, N+1
   double *Rb;         // backward output, N+1
   double *VBN;        // V_B^(n) = -L(n)/beta
+  double *val2;       // [2][N+1] solved values per direction, position order
+  double *accM, *accS;  // [2][N+1] off-block log-sum-exp accumulators
   int N;
   double beta, lnxi;
   int xi0;            // xi == 0: only k = 1 terms
@@ -186,94 +188,158 @@ __device__ __forceinline__ double rec_coef(const RecArgs &a, int p, int pp) {
   return (k - 1) * a.lnxi - a.beta * E - ln_n;
 }
 
+// ---- blocked right-looking solve (positions in blocks of RB rows) ----
+//
+// Direction dir (0 forward, 1 backward) keeps, per position p, the solved
+// value val[p] and a running log-sum-exp (accM[p], accS[p]) of the terms
+// coef(p, p') + val[p'] from every already solved position p' outside p's
+// own block.  Per block:
+//   k_rec_diag    one workgroup per direction solves the block's rows from
+//                 their accumulators plus the within-block terms;
+//   k_rec_update  one wave per later row folds the block's (<= 64) new
+//                 columns into that row's accumulator (lanes = columns).
+// The serial part is only the small diagonal solve; the O(N^2) update work
+// spreads over the whole device.
+
+__global__ void k_rec_init(RecArgs a, int dir0) {
+  const int dir = dir0 + blockIdx.y;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = a.N;
+  if (p > N) return;
+  double *val = a.val2 + (size_t)dir * (N + 1);
+  double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  if (p == 0) {
+    val[0] = 0.0;
+    if (dir == 0) {
+      a.Lf[0] = 0.0;
+      a.VBN[0] = 0.0;
+    } else {
+      a.Rb[N] = 0.0;
+    }
+    return;
+  }
+  const double c = dir == 0 ? rec_coef<true>(a, p, 0) : rec_coef<false>(a, p, 0);
+  AM[p] = c > NEG ? c : NEG;
+  AS[p] = c > NEG ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ void tri_decode(int w, int &q, int &r) {
+  q = (int)((1.0 + sqrt(1.0 + 8.0 * w)) / 2.0);
+  while (q * (q - 1) / 2 > w) --q;
+  while ((q + 1) * q / 2 <= w) ++q;
+  r = w - q * (q - 1) / 2;
+}
+
 template <bool FWD>
-__device__ void recursion_body(const RecArgs &a, double *val, double *dg, double *mo, double *so) {
+__device__ void rec_diag_body(const RecArgs &a, int dir, int p0, int nrows, double *dg, double *mo, double *so,
+                              double *sg) {
   const int N = a.N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (tid == 0) val[0] = 0.0;
-  __syncthreads();
-  for (int p0 = 1; p0 <= N; p0 += RB) {
-    const int nrows = min(RB, N - p0 + 1);
-    // (A) off-diagonal: rows p0 + r against solved positions 0..p0-1
-    for (int r = wv; r < nrows; r += RT / 64) {
-      const int p = p0 + r;
-      double mx = NEG;
-      for (int pp = lane; pp < p0; pp += 64) {
-        double c = rec_coef<FWD>(a, p, pp);
-        if (c > NEG) mx = fmax(mx, c + val[pp]);
-      }
-      mx = wave_max(mx);
-      double s = 0;
-      if (mx > NEG) {
-        for (int pp = lane; pp < p0; pp += 64) {
-          double c = rec_coef<FWD>(a, p, pp);
-          if (c > NEG) {
-            double t = c + val[pp] - mx;
-            if (t > -745.0) s += exp(t);
-          }
-        }
-      }
-      s = wave_sum(s);
-      if (lane == 0) {
-        mo[r] = mx;
-        so[r] = s;
-      }
-    }
-    // diagonal coefficients, lower triangle q > r stored at q(q-1)/2 + r
-    for (int w = tid; w < RB * (RB - 1) / 2; w += RT) {
-      int q = (int)((1.0 + sqrt(1.0 + 8.0 * w)) / 2.0);
-      while (q * (q - 1) / 2 > w) --q;
-      while ((q + 1) * q / 2 <= w) ++q;
-      int r = w - q * (q - 1) / 2;
-      dg[w] = (q < nrows) ? rec_coef<FWD>(a, p0 + q, p0 + r) : NEG;
-    }
-    __syncthreads();
-    // (B) diagonal block, one wave
-    if (wv == 0) {
-      double M = lane < nrows ? mo[lane] : NEG;
-      double S = lane < nrows ? so[lane] : 0.0;
-      for (int r = 0; r < nrows; ++r) {
-        const double Mr = readlane_d(M, r), Sr = readlane_d(S, r);
-        if (lane > r && lane < nrows) {
-          const double c = dg[lane * (lane - 1) / 2 + r];
-          if (c > NEG) {
-            const double T = c + Mr;
-            if (T > M) {
-              S = S * exp(M - T) + Sr;
-              M = T;
-            } else {
-              S += Sr * exp(T - M);
-            }
-          }
-        }
-      }
-      if (lane < nrows) val[p0 + lane] = M + log(S);
-    }
-    __syncthreads();
+  double *val = a.val2 + (size_t)dir * (N + 1);
+  const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  if (tid < RB) {
+    mo[tid] = tid < nrows ? AM[p0 + tid] : NEG;
+    so[tid] = tid < nrows ? AS[p0 + tid] : 0.0;
   }
-  for (int p = tid; p <= N; p += RT) {
-    if (FWD) {
-      a.Lf[p] = val[p];
-      a.VBN[p] = -val[p] / a.beta;
-    } else {
-      a.Rb[N - p] = val[p];
+  for (int w = tid; w < RB * (RB - 1) / 2; w += blockDim.x) {
+    int q, r;
+    tri_decode(w, q, r);
+    dg[w] = (q < nrows) ? rec_coef<FWD>(a, p0 + q, p0 + r) : NEG;
+  }
+  __syncthreads();
+  // per-row shift sigma_q = max(M_q, max_{r<q} c(q,r) + sigma_r): a (max,+)
+  // scan with no transcendental on its serial chain.  Every term obeys
+  // c(q,r) + L(r) <= L(q) <= sigma_q + (rows) ln N, so the scaled values
+  // below stay inside FP64 range.
+  if (wv == 0) {
+    double sig = lane < nrows ? mo[lane] : NEG;
+    for (int r = 0; r < nrows; ++r) {
+      const double sr = readlane_d(sig, r);
+      if (lane > r && lane < nrows) {
+        const double c = dg[lane * (lane - 1) / 2 + r];
+        if (c > NEG) sig = fmax(sig, c + sr);
+      }
+    }
+    sg[lane] = sig;
+  }
+  __syncthreads();
+  // scaled coefficients a(q,r) = exp(c(q,r) + sigma_r - sigma_q), all threads
+  for (int w = tid; w < RB * (RB - 1) / 2; w += blockDim.x) {
+    int q, r;
+    tri_decode(w, q, r);
+    const double c = dg[w];
+    dg[w] = (q < nrows && c > NEG) ? exp(c + sg[r] - sg[q]) : 0.0;
+  }
+  __syncthreads();
+  // linear forward substitution W_q = b_q + sum_{r<q} a(q,r) W_r, with
+  // W_q = exp(L(q) - sigma_q): one FMA per lane per step
+  if (wv == 0) {
+    double W = 0.0;
+    if (lane < nrows && mo[lane] > NEG && mo[lane] > sg[lane] - 745.0) W = so[lane] * exp(mo[lane] - sg[lane]);
+    for (int r = 0; r < nrows; ++r) {
+      const double wr = readlane_d(W, r);
+      if (lane > r && lane < nrows) W = fma(dg[lane * (lane - 1) / 2 + r], wr, W);
+    }
+    if (lane < nrows) {
+      const int p = p0 + lane;
+      const double v = sg[lane] + log(W);
+      val[p] = v;
+      if (FWD) {
+        a.Lf[p] = v;
+        a.VBN[p] = -v / a.beta;
+      } else {
+        a.Rb[N - p] = v;
+      }
     }
   }
 }
 
-__global__ __launch_bounds__(RT) void k_recursion(RecArgs a, int dirmask) {
-  extern __shared__ double lds[];
-  double *dg = lds;                          // RB*(RB-1)/2
-  double *mo = dg + RB * (RB - 1) / 2;       // RB
-  double *so = mo + RB;                      // RB
-  double *val = so + RB;                     // N+1
-  // block 0: forward (if requested), block 1: backward (if requested)
-  int dir = blockIdx.x;
-  if (gridDim.x == 1) dir = (dirmask & 1) ? 0 : 1;
+__global__ __launch_bounds__(256) void k_rec_diag(RecArgs a, int dir0, int p0, int nrows) {
+  __shared__ double dg[RB * (RB - 1) / 2], mo[RB], so[RB], sg[RB];
+  const int dir = dir0 + blockIdx.x;
   if (dir == 0)
-    recursion_body<true>(a, val, dg, mo, so);
+    rec_diag_body<true>(a, 0, p0, nrows, dg, mo, so, sg);
   else
-    recursion_body<false>(a, val, dg, mo, so);
+    rec_diag_body<false>(a, 1, p0, nrows, dg, mo, so, sg);
+}
+
+template <bool FWD>
+__device__ __forceinline__ void rec_update_row(const RecArgs &a, int dir, int p, int p0, int nrows, int lane) {
+  const int N = a.N;
+  const double *val = a.val2 + (size_t)dir * (N + 1);
+  double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  double t = NEG;
+  if (lane < nrows) {
+    const double c = rec_coef<FWD>(a, p, p0 + lane);
+    if (c > NEG) t = c + val[p0 + lane];
+  }
+  const double Mw = wave_max(t);
+  if (!(Mw > NEG)) return;
+  const double e = (t > Mw - LSE_SKIP) ? exp(t - Mw) : 0.0;
+  const double Sw = wave_sum(e);
+  if (lane == 0) {
+    double M = AM[p], S = AS[p];
+    if (Mw > M) {
+      S = S * exp(M - Mw) + Sw;
+      M = Mw;
+    } else if (Mw > M - LSE_SKIP) {
+      S += Sw * exp(Mw - M);
+    }
+    AM[p] = M;
+    AS[p] = S;
+  }
+}
+
+// One wave per later row p >= p0 + nrows; lanes are the block's columns.
+__global__ __launch_bounds__(256) void k_rec_update(RecArgs a, int dir0, int p0, int nrows) {
+  const int dir = dir0 + blockIdx.y;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = p0 + nrows + blockIdx.x * 4 + wv;
+  if (p > a.N) return;
+  if (dir == 0)
+    rec_update_row<true>(a, 0, p, p0, nrows, lane);
+  else
+    rec_update_row<false>(a, 1, p, p0, nrows, lane);
 }
 
 __global__ void k_lnn(double *lnn, int N) {
@@ -585,6 +651,9 @@ static RecArgs rec_args(DevPimd *p) {
   a.Lf = p->Lf;
   a.Rb = p->Rb;
   a.VBN = p->VBN;
+  a.val2 = p->rec;
+  a.accM = p->rec + 2 * (p->N + 1);
+  a.accS = p->rec + 4 * (p->N + 1);
   a.N = p->N;
   a.beta = p->host->beta;
   a.xi0 = p->host->vi == 0.0;
@@ -613,19 +682,25 @@ int launch_enk(DevPimd *p, int image, hipStream_t st) {
   return 0;
 }
 
-static size_t rec_lds(int N) { return (RB * (RB - 1) / 2 + 2 * RB + (N + 1)) * sizeof(double); }
 
 int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
-  if (p->N > MAXN_REC) {
-    fprintf(stderr, "xmd: exchange recursion supports N <= %d\n", MAXN_REC);
-    return -1;
-  }
+  const int ndir = (fwd ? 1 : 0) + (bwd ? 1 : 0);
+  if (ndir == 0) return 0;
+  const int dir0 = fwd ? 0 : 1;
+  const int N = p->N;
   RecArgs a = rec_args(p);
-  int nblk = (fwd ? 1 : 0) + (bwd ? 1 : 0);
-  if (nblk == 0) return 0;
-  int mask = (fwd ? 1 : 0) | (bwd ? 2 : 0);
-  hipLaunchKernelGGL(k_recursion, dim3(nblk), dim3(RT), rec_lds(p->N), st, a, mask);
+  hipLaunchKernelGGL(k_rec_init, dim3((N + 256) / 256, ndir), dim3(256), 0, st, a, dir0);
   XMD_CHECK(hipGetLastError());
+  for (int p0 = 1; p0 <= N; p0 += RB) {
+    const int nrows = min(RB, N - p0 + 1);
+    hipLaunchKernelGGL(k_rec_diag, dim3(ndir), dim3(256), 0, st, a, dir0, p0, nrows);
+    XMD_CHECK(hipGetLastError());
+    const int left = N - (p0 + nrows) + 1;
+    if (left > 0) {
+      hipLaunchKernelGGL(k_rec_update, dim3((left + 3) / 4, ndir), dim3(256), 0, st, a, dir0, p0, nrows);
+      XMD_CHECK(hipGetLastError());
+    }
+  }
   if (fwd) p->vbn_dev = true;
   if (bwd) {
     p->back_dev = true;

@@ -100,6 +100,7 @@ struct DevPimd {
   double *Rb = nullptr;     // backward R(m) = -beta Vt^(m),  N+1
   double *Gm = nullptr;     // optional connection matrix (debug/tests), N*N
   double *work = nullptr;   // estimator scratch
+  double *rec = nullptr;    // recursion state: values and accumulators, 6*(N+1)
   bool enk_dev = false, enk2_dev = false, vbn_dev = false, back_dev = false;
   bool enk_host_stale = false;
   unsigned enk_version = 0, back_version = 0;
