@@ -26,6 +26,21 @@ int md_hip_synchronize_x(void);
 /* Real-space Ewald cutoff in units of 1/alpha (default 6: erfc(6) ~ 2e-17). */
 int md_set_ewald_real_cutoff_x(md_num_t_x x_cut);
 
+/* Ewald split used for md_coulomb_3d_ewald_force_x / _energy_x.  The Ewald
+ * sum does not depend on the splitting parameter once its real- and
+ * reciprocal-space series are converged; when the reference parameters
+ * (md_coulomb_truc_x, md_coulomb_n_sum_x) are converged (first omitted k
+ * shell below exp(-30), real-space image cube beyond erfc(6)), the engine may
+ * evaluate the same sum with its own alpha' (real space within x_cut/alpha',
+ * k in the ball pi|G|/alpha' <= x_cut).  alpha_L = 0: always the reference's
+ * alpha and k cube; alpha_L > 0: alpha' = alpha_L / min(L) when allowed;
+ * alpha_L < 0 (default): alpha' chosen by the engine's cost model.  The
+ * _R_x / _NI_x splits always use the reference's alpha. */
+int md_set_ewald_split_x(md_num_t_x alpha_L);
+/* The split md_coulomb_3d_ewald_force_x would use for this object: alpha,
+ * real-space cutoff and the number of half-space k vectors (any may be NULL). */
+int md_pimd_get_ewald_plan_x(md_pimd_t_x *pimd, md_num_t_x *alpha, md_num_t_x *r_cut, int *n_k);
+
 /* Internal exchange quantities after md_pimd_fill_VB_x (tests, diagnostics):
  * backward potentials Vt^(0..N) and the connection-probability matrix
  * G[(l-1)*N + (j-1)] = P(bead P of l links to bead 1 of j). */

@@ -164,7 +164,9 @@ struct RecArgs {
   double *VBN;        // V_B^(n) = -L(n)/beta
   double *val2;       // [2][N+1] solved values per direction, position order
   double *accM, *accS;  // [2][N+1] off-block log-sum-exp accumulators
-  int N;
+  double *dg;         // [2][nblk][RB(RB-1)/2] in-block coefficients
+  double *la;         // [2][nblk][RB][RB] coefficients of the block on the previous block
+  int N, nblk;
   double beta, lnxi;
   int xi0;            // xi == 0: only k = 1 terms
 };
@@ -192,35 +194,37 @@ __device__ __forceinline__ double rec_coef(const RecArgs &a, int p, int pp) {
 //
 // Direction dir (0 forward, 1 backward) keeps, per position p, the solved
 // value val[p] and a running log-sum-exp (accM[p], accS[p]) of the terms
-// coef(p, p') + val[p'] from every already solved position p' outside p's
-// own block.  Per block:
-//   k_rec_diag    one workgroup per direction solves the block's rows from
-//                 their accumulators plus the within-block terms;
-//   k_rec_update  one wave per later row folds the block's (<= 64) new
-//                 columns into that row's accumulator (lanes = columns).
-// The serial part is only the small diagonal solve; the O(N^2) update work
-// spreads over the whole device.
+// coef(p, p') + val[p'] over solved positions p' outside p's own block.
+//   k_rec_prep   (one launch) initial accumulators from position 0, and the
+//                coefficients of every diagonal block and of every block on
+//                its predecessor, so the serial kernels read them with
+//                independent, coalesced loads;
+//   k_rec_block  (one launch per block b) one workgroup per direction folds
+//                block b-1 into block b's accumulators (look-ahead) and
+//                solves block b; the other workgroups fold block b-1 into
+//                64 later rows each.
+// Every row of block B thus receives column 0 (prep), blocks 0..B-2 (the
+// wide part of launches 1..B-1) and block B-1 (look-ahead of launch B).
+// Lanes are rows throughout: log-sum-exps run along each lane, so no
+// cross-lane reductions are needed.
+constexpr int RTRI = RB * (RB - 1) / 2;
 
-__global__ void k_rec_init(RecArgs a, int dir0) {
-  const int dir = dir0 + blockIdx.y;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = a.N;
-  if (p > N) return;
-  double *val = a.val2 + (size_t)dir * (N + 1);
-  double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
-  if (p == 0) {
-    val[0] = 0.0;
-    if (dir == 0) {
-      a.Lf[0] = 0.0;
-      a.VBN[0] = 0.0;
-    } else {
-      a.Rb[N] = 0.0;
-    }
-    return;
+// coef(p, p') read through the other layout (column p' of the forward
+// problem is contiguous in ET, of the backward problem in ENk), so that
+// lanes p, p+1, ... at fixed p' load consecutive doubles.
+template <bool FWD>
+__device__ __forceinline__ double rec_coef_col(const RecArgs &a, int p, int pp) {
+  const int k = p - pp;
+  if (a.xi0 && k != 1) return NEG;
+  double E, ln_n;
+  if (FWD) {
+    E = a.ET[col_off(pp, a.N) + k - 1];
+    ln_n = a.lnn[p];
+  } else {
+    E = a.ENk[tri_row(a.N - pp) + k - 1];
+    ln_n = a.lnn[a.N - pp];
   }
-  const double c = dir == 0 ? rec_coef<true>(a, p, 0) : rec_coef<false>(a, p, 0);
-  AM[p] = c > NEG ? c : NEG;
-  AS[p] = c > NEG ? 1.0 : 0.0;
+  return (k - 1) * a.lnxi - a.beta * E - ln_n;
 }
 
 __device__ __forceinline__ void tri_decode(int w, int &q, int &r) {
@@ -231,55 +235,152 @@ __device__ __forceinline__ void tri_decode(int w, int &q, int &r) {
 }
 
 template <bool FWD>
-__device__ void rec_diag_body(const RecArgs &a, int dir, int p0, int nrows, double *dg, double *mo, double *so,
-                              double *sg) {
+__device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t) {
   const int N = a.N;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  double *val = a.val2 + (size_t)dir * (N + 1);
-  const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
-  if (tid < RB) {
-    mo[tid] = tid < nrows ? AM[p0 + tid] : NEG;
-    so[tid] = tid < nrows ? AS[p0 + tid] : 0.0;
+  if (t <= N) {
+    double *val = a.val2 + (size_t)dir * (N + 1);
+    double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+    if (t == 0) {
+      val[0] = 0.0;
+      if (FWD) {
+        a.Lf[0] = 0.0;
+        a.VBN[0] = 0.0;
+      } else {
+        a.Rb[N] = 0.0;
+      }
+    } else {
+      const double c = rec_coef<FWD>(a, t, 0);
+      AM[t] = c > NEG ? c : NEG;
+      AS[t] = c > NEG ? 1.0 : 0.0;
+    }
   }
-  for (int w = tid; w < RB * (RB - 1) / 2; w += blockDim.x) {
+  const int ndg = a.nblk * RTRI;
+  if (t < ndg) {
+    const int b = t / RTRI, w = t - b * RTRI;
     int q, r;
     tri_decode(w, q, r);
-    dg[w] = (q < nrows) ? rec_coef<FWD>(a, p0 + q, p0 + r) : NEG;
+    const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
+    a.dg[(size_t)dir * ndg + t] = q < nrows ? rec_coef<FWD>(a, p0 + q, p0 + r) : NEG;
+  }
+  const int nla = a.nblk * RB * RB;
+  if (t < nla) {  // transposed: [b][r][q]
+    const int b = t / (RB * RB), w = t - b * (RB * RB), r = w / RB, q = w - r * RB;
+    const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
+    a.la[(size_t)dir * nla + t] = (b > 0 && q < nrows) ? rec_coef_col<FWD>(a, p0 + q, p0 - RB + r) : NEG;
+  }
+}
+
+__global__ void k_rec_prep(RecArgs a, int dir0) {
+  const int dir = dir0 + blockIdx.y;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dir == 0)
+    rec_prep_one<true>(a, 0, t);
+  else
+    rec_prep_one<false>(a, 1, t);
+}
+
+// (M, S) <- log-sum-exp merge with (m, s).
+__device__ __forceinline__ void lse_pair(double &M, double &S, double m, double s) {
+  if (!(m > NEG)) return;
+  if (m > M) {
+    S = S * exp(M - m) + s;
+    M = m;
+  } else if (m > M - LSE_SKIP) {
+    S += s * exp(m - M);
+  }
+}
+
+// Two-pass log-sum-exp of 16 terms held by this lane.
+__device__ __forceinline__ void lse16(const double *t, double &M, double &S) {
+  M = NEG;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) M = fmax(M, t[j]);
+  S = 0.0;
+  if (M > NEG) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) S += (t[j] > M - LSE_SKIP) ? exp(t[j] - M) : 0.0;
+  }
+}
+
+template <bool FWD>
+__device__ void rec_diag_body(const RecArgs &a, int dir, int b, double *dg, double *mo, double *so, double *sg,
+                              double (*pm)[RB], double (*ps)[RB]) {
+  const int N = a.N;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
+  double *val = a.val2 + (size_t)dir * (N + 1);
+  const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  const double *DG = a.dg + ((size_t)dir * a.nblk + b) * RTRI;
+  for (int w = tid; w < RTRI; w += blockDim.x) dg[w] = DG[w];
+  // look-ahead: block b-1 into this block's rows; wave wv takes columns
+  // 16 wv .. 16 wv + 15, lane = row
+  {
+    double M = NEG, S = 0.0;
+    if (b > 0) {
+      const double *LA = a.la + ((size_t)dir * a.nblk + b) * (RB * RB);
+      const double *vp = val + (p0 - RB);
+      double t[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = 16 * wv + j;
+        const double c = LA[r * RB + lane];
+        t[j] = c > NEG ? c + vp[r] : NEG;
+      }
+      lse16(t, M, S);
+    }
+    pm[wv][lane] = M;
+    ps[wv][lane] = S;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    double M = NEG, S = 0.0;
+    if (lane < nrows) {
+      M = AM[p0 + lane];
+      S = AS[p0 + lane];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) lse_pair(M, S, pm[w][lane], ps[w][lane]);
+    }
+    mo[lane] = M;
+    so[lane] = S;
   }
   __syncthreads();
   // per-row shift sigma_q = max(M_q, max_{r<q} c(q,r) + sigma_r): a (max,+)
-  // scan with no transcendental on its serial chain.  Every term obeys
-  // c(q,r) + L(r) <= L(q) <= sigma_q + (rows) ln N, so the scaled values
-  // below stay inside FP64 range.
+  // scan with no transcendental on its serial chain; each lane holds its
+  // row of c in registers.  Every term obeys c(q,r) + L(r) <= L(q) <=
+  // sigma_q + (rows) ln N, so the scaled values below stay in FP64 range.
+  const int rowq = lane * (lane - 1) / 2;
   if (wv == 0) {
+    double c[RB - 1];
+#pragma unroll
+    for (int r = 0; r < RB - 1; ++r) c[r] = r < lane ? dg[rowq + r] : NEG;
     double sig = lane < nrows ? mo[lane] : NEG;
-    for (int r = 0; r < nrows; ++r) {
-      const double sr = readlane_d(sig, r);
-      if (lane > r && lane < nrows) {
-        const double c = dg[lane * (lane - 1) / 2 + r];
-        if (c > NEG) sig = fmax(sig, c + sr);
-      }
-    }
-    sg[lane] = sig;
+#pragma unroll
+    for (int r = 0; r < RB - 1; ++r) sig = fmax(sig, c[r] + readlane_d(sig, r));
+    sg[lane] = lane < nrows ? sig : NEG;
   }
   __syncthreads();
-  // scaled coefficients a(q,r) = exp(c(q,r) + sigma_r - sigma_q), all threads
-  for (int w = tid; w < RB * (RB - 1) / 2; w += blockDim.x) {
-    int q, r;
-    tri_decode(w, q, r);
-    const double c = dg[w];
-    dg[w] = (q < nrows && c > NEG) ? exp(c + sg[r] - sg[q]) : 0.0;
+  // scaled coefficients a(q,r) = exp(c(q,r) + sigma_r - sigma_q): thread
+  // (q, j) takes columns r = j, j + 4, ...
+  {
+    const int q = tid >> 2, j = tid & 3;
+    const int row = q * (q - 1) / 2;
+    const double sq = sg[q];
+    for (int r = j; r < q; r += 4) {
+      const double c = dg[row + r];
+      dg[row + r] = (q < nrows && c > NEG) ? exp(c + sg[r] - sq) : 0.0;
+    }
   }
   __syncthreads();
   // linear forward substitution W_q = b_q + sum_{r<q} a(q,r) W_r, with
-  // W_q = exp(L(q) - sigma_q): one FMA per lane per step
+  // W_q = exp(L(q) - sigma_q): one FMA per step (a(q,r) = 0 for r >= q)
   if (wv == 0) {
+    double av[RB - 1];
+#pragma unroll
+    for (int r = 0; r < RB - 1; ++r) av[r] = r < lane ? dg[rowq + r] : 0.0;
     double W = 0.0;
     if (lane < nrows && mo[lane] > NEG && mo[lane] > sg[lane] - 745.0) W = so[lane] * exp(mo[lane] - sg[lane]);
-    for (int r = 0; r < nrows; ++r) {
-      const double wr = readlane_d(W, r);
-      if (lane > r && lane < nrows) W = fma(dg[lane * (lane - 1) / 2 + r], wr, W);
-    }
+#pragma unroll
+    for (int r = 0; r < RB - 1; ++r) W = fma(av[r], readlane_d(W, r), W);
     if (lane < nrows) {
       const int p = p0 + lane;
       const double v = sg[lane] + log(W);
@@ -294,52 +395,61 @@ __device__ void rec_diag_body(const RecArgs &a, int dir, int p0, int nrows, doub
   }
 }
 
-__global__ __launch_bounds__(256) void k_rec_diag(RecArgs a, int dir0, int p0, int nrows) {
-  __shared__ double dg[RB * (RB - 1) / 2], mo[RB], so[RB], sg[RB];
-  const int dir = dir0 + blockIdx.x;
-  if (dir == 0)
-    rec_diag_body<true>(a, 0, p0, nrows, dg, mo, so, sg);
-  else
-    rec_diag_body<false>(a, 1, p0, nrows, dg, mo, so, sg);
-}
-
+// Folds block b-1 (positions pprev .. pprev + 63) into the accumulators of
+// rows prow .. prow + 63: lane = row, wave wv takes 16 of the columns.
 template <bool FWD>
-__device__ __forceinline__ void rec_update_row(const RecArgs &a, int dir, int p, int p0, int nrows, int lane) {
+__device__ void rec_update_rows(const RecArgs &a, int dir, int prow, int pprev, double (*pm)[RB], double (*ps)[RB]) {
   const int N = a.N;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const double *val = a.val2 + (size_t)dir * (N + 1);
   double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
-  double t = NEG;
-  if (lane < nrows) {
-    const double c = rec_coef<FWD>(a, p, p0 + lane);
-    if (c > NEG) t = c + val[p0 + lane];
-  }
-  const double Mw = wave_max(t);
-  if (!(Mw > NEG)) return;
-  const double e = (t > Mw - LSE_SKIP) ? exp(t - Mw) : 0.0;
-  const double Sw = wave_sum(e);
-  if (lane == 0) {
-    double M = AM[p], S = AS[p];
-    if (Mw > M) {
-      S = S * exp(M - Mw) + Sw;
-      M = Mw;
-    } else if (Mw > M - LSE_SKIP) {
-      S += Sw * exp(Mw - M);
+  const int p = prow + lane;
+  const bool act = p <= N;
+  double t[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int pp = pprev + 16 * wv + j;
+    double v = NEG;
+    if (act) {
+      const double c = rec_coef_col<FWD>(a, p, pp);
+      if (c > NEG) v = c + val[pp];
     }
-    AM[p] = M;
-    AS[p] = S;
+    t[j] = v;
+  }
+  double M, S;
+  lse16(t, M, S);
+  pm[wv][lane] = M;
+  ps[wv][lane] = S;
+  __syncthreads();
+  if (wv == 0 && act) {
+    double Ma = AM[p], Sa = AS[p];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) lse_pair(Ma, Sa, pm[w][lane], ps[w][lane]);
+    AM[p] = Ma;
+    AS[p] = Sa;
   }
 }
 
-// One wave per later row p >= p0 + nrows; lanes are the block's columns.
-__global__ __launch_bounds__(256) void k_rec_update(RecArgs a, int dir0, int p0, int nrows) {
-  const int dir = dir0 + blockIdx.y;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int p = p0 + nrows + blockIdx.x * 4 + wv;
-  if (p > a.N) return;
+// Launch for block b: workgroups [0, ndir) solve it (one per direction),
+// the rest fold block b-1 into 64 later rows each.
+__global__ __launch_bounds__(256) void k_rec_block(RecArgs a, int dir0, int ndir, int b) {
+  __shared__ double dg[RTRI], mo[RB], so[RB], sg[RB], pm[4][RB], ps[4][RB];
+  if ((int)blockIdx.x < ndir) {
+    const int dir = dir0 + blockIdx.x;
+    if (dir == 0)
+      rec_diag_body<true>(a, 0, b, dg, mo, so, sg, pm, ps);
+    else
+      rec_diag_body<false>(a, 1, b, dg, mo, so, sg, pm, ps);
+    return;
+  }
+  const int u = blockIdx.x - ndir;
+  const int dir = dir0 + u % ndir;
+  const int p0 = 1 + b * RB;
+  const int prow = p0 + RB + (u / ndir) * RB;
   if (dir == 0)
-    rec_update_row<true>(a, 0, p, p0, nrows, lane);
+    rec_update_rows<true>(a, 0, prow, p0 - RB, pm, ps);
   else
-    rec_update_row<false>(a, 1, p, p0, nrows, lane);
+    rec_update_rows<false>(a, 1, prow, p0 - RB, pm, ps);
 }
 
 __global__ void k_lnn(double *lnn, int N) {
@@ -654,6 +764,9 @@ static RecArgs rec_args(DevPimd *p) {
   a.val2 = p->rec;
   a.accM = p->rec + 2 * (p->N + 1);
   a.accS = p->rec + 4 * (p->N + 1);
+  a.nblk = (p->N + RB - 1) / RB;
+  a.dg = p->rec + 6 * (p->N + 1);
+  a.la = a.dg + 2 * (size_t)a.nblk * RTRI;
   a.N = p->N;
   a.beta = p->host->beta;
   a.xi0 = p->host->vi == 0.0;
@@ -689,17 +802,14 @@ int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
   const int dir0 = fwd ? 0 : 1;
   const int N = p->N;
   RecArgs a = rec_args(p);
-  hipLaunchKernelGGL(k_rec_init, dim3((N + 256) / 256, ndir), dim3(256), 0, st, a, dir0);
+  const int nprep = std::max(N + 1, a.nblk * RB * RB);
+  hipLaunchKernelGGL(k_rec_prep, dim3((nprep + 255) / 256, ndir), dim3(256), 0, st, a, dir0);
   XMD_CHECK(hipGetLastError());
-  for (int p0 = 1; p0 <= N; p0 += RB) {
-    const int nrows = min(RB, N - p0 + 1);
-    hipLaunchKernelGGL(k_rec_diag, dim3(ndir), dim3(256), 0, st, a, dir0, p0, nrows);
+  for (int b = 0; b < a.nblk; ++b) {
+    const int later = b > 0 ? std::max(0, N - (1 + (b + 1) * RB) + 1) : 0;  // rows after block b
+    const int grid = ndir + ndir * ((later + RB - 1) / RB);
+    hipLaunchKernelGGL(k_rec_block, dim3(grid), dim3(256), 0, st, a, dir0, ndir, b);
     XMD_CHECK(hipGetLastError());
-    const int left = N - (p0 + nrows) + 1;
-    if (left > 0) {
-      hipLaunchKernelGGL(k_rec_update, dim3((left + 3) / 4, ndir), dim3(256), 0, st, a, dir0, p0, nrows);
-      XMD_CHECK(hipGetLastError());
-    }
   }
   if (fwd) p->vbn_dev = true;
   if (bwd) {

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <vector>
 
+#include "ewald_table.h"
 #include "xmd_device.h"
 #include "xmd_internal.h"
 
@@ -350,221 +351,323 @@ __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restr
 }
 
 // ------------------------------------------------------ reciprocal space
+//
+// k set: columns (mx, my) of the half space (mx > 0, or mx == 0 and my >= 0)
+// times m_z in [-MZ, MZ]; the other half follows from S(-k) = S(k)*.  The
+// coefficient coef = 4 pi A_k, A_k = q exp(-pi^2 G^2/alpha^2) / (pi V G^2),
+// G_d = m_d / L_d, is zero on the slots outside the set in use:
+//  * the reference's cube |m_d| <= n (md_simulation_x.c:266-281), or
+//  * the cylinder of a converged split: (mx, my) in the disc |G_xy| <= G_max,
+//    every m_z of the table (all terms beyond G_max are below exp(-x_cut^2)).
+// Column (0, 0) keeps m_z > 0 only.  Every column has ZR = 9 ceil((2MZ+1)/9)
+// slots (m_z = -MZ + u, u < ZR; slots past MZ are zero), so the kernels run
+// fixed-length loops without per-slot branches.
+//
+// Phase tables (k_phase), table half-width MZ:
+//   tx[m * nb + s]       e^{i 2 pi m x / L_x},  m = 0..MZ   (row-major)
+//   ty[(m + MZ) * nb + s] e^{i 2 pi m y / L_y}, m = -MZ..MZ (row-major)
+//   tz[s * ZR + m + MZ]   e^{i 2 pi m z / L_z}, m = -MZ..MZ (bead-major, zero pad)
+constexpr int SK_SEG = 9;      // m_z values per structure-factor segment
+constexpr int MAX_KM = 26;     // largest table half-width of the k kernels
 
-// k-vector bookkeeping: columns (mx, my) of the half space, mx in [0, n],
-// my in [-n, n], keeping mx > 0 or (mx == 0, my >= 0); each column carries
-// the 2n+1 values mz in [-n, n] (column (0,0) only uses mz > 0; its other
-// slots have zero weight).
+__host__ __device__ constexpr int kz_row(int MZ) { return SK_SEG * ((2 * MZ + 1 + SK_SEG - 1) / SK_SEG); }
+
 struct KSpace {
-  int n = -1, ncol = 0, nk = 0;
-  double L[3] = {0, 0, 0}, alpha = 0, q = 0;
-  double *coef = nullptr;  // [ncol*(2n+1)]: 4 pi A_k (force) ; energy uses A_k = coef / (4 pi)
-  int *cols = nullptr;     // [2*ncol]: mx, my
-  double *S = nullptr;     // [P][nk] complex (re, im)
-  double *Spart = nullptr; // [P][nchunk][nk] complex
-  size_t S_cap = 0, Spart_cap = 0;
+  double L[3] = {0, 0, 0}, alpha = 0, q = 0, gmax2 = 0;
+  int n = -2;  // cube half-width, or -1 for the cylinder of radius^2 gmax2
+  int MZ = 0, ZR = 0, ncol = 0, nk = 0;  // nk: k vectors with nonzero weight
+  int2 *cols = nullptr;    // (mx, my)
+  double *coef = nullptr;  // [ncol][ZR]
+  double2 *tx = nullptr, *ty = nullptr, *tz = nullptr, *Spart = nullptr, *S = nullptr, *W = nullptr;
+  size_t tx_cap = 0, ty_cap = 0, tz_cap = 0, part_cap = 0, s_cap = 0, w_cap = 0;
+  int stamp = 0;
 };
 
-static KSpace g_ks;
+static KSpace g_ks[2];
+static int g_ks_clock = 0;
 
-static int ensure_kspace(const double *L, double alpha, double q, int n, int P, int nchunk) {
-  KSpace &K = g_ks;
-  bool same = K.n == n && K.alpha == alpha && K.q == q && K.L[0] == L[0] && K.L[1] == L[1] && K.L[2] == L[2];
-  if (!same) {
-    std::vector<int> cols;
-    for (int mx = 0; mx <= n; ++mx)
-      for (int my = -n; my <= n; ++my)
-        if (mx > 0 || my >= 0) {
-          cols.push_back(mx);
-          cols.push_back(my);
-        }
-    int ncol = (int)cols.size() / 2, nz = 2 * n + 1;
-    std::vector<double> coef((size_t)ncol * nz, 0.0);
-    double V = L[0] * L[1] * L[2];
-    for (int c = 0; c < ncol; ++c)
-      for (int t = 0; t < nz; ++t) {
-        int mx = cols[2 * c], my = cols[2 * c + 1], mz = t - n;
-        if (mx == 0 && my == 0 && mz <= 0) continue;
-        double gx = mx / L[0], gy = my / L[1], gz = mz / L[2];
-        double G2 = gx * gx + gy * gy + gz * gz;
-        double A = q * exp(-M_PI * M_PI * G2 / (alpha * alpha)) / (G2 * V * M_PI);
-        coef[(size_t)c * nz + t] = 4 * M_PI * A;
-      }
-    if (K.coef) {
-      (void)hipFree(K.coef);
-      (void)hipFree(K.cols);
-    }
-    XMD_CHECK(hipMalloc(&K.coef, coef.size() * sizeof(double)));
-    XMD_CHECK(hipMalloc(&K.cols, cols.size() * sizeof(int)));
-    XMD_CHECK(hipMemcpy(K.coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
-    XMD_CHECK(hipMemcpy(K.cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice));
-    K.n = n;
-    K.ncol = ncol;
-    K.nk = ncol * nz;
-    K.alpha = alpha;
-    K.q = q;
-    for (int d = 0; d < 3; ++d) K.L[d] = L[d];
-  }
-  size_t need_S = (size_t)P * K.nk * 2, need_part = (size_t)P * nchunk * K.nk * 2;
-  if (need_S > K.S_cap) {
-    if (K.S) (void)hipFree(K.S);
-    XMD_CHECK(hipMalloc(&K.S, need_S * sizeof(double)));
-    K.S_cap = need_S;
-  }
-  if (need_part > K.Spart_cap) {
-    if (K.Spart) (void)hipFree(K.Spart);
-    XMD_CHECK(hipMalloc(&K.Spart, need_part * sizeof(double)));
-    K.Spart_cap = need_part;
-  }
+static int grow(double2 *&p, size_t &cap, size_t need) {
+  if (need <= cap) return 0;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  XMD_CHECK(hipMalloc(&p, need * sizeof(double2)));
+  cap = need;
   return 0;
 }
 
-constexpr int SK_CHUNK = 64;
+// Table half-widths whose 2 MZ + 1 values fill whole 9-slot segments.
+static int table_mz(int M) {
+  for (int mz : {4, 8, 13, 17, 22, 26})
+    if (M <= mz) return mz;
+  return -1;
+}
 
-// Work item = (column, half of the mz range); tables of exp(i m theta_d) for
-// the chunk's beads live in LDS: ex[b][0..n], ey[b][-n..n], ez[b][-n..n].
-__global__ __launch_bounds__(320) void k_sk_partial(const double *__restrict__ X, int N, int nb, int n,
-                                                    const int *__restrict__ cols, int ncol, double L0, double L1,
-                                                    double L2, double *__restrict__ Spart, int nchunk) {
-  extern __shared__ double tab[];
-  const int nz = 2 * n + 1, rowx = n + 1;
-  const int per_bead = 2 * (rowx + 2 * nz);
-  const int slice = blockIdx.y, chunk = blockIdx.x;
-  const int b0 = chunk * SK_CHUNK;
-  const int nbeads = min(SK_CHUNK, N - b0);
-  // tables
-  for (int w = threadIdx.x; w < nbeads * 3; w += blockDim.x) {
-    int b = w / 3, d = w % 3;
-    double L = d == 0 ? L0 : (d == 1 ? L1 : L2);
-    double x = X[d * nb + slice * N + b0 + b];
-    double th = 2 * M_PI * x / L;
-    double *t = tab + b * per_bead + (d == 0 ? 0 : (d == 1 ? 2 * rowx : 2 * (rowx + nz)));
-    int lo = d == 0 ? 0 : -n;
-    for (int m = lo; m <= n; ++m) {
-      double sn, cs;
-      sincos(m * th, &sn, &cs);
-      t[2 * (m - lo)] = cs;
-      t[2 * (m - lo) + 1] = sn;
+static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, double gmax2, int nb, int P, int nch) {
+  KSpace *K = nullptr;
+  for (auto &k : g_ks)
+    if (k.n == n && k.alpha == alpha && k.q == q && k.gmax2 == gmax2 && k.L[0] == L[0] && k.L[1] == L[1] &&
+        k.L[2] == L[2])
+      K = &k;
+  if (!K) {
+    K = g_ks[0].stamp <= g_ks[1].stamp ? &g_ks[0] : &g_ks[1];
+    int M = n;
+    if (n < 0) {
+      M = 0;
+      for (int d = 0; d < 3; ++d) M = std::max(M, (int)floor(sqrt(gmax2) * L[d]));
     }
-  }
-  __syncthreads();
-  const int nk = ncol * nz;
-  for (int item = threadIdx.x; item < 2 * ncol; item += blockDim.x) {
-    const int col = item >> 1, half = item & 1;
-    const int mx = cols[2 * col], my = cols[2 * col + 1];
-    const int zlo = half ? 1 : -n, zhi = half ? n : 0;
-    double acc_re[17], acc_im[17];
-#pragma unroll
-    for (int t = 0; t < 17; ++t) acc_re[t] = acc_im[t] = 0;
-    for (int b = 0; b < nbeads; ++b) {
-      const double *t = tab + b * per_bead;
-      double axr = t[2 * mx], axi = t[2 * mx + 1];
-      double ayr = t[2 * rowx + 2 * (my + n)], ayi = t[2 * rowx + 2 * (my + n) + 1];
-      double er = axr * ayr - axi * ayi, ei = axr * ayi + axi * ayr;
-      const double *tz = t + 2 * (rowx + nz);
-#pragma unroll
-      for (int u = 0; u < 17; ++u) {
-        int mz = zlo + u;
-        if (mz > zhi) break;
-        double zr = tz[2 * (mz + n)], zi = tz[2 * (mz + n) + 1];
-        acc_re[u] += er * zr - ei * zi;
-        acc_im[u] += er * zi + ei * zr;
+    const int MZ = table_mz(M);
+    if (MZ < 0) {
+      fprintf(stderr, "xmd: Ewald k range %d exceeds the supported %d\n", M, MAX_KM);
+      return nullptr;
+    }
+    const int ZR = kz_row(MZ);
+    std::vector<int2> cols;
+    std::vector<double> coef;
+    const double V = L[0] * L[1] * L[2];
+    int nk = 0;
+    for (int mx = 0; mx <= M; ++mx)
+      for (int my = -M; my <= M; ++my) {
+        if (mx == 0 && my < 0) continue;
+        const double gxy2 = (mx / L[0]) * (mx / L[0]) + (my / L[1]) * (my / L[1]);
+        if (n < 0 && gxy2 > gmax2) continue;
+        cols.push_back(make_int2(mx, my));
+        for (int u = 0; u < ZR; ++u) {
+          const int mz = u - MZ;
+          bool in = mz <= MZ && (n < 0 || (mz >= -n && mz <= n));
+          if (mx == 0 && my == 0 && mz <= 0) in = false;
+          double c = 0;
+          if (in) {
+            const double G2 = gxy2 + (mz / L[2]) * (mz / L[2]);
+            c = 4 * M_PI * q * exp(-M_PI * M_PI * G2 / (alpha * alpha)) / (G2 * V * M_PI);
+            ++nk;
+          }
+          coef.push_back(c);
+        }
       }
+    if (K->cols) {
+      (void)hipFree(K->cols);
+      (void)hipFree(K->coef);
     }
-    double *out = Spart + ((size_t)(slice * nchunk + chunk) * nk) * 2;
+    XMD_CHECK_NULL(hipMalloc(&K->cols, cols.size() * sizeof(int2)));
+    XMD_CHECK_NULL(hipMalloc(&K->coef, coef.size() * sizeof(double)));
+    XMD_CHECK_NULL(hipMemcpy(K->cols, cols.data(), cols.size() * sizeof(int2), hipMemcpyHostToDevice));
+    XMD_CHECK_NULL(hipMemcpy(K->coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+    K->n = n;
+    K->gmax2 = gmax2;
+    K->alpha = alpha;
+    K->q = q;
+    for (int d = 0; d < 3; ++d) K->L[d] = L[d];
+    K->MZ = MZ;
+    K->ZR = ZR;
+    K->ncol = (int)cols.size();
+    K->nk = nk;
+  }
+  K->stamp = ++g_ks_clock;
+  const size_t slots = (size_t)K->ncol * K->ZR;
+  if (grow(K->tx, K->tx_cap, (size_t)(K->MZ + 1) * nb) || grow(K->ty, K->ty_cap, (size_t)(2 * K->MZ + 1) * nb) ||
+      grow(K->tz, K->tz_cap, (size_t)K->ZR * nb) || grow(K->Spart, K->part_cap, (size_t)P * nch * slots) ||
+      grow(K->S, K->s_cap, (size_t)P * slots) || grow(K->W, K->w_cap, (size_t)P * slots))
+    return nullptr;
+  return K;
+}
+
+// One thread per (dimension, bead): e^{i pi m u}, u = 2 x / L; negative m
+// are the conjugates.
+__global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZR, double L0, double L1, double L2,
+                        double2 *__restrict__ tx, double2 *__restrict__ ty, double2 *__restrict__ tz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * nb) return;
+  const int d = t / nb, s = t - d * nb;
+  const double u = 2.0 * X[t] / (d == 0 ? L0 : (d == 1 ? L1 : L2));
+  if (d == 0) {
+    tx[s] = make_double2(1.0, 0.0);
+    for (int m = 1; m <= MZ; ++m) {
+      double sn, cs;
+      sincospi(m * u, &sn, &cs);
+      tx[(size_t)m * nb + s] = make_double2(cs, sn);
+    }
+  } else if (d == 1) {
+    double2 *r0 = ty + (size_t)MZ * nb + s;
+    r0[0] = make_double2(1.0, 0.0);
+    for (int m = 1; m <= MZ; ++m) {
+      double sn, cs;
+      sincospi(m * u, &sn, &cs);
+      r0[(ptrdiff_t)m * nb] = make_double2(cs, sn);
+      r0[-(ptrdiff_t)m * nb] = make_double2(cs, -sn);
+    }
+  } else {
+    double2 *r = tz + (size_t)s * ZR + MZ;
+    r[0] = make_double2(1.0, 0.0);
+    for (int m = 1; m <= MZ; ++m) {
+      double sn, cs;
+      sincospi(m * u, &sn, &cs);
+      r[m] = make_double2(cs, sn);
+      r[-m] = make_double2(cs, -sn);
+    }
+    for (int k = 2 * MZ + 1; k < ZR; ++k) r[k - MZ] = make_double2(0.0, 0.0);
+  }
+}
+
+// Structure-factor partials.  A wave covers 64 columns (lanes) and one
+// segment of SK_SEG m_z values; the bead's e^{i m_z theta_z} for the
+// segment are wave-uniform scalar loads, the column's e^{i (m_x theta_x +
+// m_y theta_y)} is one complex product per bead.  Grid: (column group x
+// segment, bead chunk, slice).
+__global__ __launch_bounds__(64) void k_sk(const double2 *__restrict__ tx, const double2 *__restrict__ ty,
+                                           const double2 *__restrict__ tz, const int2 *__restrict__ cols, int ncol,
+                                           int MZ, int ZR, int N, int nb, int nch, int chunk_len,
+                                           double2 *__restrict__ Spart) {
+  const int lane = threadIdx.x;
+  const int nseg = ZR / SK_SEG;
+  const int cg = blockIdx.x / nseg, seg = blockIdx.x - cg * nseg;
+  const int c = cg * 64 + lane;
+  const int chunk = blockIdx.y, slice = blockIdx.z;
+  const bool act = c < ncol;
+  const int2 cl = act ? cols[c] : make_int2(0, 0);
+  const double2 *px = tx + (size_t)cl.x * nb;
+  const double2 *py = ty + (size_t)(MZ + cl.y) * nb;
+  double ar[SK_SEG], ai[SK_SEG];
 #pragma unroll
-    for (int u = 0; u < 17; ++u) {
-      int mz = zlo + u;
-      if (mz > zhi) break;
-      int kk = col * nz + (mz + n);
-      out[2 * kk] = acc_re[u];
-      out[2 * kk + 1] = acc_im[u];
+  for (int u = 0; u < SK_SEG; ++u) ar[u] = ai[u] = 0.0;
+  const int s0 = slice * N + chunk * chunk_len;
+  const int s1 = slice * N + min(N, (chunk + 1) * chunk_len);
+  const double2 *pz = tz + seg * SK_SEG;
+  int s = s0;
+  for (; s + 1 < s1; s += 2) {
+    const double2 ex0 = px[s], ey0 = py[s], ex1 = px[s + 1], ey1 = py[s + 1];
+    const double er0 = ex0.x * ey0.x - ex0.y * ey0.y, ei0 = ex0.x * ey0.y + ex0.y * ey0.x;
+    const double er1 = ex1.x * ey1.x - ex1.y * ey1.y, ei1 = ex1.x * ey1.y + ex1.y * ey1.x;
+    const double2 *z0 = pz + (size_t)s * ZR, *z1 = z0 + ZR;
+#pragma unroll
+    for (int u = 0; u < SK_SEG; ++u) {
+      const double2 a = z0[u], b = z1[u];
+      ar[u] = fma(er0, a.x, fma(-ei0, a.y, fma(er1, b.x, fma(-ei1, b.y, ar[u]))));
+      ai[u] = fma(er0, a.y, fma(ei0, a.x, fma(er1, b.y, fma(ei1, b.x, ai[u]))));
     }
   }
-}
-
-__global__ void k_sk_reduce(const double *__restrict__ Spart, double *__restrict__ S, int nk, int nchunk, int P) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nk * P) return;
-  int slice = t / nk, kk = t % nk;
-  double re = 0, im = 0;
-  for (int c = 0; c < nchunk; ++c) {
-    const double *p = Spart + ((size_t)(slice * nchunk + c) * nk + kk) * 2;
-    re += p[0];
-    im += p[1];
+  if (s < s1) {
+    const double2 ex0 = px[s], ey0 = py[s];
+    const double er0 = ex0.x * ey0.x - ex0.y * ey0.y, ei0 = ex0.x * ey0.y + ex0.y * ey0.x;
+    const double2 *z0 = pz + (size_t)s * ZR;
+#pragma unroll
+    for (int u = 0; u < SK_SEG; ++u) {
+      const double2 a = z0[u];
+      ar[u] = fma(er0, a.x, fma(-ei0, a.y, ar[u]));
+      ai[u] = fma(er0, a.y, fma(ei0, a.x, ai[u]));
+    }
   }
-  S[2 * (size_t)t] = re;
-  S[2 * (size_t)t + 1] = im;
+  if (!act) return;
+  double2 *out = Spart + ((size_t)(slice * nch + chunk) * ncol + c) * ZR + seg * SK_SEG;
+#pragma unroll
+  for (int u = 0; u < SK_SEG; ++u) out[u] = make_double2(ar[u], ai[u]);
 }
 
-// Thread = bead; the four waves of a block share 64 beads and split the
-// columns.  exp(i k.x) advances along mz by repeated multiplication.
-__global__ __launch_bounds__(256) void k_recip_force(const double *__restrict__ X, const double *__restrict__ M,
-                                                     double *__restrict__ F, int N, int nb, int n,
-                                                     const int *__restrict__ cols, int ncol,
-                                                     const double *__restrict__ coef, const double *__restrict__ S,
-                                                     double L0, double L1, double L2, double P) {
-  __shared__ double red[3][4][64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+// S = sum of the chunk partials (fixed order); W = coef conj(S).
+__global__ void k_sk_fin(const double2 *__restrict__ Spart, const double *__restrict__ coef, int slots, int nch,
+                         int P, double2 *__restrict__ S, double2 *__restrict__ W) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= slots * P) return;
+  const int slice = t / slots, k = t - slice * slots;
+  double re = 0, im = 0;
+  for (int c = 0; c < nch; ++c) {
+    const double2 v = Spart[(size_t)(slice * nch + c) * slots + k];
+    re += v.x;
+    im += v.y;
+  }
+  S[t] = make_double2(re, im);
+  W[t] = make_double2(coef[k] * re, -coef[k] * im);
+}
+
+// Forces: lane = bead.  A workgroup holds 64 beads; each of its waves owns
+// one 9-slot m_z segment (that segment of the bead's e^{i m_z theta_z} row
+// in registers) and a share of the columns, whose weights W are
+// wave-uniform scalar loads.  For bead i:
+//   f_i = sum_k coef_k g_k Im(e^{i k.x_i} S_k*) / (m_i P),
+// the reference's per-pair sum (md_simulation_x.c:266-281) since
+// sum_{l != i} sin(k.(x_i - x_l)) = Im(e^{i k.x_i} S_k*).  The waves'
+// partial forces are summed in a fixed order.
+__host__ __device__ constexpr int rf_parts(int nseg) { return nseg >= 8 ? 1 : 8 / nseg; }
+
+template <int NSEG>
+__global__ __launch_bounds__(512) void k_recip_force(const double2 *__restrict__ tx, const double2 *__restrict__ ty,
+                                                     const double2 *__restrict__ tz, const int2 *__restrict__ cols,
+                                                     int ncol, int MZ, const double2 *__restrict__ W,
+                                                     const double *__restrict__ Mass, double *__restrict__ F, int N,
+                                                     int nb, double L0, double L1, double L2, double P) {
+  constexpr int ZR = NSEG * SK_SEG, CP = rf_parts(NSEG), NW = NSEG * CP;
+  __shared__ double red[3][NW][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: W via scalar loads
+  const int seg = wv % NSEG, part = wv / NSEG;
   const int slice = blockIdx.y;
   const int i = blockIdx.x * 64 + lane;
   const bool act = i < N;
-  const int nz = 2 * n + 1, nk = ncol * nz;
   const int s = slice * N + (act ? i : 0);
-  const double tx = 2 * M_PI * X[s] / L0, ty = 2 * M_PI * X[nb + s] / L1, tz = 2 * M_PI * X[2 * nb + s] / L2;
-  double z1r, z1i, znr, zni;
-  sincos(tz, &z1i, &z1r);
-  sincos(-n * tz, &zni, &znr);
-  const double *Ss = S + (size_t)slice * nk * 2;
-  const int c0 = (ncol * wv) / 4, c1 = (ncol * (wv + 1)) / 4;
+  double zr[SK_SEG], zi[SK_SEG];
+  const double2 *pz = tz + (size_t)s * ZR + seg * SK_SEG;
+#pragma unroll
+  for (int u = 0; u < SK_SEG; ++u) {
+    const double2 z = pz[u];
+    zr[u] = z.x;
+    zi[u] = z.y;
+  }
+  const double mz0 = (double)(seg * SK_SEG - MZ);  // m_z of slot u is mz0 + u
+  const double2 *Ws = W + (size_t)slice * ncol * ZR + seg * SK_SEG;
+  const int c0 = (ncol * part) / CP, c1 = (ncol * (part + 1)) / CP;
   double fx = 0, fy = 0, fz = 0;
   for (int c = c0; c < c1; ++c) {
-    const int mx = cols[2 * c], my = cols[2 * c + 1];
-    double axr, axi, ayr, ayi;
-    sincos(mx * tx, &axi, &axr);
-    sincos(my * ty, &ayi, &ayr);
-    double xyr = axr * ayr - axi * ayi, xyi = axr * ayi + axi * ayr;
-    double er = xyr * znr - xyi * zni, ei = xyr * zni + xyi * znr;
-    const double *Sc = Ss + (size_t)c * nz * 2;
-    const double *Kc = coef + (size_t)c * nz;
-    double acc_x = 0, acc_z = 0;
-    for (int t = 0; t < nz; ++t) {
-      const double sr = Sc[2 * t], si = Sc[2 * t + 1];
-      const double w = Kc[t] * (ei * sr - er * si);
-      acc_x += w;
-      acc_z += w * (t - n);
-      double nr = er * z1r - ei * z1i;
-      ei = er * z1i + ei * z1r;
-      er = nr;
+    const int2 cl = cols[c];
+    const double2 *Wc = Ws + (size_t)c * ZR;
+    const double2 ex = tx[(size_t)cl.x * nb + s], ey = ty[(size_t)(MZ + cl.y) * nb + s];
+    const double ar = ex.x * ey.x - ex.y * ey.y, ai = ex.x * ey.y + ex.y * ey.x;
+    double ur = 0, ui = 0, vr = 0, vi = 0;
+#pragma unroll
+    for (int u = 0; u < SK_SEG; ++u) {
+      const double2 w = Wc[u];
+      const double tr = fma(w.x, zr[u], -w.y * zi[u]);
+      const double ti = fma(w.x, zi[u], w.y * zr[u]);
+      ur += tr;
+      ui += ti;
+      if (u > 0) {
+        vr = fma((double)u, tr, vr);
+        vi = fma((double)u, ti, vi);
+      }
     }
-    fx += acc_x * mx;
-    fy += acc_x * my;
-    fz += acc_z;
+    vr = fma(mz0, ur, vr);
+    vi = fma(mz0, ui, vi);
+    const double g = fma(ar, ui, ai * ur);
+    fx = fma((double)cl.x, g, fx);
+    fy = fma((double)cl.y, g, fy);
+    fz = fma(ar, vi, fma(ai, vr, fz));
   }
   red[0][wv][lane] = fx;
   red[1][wv][lane] = fy;
   red[2][wv][lane] = fz;
   __syncthreads();
   if (wv == 0 && act) {
-    double gx = red[0][0][lane] + red[0][1][lane] + red[0][2][lane] + red[0][3][lane];
-    double gy = red[1][0][lane] + red[1][1][lane] + red[1][2][lane] + red[1][3][lane];
-    double gz = red[2][0][lane] + red[2][1][lane] + red[2][2][lane] + red[2][3][lane];
-    const double m = M[s];
+    double gx = 0, gy = 0, gz = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      gx += red[0][w][lane];
+      gy += red[1][w][lane];
+      gz += red[2][w][lane];
+    }
+    const double m = Mass[s];
     F[s] += gx / L0 / m / P;
     F[nb + s] += gy / L1 / m / P;
     F[2 * nb + s] += gz / L2 / m / P;
   }
 }
 
-// Reciprocal energy of every slice: sum_k A_k (|S_k|^2 - N), into partials.
-__global__ __launch_bounds__(256) void k_recip_energy(const double *__restrict__ S, const double *__restrict__ coef,
-                                                      int nk, int P, int N, double *__restrict__ part) {
+// Reciprocal energy of every slice: sum_{k in half} A_k (|S_k|^2 - N), into partials.
+__global__ __launch_bounds__(256) void k_recip_energy(const double2 *__restrict__ S, const double *__restrict__ coef,
+                                                      int slots, int P, int N, double *__restrict__ part) {
   __shared__ double red[256];
   double s = 0;
-  for (int t = blockIdx.x * 256 + threadIdx.x; t < nk * P; t += gridDim.x * 256) {
-    int kk = t % nk;
-    double re = S[2 * (size_t)t], im = S[2 * (size_t)t + 1];
-    s += coef[kk] / (4 * M_PI) * (re * re + im * im - N);
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < slots * P; t += gridDim.x * 256) {
+    const int kk = t % slots;
+    const double2 v = S[t];
+    s += coef[kk] / (4 * M_PI) * (v.x * v.x + v.y * v.y - N);
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -573,6 +676,162 @@ __global__ __launch_bounds__(256) void k_recip_energy(const double *__restrict__
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// ------------------------------------------ table-driven Ewald real space
+//
+// The force of one image at distance r is q h(alpha r) r_vec / r^3 with
+// h(x) = erfc(x) + 2x/sqrt(pi) exp(-x^2) (md_simulation_x.c:288-289);
+// h comes from the piecewise polynomial table of ewald_table.h (abs error
+// 1.1e-16), staged in LDS once per workgroup.
+//
+// Images: with the minimum image a_d of the separation (|a_d| <= L_d/2),
+// b_d = a_d - sign(a_d) L_d and c_d = a_d + sign(a_d) L_d, every image within
+// r_c < sqrt(2) L takes per dimension a, b or c, and at most one dimension
+// takes c (two would put it beyond sqrt(2) L).  Each lane builds the bit
+// mask of its candidates inside r_c at bit kx | ky << 2 | kz << 4 with
+// k = 0 (a), 1 (b), 3 (c); the image is then a_d - sext(k_d) s_d.  CIMG =
+// false when r_c < min L_d (no c candidates).
+struct EwReal {
+  double L[3], iL[3];
+  double q, al8, rc2;  // al8 = 8 alpha: table intervals of width 1/8 in x = alpha r
+};
+
+template <bool CIMG>
+__device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, const EwReal &ep,
+                                               const double2 *__restrict__ tab, double &gx, double &gy,
+                                               double &gz) {
+  const double ax = fma(-ep.L[0], rint(dx * ep.iL[0]), dx);
+  const double ay = fma(-ep.L[1], rint(dy * ep.iL[1]), dy);
+  const double az = fma(-ep.L[2], rint(dz * ep.iL[2]), dz);
+  const double sx = copysign(ep.L[0], ax), sy = copysign(ep.L[1], ay), sz = copysign(ep.L[2], az);
+  const double bx = ax - sx, by = ay - sy, bz = az - sz;
+  const double ax2 = ax * ax, ay2 = ay * ay, az2 = az * az;
+  const double bx2 = bx * bx, by2 = by * by, bz2 = bz * bz;
+  const double rc2 = ep.rc2;
+  const double xy[4] = {ax2 + ay2, bx2 + ay2, ax2 + by2, bx2 + by2};
+  unsigned lo = 0, hi = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double R2 = xy[q & 3] + ((q & 4) ? bz2 : az2);
+    const int pos = (q & 1) | ((q & 2) << 1) | ((q & 4) << 2);
+    lo |= (R2 <= rc2) ? (1u << pos) : 0u;
+  }
+  if (CIMG) {
+    const double cx = ax + sx, cy = ay + sy, cz = az + sz;
+    const double cx2 = cx * cx, cy2 = cy * cy, cz2 = cz * cz;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k1 = u & 1, k2 = u >> 1;
+      // c in x: (c, k1, k2); c in y: (k1, c, k2); c in z: (k1, k2, c)
+      const double Rx = cx2 + (k1 ? by2 : ay2) + (k2 ? bz2 : az2);
+      const double Ry = (k1 ? bx2 : ax2) + cy2 + (k2 ? bz2 : az2);
+      const double Rz = xy[k1 | (k2 << 1)] + cz2;
+      lo |= (Rx <= rc2) ? (1u << (3 | (k1 << 2) | (k2 << 4))) : 0u;
+      lo |= (Ry <= rc2) ? (1u << (k1 | (3 << 2) | (k2 << 4))) : 0u;
+      hi |= (Rz <= rc2) ? (1u << ((k1 | (k2 << 2) | (3 << 4)) - 32)) : 0u;
+    }
+  }
+  unsigned long long m = ((unsigned long long)hi << 32) | lo;
+  while (m) {
+    const int pos = __builtin_ctzll(m);
+    m &= m - 1;
+    const double kx = (double)((pos << 30) >> 30), ky = (double)((pos << 28) >> 30), kz = (double)((pos << 26) >> 30);
+    const double rx = fma(-kx, sx, ax), ry = fma(-ky, sy, ay), rz = fma(-kz, sz, az);
+    const double R2 = fma(rx, rx, fma(ry, ry, rz * rz));
+    const double ir = rsqrt(R2);
+    const double t8 = (R2 * ir) * ep.al8;
+    const int iv = min((int)t8, kEwTabIntervals - 1);
+    const double y = fma(2.0, t8, -(double)(2 * iv + 1));
+    const double2 *c = tab + iv * (kEwTabStride / 2);
+    const double2 c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], c4 = c[4];
+    double h = fma(c4.y, y, c4.x);
+    h = fma(h, y, c3.y);
+    h = fma(h, y, c3.x);
+    h = fma(h, y, c2.y);
+    h = fma(h, y, c2.x);
+    h = fma(h, y, c1.y);
+    h = fma(h, y, c1.x);
+    h = fma(h, y, c0.y);
+    h = fma(h, y, c0.x);
+    const double w = ep.q * h * (ir * ir * ir);
+    gx = fma(w, rx, gx);
+    gy = fma(w, ry, gy);
+    gz = fma(w, rz, gz);
+  }
+}
+
+// Four tile pairs per workgroup (one per wave) sharing the LDS table; the
+// per-wave schedule is that of k_pair_tiles.
+template <bool CIMG>
+__global__ __launch_bounds__(256) void k_ewald_real(const double *__restrict__ X, const double *__restrict__ M,
+                                                    double *__restrict__ Fpart, const int2 *__restrict__ tiles, int N,
+                                                    int nb, int T, int ntp, int nitems, EwReal ep) {
+  __shared__ double2 tab[kEwTabIntervals * kEwTabStride / 2];
+  __shared__ double jx[4][64], jy[4][64], jz[4][64], jw[4][64], rx[4][64], ry[4][64], rz[4][64];
+  const int tid = threadIdx.x, t = tid & 63, w = tid >> 6;
+  for (int k = tid; k < kEwTabIntervals * kEwTabStride / 2; k += 256)
+    tab[k] = make_double2(kEwTab[2 * k], kEwTab[2 * k + 1]);
+  const int item = blockIdx.x * 4 + w;
+  const bool valid = item < nitems;
+  int slice = 0, I = 0, J = 0;
+  if (valid) {
+    slice = item / ntp;
+    const int2 IJ = tiles[item - slice * ntp];
+    I = IJ.x;
+    J = IJ.y;
+  }
+  const int base = slice * N;
+  const int i = I * 64 + t, jl = J * 64 + t;
+  const bool act = valid && i < N, jact = valid && jl < N;
+  double xi = 0, yi = 0, zi = 0, wi = 1;
+  if (act) {
+    xi = X[base + i];
+    yi = X[nb + base + i];
+    zi = X[2 * nb + base + i];
+    wi = 1.0 / M[base + i];
+  }
+  jx[w][t] = jact ? X[base + jl] : 0.0;
+  jy[w][t] = jact ? X[nb + base + jl] : 0.0;
+  jz[w][t] = jact ? X[2 * nb + base + jl] : 0.0;
+  jw[w][t] = jact ? 1.0 / M[base + jl] : 1.0;
+  rx[w][t] = 0;
+  ry[w][t] = 0;
+  rz[w][t] = 0;
+  __syncthreads();
+  const bool diag = I == J;
+  double fx = 0, fy = 0, fz = 0;
+  if (act) {
+    const int s_lo = diag ? 1 : 0, s_hi = diag ? 32 : 63;
+    for (int s = s_lo; s <= s_hi; ++s) {
+      const int j = (t + s) & 63;
+      const int jg = J * 64 + j;
+      if (jg >= N || (diag && s == 32 && t >= 32)) continue;
+      double gx = 0, gy = 0, gz = 0;
+      ewald_pair_tab<CIMG>(xi - jx[w][j], yi - jy[w][j], zi - jz[w][j], ep, tab, gx, gy, gz);
+      const double iw = jg > i ? jw[w][j] : wi;
+      fx = fma(gx, iw, fx);
+      fy = fma(gy, iw, fy);
+      fz = fma(gz, iw, fz);
+      rx[w][j] -= gx * iw;
+      ry[w][j] -= gy * iw;
+      rz[w][j] -= gz * iw;
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  auto put = [&](int bead, int slot, double vx, double vy, double vz) {
+    const size_t s0 = (size_t)(base + bead) * T + slot;
+    Fpart[s0] = vx;
+    Fpart[(size_t)nb * T + s0] = vy;
+    Fpart[2 * (size_t)nb * T + s0] = vz;
+  };
+  if (diag) {
+    if (act) put(i, I, fx + rx[w][t], fy + ry[w][t], fz + rz[w][t]);
+  } else {
+    if (act) put(i, J, fx, fy, fz);
+    if (jact) put(jl, I, rx[w][t], ry[w][t], rz[w][t]);
+  }
 }
 
 // ------------------------------------------------------------ launchers
@@ -678,29 +937,173 @@ static int launch_pair_mode(DevPimd *p, const PairParams &pp, hipStream_t st, do
   return 0;
 }
 
-static int launch_recip(DevPimd *p, const PairParams &pp, hipStream_t st, bool force) {
+// ------------------------------------------------------------ Ewald plan
+//
+// The Ewald sum does not depend on the split alpha once both of its series
+// are converged.  When the reference's own parameters are converged -- its
+// first omitted k shell has exp(-pi^2 (n+1)^2 / (alpha L_max)^2) <= 1e-13
+// and its real-space image cube reaches erfc(alpha n L_min) < 1e-16 -- the
+// engine evaluates the same sum with a split of its own: alpha', real space
+// within x_cut / alpha' and k in the ball pi |G| / alpha' <= x_cut (dropped
+// terms below exp(-x_cut^2) relative).  The result then differs from the
+// reference's only by the reference's own truncation (<= 1e-13 relative) and
+// rounding.  g_ewald_split_aL: 0 keeps the reference's alpha and k cube,
+// > 0 fixes alpha' L_min, < 0 lets the cost model below choose.
+double g_ewald_split_aL = -1.0;
+
+struct EwaldPlan {
+  double alpha = 0, rc2 = 0, gmax2 = 0;
+  int cube_n = 0;  // >= 0: the reference's k cube; -1: the ball |G|^2 <= gmax2
+  bool own_split = false;
+};
+
+static double lmin3(const double *L) { return fmin(L[0], fmin(L[1], L[2])); }
+static double lmax3(const double *L) { return fmax(L[0], fmax(L[1], L[2])); }
+
+static bool ref_converged(const PairParams &pp) {
+  const int n = pp.nsum;
+  if (n < 1 || !(pp.alpha > 0)) return false;
+  const double a = M_PI * (n + 1) / (pp.alpha * lmax3(pp.L));
+  return a * a >= 30.0 && pp.alpha * n * lmin3(pp.L) >= 6.0;
+}
+
+// Modelled device time of a split alpha L_min = aL, per slice, from the
+// measured costs of the kernels on MI355X: real space (6 + 2 with third
+// copies) ps per pair plus 2.6 ps per image inside r_c; reciprocal space
+// 1.1 ps per (bead, padded k slot).
+static double split_cost(const PairParams &pp, int N, double aL) {
+  const double Lmin = lmin3(pp.L), V = pp.L[0] * pp.L[1] * pp.L[2];
+  const double alpha = aL / Lmin, rc = g_ewald_xcut / alpha;
+  const double images = 4.0 / 3.0 * M_PI * rc * rc * rc / V;
+  const double per_pair = (rc >= Lmin ? 8.0 : 6.0) + 2.6 * images;
+  const double gmax = g_ewald_xcut * alpha / M_PI;
+  int M = 0;
+  for (int d = 0; d < 3; ++d) M = std::max(M, (int)floor(gmax * pp.L[d]));
+  const int MZ = table_mz(M);
+  if (MZ < 0) return 1e300;
+  long ncol = 0;
+  for (int mx = 0; mx <= M; ++mx)
+    for (int my = -M; my <= M; ++my)
+      if ((mx > 0 || my >= 0) && (mx / pp.L[0]) * (mx / pp.L[0]) + (my / pp.L[1]) * (my / pp.L[1]) <= gmax * gmax)
+        ++ncol;
+  return 0.5 * N * (N - 1.0) * per_pair + 1.1 * N * (double)ncol * kz_row(MZ);
+}
+
+static bool split_ok(const PairParams &pp, double aL) {
+  const double Lmin = lmin3(pp.L), alpha = aL / Lmin, rc = g_ewald_xcut / alpha;
+  const double gmax = g_ewald_xcut * alpha / M_PI;
+  return rc * rc < 2.0 * Lmin * Lmin && g_ewald_xcut <= 6.5 && gmax * lmax3(pp.L) <= MAX_KM;
+}
+
+static EwaldPlan plan_ewald(const PairParams &pp, int N) {
+  EwaldPlan e;
+  e.alpha = pp.alpha;
+  e.rc2 = pp.rc2;
+  e.cube_n = pp.nsum;
+  if (g_ewald_split_aL == 0.0 || !ref_converged(pp)) return e;
+  double aL = g_ewald_split_aL;
+  if (aL < 0) {
+    double best = 0;
+    for (int h = 18; h <= 48; ++h) {  // alpha L_min = 4.5 .. 12 in steps of 1/4
+      const double c = 0.25 * h;
+      if (!split_ok(pp, c)) continue;
+      const double w = split_cost(pp, N, c);
+      if (aL < 0 || w < best) {
+        best = w;
+        aL = c;
+      }
+    }
+    if (aL < 0) return e;
+  } else if (!split_ok(pp, aL)) {
+    return e;
+  }
+  const double alpha = aL / lmin3(pp.L), rc = g_ewald_xcut / alpha, gmax = g_ewald_xcut * alpha / M_PI;
+  e.alpha = alpha;
+  e.rc2 = rc * rc;
+  e.gmax2 = gmax * gmax;
+  e.cube_n = -1;
+  e.own_split = true;
+  return e;
+}
+
+// Parameters of the full Ewald interaction under its plan.
+static PairParams ewald_params(DevPimd *p, EwaldPlan &pl) {
+  PairParams pp = make_params(p);
+  pl = plan_ewald(pp, p->N);
+  if (pl.own_split) {
+    pp.alpha = pl.alpha;
+    pp.rc2 = pl.rc2;
+    pp.nsum = 1 << 20;  // coverage comes from r_c alone
+  }
+  return pp;
+}
+
+static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t st) {
   DevSim *d = p->ds;
-  int n = pp.nsum;
-  if (n > 16) {
-    fprintf(stderr, "xmd: Ewald n_sum > 16 is not supported by the structure-factor kernels\n");
-    return -1;
+  if (ensure_tiles(p->N, p->P)) return -1;
+  TileBufs &B = g_tb;
+  EwReal ep;
+  for (int k = 0; k < 3; ++k) {
+    ep.L[k] = pp.L[k];
+    ep.iL[k] = 1.0 / pp.L[k];
   }
-  int nchunk = (p->N + SK_CHUNK - 1) / SK_CHUNK;
-  if (ensure_kspace(pp.L, pp.alpha, pp.q, n, p->P, nchunk)) return -1;
-  KSpace &K = g_ks;
-  int nz = 2 * n + 1;
-  size_t lds = (size_t)SK_CHUNK * 2 * ((n + 1) + 2 * nz) * sizeof(double);
-  hipLaunchKernelGGL(k_sk_partial, dim3(nchunk, p->P), dim3(320), lds, st, d->X, p->N, d->nb, n, K.cols, K.ncol,
-                     pp.L[0], pp.L[1], pp.L[2], K.Spart, nchunk);
+  ep.q = pp.q;
+  ep.al8 = kEwTabInvWidth * pp.alpha;
+  ep.rc2 = pp.rc2;
+  const int items = p->P * B.ntp;
+  const double Lmin = lmin3(pp.L);
+  if (pp.rc2 >= Lmin * Lmin)
+    hipLaunchKernelGGL(k_ewald_real<true>, dim3((items + 3) / 4), dim3(256), 0, st, d->X, d->M, B.Fpart, B.tiles,
+                       p->N, d->nb, B.T, B.ntp, items, ep);
+  else
+    hipLaunchKernelGGL(k_ewald_real<false>, dim3((items + 3) / 4), dim3(256), 0, st, d->X, d->M, B.Fpart, B.tiles,
+                       p->N, d->nb, B.T, B.ntp, items, ep);
   XMD_CHECK(hipGetLastError());
-  int tot = K.nk * p->P;
-  hipLaunchKernelGGL(k_sk_reduce, dim3((tot + 255) / 256), dim3(256), 0, st, K.Spart, K.S, K.nk, nchunk, p->P);
+  const int n3 = 3 * d->nb;
+  hipLaunchKernelGGL(k_reduce_tiles, dim3((n3 + 255) / 256), dim3(256), 0, st, B.Fpart, d->F, n3, B.T, pp.P);
   XMD_CHECK(hipGetLastError());
-  if (force) {
-    hipLaunchKernelGGL(k_recip_force, dim3((p->N + 63) / 64, p->P), dim3(256), 0, st, d->X, d->M, d->F, p->N, d->nb,
-                       n, K.cols, K.ncol, K.coef, K.S, pp.L[0], pp.L[1], pp.L[2], pp.P);
-    XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+static int launch_recip(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st, bool force,
+                        KSpace **out) {
+  DevSim *d = p->ds;
+  // bead chunks: enough waves to fill the device, chunks of >= 32 beads
+  KSpace *K = ensure_kspace(pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P, 1);
+  if (!K) return -1;
+  const int nseg = K->ZR / SK_SEG, groups = (K->ncol + 63) / 64;
+  int nch = (4096 + groups * nseg * p->P - 1) / (groups * nseg * p->P);
+  nch = std::max(1, std::min(nch, (p->N + 31) / 32));
+  const int chunk_len = (p->N + nch - 1) / nch;
+  nch = (p->N + chunk_len - 1) / chunk_len;
+  K = ensure_kspace(pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P, nch);
+  if (!K) return -1;
+  if (out) *out = K;
+  hipLaunchKernelGGL(k_phase, dim3((3 * d->nb + 255) / 256), dim3(256), 0, st, d->X, d->nb, K->MZ, K->ZR, pp.L[0],
+                     pp.L[1], pp.L[2], K->tx, K->ty, K->tz);
+  XMD_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_sk, dim3(groups * nseg, nch, p->P), dim3(64), 0, st, K->tx, K->ty, K->tz, K->cols, K->ncol,
+                     K->MZ, K->ZR, p->N, d->nb, nch, chunk_len, K->Spart);
+  XMD_CHECK(hipGetLastError());
+  const int slots = K->ncol * K->ZR, tot = slots * p->P;
+  hipLaunchKernelGGL(k_sk_fin, dim3((tot + 255) / 256), dim3(256), 0, st, K->Spart, K->coef, slots, nch, p->P, K->S,
+                     K->W);
+  XMD_CHECK(hipGetLastError());
+  if (!force) return 0;
+  const dim3 grid((p->N + 63) / 64, p->P);
+#define XMD_RECIP_FORCE(NS)                                                                                      \
+  hipLaunchKernelGGL(k_recip_force<NS>, grid, dim3(64 * NS * rf_parts(NS)), 0, st, K->tx, K->ty, K->tz, K->cols,  \
+                     K->ncol, K->MZ, K->W, d->M, d->F, p->N, d->nb, pp.L[0], pp.L[1], pp.L[2], pp.P)
+  switch (K->ZR / SK_SEG) {
+    case 1: XMD_RECIP_FORCE(1); break;
+    case 2: XMD_RECIP_FORCE(2); break;
+    case 3: XMD_RECIP_FORCE(3); break;
+    case 4: XMD_RECIP_FORCE(4); break;
+    case 5: XMD_RECIP_FORCE(5); break;
+    default: XMD_RECIP_FORCE(6); break;
   }
+#undef XMD_RECIP_FORCE
+  XMD_CHECK(hipGetLastError());
   return 0;
 }
 
@@ -718,7 +1121,12 @@ int launch_pair_real(DevPimd *p, int kind, hipStream_t st) {
         fprintf(stderr, "xmd: the Ewald kernels need equal bead masses\n");
         return -1;
       }
-      if (kind == PK_EWALD) return launch_pair_mode<PK_EWALD, false>(p, pp, st, nullptr);
+      if (kind == PK_EWALD) {
+        EwaldPlan pl;
+        pp = ewald_params(p, pl);
+        if (near_ok(pp)) return launch_ewald_real_tab(p, pp, st);
+        return launch_pair_mode<PK_EWALD, false>(p, pp, st, nullptr);
+      }
       if (kind == PK_EWALD_R) return launch_pair_mode<PK_EWALD_R, false>(p, pp, st, nullptr);
       return launch_pair_mode<PK_EWALD_NI, false>(p, pp, st, nullptr);
     default:
@@ -728,14 +1136,19 @@ int launch_pair_real(DevPimd *p, int kind, hipStream_t st) {
 
 bool pair_has_recip(int kind) { return kind == PK_EWALD || kind == PK_EWALD_R; }
 
-int launch_pair_recip(DevPimd *p, hipStream_t st) {
+int launch_pair_recip(DevPimd *p, int kind, hipStream_t st) {
+  if (kind == PK_EWALD) {
+    EwaldPlan pl;
+    PairParams pp = ewald_params(p, pl);
+    return launch_recip(p, pp, pl.cube_n, pl.gmax2, st, true, nullptr);
+  }
   PairParams pp = make_params(p);
-  return launch_recip(p, pp, st, true);
+  return launch_recip(p, pp, pp.nsum, 0.0, st, true, nullptr);
 }
 
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st) {
   if (launch_pair_real(p, kind, st)) return -1;
-  if (pair_has_recip(kind)) return launch_pair_recip(p, st);
+  if (pair_has_recip(kind)) return launch_pair_recip(p, kind, st);
   return 0;
 }
 
@@ -749,21 +1162,22 @@ int pair_energy(DevPimd *p, int kind, double *out) {
   } else if (kind == PK_PGAUSS) {
     if (launch_pair_mode<PK_PGAUSS, true>(p, pp, st, &e)) return -1;
   } else if (kind == PK_EWALD) {
+    EwaldPlan pl;
+    pp = ewald_params(p, pl);
     if (launch_pair_mode<PK_EWALD, true>(p, pp, st, &e)) return -1;
-  } else {
-    return -1;
-  }
-  if (kind == PK_EWALD) {
-    if (launch_recip(p, pp, st, false)) return -1;
-    KSpace &K = g_ks;
-    int nb2 = 256;
-    hipLaunchKernelGGL(k_recip_energy, dim3(nb2), dim3(256), 0, st, K.S, K.coef, K.nk, p->P, p->N, d->red);
+    KSpace *K = nullptr;
+    if (launch_recip(p, pp, pl.cube_n, pl.gmax2, st, false, &K)) return -1;
+    const int nb2 = 256;
+    hipLaunchKernelGGL(k_recip_energy, dim3(nb2), dim3(256), 0, st, K->S, K->coef, K->ncol * K->ZR, p->P, p->N,
+                       d->red);
     XMD_CHECK(hipGetLastError());
     double er = 0;
     if (device_sum(d->red, nb2, &er)) return -1;
-    double V = pp.L[0] * pp.L[1] * pp.L[2];
-    double bg = -pp.q * M_PI / (pp.alpha * pp.alpha * V) * 0.5 * p->N * (p->N - 1) * p->P;
+    const double V = pp.L[0] * pp.L[1] * pp.L[2];
+    const double bg = -pp.q * M_PI / (pp.alpha * pp.alpha * V) * 0.5 * p->N * (p->N - 1) * p->P;
     e += er + bg;
+  } else {
+    return -1;
   }
   *out = e / p->P;
   return 0;
@@ -783,6 +1197,21 @@ int md_pimd_calc_pair_force_x(md_pimd_t_x *pimd, md_pair_force_t_x pf) {
   }
   if (ensure_pimd_device(pimd)) return -1;
   return launch_pair_force(pimd_state(pimd, false), kind, rt().s0);
+}
+
+int md_pimd_get_ewald_plan_x(md_pimd_t_x *pimd, md_num_t_x *alpha, md_num_t_x *r_cut, int *n_k) {
+  if (ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  EwaldPlan pl;
+  PairParams pp = ewald_params(p, pl);
+  if (alpha) *alpha = pl.alpha;
+  if (r_cut) *r_cut = sqrt(pl.rc2);
+  if (n_k) {
+    KSpace *K = ensure_kspace(pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, p->ds->nb, p->P, 1);
+    if (!K) return -1;
+    *n_k = K->nk;
+  }
+  return 0;
 }
 
 int md_pimd_calc_pair_energy_x(md_pimd_t_x *pimd, md_pair_energy_t_x pe, md_stats_t_x *stats) {

@@ -74,7 +74,7 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial = false) {
   if (sp.pair_kind >= 0) {
     if (rec(8, R.s0) || launch_pair_real(p, sp.pair_kind, R.s0) || rec(9, R.s0)) return -1;
     if (pair_has_recip(sp.pair_kind)) {
-      if (rec(10, R.s0) || launch_pair_recip(p, R.s0) || rec(11, R.s0)) return -1;
+      if (rec(10, R.s0) || launch_pair_recip(p, sp.pair_kind, R.s0) || rec(11, R.s0)) return -1;
     } else if (rec(10, R.s0) || rec(11, R.s0)) {
       return -1;
     }
@@ -117,6 +117,12 @@ int md_get_nhc_length_x(void) { return g_nhc_len; }
 int md_set_ewald_real_cutoff_x(md_num_t_x x_cut) {
   if (!(x_cut > 0)) return -1;
   g_ewald_xcut = x_cut;
+  return 0;
+}
+
+int md_set_ewald_split_x(md_num_t_x alpha_L) {
+  if (!(alpha_L == alpha_L)) return -1;
+  g_ewald_split_aL = alpha_L;
   return 0;
 }
 

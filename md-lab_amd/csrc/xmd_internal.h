@@ -138,7 +138,7 @@ int pair_kind_of(md_pair_force_t_x pf);
 int pair_energy_kind_of(md_pair_energy_t_x pe);
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st);
 int launch_pair_real(DevPimd *p, int kind, hipStream_t st);
-int launch_pair_recip(DevPimd *p, hipStream_t st);
+int launch_pair_recip(DevPimd *p, int kind, hipStream_t st);
 bool pair_has_recip(int kind);
 int pair_energy(DevPimd *p, int kind, double *out);
 
@@ -149,6 +149,7 @@ int connection_matrix(DevPimd *p, double *host_G);
 int backward_potentials(DevPimd *p, double *host_Vt);
 int bead_sum(DevSim *d, int mode, const double *trap_c, double w, const double *L, double *out);
 extern double g_ewald_xcut;
+extern double g_ewald_split_aL;
 
 
 }  // namespace xmd

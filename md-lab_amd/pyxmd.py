@@ -132,6 +132,8 @@ _SIGS = {
     "md_get_nhc_length_x": (C.c_int, []),
     "md_hip_synchronize_x": (C.c_int, []),
     "md_set_ewald_real_cutoff_x": (C.c_int, [C.c_double]),
+    "md_set_ewald_split_x": (C.c_int, [C.c_double]),
+    "md_pimd_get_ewald_plan_x": (C.c_int, [PIMD_P, DP, DP, C.POINTER(C.c_int)]),
     "md_pimd_get_backward_potentials_x": (C.c_int, [PIMD_P, DP]),
     "md_pimd_get_connection_matrix_x": (C.c_int, [PIMD_P, DP]),
     "md_pimd_compute_forces_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig)]),

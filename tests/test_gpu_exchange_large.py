@@ -1,0 +1,89 @@
+"""Exchange recursion across many 64-position blocks: V_B, backward
+potentials, G and exchange forces at N up to the benchmark's N = 1024,
+against the CPU oracle on the same beads (the golden cases stop at N = 24,
+one block).  Also the size-independent identities Vt(0) = V_B(N) and G
+doubly stochastic at full size.
+
+Force tolerance: max(1e-11, twice the disagreement between the reference's
+own two exchange algorithms (fast backward pass vs slow cycle walk, both in
+the oracle) on the same input).  The G entries 1 - exp(x) carry the absolute
+round-off of x ~ beta V_B(N) ~ 1e3-1e4, so at N = 700 the reference's two
+algorithms already differ by ~2e-10.  The slow walk costs O(N^3 P); at
+N = 1024 the forces follow from the checked G and springs, and the test
+checks V_B, Vt and G only.
+"""
+import numpy as np
+import pytest
+
+import pyxmd
+from orc import Oracle, beads_from, dptr, ptr, rel_err, tri_size
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return pyxmd.Lib()
+
+
+CASES = [("trap", 130, 4, 0.0), ("trap", 200, 3, 0.5), ("trap", 300, 4, 1.0), ("ueg", 700, 2, 0.25),
+         ("ueg", 1024, 2, 1.0)]
+
+
+@pytest.mark.parametrize("kind,N,P,xi", CASES)
+def test_exchange_multiblock(lib, kind, N, P, xi):
+    if kind == "trap":
+        s = pyxmd.System(lib, "trap", N, P, 0.5, xi, seed=11)
+        L = np.array([10.0] * 3)
+    else:
+        Lb = pyxmd.ueg_box(N, 2.0)
+        s = pyxmd.System(lib, "ueg", N, P, pyxmd.ueg_temperature(N, Lb, 0.5), xi, seed=11, L=Lb, nsum=8)
+        L = np.array([Lb] * 3)
+    img = s.image
+    assert lib.md_pimd_fast_fill_ENk_x(s.pimd, img) == 0
+    assert lib.md_pimd_fill_VB_x(s.pimd) == 0
+    assert lib.md_reset_force_x(s.sim) == 0
+    assert lib.md_pimd_fast_fill_force_VB_x(s.pimd, img) == 0
+    hb = s.host_beads()
+    f = hb["f"].reshape(-1).copy()
+    s.host_pimd()
+    vbn = pyxmd.vbn(s.pimd).copy()
+    Vt_g = np.zeros(N + 1)
+    G_g = np.zeros(N * N)
+    assert lib.md_pimd_get_backward_potentials_x(s.pimd, dptr(Vt_g)) == 0
+    assert lib.md_pimd_get_connection_matrix_x(s.pimd, dptr(G_g)) == 0
+
+    o = Oracle()
+    b = beads_from(hb["x"].reshape(-1), m=hb["m"])
+    E = np.zeros(tri_size(N))
+    Ei = np.zeros(N)
+    o.fast_fill_ENk(N, P, s.omegaP, img, dptr(L), ptr(b), dptr(E), dptr(Ei))
+    V = np.zeros(N + 1)
+    o.fill_VB(N, s.beta, xi, dptr(E), dptr(V))
+    Vt = np.zeros(N + 1)
+    G = np.zeros(N * N)
+    b["f"] = 0
+    o.fast_force_VB(N, P, s.beta, xi, s.omegaP, img, dptr(L), dptr(E), dptr(V), ptr(b), dptr(Vt), dptr(G))
+    fo = b["f"].reshape(-1).copy()
+
+    np.testing.assert_allclose(pyxmd.tri(s.pimd), E, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(vbn, V, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(Vt_g, Vt, rtol=1e-12, atol=1e-12)
+    scale = max(1.0, abs(s.beta * V[N]))
+    np.testing.assert_allclose(G_g, G, rtol=1e-10, atol=max(1e-12, 64 * 2.2e-16 * scale))
+    err = rel_err(f, fo)
+    if N <= 700:
+        b2 = beads_from(hb["x"].reshape(-1), m=hb["m"])
+        o.slow_force_VB(N, P, s.beta, xi, s.omegaP, img, dptr(L), dptr(E), dptr(V), ptr(b2))
+        floor = rel_err(fo, b2["f"].reshape(-1))
+        tol = max(1e-11, 2 * floor)
+        print(f"{kind} N={N} P={P} xi={xi}: beta V_B(N)={s.beta * V[N]:.1f}, exchange force rel err {err:.2e} "
+              f"(reference fast vs slow {floor:.2e})")
+        assert err < tol
+    else:
+        print(f"{kind} N={N} P={P} xi={xi}: beta V_B(N)={s.beta * V[N]:.1f}, exchange force rel err {err:.2e}")
+    # identities at full size
+    assert Vt_g[0] == pytest.approx(vbn[N], rel=1e-12, abs=1e-12)
+    Gm = G_g.reshape(N, N)
+    np.testing.assert_allclose(Gm.sum(0), 1.0, atol=1e-9)
+    np.testing.assert_allclose(Gm.sum(1), 1.0, atol=1e-9)
