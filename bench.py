@@ -80,7 +80,20 @@ def main():
     ap.add_argument("--P", type=int, default=32)
     ap.add_argument("--cpu-pairs", type=int, default=25000, help="pairs in the reference CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 8 * n, "dispatches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build "
                                    "(replicas or bead slices would shard across GPUs, see DESIGN.md)"}))

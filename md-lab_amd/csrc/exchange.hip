@@ -24,14 +24,9 @@
 // pass and the host mirror, and a column layout (cycles sharing their first
 // particle contiguous) for the backward pass and the exchange-force columns.
 //
-// Recursion kernel (one 1024-thread workgroup per direction, both directions
-// run concurrently as the two blocks of one launch).  Positions are
-// processed in blocks of 64: all 16 waves reduce the off-diagonal part of
-// the block's 64 rows (terms from finished blocks, two-pass LSE per row),
-// then one wave runs the 64-step diagonal with the right-looking update
-//   (M_q, S_q) <- (M_q, S_q) (+) S_r exp(c(q, r) + M_r)
-// that needs one exp per lane per step and no log on the critical path;
-// values are broadcast between steps with scalar readlanes.
+// The recursions are solved as a blocked right-looking triangular solve in
+// log-sum-exp form (see the block-solve section below): both directions
+// share each launch, one launch per 64 positions.
 #include <cmath>
 
 #include "xmd_device.h"
@@ -49,8 +44,28 @@ __device__ __forceinline__ size_t col_off(int m, int N) { return (size_t)m * N -
 
 struct Geo {
   double L0, L1, L2;
+  double iL0, iL1, iL2;
   int image;
 };
+
+// Minimum image by rounding, for squared distances only: at an exact
+// +-L/2 tie it may pick the other sign than md_minimum_image_x, which
+// leaves d^2 unchanged.
+__device__ __forceinline__ double comp_sq(double a, double L, double iL, int image) {
+  return image ? fma(-L, rint(a * iL), a) : a;
+}
+
+// s(a, b) = 1/2 m_a w^2 d^2 (energy only; md_pimd_x.c:2120-2124).
+__device__ __forceinline__ double spring_e(const double *__restrict__ X, int nb, int sa, int sb, double m, double om,
+                                           const Geo &g) {
+  const double dx = comp_sq(X[sa] - X[sb], g.L0, g.iL0, g.image);
+  const double dy = comp_sq(X[nb + sa] - X[nb + sb], g.L1, g.iL1, g.image);
+  const double dz = comp_sq(X[2 * nb + sa] - X[2 * nb + sb], g.L2, g.iL2, g.image);
+  // distance, then its square, in the reference's order of operations: on
+  // ill-conditioned exchange inputs the forces follow ENk's last bits
+  const double d = sqrt(dx * dx + dy * dy + dz * dz);
+  return 0.5 * m * om * om * d * d;
+}
 
 __device__ __forceinline__ double comp(double a, double L, int image) { return image ? dv_min_image(a, L) : a; }
 
@@ -81,21 +96,32 @@ __device__ __forceinline__ double e2(const double *X, const double *M, int nb, i
 
 // ------------------------------------------------------------ E_n^(k)
 
-// Thread per particle l (0-based): Eint_l, E_l^(1), link C_l.
-__global__ void k_chain(const double *__restrict__ X, const double *__restrict__ M, int N, int P, double om, Geo g,
-                        double *__restrict__ Eint, double *__restrict__ Clink, double *__restrict__ ENk,
-                        double *__restrict__ ET) {
-  int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= N) return;
+// Particle l (0-based): Eint_l, E_l^(1), link C_l.  A block holds 64
+// particles (lanes); its four waves split the P - 1 interior springs and
+// the partial sums are added in a fixed order.
+__global__ __launch_bounds__(256) void k_chain(const double *__restrict__ X, const double *__restrict__ M, int N,
+                                               int P, double om, Geo g, double *__restrict__ Eint,
+                                               double *__restrict__ Clink, double *__restrict__ ENk,
+                                               double *__restrict__ ET) {
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l = blockIdx.x * 64 + lane;
   const int nb = N * P;
   double e = 0;
-  for (int j = 0; j < P - 1; ++j) e += spring(X, nb, j * N + l, (j + 1) * N + l, M[j * N + l], om, g);
+  if (l < N) {
+    const int j0 = ((P - 1) * wv) / 4, j1 = ((P - 1) * (wv + 1)) / 4;
+    for (int j = j0; j < j1; ++j) e += spring_e(X, nb, j * N + l, (j + 1) * N + l, M[j * N + l], om, g);
+  }
+  part[wv][lane] = e;
+  __syncthreads();
+  if (wv != 0 || l >= N) return;
+  e = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
   Eint[l] = e;
   const int sP = (P - 1) * N + l;
-  double e1 = e + spring(X, nb, sP, l, M[sP], om, g);
+  const double e1 = e + spring_e(X, nb, sP, l, M[sP], om, g);
   ENk[tri_row(l + 1)] = e1;
   ET[col_off(l, N)] = e1;
-  Clink[l] = (l < N - 1) ? e + spring(X, nb, sP, l + 1, M[sP], om, g) : 0.0;
+  Clink[l] = (l < N - 1) ? e + spring_e(X, nb, sP, l + 1, M[sP], om, g) : 0.0;
 }
 
 // Exclusive double-double prefix sum of in[0..n-1] into pre[2*i] (hi, lo).
@@ -138,7 +164,7 @@ __global__ __launch_bounds__(256) void k_enk_fill(const double *__restrict__ X, 
   for (int k = 2 + threadIdx.x; k <= r + 1; k += blockDim.x) {
     const int m0 = r - k + 1;  // 0-based first particle
     double chain = dd_sub_to_double(ar, dd{pre[2 * m0], pre[2 * m0 + 1]});
-    double E = chain + ein + spring(X, nb, sP, m0, mP, om, g);
+    double E = chain + ein + spring_e(X, nb, sP, m0, mP, om, g);
     ENk[tri_row(r + 1) + k - 1] = E;
     ET[col_off(m0, N) + k - 1] = E;
   }
@@ -749,6 +775,9 @@ static Geo geo_of(DevPimd *p, int image) {
   g.L0 = box_L(sim, 0);
   g.L1 = box_L(sim, 1);
   g.L2 = box_L(sim, 2);
+  g.iL0 = g.L0 != 0 ? 1.0 / g.L0 : 0.0;
+  g.iL1 = g.L1 != 0 ? 1.0 / g.L1 : 0.0;
+  g.iL2 = g.L2 != 0 ? 1.0 / g.L2 : 0.0;
   g.image = image;
   return g;
 }
@@ -780,7 +809,7 @@ int launch_enk(DevPimd *p, int image, hipStream_t st) {
   Geo g = geo_of(p, image);
   double om = p->host->omegaP;
   double *clink = p->work + (N + 1);
-  hipLaunchKernelGGL(k_chain, dim3((N + 255) / 256), dim3(256), 0, st, d->X, d->M, N, P, om, g, p->Eint, clink,
+  hipLaunchKernelGGL(k_chain, dim3((N + 63) / 64), dim3(256), 0, st, d->X, d->M, N, P, om, g, p->Eint, clink,
                      p->ENk, p->ET);
   XMD_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_scan_dd, dim3(1), dim3(1024), 0, st, clink, N, p->Cpre);

@@ -3,28 +3,22 @@
 // (md_simulation_x.c:166-346, md_statistics_x.c:340-396).
 //
 // Kernel shapes (gfx950, wave64):
-//  * k_pair<MODE>: grid (ceil(N/64), P), 256 threads.  The 64 lanes of every
-//    wave own 64 beads i of one slice; the four waves split the partner list
-//    (stride 4) of an LDS-staged partner tile, so every partner read is an
-//    LDS broadcast.  Partials are combined through LDS and written once.
-//    Result per bead: f_i += sum_{j != i} F(x_i, x_j) / m_{max(i,j)} / P,
-//    i.e. the reference's l > l' loop with Newton's third law unrolled into
-//    ordered pairs (the reaction is divided by the higher-index bead's mass
-//    exactly as at md_pimd_x.c:1669-1673).
-//  * Ewald real space uses the reference's own image set -- the cube
-//    |n_d| <= n_sum around the raw separation p1 - p2 (md_simulation_x.c:282)
-//    -- pruned to images within r_c = 6/alpha (erfc(6) = 2e-17; the dropped
-//    terms are below 1e-16 of a typical pair force).
+//  * k_pair_tiles / k_ewald_real: one wave per (slice, tile I <= tile J) of
+//    64 x 64 beads; lane t meets partner (t + s) mod 64 at rotation step s,
+//    so each pair is evaluated once and both reactions accumulate without
+//    atomics (i in registers, j in LDS).  Result per bead:
+//    f_i += sum_{j != i} F(x_i, x_j) / m_{max(i,j)} / P, i.e. the reference's
+//    l > l' loop with Newton's third law (md_pimd_x.c:1669-1673).
+//  * Ewald real space sums the images within r_c = x_cut/alpha (erfc(6) =
+//    2e-17; the reference sums its whole image cube |n_d| <= n_sum, whose
+//    other terms are below that).  k_ewald_real evaluates the force kernel
+//    from a piecewise-polynomial table in LDS; k_pair_tiles covers the R/NI
+//    splits, energies and the general cube-clipped case.
 //  * Ewald reciprocal space uses per-slice structure factors
-//    S(k) = sum_l exp(i k.x_l) over the reference's k cube
-//    (|m_d| <= n_sum, k != 0) folded to a half space:
-//      f_i = sum_{k in half} 4 pi A_k g_k Im(exp(i k.x_i) S(k)*) / m,
-//    A_k = q exp(-pi^2 G^2/alpha^2) / (pi V G^2), which equals the
-//    reference's per-pair sum (md_simulation_x.c:266-281) because
-//    sum_{l != i} sin(k.(x_i - x_l)) = Im(exp(i k.x_i) S(k)*).
-//    k_sk_partial builds S for 64-bead chunks (tables of exp(i m theta) in
-//    LDS), k_sk_reduce sums the chunks, k_recip_force applies them with the
-//    wave's S(k) read through scalar loads.
+//    S(k) = sum_l exp(i k.x_l) over half of the k set (k_phase, k_sk,
+//    k_sk_fin, k_recip_force, k_recip_energy).
+//  * When the reference's own Ewald parameters are converged, the same sum is
+//    evaluated with the engine's split (plan_ewald).
 #include <cmath>
 #include <vector>
 
@@ -266,9 +260,10 @@ __device__ __forceinline__ void pair_term(double dx, double dy, double dz, const
 // is evaluated once (Newton's third law), divided by the mass of the
 // higher-index bead as in md_pimd_x.c:1669-1673.
 //
-// Output: per-bead partial sums in Fpart[(d*nb + bead)*T + slot], where
+// Output: per-bead partial sums in Fpart[(slot*3 + d)*nb + bead], where
 // slot is the other tile of the pair, so every slot is written exactly once
-// and k_reduce_tiles sums them in a fixed order (deterministic forces).
+// (a wave writes 64 consecutive doubles) and k_reduce_tiles sums them in a
+// fixed order (deterministic forces).
 template <int MODE, bool ENERGY, bool NEAR>
 __global__ __launch_bounds__(64) void k_pair_tiles(const double *__restrict__ X, const double *__restrict__ M,
                                                    double *__restrict__ Fpart, double *__restrict__ Epart,
@@ -327,10 +322,10 @@ __global__ __launch_bounds__(64) void k_pair_tiles(const double *__restrict__ X,
     return;
   }
   auto put = [&](int bead, int slot, double vx, double vy, double vz) {
-    const size_t s0 = (size_t)(base + bead) * T + slot;
+    const size_t s0 = (size_t)slot * 3 * nb + base + bead;
     Fpart[s0] = vx;
-    Fpart[(size_t)nb * T + s0] = vy;
-    Fpart[2 * (size_t)nb * T + s0] = vz;
+    Fpart[s0 + nb] = vy;
+    Fpart[s0 + 2 * (size_t)nb] = vz;
   };
   if (diag) {
     if (act) put(i, I, fx + rx[t], fy + ry[t], fz + rz[t]);
@@ -344,9 +339,8 @@ __global__ __launch_bounds__(64) void k_pair_tiles(const double *__restrict__ X,
 __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restrict__ F, int n3, int T, double P) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n3) return;
-  const double *src = Fpart + (size_t)k * T;
   double s = 0;
-  for (int q = 0; q < T; ++q) s += src[q];
+  for (int q = 0; q < T; ++q) s += Fpart[(size_t)q * n3 + k];
   F[k] += s / P;
 }
 
@@ -821,10 +815,10 @@ __global__ __launch_bounds__(256) void k_ewald_real(const double *__restrict__ X
   __syncthreads();
   if (!valid) return;
   auto put = [&](int bead, int slot, double vx, double vy, double vz) {
-    const size_t s0 = (size_t)(base + bead) * T + slot;
+    const size_t s0 = (size_t)slot * 3 * nb + base + bead;
     Fpart[s0] = vx;
-    Fpart[(size_t)nb * T + s0] = vy;
-    Fpart[2 * (size_t)nb * T + s0] = vz;
+    Fpart[s0 + nb] = vy;
+    Fpart[s0 + 2 * (size_t)nb] = vz;
   };
   if (diag) {
     if (act) put(i, I, fx + rx[w][t], fy + ry[w][t], fz + rz[w][t]);
