@@ -347,33 +347,45 @@ __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restr
 // ------------------------------------------------------ reciprocal space
 //
 // k set: columns (mx, my) of the half space (mx > 0, or mx == 0 and my >= 0)
-// times m_z in [-MZ, MZ]; the other half follows from S(-k) = S(k)*.  The
-// coefficient coef = 4 pi A_k, A_k = q exp(-pi^2 G^2/alpha^2) / (pi V G^2),
+// times m_z = z - MZ, z in [0, ZP); the other half follows from
+// S(-k) = S(k)*.  coef = 4 pi A_k, A_k = q exp(-pi^2 G^2/alpha^2)/(pi V G^2),
 // G_d = m_d / L_d, is zero on the slots outside the set in use:
 //  * the reference's cube |m_d| <= n (md_simulation_x.c:266-281), or
 //  * the cylinder of a converged split: (mx, my) in the disc |G_xy| <= G_max,
-//    every m_z of the table (all terms beyond G_max are below exp(-x_cut^2)).
-// Column (0, 0) keeps m_z > 0 only.  Every column has ZR = 9 ceil((2MZ+1)/9)
-// slots (m_z = -MZ + u, u < ZR; slots past MZ are zero), so the kernels run
-// fixed-length loops without per-slot branches.
+//    |m_z| <= MZ (all terms beyond G_max are below exp(-x_cut^2)).
+// Column (0, 0) keeps m_z > 0 only; padding columns and the slot z = 2MZ+1
+// carry zero weight.  MZ is 7, 15, 23 or 31, so ZP = 2 MZ + 2 is a multiple
+// of the 16-wide matrix-core tiles.
+//
+// Both k kernels are complex GEMMs on the FP64 matrix cores
+// (V_MFMA_F64_16X16X4: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16 (i mod 4), register i div 4):
+//   structure factors  S[c][z] = sum_beads A[c][s] Z[s][z]   (M = columns,
+//                      N = m_z, K = beads), A = e^{i(mx tx + my ty)}, Z = e^{i mz tz}
+//   forces             U[s][c] = sum_z Z[s][z] W[c][z]  and  V with W -> m_z W
+//                      (M = beads, N = columns, K = m_z), W = coef conj(S),
+// then f_s = sum_c (mx, my) Im(A U) / L + (0, 0, Im(A V)).
 //
 // Phase tables (k_phase), table half-width MZ:
-//   tx[m * nb + s]       e^{i 2 pi m x / L_x},  m = 0..MZ   (row-major)
-//   ty[(m + MZ) * nb + s] e^{i 2 pi m y / L_y}, m = -MZ..MZ (row-major)
-//   tz[s * ZR + m + MZ]   e^{i 2 pi m z / L_z}, m = -MZ..MZ (bead-major, zero pad)
-constexpr int SK_SEG = 9;      // m_z values per structure-factor segment
-constexpr int MAX_KM = 26;     // largest table half-width of the k kernels
+//   tx[m * nb + s]        e^{i 2 pi m x / L_x},  m = 0..MZ    (row-major)
+//   ty[(m + MZ) * nb + s] e^{i 2 pi m y / L_y},  m = -MZ..MZ  (row-major)
+//   tz[s * ZP + m + MZ]   e^{i 2 pi m z / L_z},  m = -MZ..MZ  (bead-major, slot ZP-1 zero)
+constexpr int MAX_KM = 31;     // largest table half-width of the k kernels
 
-__host__ __device__ constexpr int kz_row(int MZ) { return SK_SEG * ((2 * MZ + 1 + SK_SEG - 1) / SK_SEG); }
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
 struct KSpace {
   double L[3] = {0, 0, 0}, alpha = 0, q = 0, gmax2 = 0;
   int n = -2;  // cube half-width, or -1 for the cylinder of radius^2 gmax2
-  int MZ = 0, ZR = 0, ncol = 0, nk = 0;  // nk: k vectors with nonzero weight
-  int2 *cols = nullptr;    // (mx, my)
-  double *coef = nullptr;  // [ncol][ZR]
-  double2 *tx = nullptr, *ty = nullptr, *tz = nullptr, *Spart = nullptr, *S = nullptr, *W = nullptr;
-  size_t tx_cap = 0, ty_cap = 0, tz_cap = 0, part_cap = 0, s_cap = 0, w_cap = 0;
+  int MZ = 0, ZP = 0, ncol = 0, ncolp = 0, nk = 0;  // nk: k vectors with nonzero weight
+  int2 *cols = nullptr;    // [ncolp] (mx, my)
+  double *coef = nullptr;  // [ncolp][ZP]
+  double2 *tx = nullptr, *ty = nullptr, *tz = nullptr, *S = nullptr, *W = nullptr;
+  size_t tx_cap = 0, ty_cap = 0, tz_cap = 0, s_cap = 0, w_cap = 0;
   int stamp = 0;
 };
 
@@ -389,14 +401,14 @@ static int grow(double2 *&p, size_t &cap, size_t need) {
   return 0;
 }
 
-// Table half-widths whose 2 MZ + 1 values fill whole 9-slot segments.
+// Table half-widths whose 2 MZ + 2 slots fill whole 16-wide tiles.
 static int table_mz(int M) {
-  for (int mz : {4, 8, 13, 17, 22, 26})
+  for (int mz : {7, 15, 23, 31})
     if (M <= mz) return mz;
   return -1;
 }
 
-static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, double gmax2, int nb, int P, int nch) {
+static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, double gmax2, int nb, int P) {
   KSpace *K = nullptr;
   for (auto &k : g_ks)
     if (k.n == n && k.alpha == alpha && k.q == q && k.gmax2 == gmax2 && k.L[0] == L[0] && k.L[1] == L[1] &&
@@ -414,7 +426,7 @@ static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, dou
       fprintf(stderr, "xmd: Ewald k range %d exceeds the supported %d\n", M, MAX_KM);
       return nullptr;
     }
-    const int ZR = kz_row(MZ);
+    const int ZP = 2 * MZ + 2;
     std::vector<int2> cols;
     std::vector<double> coef;
     const double V = L[0] * L[1] * L[2];
@@ -425,8 +437,8 @@ static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, dou
         const double gxy2 = (mx / L[0]) * (mx / L[0]) + (my / L[1]) * (my / L[1]);
         if (n < 0 && gxy2 > gmax2) continue;
         cols.push_back(make_int2(mx, my));
-        for (int u = 0; u < ZR; ++u) {
-          const int mz = u - MZ;
+        for (int z = 0; z < ZP; ++z) {
+          const int mz = z - MZ;
           bool in = mz <= MZ && (n < 0 || (mz >= -n && mz <= n));
           if (mx == 0 && my == 0 && mz <= 0) in = false;
           double c = 0;
@@ -438,6 +450,11 @@ static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, dou
           coef.push_back(c);
         }
       }
+    const int ncol = (int)cols.size(), ncolp = (ncol + 15) / 16 * 16;
+    while ((int)cols.size() < ncolp) {
+      cols.push_back(make_int2(0, 0));
+      for (int z = 0; z < ZP; ++z) coef.push_back(0.0);
+    }
     if (K->cols) {
       (void)hipFree(K->cols);
       (void)hipFree(K->coef);
@@ -452,22 +469,23 @@ static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, dou
     K->q = q;
     for (int d = 0; d < 3; ++d) K->L[d] = L[d];
     K->MZ = MZ;
-    K->ZR = ZR;
-    K->ncol = (int)cols.size();
+    K->ZP = ZP;
+    K->ncol = ncol;
+    K->ncolp = ncolp;
     K->nk = nk;
   }
   K->stamp = ++g_ks_clock;
-  const size_t slots = (size_t)K->ncol * K->ZR;
+  const size_t slots = (size_t)K->ncolp * K->ZP;
   if (grow(K->tx, K->tx_cap, (size_t)(K->MZ + 1) * nb) || grow(K->ty, K->ty_cap, (size_t)(2 * K->MZ + 1) * nb) ||
-      grow(K->tz, K->tz_cap, (size_t)K->ZR * nb) || grow(K->Spart, K->part_cap, (size_t)P * nch * slots) ||
-      grow(K->S, K->s_cap, (size_t)P * slots) || grow(K->W, K->w_cap, (size_t)P * slots))
+      grow(K->tz, K->tz_cap, (size_t)K->ZP * nb) || grow(K->S, K->s_cap, (size_t)P * slots) ||
+      grow(K->W, K->w_cap, (size_t)P * slots))
     return nullptr;
   return K;
 }
 
 // One thread per (dimension, bead): e^{i pi m u}, u = 2 x / L; negative m
 // are the conjugates.
-__global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZR, double L0, double L1, double L2,
+__global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZP, double L0, double L1, double L2,
                         double2 *__restrict__ tx, double2 *__restrict__ ty, double2 *__restrict__ tz) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * nb) return;
@@ -490,7 +508,7 @@ __global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZR, do
       r0[-(ptrdiff_t)m * nb] = make_double2(cs, -sn);
     }
   } else {
-    double2 *r = tz + (size_t)s * ZR + MZ;
+    double2 *r = tz + (size_t)s * ZP + MZ;
     r[0] = make_double2(1.0, 0.0);
     for (int m = 1; m <= MZ; ++m) {
       double sn, cs;
@@ -498,158 +516,196 @@ __global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZR, do
       r[m] = make_double2(cs, sn);
       r[-m] = make_double2(cs, -sn);
     }
-    for (int k = 2 * MZ + 1; k < ZR; ++k) r[k - MZ] = make_double2(0.0, 0.0);
+    r[MZ + 1] = make_double2(0.0, 0.0);
   }
 }
 
-// Structure-factor partials.  A wave covers 64 columns (lanes) and one
-// segment of SK_SEG m_z values; the bead's e^{i m_z theta_z} for the
-// segment are wave-uniform scalar loads, the column's e^{i (m_x theta_x +
-// m_y theta_y)} is one complex product per bead.  Grid: (column group x
-// segment, bead chunk, slice).
-__global__ __launch_bounds__(64) void k_sk(const double2 *__restrict__ tx, const double2 *__restrict__ ty,
-                                           const double2 *__restrict__ tz, const int2 *__restrict__ cols, int ncol,
-                                           int MZ, int ZR, int N, int nb, int nch, int chunk_len,
-                                           double2 *__restrict__ Spart) {
-  const int lane = threadIdx.x;
-  const int nseg = ZR / SK_SEG;
-  const int cg = blockIdx.x / nseg, seg = blockIdx.x - cg * nseg;
-  const int c = cg * 64 + lane;
-  const int chunk = blockIdx.y, slice = blockIdx.z;
-  const bool act = c < ncol;
-  const int2 cl = act ? cols[c] : make_int2(0, 0);
+// Workgroups are numbered so that the tiles of one slice share an XCD
+// (consecutive workgroup ids go to consecutive XCDs) when the slice count
+// is a multiple of 8; otherwise the plain order.
+__device__ __forceinline__ void xcd_tile(int w, int ntile, int P, int &slice, int &tile) {
+  if ((P & 7) == 0) {
+    const int x = w & 7, loc = w >> 3;
+    slice = x + 8 * (loc / ntile);
+    tile = loc % ntile;
+  } else {
+    slice = w / ntile;
+    tile = w % ntile;
+  }
+}
+
+// Structure factors of one 16 x 16 tile (16 columns x 16 m_z slots) of one
+// slice; the four waves split the beads and their partial sums are added in
+// a fixed order.  Writes S and W = coef conj(S).  Workgroup w covers
+// (slice, column tile, slot tile), slot tile fastest.
+__global__ __launch_bounds__(256) void k_sk_mfma(const double2 *__restrict__ tx, const double2 *__restrict__ ty,
+                                                 const double2 *__restrict__ tz, const int2 *__restrict__ cols,
+                                                 const double *__restrict__ coef, int ncolp, int MZ, int ZP, int N,
+                                                 int nb, int P, double2 *__restrict__ S, double2 *__restrict__ W) {
+  __shared__ double red[4][2][256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nzt = ZP / 16;
+  int slice, tile;
+  xcd_tile(blockIdx.x, (ncolp / 16) * nzt, P, slice, tile);
+  const int c0 = (tile / nzt) * 16, z0 = (tile % nzt) * 16;
+  const int2 cl = cols[c0 + (lane & 15)];
   const double2 *px = tx + (size_t)cl.x * nb;
   const double2 *py = ty + (size_t)(MZ + cl.y) * nb;
-  double ar[SK_SEG], ai[SK_SEG];
-#pragma unroll
-  for (int u = 0; u < SK_SEG; ++u) ar[u] = ai[u] = 0.0;
-  const int s0 = slice * N + chunk * chunk_len;
-  const int s1 = slice * N + min(N, (chunk + 1) * chunk_len);
-  const double2 *pz = tz + seg * SK_SEG;
-  int s = s0;
-  for (; s + 1 < s1; s += 2) {
-    const double2 ex0 = px[s], ey0 = py[s], ex1 = px[s + 1], ey1 = py[s + 1];
-    const double er0 = ex0.x * ey0.x - ex0.y * ey0.y, ei0 = ex0.x * ey0.y + ex0.y * ey0.x;
-    const double er1 = ex1.x * ey1.x - ex1.y * ey1.y, ei1 = ex1.x * ey1.y + ex1.y * ey1.x;
-    const double2 *z0 = pz + (size_t)s * ZR, *z1 = z0 + ZR;
-#pragma unroll
-    for (int u = 0; u < SK_SEG; ++u) {
-      const double2 a = z0[u], b = z1[u];
-      ar[u] = fma(er0, a.x, fma(-ei0, a.y, fma(er1, b.x, fma(-ei1, b.y, ar[u]))));
-      ai[u] = fma(er0, a.y, fma(ei0, a.x, fma(er1, b.y, fma(ei1, b.x, ai[u]))));
+  const int kb = lane >> 4;  // bead of this lane within a K step
+  v4d sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+  const int q0 = (N * wv) / 4, q1 = (N * (wv + 1)) / 4;
+  // software pipeline: the loads of K step b + 4 are in flight while the
+  // matrix cores work on step b
+  auto fetch = [&](int b, double2 &ex, double2 &ey, double2 &z, bool &ok) {
+    const int bead = b + kb;
+    ok = bead < q1;
+    const int s = slice * N + (ok ? bead : q0);
+    ex = px[s];
+    ey = py[s];
+    z = tz[(size_t)s * ZP + z0 + (lane & 15)];
+  };
+  double2 ex, ey, z;
+  bool ok;
+  if (q0 < q1) fetch(q0, ex, ey, z, ok);
+  for (int b = q0; b < q1; b += 4) {
+    const double2 cex = ex, cey = ey, cz = z;
+    const bool cok = ok;
+    if (b + 4 < q1) fetch(b + 4, ex, ey, z, ok);
+    double ar = 0, ai = 0;
+    if (cok) {
+      ar = cex.x * cey.x - cex.y * cey.y;
+      ai = cex.x * cey.y + cex.y * cey.x;
     }
+    sr = mfma4(ar, cz.x, sr);
+    si = mfma4(ar, cz.y, si);
+    sr = mfma4(-ai, cz.y, sr);
+    si = mfma4(ai, cz.x, si);
   }
-  if (s < s1) {
-    const double2 ex0 = px[s], ey0 = py[s];
-    const double er0 = ex0.x * ey0.x - ex0.y * ey0.y, ei0 = ex0.x * ey0.y + ex0.y * ey0.x;
-    const double2 *z0 = pz + (size_t)s * ZR;
+  // D[i = column][j = slot]: lane j + 16 (i%4), register i/4
 #pragma unroll
-    for (int u = 0; u < SK_SEG; ++u) {
-      const double2 a = z0[u];
-      ar[u] = fma(er0, a.x, fma(-ei0, a.y, ar[u]));
-      ai[u] = fma(er0, a.y, fma(ei0, a.x, ai[u]));
-    }
+  for (int r = 0; r < 4; ++r) {
+    const int e = (4 * r + (lane >> 4)) * 16 + (lane & 15);
+    red[wv][0][e] = sr[r];
+    red[wv][1][e] = si[r];
   }
-  if (!act) return;
-  double2 *out = Spart + ((size_t)(slice * nch + chunk) * ncol + c) * ZR + seg * SK_SEG;
-#pragma unroll
-  for (int u = 0; u < SK_SEG; ++u) out[u] = make_double2(ar[u], ai[u]);
-}
-
-// S = sum of the chunk partials (fixed order); W = coef conj(S).
-__global__ void k_sk_fin(const double2 *__restrict__ Spart, const double *__restrict__ coef, int slots, int nch,
-                         int P, double2 *__restrict__ S, double2 *__restrict__ W) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= slots * P) return;
-  const int slice = t / slots, k = t - slice * slots;
-  double re = 0, im = 0;
-  for (int c = 0; c < nch; ++c) {
-    const double2 v = Spart[(size_t)(slice * nch + c) * slots + k];
-    re += v.x;
-    im += v.y;
-  }
-  S[t] = make_double2(re, im);
-  W[t] = make_double2(coef[k] * re, -coef[k] * im);
-}
-
-// Forces: lane = bead.  A workgroup holds 64 beads; each of its waves owns
-// one 9-slot m_z segment (that segment of the bead's e^{i m_z theta_z} row
-// in registers) and a share of the columns, whose weights W are
-// wave-uniform scalar loads.  For bead i:
-//   f_i = sum_k coef_k g_k Im(e^{i k.x_i} S_k*) / (m_i P),
-// the reference's per-pair sum (md_simulation_x.c:266-281) since
-// sum_{l != i} sin(k.(x_i - x_l)) = Im(e^{i k.x_i} S_k*).  The waves'
-// partial forces are summed in a fixed order.
-__host__ __device__ constexpr int rf_parts(int nseg) { return nseg >= 8 ? 1 : 8 / nseg; }
-
-template <int NSEG>
-__global__ __launch_bounds__(512) void k_recip_force(const double2 *__restrict__ tx, const double2 *__restrict__ ty,
-                                                     const double2 *__restrict__ tz, const int2 *__restrict__ cols,
-                                                     int ncol, int MZ, const double2 *__restrict__ W,
-                                                     const double *__restrict__ Mass, double *__restrict__ F, int N,
-                                                     int nb, double L0, double L1, double L2, double P) {
-  constexpr int ZR = NSEG * SK_SEG, CP = rf_parts(NSEG), NW = NSEG * CP;
-  __shared__ double red[3][NW][64];
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: W via scalar loads
-  const int seg = wv % NSEG, part = wv / NSEG;
-  const int slice = blockIdx.y;
-  const int i = blockIdx.x * 64 + lane;
-  const bool act = i < N;
-  const int s = slice * N + (act ? i : 0);
-  double zr[SK_SEG], zi[SK_SEG];
-  const double2 *pz = tz + (size_t)s * ZR + seg * SK_SEG;
-#pragma unroll
-  for (int u = 0; u < SK_SEG; ++u) {
-    const double2 z = pz[u];
-    zr[u] = z.x;
-    zi[u] = z.y;
-  }
-  const double mz0 = (double)(seg * SK_SEG - MZ);  // m_z of slot u is mz0 + u
-  const double2 *Ws = W + (size_t)slice * ncol * ZR + seg * SK_SEG;
-  const int c0 = (ncol * part) / CP, c1 = (ncol * (part + 1)) / CP;
-  double fx = 0, fy = 0, fz = 0;
-  for (int c = c0; c < c1; ++c) {
-    const int2 cl = cols[c];
-    const double2 *Wc = Ws + (size_t)c * ZR;
-    const double2 ex = tx[(size_t)cl.x * nb + s], ey = ty[(size_t)(MZ + cl.y) * nb + s];
-    const double ar = ex.x * ey.x - ex.y * ey.y, ai = ex.x * ey.y + ex.y * ey.x;
-    double ur = 0, ui = 0, vr = 0, vi = 0;
-#pragma unroll
-    for (int u = 0; u < SK_SEG; ++u) {
-      const double2 w = Wc[u];
-      const double tr = fma(w.x, zr[u], -w.y * zi[u]);
-      const double ti = fma(w.x, zi[u], w.y * zr[u]);
-      ur += tr;
-      ui += ti;
-      if (u > 0) {
-        vr = fma((double)u, tr, vr);
-        vi = fma((double)u, ti, vi);
-      }
-    }
-    vr = fma(mz0, ur, vr);
-    vi = fma(mz0, ui, vi);
-    const double g = fma(ar, ui, ai * ur);
-    fx = fma((double)cl.x, g, fx);
-    fy = fma((double)cl.y, g, fy);
-    fz = fma(ar, vi, fma(ai, vr, fz));
-  }
-  red[0][wv][lane] = fx;
-  red[1][wv][lane] = fy;
-  red[2][wv][lane] = fz;
   __syncthreads();
-  if (wv == 0 && act) {
-    double gx = 0, gy = 0, gz = 0;
+  const int e = threadIdx.x, i = e >> 4, j = e & 15;
+  const double re = ((red[0][0][e] + red[1][0][e]) + red[2][0][e]) + red[3][0][e];
+  const double im = ((red[0][1][e] + red[1][1][e]) + red[2][1][e]) + red[3][1][e];
+  const size_t k = (size_t)(c0 + i) * ZP + z0 + j;
+  const double c = coef[k];
+  S[(size_t)slice * ncolp * ZP + k] = make_double2(re, im);
+  W[(size_t)slice * ncolp * ZP + k] = make_double2(c * re, -c * im);
+}
+
+// Forces on 16 beads of one slice.  Work units are (column tile, slot
+// tile) pairs, dealt round-robin to the four waves; per unit
+//   U[s][c] = sum_z Z[s][z] W[c][z],  V[s][c] = sum_z Z[s][z] m_z W[c][z]
+// over the unit's 16 slots, and f_s += sum_c (mx Im(A U) / L_x,
+// my Im(A U) / L_y, Im(A V) / L_z) / (m P) (linear in U, V, so the slot
+// tiles add up).  For bead i this is sum_k coef_k g_k Im(e^{i k.x_i} S_k*) /
+// (m_i P), the reference's per-pair sum (md_simulation_x.c:266-281) since
+// sum_{l != i} sin(k.(x_i - x_l)) = Im(e^{i k.x_i} S_k*).
+__global__ __launch_bounds__(256) void k_recip_force_mfma(
+    const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
+    const int2 *__restrict__ cols, int ncolp, int MZ, int ZP, const double2 *__restrict__ W,
+    const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
+    double Pd) {
+  __shared__ double red[3][4][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nbt = (N + 15) / 16;
+  int slice, bt;
+  xcd_tile(blockIdx.x, nbt, P, slice, bt);
+  const int b0 = bt * 16;
+  const int abead = b0 + (lane & 15);  // A operand row of this lane
+  const double2 *pza = tz + (size_t)(slice * N + (abead < N ? abead : 0)) * ZP + (lane >> 4);
+  const bool aok = abead < N;
+  // D[i = bead][j = column]: lane j + 16 (i%4), register i/4
+  const int beadq = lane >> 4;  // this lane's beads: b0 + 4 r + beadq
+  int sb[4];
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      gx += red[0][w][lane];
-      gy += red[1][w][lane];
-      gz += red[2][w][lane];
+  for (int r = 0; r < 4; ++r) {
+    const int bead = b0 + 4 * r + beadq;
+    sb[r] = slice * N + (bead < N ? bead : 0);
+  }
+  double fx[4] = {0, 0, 0, 0}, fy[4] = {0, 0, 0, 0}, fz[4] = {0, 0, 0, 0};
+  const double2 *Ws = W + (size_t)slice * ncolp * ZP;
+  const int nzt = ZP / 16, nunit = (ncolp / 16) * nzt;
+  for (int u = wv; u < nunit; u += 4) {
+    const int ct = u / nzt, z0 = (u - ct * nzt) * 16;
+    const int c = ct * 16 + (lane & 15);
+    // operands: Z rows of the beads, W rows of the columns, and the
+    // epilogue's phase factors, all issued before the matrix-core chain
+    double2 za[4], wb[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      za[ks] = aok ? pza[z0 + 4 * ks] : make_double2(0.0, 0.0);
+      wb[ks] = Ws[(size_t)c * ZP + z0 + 4 * ks + (lane >> 4)];
     }
-    const double m = Mass[s];
-    F[s] += gx / L0 / m / P;
-    F[nb + s] += gy / L1 / m / P;
-    F[2 * nb + s] += gz / L2 / m / P;
+    const int2 cl = cols[c];
+    const double2 *px = tx + (size_t)cl.x * nb;
+    const double2 *py = ty + (size_t)(MZ + cl.y) * nb;
+    double2 exs[4], eys[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      exs[r] = px[sb[r]];
+      eys[r] = py[sb[r]];
+    }
+    v4d ur = {0, 0, 0, 0}, ui = {0, 0, 0, 0}, vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const double2 w = wb[ks];
+      const double mz = (double)(z0 + 4 * ks + (lane >> 4) - MZ);
+      const double wzr = mz * w.x, wzi = mz * w.y;
+      ur = mfma4(za[ks].x, w.x, ur);
+      ui = mfma4(za[ks].x, w.y, ui);
+      vr = mfma4(za[ks].x, wzr, vr);
+      vi = mfma4(za[ks].x, wzi, vi);
+      ur = mfma4(-za[ks].y, w.y, ur);
+      ui = mfma4(za[ks].y, w.x, ui);
+      vr = mfma4(-za[ks].y, wzi, vr);
+      vi = mfma4(za[ks].y, wzr, vi);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double2 ex = exs[r], ey = eys[r];
+      const double ar = ex.x * ey.x - ex.y * ey.y, ai = ex.x * ey.y + ex.y * ey.x;
+      const double g = fma(ar, ui[r], ai * ur[r]);
+      fx[r] = fma((double)cl.x, g, fx[r]);
+      fy[r] = fma((double)cl.y, g, fy[r]);
+      fz[r] = fma(ar, vi[r], fma(ai, vr[r], fz[r]));
+    }
+  }
+  // sum over the 16 columns held by the lanes of each 16-lane row
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      fx[r] += __shfl_xor(fx[r], o, 64);
+      fy[r] += __shfl_xor(fy[r], o, 64);
+      fz[r] += __shfl_xor(fz[r], o, 64);
+    }
+  if ((lane & 15) == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      red[0][wv][4 * r + beadq] = fx[r];
+      red[1][wv][4 * r + beadq] = fy[r];
+      red[2][wv][4 * r + beadq] = fz[r];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int t = threadIdx.x, bead = b0 + t;
+    if (bead < N) {
+      const int s = slice * N + bead;
+      const double gx = ((red[0][0][t] + red[0][1][t]) + red[0][2][t]) + red[0][3][t];
+      const double gy = ((red[1][0][t] + red[1][1][t]) + red[1][2][t]) + red[1][3][t];
+      const double gz = ((red[2][0][t] + red[2][1][t]) + red[2][2][t]) + red[2][3][t];
+      const double m = Mass[s];
+      F[s] += gx / L0 / m / Pd;
+      F[nb + s] += gy / L1 / m / Pd;
+      F[2 * nb + s] += gz / L2 / m / Pd;
+    }
   }
 }
 
@@ -964,7 +1020,7 @@ static bool ref_converged(const PairParams &pp) {
 // Modelled device time of a split alpha L_min = aL, per slice, from the
 // measured costs of the kernels on MI355X: real space (6 + 2 with third
 // copies) ps per pair plus 2.6 ps per image inside r_c; reciprocal space
-// 1.1 ps per (bead, padded k slot).
+// 0.45 ps per (bead, padded k slot).
 static double split_cost(const PairParams &pp, int N, double aL) {
   const double Lmin = lmin3(pp.L), V = pp.L[0] * pp.L[1] * pp.L[2];
   const double alpha = aL / Lmin, rc = g_ewald_xcut / alpha;
@@ -980,7 +1036,7 @@ static double split_cost(const PairParams &pp, int N, double aL) {
     for (int my = -M; my <= M; ++my)
       if ((mx > 0 || my >= 0) && (mx / pp.L[0]) * (mx / pp.L[0]) + (my / pp.L[1]) * (my / pp.L[1]) <= gmax * gmax)
         ++ncol;
-  return 0.5 * N * (N - 1.0) * per_pair + 1.1 * N * (double)ncol * kz_row(MZ);
+  return 0.5 * N * (N - 1.0) * per_pair + 0.45 * N * (double)((ncol + 15) / 16 * 16) * (2 * MZ + 2);
 }
 
 static bool split_ok(const PairParams &pp, double aL) {
@@ -1062,42 +1118,22 @@ static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t s
 static int launch_recip(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st, bool force,
                         KSpace **out) {
   DevSim *d = p->ds;
-  // bead chunks: enough waves to fill the device, chunks of >= 32 beads
-  KSpace *K = ensure_kspace(pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P, 1);
-  if (!K) return -1;
-  const int nseg = K->ZR / SK_SEG, groups = (K->ncol + 63) / 64;
-  int nch = (4096 + groups * nseg * p->P - 1) / (groups * nseg * p->P);
-  nch = std::max(1, std::min(nch, (p->N + 31) / 32));
-  const int chunk_len = (p->N + nch - 1) / nch;
-  nch = (p->N + chunk_len - 1) / chunk_len;
-  K = ensure_kspace(pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P, nch);
+  KSpace *K = ensure_kspace(pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P);
   if (!K) return -1;
   if (out) *out = K;
-  hipLaunchKernelGGL(k_phase, dim3((3 * d->nb + 255) / 256), dim3(256), 0, st, d->X, d->nb, K->MZ, K->ZR, pp.L[0],
+  hipLaunchKernelGGL(k_phase, dim3((3 * d->nb + 255) / 256), dim3(256), 0, st, d->X, d->nb, K->MZ, K->ZP, pp.L[0],
                      pp.L[1], pp.L[2], K->tx, K->ty, K->tz);
   XMD_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_sk, dim3(groups * nseg, nch, p->P), dim3(64), 0, st, K->tx, K->ty, K->tz, K->cols, K->ncol,
-                     K->MZ, K->ZR, p->N, d->nb, nch, chunk_len, K->Spart);
+  const int nzt = K->ZP / 16;
+  const dim3 gf(((p->N + 15) / 16) * p->P);
+  hipLaunchKernelGGL(k_sk_mfma, dim3((K->ncolp / 16) * nzt * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz, K->cols,
+                     K->coef, K->ncolp, K->MZ, K->ZP, p->N, d->nb, p->P, K->S, K->W);
   XMD_CHECK(hipGetLastError());
-  const int slots = K->ncol * K->ZR, tot = slots * p->P;
-  hipLaunchKernelGGL(k_sk_fin, dim3((tot + 255) / 256), dim3(256), 0, st, K->Spart, K->coef, slots, nch, p->P, K->S,
-                     K->W);
-  XMD_CHECK(hipGetLastError());
-  if (!force) return 0;
-  const dim3 grid((p->N + 63) / 64, p->P);
-#define XMD_RECIP_FORCE(NS)                                                                                      \
-  hipLaunchKernelGGL(k_recip_force<NS>, grid, dim3(64 * NS * rf_parts(NS)), 0, st, K->tx, K->ty, K->tz, K->cols,  \
-                     K->ncol, K->MZ, K->W, d->M, d->F, p->N, d->nb, pp.L[0], pp.L[1], pp.L[2], pp.P)
-  switch (K->ZR / SK_SEG) {
-    case 1: XMD_RECIP_FORCE(1); break;
-    case 2: XMD_RECIP_FORCE(2); break;
-    case 3: XMD_RECIP_FORCE(3); break;
-    case 4: XMD_RECIP_FORCE(4); break;
-    case 5: XMD_RECIP_FORCE(5); break;
-    default: XMD_RECIP_FORCE(6); break;
+  if (force) {
+    hipLaunchKernelGGL(k_recip_force_mfma, gf, dim3(256), 0, st, K->tx, K->ty, K->tz, K->cols, K->ncolp, K->MZ, K->ZP,
+                       K->W, d->M, d->F, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P);
+    XMD_CHECK(hipGetLastError());
   }
-#undef XMD_RECIP_FORCE
-  XMD_CHECK(hipGetLastError());
   return 0;
 }
 
@@ -1162,7 +1198,7 @@ int pair_energy(DevPimd *p, int kind, double *out) {
     KSpace *K = nullptr;
     if (launch_recip(p, pp, pl.cube_n, pl.gmax2, st, false, &K)) return -1;
     const int nb2 = 256;
-    hipLaunchKernelGGL(k_recip_energy, dim3(nb2), dim3(256), 0, st, K->S, K->coef, K->ncol * K->ZR, p->P, p->N,
+    hipLaunchKernelGGL(k_recip_energy, dim3(nb2), dim3(256), 0, st, K->S, K->coef, K->ncolp * K->ZP, p->P, p->N,
                        d->red);
     XMD_CHECK(hipGetLastError());
     double er = 0;
@@ -1201,7 +1237,7 @@ int md_pimd_get_ewald_plan_x(md_pimd_t_x *pimd, md_num_t_x *alpha, md_num_t_x *r
   if (alpha) *alpha = pl.alpha;
   if (r_cut) *r_cut = sqrt(pl.rc2);
   if (n_k) {
-    KSpace *K = ensure_kspace(pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, p->ds->nb, p->P, 1);
+    KSpace *K = ensure_kspace(pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, p->ds->nb, p->P);
     if (!K) return -1;
     *n_k = K->nk;
   }
