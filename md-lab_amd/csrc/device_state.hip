@@ -267,6 +267,8 @@ static int alloc_pimd(DevPimd *p) {
   // 64-row block the in-block (64*63/2) and previous-block (64*64) coefficients
   const size_t nblk = (N + 63) / 64;
   XMD_CHECK(hipMalloc(&p->rec, (6 * (size_t)(N + 1) + 2 * nblk * (64 * 63 / 2 + 64 * 64)) * sizeof(double)));
+  XMD_CHECK(hipMalloc(&p->recsync, (4 * nblk + 1) * sizeof(int)));
+  XMD_CHECK(hipMemset(p->recsync, 0, (4 * nblk + 1) * sizeof(int)));
   return 0;
 }
 

@@ -192,6 +192,7 @@ struct RecArgs {
   double *accM, *accS;  // [2][N+1] off-block log-sum-exp accumulators
   double *dg;         // [2][nblk][RB(RB-1)/2] in-block coefficients
   double *la;         // [2][nblk][RB][RB] coefficients of the block on the previous block
+  int *sync;          // solved[2][nblk], folded[2][nblk], error flag
   int N, nblk;
   double beta, lnxi;
   int xi0;            // xi == 0: only k = 1 terms
@@ -299,6 +300,7 @@ __device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t) {
 __global__ void k_rec_prep(RecArgs a, int dir0) {
   const int dir = dir0 + blockIdx.y;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.y == 0 && t < 4 * a.nblk + 1) a.sync[t] = 0;
   if (dir == 0)
     rec_prep_one<true>(a, 0, t);
   else
@@ -476,6 +478,117 @@ __global__ __launch_bounds__(256) void k_rec_block(RecArgs a, int dir0, int ndir
     rec_update_rows<true>(a, 0, prow, p0 - RB, pm, ps);
   else
     rec_update_rows<false>(a, 1, prow, p0 - RB, pm, ps);
+}
+
+// ---- one persistent launch for the whole solve ----
+//
+// Per direction one solver workgroup (blocks 0, 1, 2, ... in order, with
+// the look-ahead fold) and one updater workgroup per row block G >= 2 (folds
+// blocks 0 .. G-2 into its 64 rows as soon as each is solved).  All of the
+// launch's workgroups are resident from the start, so no block waits for a
+// new dispatch while other kernels fill the device.  Hand-off through
+// device-scope release/acquire flags:
+//   solved[dir][b] = 1       block b's values are in val[]
+//   folded[dir][G] = G - 1   row block G's accumulators hold blocks 0..G-2
+// Every wait is bounded; a timeout raises the error flag (checked by the
+// host at synchronisation points) and lets all waits fall through, so the
+// grid always drains.
+constexpr int REC_SPIN_MAX = 1 << 22;
+
+__device__ void rec_wait(int *flag, int target, int *err) {
+  if (threadIdx.x == 0) {
+    int it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (++it > REC_SPIN_MAX) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+  __threadfence();
+}
+
+__device__ void rec_post(int *flag, int value) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool FWD>
+__device__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[RB], double (*ps)[RB]) {
+  const int N = a.N;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int *solved = a.sync + dir * a.nblk, *folded = a.sync + (2 + dir) * a.nblk, *err = a.sync + 4 * a.nblk;
+  const double *val = a.val2 + (size_t)dir * (N + 1);
+  double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  const int p = 1 + G * RB + lane;
+  const bool act = p <= N;
+  double Ma = NEG, Sa = 0.0;
+  if (wv == 0 && act) {
+    Ma = AM[p];
+    Sa = AS[p];
+  }
+  for (int b = 0; b <= G - 2; ++b) {
+    rec_wait(solved + b, 1, err);
+    const int pprev = 1 + b * RB;
+    double t[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int pp = pprev + 16 * wv + j;
+      double v = NEG;
+      if (act) {
+        const double c = rec_coef_col<FWD>(a, p, pp);
+        if (c > NEG) v = c + val[pp];
+      }
+      t[j] = v;
+    }
+    double M, S;
+    lse16(t, M, S);
+    pm[wv][lane] = M;
+    ps[wv][lane] = S;
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) lse_pair(Ma, Sa, pm[w][lane], ps[w][lane]);
+    }
+    __syncthreads();
+  }
+  if (wv == 0 && act) {
+    AM[p] = Ma;
+    AS[p] = Sa;
+  }
+  rec_post(folded + G, G - 1);
+}
+
+template <bool FWD>
+__device__ void rec_solver(const RecArgs &a, int dir, double *dg, double *mo, double *so, double *sg,
+                           double (*pm)[RB], double (*ps)[RB]) {
+  int *solved = a.sync + dir * a.nblk, *folded = a.sync + (2 + dir) * a.nblk, *err = a.sync + 4 * a.nblk;
+  for (int b = 0; b < a.nblk; ++b) {
+    if (b >= 2) rec_wait(folded + b, b - 1, err);
+    rec_diag_body<FWD>(a, dir, b, dg, mo, so, sg, pm, ps);
+    rec_post(solved + b, 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rec_persist(RecArgs a, int dir0) {
+  __shared__ double dg[RTRI], mo[RB], so[RB], sg[RB], pm[4][RB], ps[4][RB];
+  const int per = 1 + max(0, a.nblk - 2);
+  const int dir = dir0 + blockIdx.x / per, role = blockIdx.x % per;
+  if (role == 0) {
+    if (dir == 0)
+      rec_solver<true>(a, 0, dg, mo, so, sg, pm, ps);
+    else
+      rec_solver<false>(a, 1, dg, mo, so, sg, pm, ps);
+  } else {
+    if (dir == 0)
+      rec_updater<true>(a, 0, role + 1, pm, ps);
+    else
+      rec_updater<false>(a, 1, role + 1, pm, ps);
+  }
 }
 
 __global__ void k_lnn(double *lnn, int N) {
@@ -796,6 +909,7 @@ static RecArgs rec_args(DevPimd *p) {
   a.nblk = (p->N + RB - 1) / RB;
   a.dg = p->rec + 6 * (p->N + 1);
   a.la = a.dg + 2 * (size_t)a.nblk * RTRI;
+  a.sync = p->recsync;
   a.N = p->N;
   a.beta = p->host->beta;
   a.xi0 = p->host->vi == 0.0;
@@ -825,6 +939,24 @@ int launch_enk(DevPimd *p, int image, hipStream_t st) {
 }
 
 
+// One persistent launch (default) or one launch per 64-position block.
+int g_rec_persistent = 1;
+DevPimd *g_rec_last = nullptr;
+
+// Error flag of the persistent solve (timed-out hand-off); call after a
+// synchronisation.
+int recursion_status(DevPimd *p) {
+  if (!p || !p->recsync) return 0;
+  const int nblk = (p->N + RB - 1) / RB;
+  int err = 0;
+  XMD_CHECK(hipMemcpy(&err, p->recsync + 4 * nblk, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) {
+    fprintf(stderr, "xmd: exchange recursion hand-off timed out\n");
+    return -1;
+  }
+  return 0;
+}
+
 int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
   const int ndir = (fwd ? 1 : 0) + (bwd ? 1 : 0);
   if (ndir == 0) return 0;
@@ -834,11 +966,18 @@ int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
   const int nprep = std::max(N + 1, a.nblk * RB * RB);
   hipLaunchKernelGGL(k_rec_prep, dim3((nprep + 255) / 256, ndir), dim3(256), 0, st, a, dir0);
   XMD_CHECK(hipGetLastError());
-  for (int b = 0; b < a.nblk; ++b) {
-    const int later = b > 0 ? std::max(0, N - (1 + (b + 1) * RB) + 1) : 0;  // rows after block b
-    const int grid = ndir + ndir * ((later + RB - 1) / RB);
-    hipLaunchKernelGGL(k_rec_block, dim3(grid), dim3(256), 0, st, a, dir0, ndir, b);
+  if (g_rec_persistent) {
+    const int per = 1 + std::max(0, a.nblk - 2);
+    hipLaunchKernelGGL(k_rec_persist, dim3(ndir * per), dim3(256), 0, st, a, dir0);
     XMD_CHECK(hipGetLastError());
+    g_rec_last = p;
+  } else {
+    for (int b = 0; b < a.nblk; ++b) {
+      const int later = b > 0 ? std::max(0, N - (1 + (b + 1) * RB) + 1) : 0;  // rows after block b
+      const int grid = ndir + ndir * ((later + RB - 1) / RB);
+      hipLaunchKernelGGL(k_rec_block, dim3(grid), dim3(256), 0, st, a, dir0, ndir, b);
+      XMD_CHECK(hipGetLastError());
+    }
   }
   if (fwd) p->vbn_dev = true;
   if (bwd) {

@@ -1017,15 +1017,16 @@ static bool ref_converged(const PairParams &pp) {
   return a * a >= 30.0 && pp.alpha * n * lmin3(pp.L) >= 6.0;
 }
 
-// Modelled device time of a split alpha L_min = aL, per slice, from the
-// measured costs of the kernels on MI355X: real space (6 + 2 with third
-// copies) ps per pair plus 2.6 ps per image inside r_c; reciprocal space
-// 0.45 ps per (bead, padded k slot).
-static double split_cost(const PairParams &pp, int N, double aL) {
+// Modelled device time (ps) of a split alpha L_min = aL, fitted to the
+// kernels' measured times on MI355X at N = 1024, P = 32: real space 4 ps per
+// pair plus 3.7 ps per image inside r_c (11 + 2.1 when r_c >= L admits third
+// copies); reciprocal space 0.55 ps per (bead, padded k slot) plus a fixed
+// 30 us per 32 slices.
+static double split_cost(const PairParams &pp, int N, int P, double aL) {
   const double Lmin = lmin3(pp.L), V = pp.L[0] * pp.L[1] * pp.L[2];
   const double alpha = aL / Lmin, rc = g_ewald_xcut / alpha;
   const double images = 4.0 / 3.0 * M_PI * rc * rc * rc / V;
-  const double per_pair = (rc >= Lmin ? 8.0 : 6.0) + 2.6 * images;
+  const double per_pair = rc >= Lmin ? 11.0 + 2.1 * images : 4.0 + 3.7 * images;
   const double gmax = g_ewald_xcut * alpha / M_PI;
   int M = 0;
   for (int d = 0; d < 3; ++d) M = std::max(M, (int)floor(gmax * pp.L[d]));
@@ -1036,7 +1037,8 @@ static double split_cost(const PairParams &pp, int N, double aL) {
     for (int my = -M; my <= M; ++my)
       if ((mx > 0 || my >= 0) && (mx / pp.L[0]) * (mx / pp.L[0]) + (my / pp.L[1]) * (my / pp.L[1]) <= gmax * gmax)
         ++ncol;
-  return 0.5 * N * (N - 1.0) * per_pair + 0.45 * N * (double)((ncol + 15) / 16 * 16) * (2 * MZ + 2);
+  const double slots = (double)((ncol + 15) / 16 * 16) * (2 * MZ + 2);
+  return P * (0.5 * N * (N - 1.0) * per_pair + 0.55 * N * slots) + 30e6 * P / 32.0;
 }
 
 static bool split_ok(const PairParams &pp, double aL) {
@@ -1045,7 +1047,7 @@ static bool split_ok(const PairParams &pp, double aL) {
   return rc * rc < 2.0 * Lmin * Lmin && g_ewald_xcut <= 6.5 && gmax * lmax3(pp.L) <= MAX_KM;
 }
 
-static EwaldPlan plan_ewald(const PairParams &pp, int N) {
+static EwaldPlan plan_ewald(const PairParams &pp, int N, int P) {
   EwaldPlan e;
   e.alpha = pp.alpha;
   e.rc2 = pp.rc2;
@@ -1057,7 +1059,7 @@ static EwaldPlan plan_ewald(const PairParams &pp, int N) {
     for (int h = 18; h <= 48; ++h) {  // alpha L_min = 4.5 .. 12 in steps of 1/4
       const double c = 0.25 * h;
       if (!split_ok(pp, c)) continue;
-      const double w = split_cost(pp, N, c);
+      const double w = split_cost(pp, N, P, c);
       if (aL < 0 || w < best) {
         best = w;
         aL = c;
@@ -1079,7 +1081,7 @@ static EwaldPlan plan_ewald(const PairParams &pp, int N) {
 // Parameters of the full Ewald interaction under its plan.
 static PairParams ewald_params(DevPimd *p, EwaldPlan &pl) {
   PairParams pp = make_params(p);
-  pl = plan_ewald(pp, p->N);
+  pl = plan_ewald(pp, p->N, p->P);
   if (pl.own_split) {
     pp.alpha = pl.alpha;
     pp.rc2 = pl.rc2;

@@ -130,7 +130,7 @@ int md_hip_synchronize_x(void) {
   if (ensure_runtime()) return -1;
   XMD_CHECK(hipStreamSynchronize(rt().s1));
   XMD_CHECK(hipStreamSynchronize(rt().s0));
-  return 0;
+  return recursion_status(g_rec_last);
 }
 
 int md_pimd_get_backward_potentials_x(md_pimd_t_x *pimd, md_num_t_x *Vt) {
@@ -180,7 +180,7 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   XMD_CHECK(hipStreamSynchronize(R.s0));
   XMD_CHECK(hipGraphExecDestroy(exec));
   XMD_CHECK(hipGraphDestroy(graph));
-  return 0;
+  return recursion_status(sp.p);
 }
 
 int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, int serial,
@@ -196,6 +196,7 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
   for (int s = 0; s < nsteps; ++s) {
     if (enqueue_step(sp, ev, serial != 0)) return -1;
     XMD_CHECK(hipStreamSynchronize(rt().s0));
+    if (recursion_status(sp.p)) return -1;
     for (int k = 0; k < 8; ++k) {
       float ms = 0;
       XMD_CHECK(hipEventElapsedTime(&ms, ev[pairs[k][0]], ev[pairs[k][1]]));

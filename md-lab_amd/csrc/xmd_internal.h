@@ -100,7 +100,8 @@ struct DevPimd {
   double *Rb = nullptr;     // backward R(m) = -beta Vt^(m),  N+1
   double *Gm = nullptr;     // optional connection matrix (debug/tests), N*N
   double *work = nullptr;   // estimator scratch
-  double *rec = nullptr;    // recursion state: values and accumulators, 6*(N+1)
+  double *rec = nullptr;    // recursion state: values, accumulators, block coefficients
+  int *recsync = nullptr;   // recursion hand-off flags: 4*nblk + 1
   bool enk_dev = false, enk2_dev = false, vbn_dev = false, back_dev = false;
   bool enk_host_stale = false;
   unsigned enk_version = 0, back_version = 0;
@@ -150,6 +151,9 @@ int backward_potentials(DevPimd *p, double *host_Vt);
 int bead_sum(DevSim *d, int mode, const double *trap_c, double w, const double *L, double *out);
 extern double g_ewald_xcut;
 extern double g_ewald_split_aL;
+extern int g_rec_persistent;
+extern DevPimd *g_rec_last;
+int recursion_status(DevPimd *p);
 
 
 }  // namespace xmd
