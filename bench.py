@@ -44,6 +44,22 @@ def expected_images(alpha, L, xcut):
     return 4.0 / 3.0 * math.pi * rc ** 3 / L ** 3
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (profiles/hbm_traffic.json),
+    or None when that file holds no entry for this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in table.get("entries", []):
+        if e.get("kernel") == kernel and e.get("workload") == workload:
+            return e.get("hbm_bytes_per_launch")
+    return None
+
+
 def cpu_baseline(N, P, rs, theta, xi, nsum, pairs):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
     if not os.path.exists(exe):
@@ -137,6 +153,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(N, P, rs, theta, xi, nsum, args.cpu_pairs)
 
+    traffic_key = f"UEG N={N} P={P} alpha'L={alpha * L:.2f}"
+    traffic = measured_traffic("k_ewald_real", traffic_key)
     line = {
         "metric": METRIC, "value": value, "unit": "particle-bead-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -147,7 +165,7 @@ def main():
                    "path": "plain FP64 (no surrogate)", "energy_error_vs_fp64_ewald": 0.0,
                    "hipgraph": True, "finite_state": finite},
         "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "k_ewald_real (Ewald real space)", "avg_ms": real_ms,
                      "work_flop_eq": work},
         "stages_ms": {k: round(float(v), 5) for k, v in zip(names, stages)},
