@@ -41,6 +41,15 @@ int md_set_ewald_split_x(md_num_t_x alpha_L);
  * real-space cutoff and the number of half-space k vectors (any may be NULL). */
 int md_pimd_get_ewald_plan_x(md_pimd_t_x *pimd, md_num_t_x *alpha, md_num_t_x *r_cut, int *n_k);
 
+/* Lossless checkpoint / restart of a simulation's dynamical state: the x,
+ * v, f and nhc blocks in the layout of md_fprint_particle_pos_x / _vel_x /
+ * _force_x / md_fprint_nhcs_x (md_util_x.c:838-915), every number a C99
+ * hexadecimal float ("%a"), so a run resumes bit for bit and the reference's
+ * own md_fread_* readers parse the file.  Reading hands authority back to the
+ * host mirror (the next device call uploads it). */
+int md_checkpoint_write_x(FILE *out, md_simulation_t_x *sim);
+int md_checkpoint_read_x(FILE *in, md_simulation_t_x *sim);
+
 /* Internal exchange quantities after md_pimd_fill_VB_x (tests, diagnostics):
  * backward potentials Vt^(0..N) and the connection-probability matrix
  * G[(l-1)*N + (j-1)] = P(bead P of l links to bead 1 of j). */

@@ -76,6 +76,22 @@ int md_pimd_calc_pair_force_x(md_pimd_t_x *pimd, md_pair_force_t_x pf);
 int md_pimd_calc_VBN_energy_x(md_pimd_t_x *pimd, md_stats_t_x *stats);
 int md_pimd_calc_trap_energy_x(md_pimd_t_x *pimd, md_trap_energy_t_x te, md_stats_t_x *stats);
 int md_pimd_calc_pair_energy_x(md_pimd_t_x *pimd, md_pair_energy_t_x pe, md_stats_t_x *stats);
+/* Pair forces / energy between two species with the same P (md_pimd_x.h:146-147,
+ * md_pimd_x.c:3282-3510): every bead of pimd against every bead of pimd2 in
+ * the same slice; device kernels for the Gaussian callbacks. */
+/* Per-sample observables into stats->fxs[stats->N-1][0..points): density
+ * distribution around center (md_pimd_x.h:135), pair correlation of bead 1
+ * against bead pi (md_pimd_x.h:143), structure factor S(q), q = q0 + j qincre
+ * (md_pimd_x.h:185); nocl semantics of md_pimd_x.c:1826-1967, 2974-3040,
+ * 4958-5000. */
+int md_pimd_calc_density_distribution_x(md_pimd_t_x *pimd, md_stats_t_x *stats, md_num_t_x rmax, int image,
+                                        md_num_t_x *center);
+int md_pimd_calc_pair_correlation_x(md_pimd_t_x *pimd, md_stats_t_x *stats, md_num_t_x rmax, md_num_t_x norm,
+                                    int pi, int image);
+int md_pimd_calc_Sk_structure_x(md_pimd_t_x *pimd, md_stats_t_x *stats, md_num_t_x q0, md_num_t_x qincre, int pi);
+
+int md_pimd_calc_pair_force_2_x(md_pimd_t_x *pimd, md_pimd_t_x *pimd2, md_pair_force_t_x pf);
+int md_pimd_calc_pair_energy_2_x(md_pimd_t_x *pimd, md_pimd_t_x *pimd2, md_pair_energy_t_x pe, md_stats_t_x *stats);
 int md_pimd_calc_virial_energy_x(md_pimd_t_x *pimd, md_stats_t_x *stats);
 int md_pimd_calc_centroid_virial_energy_x(md_pimd_t_x *pimd, md_stats_t_x *stats, int fc);
 int md_pimd_fill_ENk2_x(md_pimd_t_x *pimd, int image);

@@ -398,12 +398,13 @@ int md_simulation_copy_to_x(md_simulation_t_x *sim, md_simulation_t_x *sim2) {
 
 // ------------------------------------------------------------ text I/O
 // Formats of md_util_x.c:838-1011 ("%f", six decimals: lossy by design).
-static int print_vec(FILE *out, md_simulation_t_x *sim, const char *tag, int which) {
+// The checkpoint pair below writes the same layout with "%a" (exact).
+static int print_vec(FILE *out, md_simulation_t_x *sim, const char *tag, int which, const char *fmt = "%f ") {
   fprintf(out, "%s %d %d\n", tag, sim->N, MD_DIMENSION_X);
   if (!md_simulation_sync_host_x(sim, 1)) return -1;
   for (int i = 0; i < sim->N; ++i) {
     const double *v = which == 0 ? sim->particles[i].x : (which == 1 ? sim->particles[i].v : sim->particles[i].f);
-    for (int d = 0; d < MD_DIMENSION_X; ++d) fprintf(out, "%f ", v[d]);
+    for (int d = 0; d < MD_DIMENSION_X; ++d) fprintf(out, fmt, v[d]);
     fprintf(out, "\n");
   }
   return ferror(out);
@@ -412,21 +413,23 @@ int md_fprint_particle_pos_x(FILE *out, md_simulation_t_x *sim) { return print_v
 int md_fprint_particle_vel_x(FILE *out, md_simulation_t_x *sim) { return print_vec(out, sim, "v", 1); }
 int md_fprint_particle_force_x(FILE *out, md_simulation_t_x *sim) { return print_vec(out, sim, "f", 2); }
 
-int md_fprint_nhcs_x(FILE *out, md_simulation_t_x *sim) {
+static int print_nhcs(FILE *out, md_simulation_t_x *sim, const char *fmt) {
   fprintf(out, "nhc %d %d\n", sim->Nf, MD_NHC_LENGTH_X);
   if (!md_simulation_sync_host_x(sim, 1)) return -1;
   for (int i = 0; i < sim->Nf; ++i) {
     const md_nhc_t_x *c = &sim->nhcs[i];
     fprintf(out, "%d\n", c->f);
-    for (int j = 0; j < MD_NHC_LENGTH_X; ++j) fprintf(out, "%f ", c->theta[j]);
+    for (int j = 0; j < MD_NHC_LENGTH_X; ++j) fprintf(out, fmt, c->theta[j]);
     fprintf(out, "\n");
-    for (int j = 0; j < MD_NHC_LENGTH_X; ++j) fprintf(out, "%f ", c->vtheta[j]);
+    for (int j = 0; j < MD_NHC_LENGTH_X; ++j) fprintf(out, fmt, c->vtheta[j]);
     fprintf(out, "\n");
-    for (int j = 0; j < MD_NHC_LENGTH_X; ++j) fprintf(out, "%f ", c->Q[j]);
+    for (int j = 0; j < MD_NHC_LENGTH_X; ++j) fprintf(out, fmt, c->Q[j]);
     fprintf(out, "\n");
   }
   return ferror(out);
 }
+
+int md_fprint_nhcs_x(FILE *out, md_simulation_t_x *sim) { return print_nhcs(out, sim, "%f "); }
 
 int md_fprint_sim_box_x(FILE *out, simulation_box_t_x *box) {
   fprintf(out, "box %d %d\n", box->type, MD_DIMENSION_X);
@@ -454,7 +457,7 @@ static int read_vec(FILE *in, md_simulation_t_x *sim, int which) {
     for (int d = 0; d < MD_DIMENSION_X; ++d) {
       double v;
       if (fscanf(in, "%lf", &v) != 1) return -1;
-      (which == 0 ? sim->particles[i].x : sim->particles[i].v)[d] = v;
+      (which == 0 ? sim->particles[i].x : (which == 1 ? sim->particles[i].v : sim->particles[i].f))[d] = v;
     }
   return ferror(in);
 }
@@ -477,6 +480,23 @@ int md_fread_nhcs_x(FILE *in, md_simulation_t_x *sim) {
       }
   }
   return ferror(in);
+}
+
+// Lossless checkpoint: the x, v, f and nhc blocks of the reference's text
+// layout (md_util_x.c:838-915), every number a C99 hexadecimal float.  The
+// reference's readers (fscanf "%lf") parse it too.
+int md_checkpoint_write_x(FILE *out, md_simulation_t_x *sim) {
+  if (!out || !sim) return -1;
+  if (print_vec(out, sim, "x", 0, "%a ") || print_vec(out, sim, "v", 1, "%a ") || print_vec(out, sim, "f", 2, "%a ") ||
+      print_nhcs(out, sim, "%a "))
+    return -1;
+  return fflush(out) ? -1 : 0;
+}
+
+int md_checkpoint_read_x(FILE *in, md_simulation_t_x *sim) {
+  if (!in || !sim) return -1;
+  if (read_vec(in, sim, 0) || read_vec(in, sim, 1) || read_vec(in, sim, 2) || md_fread_nhcs_x(in, sim)) return -1;
+  return 0;
 }
 
 // ------------------------------------------------------- key/value input

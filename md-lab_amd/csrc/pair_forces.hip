@@ -1215,7 +1215,83 @@ int pair_energy(DevPimd *p, int kind, double *out) {
   return 0;
 }
 
+// ------------------------------------------------------- two species
+//
+// md_pimd_x.c:3282-3510 (nocl): every pair of bead (l, j) of species A and
+// bead (l2, j) of species B in the same slice j,
+//   f_A += F(x_A, x_B) / m_A / P,   f_B -= m_A (F / m_A) / P / m_B,
+//   E   += e(x_A, x_B) / P,
+// with F, e the pair callback's force and energy.  Two passes (lane = bead
+// of A, then lane = bead of B), each summing its partners of the slice in a
+// fixed order: deterministic without atomics.
+template <int MODE, bool ON_B>
+__global__ __launch_bounds__(256) void k_pair2_force(const double *__restrict__ XA, const double *__restrict__ MA,
+                                                     double *__restrict__ FA, int NA, int nbA,
+                                                     const double *__restrict__ XB, const double *__restrict__ MB,
+                                                     double *__restrict__ FB, int NB, int nbB, PairParams pp) {
+  const int slice = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Nt = ON_B ? NB : NA, No = ON_B ? NA : NB;
+  if (i >= Nt) return;
+  const double *Xt = ON_B ? XB : XA, *Xo = ON_B ? XA : XB;
+  const int nbt = ON_B ? nbB : nbA, nbo = ON_B ? nbA : nbB;
+  const int st = slice * Nt + i;
+  const double xt = Xt[st], yt = Xt[nbt + st], zt = Xt[2 * nbt + st];
+  double gx = 0, gy = 0, gz = 0, e = 0;
+  for (int j = 0; j < No; ++j) {
+    const int so = slice * No + j;
+    const double xo = Xo[so], yo = Xo[nbo + so], zo = Xo[2 * nbo + so];
+    // force on the A bead of the pair, from the B bead
+    if (ON_B)
+      pair_term<MODE, false, true>(xo - xt, yo - yt, zo - zt, pp, gx, gy, gz, e);
+    else
+      pair_term<MODE, false, true>(xt - xo, yt - yo, zt - zo, pp, gx, gy, gz, e);
+  }
+  double *Ft = ON_B ? FB : FA;
+  const double s = ON_B ? -1.0 / (pp.P * MB[st]) : 1.0 / (MA[st] * pp.P);
+  Ft[st] += gx * s;
+  Ft[nbt + st] += gy * s;
+  Ft[2 * nbt + st] += gz * s;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pair2_energy(const double *__restrict__ XA, int NA, int nbA,
+                                                      const double *__restrict__ XB, int NB, int nbB, PairParams pp,
+                                                      double *__restrict__ part) {
+  __shared__ double red[256];
+  const int slice = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (i < NA) {
+    const int sa = slice * NA + i;
+    const double xa = XA[sa], ya = XA[nbA + sa], za = XA[2 * nbA + sa];
+    double gx, gy, gz;
+    for (int j = 0; j < NB; ++j) {
+      const int sb = slice * NB + j;
+      pair_term<MODE, true, true>(xa - XB[sb], ya - XB[nbB + sb], za - XB[2 * nbB + sb], pp, gx, gy, gz, e);
+    }
+  }
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+static int pair2_check(md_pimd_t_x *a, md_pimd_t_x *b, int kind) {
+  if (a->P != b->P) return -1;
+  if (kind != PK_GAUSS && kind != PK_PGAUSS) {
+    fprintf(stderr, "xmd: two-species pair kernels cover the Gaussian interactions only\n");
+    return -1;
+  }
+  if (ensure_pimd_device(a) || ensure_pimd_device(b)) return -1;
+  return 0;
+}
+
 }  // namespace xmd
+
 
 using namespace xmd;
 
@@ -1243,6 +1319,56 @@ int md_pimd_get_ewald_plan_x(md_pimd_t_x *pimd, md_num_t_x *alpha, md_num_t_x *r
     if (!K) return -1;
     *n_k = K->nk;
   }
+  return 0;
+}
+
+int md_pimd_calc_pair_force_2_x(md_pimd_t_x *pimd, md_pimd_t_x *pimd2, md_pair_force_t_x pf) {
+  const int kind = pair_kind_of(pf);
+  if (pair2_check(pimd, pimd2, kind)) return -1;
+  DevPimd *a = pimd_state(pimd, false), *b = pimd_state(pimd2, false);
+  PairParams pp = make_params(a);
+  hipStream_t st = rt().s0;
+  DevSim *da = a->ds, *db = b->ds;
+  const dim3 ga((a->N + 255) / 256, a->P), gb((b->N + 255) / 256, b->P);
+  if (kind == PK_GAUSS) {
+    hipLaunchKernelGGL((k_pair2_force<PK_GAUSS, false>), ga, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb,
+                       db->X, db->M, db->F, b->N, db->nb, pp);
+    hipLaunchKernelGGL((k_pair2_force<PK_GAUSS, true>), gb, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb, db->X,
+                       db->M, db->F, b->N, db->nb, pp);
+  } else {
+    hipLaunchKernelGGL((k_pair2_force<PK_PGAUSS, false>), ga, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb,
+                       db->X, db->M, db->F, b->N, db->nb, pp);
+    hipLaunchKernelGGL((k_pair2_force<PK_PGAUSS, true>), gb, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb,
+                       db->X, db->M, db->F, b->N, db->nb, pp);
+  }
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+int md_pimd_calc_pair_energy_2_x(md_pimd_t_x *pimd, md_pimd_t_x *pimd2, md_pair_energy_t_x pe,
+                                 md_stats_t_x *stats) {
+  const int kind = pair_energy_kind_of(pe);
+  if (pair2_check(pimd, pimd2, kind)) return -1;
+  DevPimd *a = pimd_state(pimd, false), *b = pimd_state(pimd2, false);
+  PairParams pp = make_params(a);
+  hipStream_t st = rt().s0;
+  DevSim *da = a->ds, *db = b->ds;
+  const dim3 g((a->N + 255) / 256, a->P);
+  const int nblk = g.x * g.y;
+  double *part = nullptr;
+  XMD_CHECK(hipMallocAsync((void **)&part, nblk * sizeof(double), st));
+  if (kind == PK_GAUSS)
+    hipLaunchKernelGGL(k_pair2_energy<PK_GAUSS>, g, dim3(256), 0, st, da->X, a->N, da->nb, db->X, b->N, db->nb, pp,
+                       part);
+  else
+    hipLaunchKernelGGL(k_pair2_energy<PK_PGAUSS>, g, dim3(256), 0, st, da->X, a->N, da->nb, db->X, b->N, db->nb, pp,
+                       part);
+  XMD_CHECK(hipGetLastError());
+  double e = 0;
+  const int rc = device_sum(part, nblk, &e);
+  XMD_CHECK(hipFreeAsync(part, st));
+  if (rc) return -1;
+  stats->es[stats->N - 1] += e / a->P;
   return 0;
 }
 

@@ -134,6 +134,13 @@ _SIGS = {
     "md_set_ewald_real_cutoff_x": (C.c_int, [C.c_double]),
     "md_set_ewald_split_x": (C.c_int, [C.c_double]),
     "md_pimd_get_ewald_plan_x": (C.c_int, [PIMD_P, DP, DP, C.POINTER(C.c_int)]),
+    "md_checkpoint_write_x": (C.c_int, [C.c_void_p, SIM_P]),
+    "md_pimd_calc_pair_force_2_x": (C.c_int, [PIMD_P, PIMD_P, C.c_void_p]),
+    "md_pimd_calc_density_distribution_x": (C.c_int, [PIMD_P, STATS_P, C.c_double, C.c_int, DP]),
+    "md_pimd_calc_pair_correlation_x": (C.c_int, [PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int, C.c_int]),
+    "md_pimd_calc_Sk_structure_x": (C.c_int, [PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int]),
+    "md_pimd_calc_pair_energy_2_x": (C.c_int, [PIMD_P, PIMD_P, C.c_void_p, STATS_P]),
+    "md_checkpoint_read_x": (C.c_int, [C.c_void_p, SIM_P]),
     "md_pimd_get_backward_potentials_x": (C.c_int, [PIMD_P, DP]),
     "md_pimd_get_connection_matrix_x": (C.c_int, [PIMD_P, DP]),
     "md_pimd_compute_forces_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig)]),

@@ -231,6 +231,87 @@ void orc_pair_force(int kind, const orc_params_t *p, int N, int P, const double 
       }
 }
 
+/* md_pimd_x.c:3452-3467 (two species, same P): bead (l, j) of a against
+ * bead (l2, j) of c; force on a divided by its mass (by the callback), the
+ * reaction m_a f / P / m_c on c. */
+void orc_pair_force_2(int kind, const orc_params_t *p, int NA, int NB, int P, const double *L, orc_particle_t *a,
+                      orc_particle_t *c) {
+  for (int j = 1; j <= P; ++j)
+    for (int l = 1; l <= NA; ++l)
+      for (int l2 = 1; l2 <= NB; ++l2) {
+        orc_particle_t *u = &a[BEAD(l, j, P)], *w = &c[BEAD(l2, j, P)];
+        double f[3] = {0, 0, 0};
+        orc_pair_force_one(kind, p, u->x, w->x, f, L, u->m);
+        for (int i = 0; i < 3; ++i) {
+          u->f[i] += f[i] / P;
+          w->f[i] -= u->m * f[i] / P / w->m;
+        }
+      }
+}
+
+/* md_pimd_x.c:3614-3621 */
+double orc_pair_energy_2(int kind, const orc_params_t *p, int NA, int NB, int P, const double *L,
+                         const orc_particle_t *a, const orc_particle_t *c) {
+  double e = 0;
+  for (int j = 1; j <= P; ++j)
+    for (int l = 1; l <= NA; ++l)
+      for (int l2 = 1; l2 <= NB; ++l2)
+        e += orc_pair_energy_one(kind, p, a[BEAD(l, j, P)].x, c[BEAD(l2, j, P)].x, L) / P;
+  return e;
+}
+
+/* ------------------------------------------------------------ observables */
+
+static double obs_dist(const double *a, const double *b, const double *L, int image) {
+  return image ? mi_dist(a, b, L) : dist(a, b);
+}
+
+/* md_pimd_x.c:1944-1962: histogram of bead distances to center, += 1/P. */
+void orc_density_distribution(int N, int P, const orc_particle_t *b, const double *L, int image,
+                              const double *center, double rmax, int points, double *fx) {
+  double incre = rmax / points;
+  for (int j = 1; j <= P; ++j)
+    for (int l = 1; l <= N; ++l) {
+      double r = obs_dist(b[BEAD(l, j, P)].x, center, L, image);
+      int k = (int)(r / incre);
+      if (k >= points) continue;
+      fx[k] += 1.0 / P;
+    }
+}
+
+/* md_pimd_x.c:3019-3036: bead 1 of l against bead pi of every j != l, += 1/norm. */
+void orc_pair_correlation(int N, int P, const orc_particle_t *b, const double *L, int image, double rmax,
+                          double norm, int pi, int points, double *fx) {
+  double incre = rmax / points;
+  for (int l = 1; l <= N; ++l)
+    for (int j = 1; j <= N; ++j) {
+      if (l == j) continue;
+      double r = obs_dist(b[BEAD(l, 1, P)].x, b[BEAD(j, pi, P)].x, L, image);
+      int k = (int)(r / incre);
+      if (k >= points) continue;
+      fx[k] += 1.0 / norm;
+    }
+}
+
+/* md_pimd_x.c:4967-4996 */
+void orc_sk_structure(int N, int P, const orc_particle_t *b, double q0, double qincre, int pi, int points,
+                      double *fx) {
+  for (int j = 0; j < points; ++j) {
+    double q = q0 + j * qincre, res = 0;
+    for (int i = 0; i < 3; ++i) {
+      double sum = 0, sum2 = 0;
+      for (int l = 1; l <= N; ++l) sum += cos(q * b[BEAD(l, 1, P)].x[i]);
+      for (int l = 1; l <= N; ++l) sum2 += cos(q * b[BEAD(l, pi, P)].x[i]);
+      res += sum * sum2 / N;
+      sum = sum2 = 0;
+      for (int l = 1; l <= N; ++l) sum += sin(q * b[BEAD(l, 1, P)].x[i]);
+      for (int l = 1; l <= N; ++l) sum2 += sin(q * b[BEAD(l, pi, P)].x[i]);
+      res += sum * sum2 / N;
+    }
+    fx[j] += res / 3;
+  }
+}
+
 /* md_pimd_x.c:1518-1522 with md_harmonic_trap_energy_x (md_statistics_x.c:322-325) */
 double orc_trap_energy(const orc_params_t *p, int N, int P, const orc_particle_t *b) {
   double e = 0;

@@ -82,6 +82,16 @@ void orc_pair_force(int kind, const orc_params_t *p, int N, int P, const double 
 double orc_trap_energy(const orc_params_t *p, int N, int P, const orc_particle_t *b);
 double orc_pair_energy(int kind, const orc_params_t *p, int N, int P, const double *L,
                        const orc_particle_t *b);
+void orc_pair_force_2(int kind, const orc_params_t *p, int NA, int NB, int P, const double *L, orc_particle_t *a,
+                      orc_particle_t *c);
+double orc_pair_energy_2(int kind, const orc_params_t *p, int NA, int NB, int P, const double *L,
+                         const orc_particle_t *a, const orc_particle_t *c);
+void orc_density_distribution(int N, int P, const orc_particle_t *b, const double *L, int image,
+                              const double *center, double rmax, int points, double *fx);
+void orc_pair_correlation(int N, int P, const orc_particle_t *b, const double *L, int image, double rmax,
+                          double norm, int pi, int points, double *fx);
+void orc_sk_structure(int N, int P, const orc_particle_t *b, double q0, double qincre, int pi, int points,
+                      double *fx);
 
 /* Exchange.  ENk is a packed triangle: E_n^(k) at (n-1)n/2 + k-1. */
 void orc_fast_fill_ENk(int N, int P, double omegaP, int image, const double *L,

@@ -297,6 +297,82 @@ static void rng_case(void) {
   dump("rng/uniform_pos", x, 45);
 }
 
+/* Two species with the same P (md_pimd_calc_pair_force_2_x / _energy_2_x):
+ * A = 5 particles (mass 1), B = 3 particles (mass 2), P = 4, positions
+ * Normal(0,1) in particle, bead, dimension order (A then B); Gaussian and
+ * periodic Gaussian (box 3 x 3.5 x 4). */
+static void two_species_case(void) {
+  const int NA = 5, NB = 3, P = 4;
+  md_num_t_x L3[3] = {3.0, 3.5, 4.0}, zero[3] = {0, 0, 0};
+  md_set_gaussian_force_strength_x(3.0);
+  md_set_guassian_force_range_x(0.5);
+  md_set_harmonic_trap_center_x(zero);
+  md_set_seed_x(21);
+  md_simulation_t_x *sa = md_alloc_simulation_x(NA * P, 1, 0, MD_ND_RECT_BOX_X);
+  md_simulation_t_x *sb = md_alloc_simulation_x(NB * P, 1, 0, MD_ND_RECT_BOX_X);
+  md_init_nd_rect_box_x(sa->box, L3);
+  md_init_nd_rect_box_x(sb->box, L3);
+  md_pimd_t_x *pa = md_alloc_pimd_x(sa, NA, P, 1.0, 1.0);
+  md_pimd_t_x *pb = md_alloc_pimd_x(sb, NB, P, 1.0, 1.0);
+  md_init_particle_mass_x(sa, 1.0, 0, NA * P);
+  md_init_particle_mass_x(sb, 2.0, 0, NB * P);
+  for (int i = 0; i < NA * P; ++i)
+    for (int d = 0; d < 3; ++d) sa->particles[i].x[d] = md_random_gaussian_x();
+  for (int i = 0; i < NB * P; ++i)
+    for (int d = 0; d < 3; ++d) sb->particles[i].x[d] = md_random_gaussian_x();
+  dump_beads("two", "A0", sa);
+  dump_beads("two", "B0", sb);
+  md_pair_force_t_x pfs[2] = {md_gaussian_force_x, md_periodic_gaussian_force_x};
+  md_pair_energy_t_x pes[2] = {md_gaussian_energy_x, md_periodic_gaussian_energy_x};
+  const char *tag[2] = {"g", "pg"};
+  for (int k = 0; k < 2; ++k) {
+    char w[32];
+    md_reset_force_x(sa);
+    md_reset_force_x(sb);
+    md_pimd_calc_pair_force_2_x(pa, pb, pfs[k]);
+    snprintf(w, sizeof w, "fA_%s", tag[k]);
+    dump_forces("two", w, sa);
+    snprintf(w, sizeof w, "fB_%s", tag[k]);
+    dump_forces("two", w, sb);
+    md_stats_t_x *st = md_alloc_stats_x(1);
+    st->es[0] = 0;
+    md_pimd_calc_pair_energy_2_x(pa, pb, pes[k], st);
+    snprintf(w, sizeof w, "e_%s", tag[k]);
+    dumpf("two", w, (double[]){stat_of(st)}, 1);
+  }
+}
+
+/* Observables (density distribution, pair correlation, S(q)) of two seeded
+ * configurations: periodic (N = 20, P = 6, box 5 x 5.5 x 6, uniform
+ * positions) and open (Gaussian positions). */
+static void observables_case(void) {
+  const int N = 20, P = 6, nb = N * P;
+  md_num_t_x L3[3] = {5.0, 5.5, 6.0}, ctr[3] = {2.5, 2.75, 3.0}, c0[3] = {0.1, -0.2, 0.3};
+  md_set_seed_x(31);
+  md_simulation_t_x *sim = md_alloc_simulation_x(nb, 1, 0, MD_ND_RECT_BOX_X);
+  md_init_nd_rect_box_x(sim->box, L3);
+  md_pimd_t_x *pimd = md_alloc_pimd_x(sim, N, P, 1.0, 1.0);
+  md_init_particle_mass_x(sim, 1.0, 0, nb);
+  for (int pass = 0; pass < 2; ++pass) {
+    const char *pf = pass == 0 ? "obs_pbc" : "obs_open";
+    if (pass == 0) {
+      md_pimd_init_particle_uniform_pos_x(pimd, ctr, L3, 0.3);
+      md_periodic_boundary_x(sim);
+    } else {
+      for (int i = 0; i < nb; ++i)
+        for (int d = 0; d < 3; ++d) sim->particles[i].x[d] = md_random_gaussian_x();
+    }
+    dump_beads(pf, "0", sim);
+    md_stats_t_x *a = md_alloc_stats_x(24), *b = md_alloc_stats_x(30), *c = md_alloc_stats_x(16);
+    md_pimd_calc_density_distribution_x(pimd, a, 3.0, 1 - pass, pass == 0 ? ctr : c0);
+    md_pimd_calc_pair_correlation_x(pimd, b, 3.5, 7.0, 4, 1 - pass);
+    md_pimd_calc_Sk_structure_x(pimd, c, 0.2, 0.35, 3);
+    dumpf(pf, "density", a->fxs[0], 24);
+    dumpf(pf, "paircorr", b->fxs[0], 30);
+    dumpf(pf, "sk", c->fxs[0], 16);
+  }
+}
+
 int main(int argc, char **argv) {
   const char *path = argc > 1 ? argv[1] : "golden.bin";
   out = fopen(path, "wb");
@@ -323,6 +399,8 @@ int main(int argc, char **argv) {
       {"ueg16_n8",    16, 2,  0.25 * kF16 * kF16, 1.0, 1, 1, L16, 8, 1, 0, 0.05, 7},
   };
   for (size_t i = 0; i < sizeof cases / sizeof cases[0]; ++i) run_case(&cases[i]);
+  two_species_case();
+  observables_case();
   fclose(out);
   return 0;
 }

@@ -100,6 +100,11 @@ class Oracle:
             "orc_pair_force": (None, [i, vp, i, i, DP, vp]),
             "orc_trap_energy": (d, [vp, i, i, vp]),
             "orc_pair_energy": (d, [i, vp, i, i, DP, vp]),
+            "orc_pair_force_2": (None, [i, vp, i, i, i, DP, vp, vp]),
+            "orc_pair_energy_2": (d, [i, vp, i, i, i, DP, vp, vp]),
+            "orc_density_distribution": (None, [i, i, vp, DP, i, DP, d, i, DP]),
+            "orc_pair_correlation": (None, [i, i, vp, DP, i, d, d, i, i, DP]),
+            "orc_sk_structure": (None, [i, i, vp, d, d, i, i, DP]),
             "orc_madelung": (d, [vp, DP]),
             "orc_fast_fill_ENk": (None, [i, i, d, i, DP, vp, DP, DP]),
             "orc_fill_ENk_slow": (None, [i, i, d, i, DP, vp, DP]),

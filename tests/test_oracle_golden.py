@@ -4,6 +4,8 @@ Every per-step stage of the reference (nocl build) is reproduced by the oracle
 on the same seeded inputs.  Golden vectors: tests/golden/golden.bin, written by
 tests/golden/make_golden.c linked against the reference library.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -183,3 +185,51 @@ def test_rng(golden, oracle):
     b = beads_from(np.zeros(45))
     oracle.init_uniform_pos(5, 3, dptr(np.array([0.5, 1.0, -0.5])), dptr(np.array([2.0, 3.0, 4.0])), 0.1, ptr(b))
     np.testing.assert_array_equal(b["x"].reshape(-1), golden["rng/uniform_pos"])
+
+
+def test_two_species_pair_force_and_energy(golden, oracle):
+    """orc_pair_force_2 / orc_pair_energy_2 against the compiled reference's
+    md_pimd_calc_pair_force_2_x / _energy_2_x, plus momentum balance."""
+    o = oracle
+    g = {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith("two/")}
+    NA, NB, P = 5, 3, 4
+    L = np.array([3.0, 3.5, 4.0])
+    p = make_params(g=3.0, s=0.5)
+    for tag, kind in (("g", PAIR_KIND["gaussian"]), ("pg", PAIR_KIND["periodic_gaussian"])):
+        a = beads_from(g["xA0"], m=g["mA0"])
+        b = beads_from(g["xB0"], m=g["mB0"])
+        o.pair_force_2(kind, C.byref(p), NA, NB, P, dptr(L), ptr(a), ptr(b))
+        assert rel_err(a["f"].reshape(-1), g["fA_" + tag]) < 1e-14
+        assert rel_err(b["f"].reshape(-1), g["fB_" + tag]) < 1e-14
+        mom = (a["m"][:, None] * a["f"]).sum(0) + (b["m"][:, None] * b["f"]).sum(0)
+        assert np.abs(mom).max() < 1e-13 * np.abs(a["f"]).max()
+        e = o.pair_energy_2(kind, C.byref(p), NA, NB, P, dptr(L), ptr(a), ptr(b))
+        assert e == pytest.approx(g["e_" + tag][0], rel=1e-14)
+
+
+OBS_SETUP = {"obs_pbc": dict(image=1, center=[2.5, 2.75, 3.0]), "obs_open": dict(image=0, center=[0.1, -0.2, 0.3])}
+
+
+@pytest.mark.parametrize("name", list(OBS_SETUP))
+def test_observables(golden, oracle, name):
+    """orc_density_distribution / orc_pair_correlation / orc_sk_structure
+    against the compiled reference (md_pimd_calc_density_distribution_x,
+    _pair_correlation_x, _Sk_structure_x); histograms bit for bit."""
+    g = {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(name + "/")}
+    N, P = 20, 6
+    L = np.array([5.0, 5.5, 6.0])
+    b = beads_from(g["x0"], m=g["m0"])
+    su = OBS_SETUP[name]
+    fx = np.zeros(24)
+    oracle.density_distribution(N, P, ptr(b), dptr(L), su["image"], dptr(np.array(su["center"])), 3.0, 24, dptr(fx))
+    assert np.array_equal(fx, g["density"])
+    fx = np.zeros(30)
+    oracle.pair_correlation(N, P, ptr(b), dptr(L), su["image"], 3.5, 7.0, 4, 30, dptr(fx))
+    assert np.array_equal(fx, g["paircorr"])
+    fx = np.zeros(16)
+    oracle.sk_structure(N, P, ptr(b), 0.2, 0.35, 3, 16, dptr(fx))
+    np.testing.assert_allclose(fx, g["sk"], rtol=1e-14, atol=1e-14)
+    # S(q -> 0) of bead 1 against bead 1 is N (all phases aligned)
+    fx = np.zeros(1)
+    oracle.sk_structure(N, P, ptr(b), 0.0, 1.0, 1, 1, dptr(fx))
+    assert fx[0] == pytest.approx(N / 3.0 * 3, rel=1e-15)
