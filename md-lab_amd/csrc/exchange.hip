@@ -226,12 +226,12 @@ __device__ __forceinline__ double rec_coef(const RecArgs &a, int p, int pp) {
 //                coefficients of every diagonal block and of every block on
 //                its predecessor, so the serial kernels read them with
 //                independent, coalesced loads;
-//   k_rec_block  (one launch per block b) one workgroup per direction folds
+//   k_rec_persist (one launch) per direction a solver workgroup that folds
 //                block b-1 into block b's accumulators (look-ahead) and
-//                solves block b; the other workgroups fold block b-1 into
-//                64 later rows each.
-// Every row of block B thus receives column 0 (prep), blocks 0..B-2 (the
-// wide part of launches 1..B-1) and block B-1 (look-ahead of launch B).
+//                solves block b, and one updater workgroup per later row
+//                block that folds each finished block into its rows.
+// Every row of block B thus receives column 0 (prep), blocks 0..B-2 (its
+// updater) and block B-1 (the solver's look-ahead).
 // Lanes are rows throughout: log-sum-exps run along each lane, so no
 // cross-lane reductions are needed.
 constexpr int RTRI = RB * (RB - 1) / 2;
@@ -330,156 +330,6 @@ __device__ __forceinline__ void lse16(const double *t, double &M, double &S) {
   }
 }
 
-template <bool FWD>
-__device__ void rec_diag_body(const RecArgs &a, int dir, int b, double *dg, double *mo, double *so, double *sg,
-                              double (*pm)[RB], double (*ps)[RB]) {
-  const int N = a.N;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
-  double *val = a.val2 + (size_t)dir * (N + 1);
-  const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
-  const double *DG = a.dg + ((size_t)dir * a.nblk + b) * RTRI;
-  for (int w = tid; w < RTRI; w += blockDim.x) dg[w] = DG[w];
-  // look-ahead: block b-1 into this block's rows; wave wv takes columns
-  // 16 wv .. 16 wv + 15, lane = row
-  {
-    double M = NEG, S = 0.0;
-    if (b > 0) {
-      const double *LA = a.la + ((size_t)dir * a.nblk + b) * (RB * RB);
-      const double *vp = val + (p0 - RB);
-      double t[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int r = 16 * wv + j;
-        const double c = LA[r * RB + lane];
-        t[j] = c > NEG ? c + vp[r] : NEG;
-      }
-      lse16(t, M, S);
-    }
-    pm[wv][lane] = M;
-    ps[wv][lane] = S;
-  }
-  __syncthreads();
-  if (wv == 0) {
-    double M = NEG, S = 0.0;
-    if (lane < nrows) {
-      M = AM[p0 + lane];
-      S = AS[p0 + lane];
-#pragma unroll
-      for (int w = 0; w < 4; ++w) lse_pair(M, S, pm[w][lane], ps[w][lane]);
-    }
-    mo[lane] = M;
-    so[lane] = S;
-  }
-  __syncthreads();
-  // per-row shift sigma_q = max(M_q, max_{r<q} c(q,r) + sigma_r): a (max,+)
-  // scan with no transcendental on its serial chain; each lane holds its
-  // row of c in registers.  Every term obeys c(q,r) + L(r) <= L(q) <=
-  // sigma_q + (rows) ln N, so the scaled values below stay in FP64 range.
-  const int rowq = lane * (lane - 1) / 2;
-  if (wv == 0) {
-    double c[RB - 1];
-#pragma unroll
-    for (int r = 0; r < RB - 1; ++r) c[r] = r < lane ? dg[rowq + r] : NEG;
-    double sig = lane < nrows ? mo[lane] : NEG;
-#pragma unroll
-    for (int r = 0; r < RB - 1; ++r) sig = fmax(sig, c[r] + readlane_d(sig, r));
-    sg[lane] = lane < nrows ? sig : NEG;
-  }
-  __syncthreads();
-  // scaled coefficients a(q,r) = exp(c(q,r) + sigma_r - sigma_q): thread
-  // (q, j) takes columns r = j, j + 4, ...
-  {
-    const int q = tid >> 2, j = tid & 3;
-    const int row = q * (q - 1) / 2;
-    const double sq = sg[q];
-    for (int r = j; r < q; r += 4) {
-      const double c = dg[row + r];
-      dg[row + r] = (q < nrows && c > NEG) ? exp(c + sg[r] - sq) : 0.0;
-    }
-  }
-  __syncthreads();
-  // linear forward substitution W_q = b_q + sum_{r<q} a(q,r) W_r, with
-  // W_q = exp(L(q) - sigma_q): one FMA per step (a(q,r) = 0 for r >= q)
-  if (wv == 0) {
-    double av[RB - 1];
-#pragma unroll
-    for (int r = 0; r < RB - 1; ++r) av[r] = r < lane ? dg[rowq + r] : 0.0;
-    double W = 0.0;
-    if (lane < nrows && mo[lane] > NEG && mo[lane] > sg[lane] - 745.0) W = so[lane] * exp(mo[lane] - sg[lane]);
-#pragma unroll
-    for (int r = 0; r < RB - 1; ++r) W = fma(av[r], readlane_d(W, r), W);
-    if (lane < nrows) {
-      const int p = p0 + lane;
-      const double v = sg[lane] + log(W);
-      val[p] = v;
-      if (FWD) {
-        a.Lf[p] = v;
-        a.VBN[p] = -v / a.beta;
-      } else {
-        a.Rb[N - p] = v;
-      }
-    }
-  }
-}
-
-// Folds block b-1 (positions pprev .. pprev + 63) into the accumulators of
-// rows prow .. prow + 63: lane = row, wave wv takes 16 of the columns.
-template <bool FWD>
-__device__ void rec_update_rows(const RecArgs &a, int dir, int prow, int pprev, double (*pm)[RB], double (*ps)[RB]) {
-  const int N = a.N;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const double *val = a.val2 + (size_t)dir * (N + 1);
-  double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
-  const int p = prow + lane;
-  const bool act = p <= N;
-  double t[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int pp = pprev + 16 * wv + j;
-    double v = NEG;
-    if (act) {
-      const double c = rec_coef_col<FWD>(a, p, pp);
-      if (c > NEG) v = c + val[pp];
-    }
-    t[j] = v;
-  }
-  double M, S;
-  lse16(t, M, S);
-  pm[wv][lane] = M;
-  ps[wv][lane] = S;
-  __syncthreads();
-  if (wv == 0 && act) {
-    double Ma = AM[p], Sa = AS[p];
-#pragma unroll
-    for (int w = 0; w < 4; ++w) lse_pair(Ma, Sa, pm[w][lane], ps[w][lane]);
-    AM[p] = Ma;
-    AS[p] = Sa;
-  }
-}
-
-// Launch for block b: workgroups [0, ndir) solve it (one per direction),
-// the rest fold block b-1 into 64 later rows each.
-__global__ __launch_bounds__(256) void k_rec_block(RecArgs a, int dir0, int ndir, int b) {
-  __shared__ double dg[RTRI], mo[RB], so[RB], sg[RB], pm[4][RB], ps[4][RB];
-  if ((int)blockIdx.x < ndir) {
-    const int dir = dir0 + blockIdx.x;
-    if (dir == 0)
-      rec_diag_body<true>(a, 0, b, dg, mo, so, sg, pm, ps);
-    else
-      rec_diag_body<false>(a, 1, b, dg, mo, so, sg, pm, ps);
-    return;
-  }
-  const int u = blockIdx.x - ndir;
-  const int dir = dir0 + u % ndir;
-  const int p0 = 1 + b * RB;
-  const int prow = p0 + RB + (u / ndir) * RB;
-  if (dir == 0)
-    rec_update_rows<true>(a, 0, prow, p0 - RB, pm, ps);
-  else
-    rec_update_rows<false>(a, 1, prow, p0 - RB, pm, ps);
-}
-
 // ---- one persistent launch for the whole solve ----
 //
 // Per direction one solver workgroup (blocks 0, 1, 2, ... in order, with
@@ -495,7 +345,7 @@ __global__ __launch_bounds__(256) void k_rec_block(RecArgs a, int dir0, int ndir
 // grid always drains.
 constexpr int REC_SPIN_MAX = 1 << 22;
 
-__device__ void rec_wait(int *flag, int target, int *err) {
+__device__ __forceinline__ void rec_wait(int *flag, int target, int *err) {
   if (threadIdx.x == 0) {
     int it = 0;
     while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -511,14 +361,14 @@ __device__ void rec_wait(int *flag, int target, int *err) {
   __threadfence();
 }
 
-__device__ void rec_post(int *flag, int value) {
+__device__ __forceinline__ void rec_post(int *flag, int value) {
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <bool FWD>
-__device__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[RB], double (*ps)[RB]) {
+__device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[RB], double (*ps)[RB]) {
   const int N = a.N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int *solved = a.sync + dir * a.nblk, *folded = a.sync + (2 + dir) * a.nblk, *err = a.sync + 4 * a.nblk;
@@ -563,31 +413,168 @@ __device__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[RB], 
   rec_post(folded + G, G - 1);
 }
 
+// (q, r) of the in-block coefficient w = q (q - 1) / 2 + r, packed q << 8 | r.
+struct TriTable {
+  unsigned short v[RTRI];
+  constexpr TriTable() : v() {
+    int w = 0;
+    for (int q = 1; q < RB; ++q)
+      for (int r = 0; r < q; ++r) v[w++] = (unsigned short)((q << 8) | r);
+  }
+};
+__constant__ TriTable kTri = TriTable();
+
+struct RecLds {
+  double dg[2][RTRI];       // in-block coefficients, then scaled exps (ping-pong)
+  double la[2][RB * RB];    // look-ahead coefficients [r][q] (ping-pong)
+  double vlast[RB];         // values of the block solved last
+  double mo[RB], so[RB];    // row accumulators (max, scaled sum)
+  double sg[RB];            // per-row shifts
+  double pm[4][RB], ps[4][RB];
+};
+
+// The solver walks the blocks in order.  Per block b: stage its in-block
+// coefficients (prefetched into registers during block b-1), fold block b-1
+// into the rows from the prefetched look-ahead coefficients and the values
+// kept in LDS, prefetch block b+1, wait for the updater of block b, merge,
+// then the (max,+) shift scan, scaled exps and the linear substitution.
 template <bool FWD>
-__device__ void rec_solver(const RecArgs &a, int dir, double *dg, double *mo, double *so, double *sg,
-                           double (*pm)[RB], double (*ps)[RB]) {
+__device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S) {
+  const int N = a.N;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int *solved = a.sync + dir * a.nblk, *folded = a.sync + (2 + dir) * a.nblk, *err = a.sync + 4 * a.nblk;
+  double *val = a.val2 + (size_t)dir * (N + 1);
+  const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  const double *DGd = a.dg + (size_t)dir * a.nblk * RTRI;
+  const double *LAd = a.la + (size_t)dir * a.nblk * (RB * RB);
+  constexpr int DPT = (RTRI + 255) / 256;  // in-block coefficients per thread
+  constexpr int LPT = RB * RB / 256;       // look-ahead coefficients per thread
+  // block 0's coefficients
+  for (int w = tid; w < RTRI; w += 256) S.dg[0][w] = DGd[w];
+  for (int w = tid; w < RB * RB; w += 256) S.la[0][w] = LAd[w];
+  const int rowq = lane * (lane - 1) / 2;
   for (int b = 0; b < a.nblk; ++b) {
-    if (b >= 2) rec_wait(folded + b, b - 1, err);
-    rec_diag_body<FWD>(a, dir, b, dg, mo, so, sg, pm, ps);
+    const int cur = b & 1;
+    const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
+    __syncthreads();
+    // prefetch block b+1: loads in flight during the look-ahead
+    double dgr[DPT], lar[LPT];
+    const bool more = b + 1 < a.nblk;
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const int w = tid + 256 * k;
+        dgr[k] = w < RTRI ? DGd[(size_t)(b + 1) * RTRI + w] : NEG;
+      }
+#pragma unroll
+      for (int k = 0; k < LPT; ++k) lar[k] = LAd[(size_t)(b + 1) * (RB * RB) + tid + 256 * k];
+    }
+    {
+      double t[16], M, Sm;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = 16 * wv + j;
+        const double c = S.la[cur][r * RB + lane];
+        t[j] = (b > 0 && c > NEG) ? c + S.vlast[r] : NEG;
+      }
+      lse16(t, M, Sm);
+      S.pm[wv][lane] = M;
+      S.ps[wv][lane] = Sm;
+    }
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const int w = tid + 256 * k;
+        if (w < RTRI) S.dg[cur ^ 1][w] = dgr[k];
+      }
+#pragma unroll
+      for (int k = 0; k < LPT; ++k) S.la[cur ^ 1][tid + 256 * k] = lar[k];
+    }
+    if (b >= 2)
+      rec_wait(folded + b, b - 1, err);
+    else
+      __syncthreads();
+    if (wv == 0) {
+      double M = NEG, Sm = 0.0;
+      if (lane < nrows) {
+        M = AM[p0 + lane];
+        Sm = AS[p0 + lane];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) lse_pair(M, Sm, S.pm[w][lane], S.ps[w][lane]);
+      }
+      S.mo[lane] = M;
+      S.so[lane] = Sm;
+    }
+    __syncthreads();
+    double *dg = S.dg[cur];
+    // per-row shift sigma_q = max(M_q, max_{r<q} c(q,r) + sigma_r): a (max,+)
+    // scan with no transcendental on its serial chain; each lane holds its
+    // row of c in registers.  Every term obeys c(q,r) + L(r) <= L(q) <=
+    // sigma_q + (rows) ln N, so the scaled values below stay in FP64 range.
+    if (wv == 0) {
+      double c[RB - 1];
+#pragma unroll
+      for (int r = 0; r < RB - 1; ++r) c[r] = r < lane ? dg[rowq + r] : NEG;
+      double sig = lane < nrows ? S.mo[lane] : NEG;
+#pragma unroll
+      for (int r = 0; r < RB - 1; ++r) sig = fmax(sig, c[r] + readlane_d(sig, r));
+      S.sg[lane] = lane < nrows ? sig : NEG;
+    }
+    __syncthreads();
+    // scaled coefficients a(q,r) = exp(c(q,r) + sigma_r - sigma_q), evenly
+    // spread over the workgroup
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+      const int w = tid + 256 * k;
+      if (w < RTRI) {
+        const int qr = kTri.v[w], q = qr >> 8, r = qr & 255;
+        const double c = dg[w];
+        dg[w] = (q < nrows && c > NEG) ? exp(c + S.sg[r] - S.sg[q]) : 0.0;
+      }
+    }
+    __syncthreads();
+    // linear forward substitution W_q = b_q + sum_{r<q} a(q,r) W_r, with
+    // W_q = exp(L(q) - sigma_q): one FMA per step (a(q,r) = 0 for r >= q)
+    if (wv == 0) {
+      double av[RB - 1];
+#pragma unroll
+      for (int r = 0; r < RB - 1; ++r) av[r] = r < lane ? dg[rowq + r] : 0.0;
+      double W = 0.0;
+      if (lane < nrows && S.mo[lane] > NEG && S.mo[lane] > S.sg[lane] - 745.0)
+        W = S.so[lane] * exp(S.mo[lane] - S.sg[lane]);
+#pragma unroll
+      for (int r = 0; r < RB - 1; ++r) W = fma(av[r], readlane_d(W, r), W);
+      const double v = S.sg[lane] + log(W);
+      S.vlast[lane] = v;
+      if (lane < nrows) {
+        const int p = p0 + lane;
+        val[p] = v;
+        if (FWD) {
+          a.Lf[p] = v;
+          a.VBN[p] = -v / a.beta;
+        } else {
+          a.Rb[N - p] = v;
+        }
+      }
+    }
     rec_post(solved + b, 1);
   }
 }
 
 __global__ __launch_bounds__(256) void k_rec_persist(RecArgs a, int dir0) {
-  __shared__ double dg[RTRI], mo[RB], so[RB], sg[RB], pm[4][RB], ps[4][RB];
+  __shared__ RecLds S;
   const int per = 1 + max(0, a.nblk - 2);
   const int dir = dir0 + blockIdx.x / per, role = blockIdx.x % per;
   if (role == 0) {
     if (dir == 0)
-      rec_solver<true>(a, 0, dg, mo, so, sg, pm, ps);
+      rec_solver<true>(a, 0, S);
     else
-      rec_solver<false>(a, 1, dg, mo, so, sg, pm, ps);
+      rec_solver<false>(a, 1, S);
   } else {
     if (dir == 0)
-      rec_updater<true>(a, 0, role + 1, pm, ps);
+      rec_updater<true>(a, 0, role + 1, S.pm, S.ps);
     else
-      rec_updater<false>(a, 1, role + 1, pm, ps);
+      rec_updater<false>(a, 1, role + 1, S.pm, S.ps);
   }
 }
 
@@ -939,9 +926,7 @@ int launch_enk(DevPimd *p, int image, hipStream_t st) {
 }
 
 
-// One persistent launch (default) or one launch per 64-position block.
-int g_rec_persistent = 1;
-DevPimd *g_rec_last = nullptr;
+DevPimd *g_rec_last = nullptr;  // object of the last recursion launch (error checks)
 
 // Error flag of the persistent solve (timed-out hand-off); call after a
 // synchronisation.
@@ -966,19 +951,10 @@ int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
   const int nprep = std::max(N + 1, a.nblk * RB * RB);
   hipLaunchKernelGGL(k_rec_prep, dim3((nprep + 255) / 256, ndir), dim3(256), 0, st, a, dir0);
   XMD_CHECK(hipGetLastError());
-  if (g_rec_persistent) {
-    const int per = 1 + std::max(0, a.nblk - 2);
-    hipLaunchKernelGGL(k_rec_persist, dim3(ndir * per), dim3(256), 0, st, a, dir0);
-    XMD_CHECK(hipGetLastError());
-    g_rec_last = p;
-  } else {
-    for (int b = 0; b < a.nblk; ++b) {
-      const int later = b > 0 ? std::max(0, N - (1 + (b + 1) * RB) + 1) : 0;  // rows after block b
-      const int grid = ndir + ndir * ((later + RB - 1) / RB);
-      hipLaunchKernelGGL(k_rec_block, dim3(grid), dim3(256), 0, st, a, dir0, ndir, b);
-      XMD_CHECK(hipGetLastError());
-    }
-  }
+  const int per = 1 + std::max(0, a.nblk - 2);
+  hipLaunchKernelGGL(k_rec_persist, dim3(ndir * per), dim3(256), 0, st, a, dir0);
+  XMD_CHECK(hipGetLastError());
+  g_rec_last = p;
   if (fwd) p->vbn_dev = true;
   if (bwd) {
     p->back_dev = true;

@@ -151,7 +151,6 @@ int backward_potentials(DevPimd *p, double *host_Vt);
 int bead_sum(DevSim *d, int mode, const double *trap_c, double w, const double *L, double *out);
 extern double g_ewald_xcut;
 extern double g_ewald_split_aL;
-extern int g_rec_persistent;
 extern DevPimd *g_rec_last;
 int recursion_status(DevPimd *p);
 
