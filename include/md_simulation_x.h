@@ -47,6 +47,14 @@ void md_periodic_gaussian_force_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *f,
 void md_set_gaussian_force_strength_x(md_num_t_x g);
 void md_set_guassian_force_range_x(md_num_t_x s);   /* reference spelling */
 
+/* Lennard-Jones 4 (r^-12 - r^-6), minimum image (md_simulation_x.c:116-122). */
+void md_LJ_periodic_force_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *f, md_num_t_x *L, md_num_t_x m);
+
+/* Helium pair potential eps [A e^{-alpha x} - (C6 x^-6 + C8 x^-8 + C10 x^-10) F(x)], x = r / r_m,
+ * minimum image (md_simulation_x.c:365-399). */
+void md_set_helium_parameters_x(md_num_t_x eps, md_num_t_x A, md_num_t_x alpha, md_num_t_x C6, md_num_t_x C8, md_num_t_x C10, md_num_t_x D, md_num_t_x rm);
+void md_periodic_helium_force_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *f, md_num_t_x *L, md_num_t_x m);
+
 /* 3-D Ewald Coulomb (md_simulation_x.c:206-346). */
 void md_set_coulomb_force_strength_x(md_num_t_x g);
 void md_set_coulomb_truc_x(md_num_t_x k);

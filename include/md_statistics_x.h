@@ -26,6 +26,16 @@ typedef struct {
   md_num_t_x **fxs;
 } md_stats_t_x;
 
+/* q vectors of a spherical S(q) grid, one representative per |q| shell, sorted by |q|
+ * (md_statistics_x.h:28-34). */
+typedef struct {
+  int N;
+  md_num_t_x *qs;   /* |q|, ascending */
+  md_num_t_x *qis;  /* representative (qx, qy, qz) per shell */
+} md_sk_sphere_t_x;
+
+md_sk_sphere_t_x *md_alloc_sk_sphere_3d_x(md_num_t_x q0, md_num_t_x qincre, md_num_t_x qmax);
+
 md_stats_t_x *md_stats_sync_host_x(md_stats_t_x *stats);
 
 typedef md_num_t_x (*md_pair_energy_t_x)(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
@@ -39,6 +49,8 @@ md_num_t_x md_harmonic_trap_energy_x(md_num_t_x *p1, md_num_t_x *L, md_num_t_x m
 md_num_t_x md_gaussian_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
 md_num_t_x md_periodic_gaussian_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
 md_num_t_x md_coulomb_3d_ewald_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+md_num_t_x md_LJ_periodic_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+md_num_t_x md_periodic_helium_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
 
 int md_calc_kinetic_x(md_simulation_t_x *sim, md_stats_t_x *stats);
 int md_calc_nhc_kinetic_x(md_simulation_t_x *sim, md_stats_t_x *stats);

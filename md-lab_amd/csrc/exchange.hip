@@ -815,6 +815,27 @@ __global__ __launch_bounds__(256) void k_vir_term(const double *__restrict__ X, 
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Box-centred virial sum_{all beads} (x - L/2) . f m (md_pimd_x.c:5508-5515).
+__global__ __launch_bounds__(256) void k_box_virial(const double *__restrict__ X, const double *__restrict__ F,
+                                                    const double *__restrict__ M, int nb, Geo g,
+                                                    double *__restrict__ part) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < nb; i += gridDim.x * 256) {
+    const double m = M[i];
+    s += (X[i] - g.L0 / 2.0) * F[i] * m;
+    s += (X[nb + i] - g.L1 / 2.0) * F[nb + i] * m;
+    s += (X[2 * nb + i] - g.L2 / 2.0) * F[2 * nb + i] * m;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // Sign-tracked forward recursion at |xi'| (md_pimd_x.c:4675-4701): one
 // workgroup, one n at a time (a per-sample estimator, not per step).
 __global__ __launch_bounds__(256) void k_vi_sign(RecArgs a, double vi2, double *__restrict__ out) {
@@ -1050,9 +1071,171 @@ int backward_potentials(DevPimd *p, double *host_Vt) {
   return 0;
 }
 
+// ------------------------------------------------ single-value helpers
+//
+// The reference's per-element helpers of the exchange path, evaluated on the
+// device state: one cycle energy, one link-virial cycle term, the log-sum-exp
+// shifts and one bead's spring gradient.  A single workgroup each; the sums
+// are tree reductions (the reference sums sequentially).
+
+__device__ double block_sum(double v, double *red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+__device__ double block_min(double v, double *red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// E_{N2}^{(k)} by the explicit cycle walk (md_pimd_x.c:569-591): springs from
+// bead (l, j) to its successor on the ring of particles N2-k+1..N2.
+__global__ __launch_bounds__(256) void k_cycle_energy(const double *__restrict__ X, const double *__restrict__ M, int N,
+                                                     int P, int N2, int k, double om, Geo g, double *__restrict__ out) {
+  __shared__ double red[256];
+  const int nb = N * P, l0 = N2 - k + 1;
+  double e = 0;
+  for (int t = threadIdx.x; t < k * P; t += blockDim.x) {
+    const int l = l0 + t / P, j = t % P + 1;  // 1-based
+    const int sa = (j - 1) * N + (l - 1);
+    int sb;
+    if (j < P)
+      sb = j * N + (l - 1);
+    else
+      sb = (l == N2) ? (l0 - 1) : l;  // bead 1 of the next particle on the ring
+    e += spring(X, nb, sa, sb, M[sa], om, g);
+  }
+  e = block_sum(e, red);
+  if (threadIdx.x == 0) out[0] = e;
+}
+
+// Link-virial cycle term (md_pimd_x.c:3959-3987 with forward = 1, and
+// md_pimd_x.c:4376-4420 for one explicit bead triple, pbc = 0):
+// sum over the listed triples (a, b, c) of 1/2 m_a w^2 (x_a - x_b).(x_a - x_c).
+__global__ __launch_bounds__(256) void k_cycle_virial(const double *__restrict__ X, const double *__restrict__ M, int N,
+                                                     int P, int N2, int k, int sa1, int sb1, int sc1, double om, Geo g,
+                                                     double *__restrict__ out) {
+  __shared__ double red[256];
+  const int nb = N * P;
+  double e = 0;
+  const int n = sa1 >= 0 ? 1 : k;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    int sa, sb, sc;
+    if (sa1 >= 0) {
+      sa = sa1;
+      sb = sb1;
+      sc = sc1;
+    } else {
+      const int l = N2 - k + 1 + t;  // 1-based
+      sa = (l == N2) ? (N2 - k) : l;  // successor of bead (l, P): bead 1 of the next particle
+      sb = (P - 1) * N + (l - 1);
+      sc = l - 1;
+    }
+    const double m = M[sa];
+    const double dx = comp(X[sa] - X[sb], g.L0, g.image), ex = comp(X[sa] - X[sc], g.L0, g.image);
+    const double dy = comp(X[nb + sa] - X[nb + sb], g.L1, g.image), ey = comp(X[nb + sa] - X[nb + sc], g.L1, g.image);
+    const double dz = comp(X[2 * nb + sa] - X[2 * nb + sb], g.L2, g.image),
+                 ez = comp(X[2 * nb + sa] - X[2 * nb + sc], g.L2, g.image);
+    e += 0.5 * m * om * om * dx * ex;
+    e += 0.5 * m * om * om * dy * ey;
+    e += 0.5 * m * om * om * dz * ez;
+  }
+  e = block_sum(e, red);
+  if (threadIdx.x == 0) out[0] = e;
+}
+
+// Log-sum-exp shifts of the recursions (md_pimd_x.c:688-808 forward,
+// 2213-2335 backward, 4652-4663 at a second xi):
+//  forward : min_k [-(k-1) ln xi + beta (E_n^(k) + V(n-k))],              k = 1..n
+//  backward: min_l [-(l-u) ln xi + ln l + beta (E_l^(l-u+1) + V(l))],       l = u..N
+// (xi = 0: the k = 1 / l = u term alone); the running minimum starts at 1e10
+// as in the reference, so terms with a NaN logarithm never enter.
+__global__ __launch_bounds__(256) void k_lse_shift(const double *__restrict__ ENk, const double *__restrict__ V, int N,
+                                                  int n, int backward, double beta, double vi,
+                                                  double *__restrict__ out) {
+  __shared__ double red[256];
+  double res = 1e10;
+  if (vi == 0.0) {
+    if (threadIdx.x == 0)
+      res = backward ? log((double)n) + beta * (ENk[(size_t)(n - 1) * n / 2] + V[n])
+                     : beta * (ENk[(size_t)(n - 1) * n / 2] + V[n - 1]);
+  } else {
+    const double lv = log(vi);
+    const int cnt = backward ? N - n + 1 : n;
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      double tmp;
+      if (backward) {
+        const int l = n + t, k = t + 1;
+        tmp = -(k - 1) * lv + log((double)l) + beta * (ENk[(size_t)(l - 1) * l / 2 + k - 1] + V[l]);
+      } else {
+        const int k = t + 1;
+        tmp = -(k - 1) * lv + beta * (ENk[(size_t)(n - 1) * n / 2 + k - 1] + V[n - k]);
+      }
+      if (tmp < res) res = tmp;
+    }
+  }
+  res = block_min(res, red);
+  if (threadIdx.x == 0) out[0] = res;
+}
+
+// Spring gradient of one bead (md_pimd_x.c:967-995, 2344-2367):
+// m_a w^2 [(x_a - x_b) w_b + (x_a - x_c) w_c], w = 1 or 0 per link.
+__global__ void k_spring_grad(const double *__restrict__ X, const double *__restrict__ M, int nb, int sa, int sb,
+                              int sc, int wb, int wc, double om, Geo g, double *__restrict__ out) {
+  const int d = threadIdx.x;
+  if (d >= 3) return;
+  const double L = d == 0 ? g.L0 : (d == 1 ? g.L1 : g.L2);
+  const double m = M[sa];
+  const double xa = X[d * nb + sa];
+  double r = 0;
+  if (wb) r += m * om * om * comp(xa - X[d * nb + sb], L, g.image);
+  if (wc) r += m * om * om * comp(xa - X[d * nb + sc], L, g.image);
+  out[d] = r;
+}
+
 }  // namespace xmd
 
 using namespace xmd;
+
+namespace {
+
+// Reference bead index (l-1) P + (j-1) -> device slot (j-1) N + (l-1).
+int ref_to_slot(int index, int N, int P) { return (index % P) * N + index / P; }
+
+double fetch_scalar(DevPimd *p, int n, double *host) {
+  double v[3] = {0, 0, 0};
+  if (hipMemcpyAsync(v, p->work + 4 * (p->N + 1), n * sizeof(double), hipMemcpyDeviceToHost, rt().s0) !=
+          hipSuccess ||
+      hipStreamSynchronize(rt().s0) != hipSuccess)
+    return NAN;
+  if (host)
+    for (int i = 0; i < n; ++i) host[i] = v[i];
+  return v[0];
+}
+
+DevPimd *helper_pimd(md_pimd_t_x *pimd) {
+  if (!pimd || ensure_pimd_device(pimd)) return nullptr;
+  return pimd_state(pimd, false);
+}
+
+// V (N+1 host values) into device scratch after the helpers' output slot.
+double *stage_host_vector(DevPimd *p, const double *V) {
+  double *d = p->work + 4 * (p->N + 1) + 8;
+  if (hipMemcpyAsync(d, V, (p->N + 1) * sizeof(double), hipMemcpyHostToDevice, rt().s0) != hipSuccess) return nullptr;
+  return d;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1135,6 +1318,45 @@ int md_pimd_calc_vi_virial_energy_x(md_pimd_t_x *pimd, md_stats_t_x *stats, int 
   return 0;
 }
 
+// md_pimd_x.c:5452-5517 (pbc = 0): 2 res[N] / 3 + N / beta + (bead-1-relative
+// virial) / 3 + (box-centred virial) / 3, with res the energy estimator's
+// ENk2 recursion.
+int md_pimd_calc_vi_virial_pressure_x(md_pimd_t_x *pimd, md_stats_t_x *stats, int image) {
+  if (ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  if (!(pimd->sim->box && pimd->sim->box->type == MD_ND_RECT_BOX_X)) {
+    fprintf(stderr, "xmd: the virial pressure needs a rectangular box\n");
+    return -1;
+  }
+  if (ensure_enk(p) || ensure_vbn(p)) return -1;
+  if (!p->enk2_dev) {
+    size_t tri = (size_t)p->N * (p->N + 1) / 2;
+    XMD_CHECK(hipMemcpy(p->ENk2, pimd->ENk2[0], tri * sizeof(double), hipMemcpyHostToDevice));
+    p->enk2_dev = true;
+  }
+  double r = 0;
+  if (run_estimator(p, p->ENk2, &r)) return -1;
+  DevSim *d = p->ds;
+  const Geo g = geo_of(p, image);
+  int nblk = (int)std::min<long>((p->N * (long)(p->P - 1) + 255) / 256, 1024);
+  if (nblk < 1) nblk = 1;
+  hipLaunchKernelGGL(k_vir_term, dim3(nblk), dim3(256), 0, rt().s0, d->X, d->F, d->M, p->N, p->P, g, d->red);
+  XMD_CHECK(hipGetLastError());
+  double v = 0;
+  if (device_sum(d->red, nblk, &v)) return -1;
+  const int nb = p->N * p->P;
+  int nblk2 = std::min((nb + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_box_virial, dim3(nblk2), dim3(256), 0, rt().s0, d->X, d->F, d->M, nb, g, d->red);
+  XMD_CHECK(hipGetLastError());
+  double w = 0;
+  if (device_sum(d->red, nblk2, &w)) return -1;
+  stats->es[stats->N - 1] += 2.0 * r / MD_DIMENSION_X;
+  stats->es[stats->N - 1] += pimd->N / pimd->beta;
+  stats->es[stats->N - 1] += v / MD_DIMENSION_X;
+  stats->es[stats->N - 1] += w / MD_DIMENSION_X;
+  return 0;
+}
+
 int md_pimd_calc_vi_sign_x(md_pimd_t_x *pimd, md_stats_t_x *e_s, md_stats_t_x *s_s, md_num_t_x vi2) {
   if (ensure_pimd_device(pimd)) return -1;
   DevPimd *p = pimd_state(pimd, false);
@@ -1153,6 +1375,103 @@ int md_pimd_calc_vi_sign_x(md_pimd_t_x *pimd, md_stats_t_x *e_s, md_stats_t_x *s
   e_s->es[0] *= s;
   s_s->es[0] *= s;
   return 0;
+}
+
+// Single-value helpers of the exchange path (md_pimd_x.h:121,125,128,137,139,156,161,171),
+// evaluated on the device state.  Particle indices are the reference's
+// (l - 1) P + (j - 1).  NaN is returned when the device call fails.
+md_num_t_x md_pimd_ENk_x(md_pimd_t_x *pimd, int N2, int k, int image) {
+  DevPimd *p = helper_pimd(pimd);
+  if (!p || N2 < 1 || N2 > p->N || k < 1 || k > N2) return NAN;
+  hipLaunchKernelGGL(k_cycle_energy, dim3(1), dim3(256), 0, rt().s0, p->ds->X, p->ds->M, p->N, p->P, N2, k,
+                     pimd->omegaP, geo_of(p, image), p->work + 4 * (p->N + 1));
+  if (hipGetLastError() != hipSuccess) return NAN;
+  return fetch_scalar(p, 1, nullptr);
+}
+
+md_num_t_x md_pimd_ENk2_x(md_pimd_t_x *pimd, int N2, int k, int image) {
+  DevPimd *p = helper_pimd(pimd);
+  if (!p || N2 < 1 || N2 > p->N || k < 1 || k > N2) return NAN;
+  hipLaunchKernelGGL(k_cycle_virial, dim3(1), dim3(256), 0, rt().s0, p->ds->X, p->ds->M, p->N, p->P, N2, k, -1, -1,
+                     -1, pimd->omegaP, geo_of(p, image), p->work + 4 * (p->N + 1));
+  if (hipGetLastError() != hipSuccess) return NAN;
+  return fetch_scalar(p, 1, nullptr);
+}
+
+md_num_t_x md_pimd_fast_ENk2_x(md_pimd_t_x *pimd, int index, int index2, int index3, int image) {
+  DevPimd *p = helper_pimd(pimd);
+  const int nb = p ? p->N * p->P : 0;
+  if (!p || index < 0 || index >= nb || index2 < 0 || index2 >= nb || index3 < 0 || index3 >= nb) return NAN;
+  if (pimd->pbc) {
+    fprintf(stderr, "xmd: winding-number (pbc) cycle terms are not part of this engine\n");
+    return NAN;
+  }
+  hipLaunchKernelGGL(k_cycle_virial, dim3(1), dim3(64), 0, rt().s0, p->ds->X, p->ds->M, p->N, p->P, 1, 1,
+                     ref_to_slot(index, p->N, p->P), ref_to_slot(index2, p->N, p->P),
+                     ref_to_slot(index3, p->N, p->P), pimd->omegaP, geo_of(p, image), p->work + 4 * (p->N + 1));
+  if (hipGetLastError() != hipSuccess) return NAN;
+  return fetch_scalar(p, 1, nullptr);
+}
+
+md_num_t_x md_pimd_xminE_x(md_pimd_t_x *pimd, int N2) {
+  DevPimd *p = helper_pimd(pimd);
+  if (!p || N2 < 1 || N2 > p->N || ensure_enk(p) || ensure_vbn(p)) return NAN;
+  hipLaunchKernelGGL(k_lse_shift, dim3(1), dim3(256), 0, rt().s0, p->ENk, p->VBN, p->N, N2, 0, pimd->beta, pimd->vi,
+                     p->work + 4 * (p->N + 1));
+  if (hipGetLastError() != hipSuccess) return NAN;
+  return fetch_scalar(p, 1, nullptr);
+}
+
+md_num_t_x md_pimd_xminE2_x(md_pimd_t_x *pimd, int N2, md_num_t_x *VBN2, md_num_t_x vi2) {
+  DevPimd *p = helper_pimd(pimd);
+  if (!p || !VBN2 || N2 < 1 || N2 > p->N || ensure_enk(p)) return NAN;
+  double *V = stage_host_vector(p, VBN2);
+  if (!V) return NAN;
+  hipLaunchKernelGGL(k_lse_shift, dim3(1), dim3(256), 0, rt().s0, p->ENk, V, p->N, N2, 0, pimd->beta, vi2,
+                     p->work + 4 * (p->N + 1));
+  if (hipGetLastError() != hipSuccess) return NAN;
+  return fetch_scalar(p, 1, nullptr);
+}
+
+md_num_t_x md_pimd_fast_xminE_x(md_pimd_t_x *pimd, int u, md_num_t_x *V) {
+  DevPimd *p = helper_pimd(pimd);
+  if (!p || !V || u < 1 || u > p->N || ensure_enk(p)) return NAN;
+  double *Vd = stage_host_vector(p, V);
+  if (!Vd) return NAN;
+  hipLaunchKernelGGL(k_lse_shift, dim3(1), dim3(256), 0, rt().s0, p->ENk, Vd, p->N, u, 1, pimd->beta, pimd->vi,
+                     p->work + 4 * (p->N + 1));
+  if (hipGetLastError() != hipSuccess) return NAN;
+  return fetch_scalar(p, 1, nullptr);
+}
+
+void md_pimd_dENk_x(md_pimd_t_x *pimd, int N2, int k, int l, int j, int image, md_num_t_x *dENk) {
+  for (int i = 0; i < MD_DIMENSION_X; ++i) dENk[i] = 0;
+  DevPimd *p = helper_pimd(pimd);
+  if (!p || !(l >= N2 - k + 1 && l <= N2) || j < 1 || j > p->P || N2 > p->N) return;
+  const int P = p->P;
+  const int a = MD_PIMD_INDEX_X(l, j, P), b = md_pimd_next_index_x(l, j, N2, k, P),
+            c = md_pimd_prev_index_x(l, j, N2, k, P);
+  hipLaunchKernelGGL(k_spring_grad, dim3(1), dim3(64), 0, rt().s0, p->ds->X, p->ds->M, p->ds->nb,
+                     ref_to_slot(a, p->N, P), ref_to_slot(b, p->N, P), ref_to_slot(c, p->N, P), 1, 1, pimd->omegaP,
+                     geo_of(p, image), p->work + 4 * (p->N + 1));
+  if (hipGetLastError() != hipSuccess) return;
+  fetch_scalar(p, 3, dENk);
+}
+
+void md_pimd_fast_dENk_x(md_pimd_t_x *pimd, int index, int index2, int index3, int image, md_num_t_x *dENk) {
+  for (int i = 0; i < MD_DIMENSION_X; ++i) dENk[i] = 0;
+  DevPimd *p = helper_pimd(pimd);
+  const int nb = p ? p->N * p->P : 0;
+  if (!p || index < 0 || index >= nb || index2 < 0 || index2 >= nb || index3 < 0 || index3 >= nb) return;
+  const int P = p->P;
+  // a link that is cut at slice cut_Pj is dropped (md_pimd_x.c:2354-2357)
+  const int wb = (index + 1 != index2 || (index % P) + 1 != pimd->cut_Pj) ? 1 : 0;
+  const int wc = (index - 1 != index3 || (index % P) + 1 != pimd->cut_Pj + 1) ? 1 : 0;
+  hipLaunchKernelGGL(k_spring_grad, dim3(1), dim3(64), 0, rt().s0, p->ds->X, p->ds->M, p->ds->nb,
+                     ref_to_slot(index, p->N, P), ref_to_slot(index2, p->N, P), ref_to_slot(index3, p->N, P), wb, wc,
+                     pimd->omegaP, geo_of(p, image), p->work + 4 * (p->N + 1));
+  if (hipGetLastError() != hipSuccess) return;
+  fetch_scalar(p, 3, dENk);
 }
 
 }  // extern "C"

@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <vector>
 
 #include "xmd_internal.h"
 
@@ -34,6 +36,20 @@ void md_set_guassian_force_range_x(md_num_t_x s) { md_gaussian_range_x = s; }
 void md_set_coulomb_force_strength_x(md_num_t_x g) { md_coulomb_strength_x = g; }
 void md_set_coulomb_truc_x(md_num_t_x k) { md_coulomb_truc_x = k; }
 void md_set_coulomb_n_sum_x(int n) { md_coulomb_n_sum_x = n; }
+
+// md_simulation_x.c:365-383
+md_num_t_x md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C8_x, md_he_C10_x, md_he_D_x, md_he_rm_x;
+void md_set_helium_parameters_x(md_num_t_x eps, md_num_t_x A, md_num_t_x alpha, md_num_t_x C6, md_num_t_x C8,
+                                md_num_t_x C10, md_num_t_x D, md_num_t_x rm) {
+  md_he_eps_x = eps;
+  md_he_A_x = A;
+  md_he_alpha_x = alpha;
+  md_he_C6_x = C6;
+  md_he_C8_x = C8;
+  md_he_C10_x = C10;
+  md_he_D_x = D;
+  md_he_rm_x = rm;
+}
 
 // --------------------------------------------------------- image helpers
 // md_simulation_x.c:858-865
@@ -96,6 +112,34 @@ void md_periodic_gaussian_force_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *f,
   double d = md_minimum_image_distance_x(p1, p2, L);
   double w = (md_gaussian_strength_x / (M_PI * s2)) * exp(-d * d / s2);
   for (int i = 0; i < MD_DIMENSION_X; ++i) f[i] += w * (2 * md_minimum_image_x(p1[i] - p2[i], L[i])) / s2 / m;
+}
+
+// Lennard-Jones, minimum image (md_simulation_x.c:116-122)
+void md_LJ_periodic_force_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *f, md_num_t_x *L, md_num_t_x m) {
+  const double r = md_minimum_image_distance_x(p1, p2, L);
+  const double w = 48 * pow(r, -14.0) - 24 * pow(r, -8.0);
+  for (int i = 0; i < MD_DIMENSION_X; ++i) f[i] += md_minimum_image_x(p1[i] - p2[i], L[i]) * w / m;
+}
+
+// helium damping function and the scalar parts of force and energy (md_simulation_x.c:385-399)
+static double he_damp(double x) {
+  if (x < md_he_D_x) {
+    const double u = md_he_D_x / x - 1;
+    return exp(-u * u);
+  }
+  return 1.0;
+}
+
+void md_periodic_helium_force_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *f, md_num_t_x *L, md_num_t_x m) {
+  const double r = md_minimum_image_distance_x(p1, p2, L);
+  const double x = r / md_he_rm_x, F = he_damp(x);
+  double w = md_he_alpha_x * md_he_A_x * exp(-md_he_alpha_x * x) -
+             (6 * md_he_C6_x * pow(x, -7) + 8 * md_he_C8_x * pow(x, -9) + 10 * md_he_C10_x * pow(x, -11)) * F;
+  if (x < md_he_D_x)
+    w += (md_he_C6_x * pow(x, -6) + md_he_C8_x * pow(x, -8) + md_he_C10_x * pow(x, -10)) * F * 2 * md_he_D_x *
+         (md_he_D_x / x - 1) / (x * x);
+  for (int i = 0; i < MD_DIMENSION_X; ++i)
+    f[i] += md_he_eps_x * w * md_minimum_image_x(p1[i] - p2[i], L[i]) / (r * md_he_rm_x * m);
 }
 
 static void ewald_pair(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *f, md_num_t_x *L, md_num_t_x m, int mode) {
@@ -164,6 +208,19 @@ md_num_t_x md_periodic_gaussian_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_
   double s2 = md_gaussian_range_x * md_gaussian_range_x;
   double d = md_minimum_image_distance_x(p1, p2, L);
   return (md_gaussian_strength_x / (M_PI * s2)) * exp(-d * d / s2);
+}
+
+// md_statistics_x.c:317-320
+md_num_t_x md_LJ_periodic_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x MD_UNUSED_X(m)) {
+  const double r = md_minimum_image_distance_x(p1, p2, L);
+  return 4 * (pow(r, -12.0) - pow(r, -6.0));
+}
+
+// md_statistics_x.c:496-506
+md_num_t_x md_periodic_helium_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x MD_UNUSED_X(m)) {
+  const double x = md_minimum_image_distance_x(p1, p2, L) / md_he_rm_x;
+  const double disp = md_he_C6_x * pow(x, -6) + md_he_C8_x * pow(x, -8) + md_he_C10_x * pow(x, -10);
+  return md_he_eps_x * (md_he_A_x * exp(-md_he_alpha_x * x) - disp * he_damp(x));
 }
 
 md_num_t_x md_coulomb_3d_ewald_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x MD_UNUSED_X(m)) {
@@ -554,6 +611,52 @@ md_stats_t_x *md_alloc_stats_x(int points) {
 
 md_stats_t_x *md_stats_sync_host_x(md_stats_t_x *stats) { return stats; }
 
+// Spherical S(q) grid (md_statistics_x.c:104-161): q = (qx, qy, qz) on the
+// cube q0, q0 + qincre, ... <= qmax per axis (the axis values accumulate by
+// repeated addition, as in the reference's loops), |q| < 1e-4 dropped; the
+// first vector of each |q| shell (shells closer than 1e-4 merged) is its
+// representative, and the shells are sorted by |q|.
+md_sk_sphere_t_x *md_alloc_sk_sphere_3d_x(md_num_t_x q0, md_num_t_x qincre, md_num_t_x qmax) {
+  if (!(qincre > 0)) return nullptr;
+  std::vector<double> axis;
+  for (double q = q0; q <= qmax; q += qincre) axis.push_back(q);
+  std::vector<double> qs, qis;
+  for (double qx : axis)
+    for (double qy : axis)
+      for (double qz : axis) {
+        const double q = sqrt(qx * qx + qy * qy + qz * qz);
+        if (fabs(q) < 1e-4) continue;
+        bool found = false;
+        for (double v : qs)
+          if (fabs(v - q) < 1e-4) {
+            found = true;
+            break;
+          }
+        if (found) continue;
+        qs.push_back(q);
+        qis.insert(qis.end(), {qx, qy, qz});
+      }
+  std::vector<int> ord(qs.size());
+  for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return qs[a] < qs[b]; });
+  md_sk_sphere_t_x *res = (md_sk_sphere_t_x *)malloc(sizeof(md_sk_sphere_t_x));
+  if (!res) return nullptr;
+  res->N = (int)qs.size();
+  res->qs = (md_num_t_x *)malloc(sizeof(md_num_t_x) * (qs.size() + 1));
+  res->qis = (md_num_t_x *)malloc(sizeof(md_num_t_x) * (qis.size() + 3));
+  if (!res->qs || !res->qis) {
+    free(res->qs);
+    free(res->qis);
+    free(res);
+    return nullptr;
+  }
+  for (int i = 0; i < res->N; ++i) {
+    res->qs[i] = qs[ord[i]];
+    for (int d = 0; d < 3; ++d) res->qis[3 * i + d] = qis[3 * ord[i] + d];
+  }
+  return res;
+}
+
 md_stats_t_x *md_update_stats_x(md_stats_t_x *s) {
   s->count++;
   if (s->count >= MD_MAX_STATS_COUNT) {
@@ -611,6 +714,25 @@ int md_stats_copy_to_x(md_stats_t_x *stats, md_stats_t_x *stats2) {
 }
 int md_stats_mult_to_x(md_stats_t_x *stats, md_stats_t_x *stats2) {
   stats->es[0] *= stats2->es[0];
+  return 0;
+}
+// Histogram forms of the same helpers (md_pimd_x.c:4565-4650).
+int md_reset_stats_f_x(md_stats_t_x *stats, md_num_t_x e) {
+  for (int i = 0; i < stats->points; ++i) stats->fxs[0][i] = e;
+  return 0;
+}
+int md_stats_add_to_f_x(md_stats_t_x *stats, md_stats_t_x *stats2) {
+  if (stats->points != stats2->points) return -1;
+  for (int i = 0; i < stats->points; ++i) stats->fxs[stats->N - 1][i] += stats2->fxs[0][i];
+  return 0;
+}
+int md_stats_copy_to_f_x(md_stats_t_x *stats, md_stats_t_x *stats2) {
+  if (stats->points != stats2->points) return -1;
+  for (int i = 0; i < stats->points; ++i) stats->fxs[0][i] = stats2->fxs[0][i];
+  return 0;
+}
+int md_stats_mult_to_f_x(md_stats_t_x *stats, md_stats_t_x *stats2) {
+  for (int i = 0; i < stats->points; ++i) stats->fxs[0][i] *= stats2->es[0];
   return 0;
 }
 

@@ -48,6 +48,7 @@ struct PairParams {
   double rc2;          // real-space cutoff^2
   int nsum;            // Ewald cube half-width
   double P;            // beads (pair forces are divided by P)
+  double he[8];        // helium: eps, A, alpha, C6, C8, C10, D, r_m
 };
 
 // Real-space Ewald contribution of one raw separation (p1 - p2) to the
@@ -244,6 +245,49 @@ __device__ __forceinline__ void pair_term(double dx, double dy, double dz, const
       gx += w * (2 * dx) / pp.s2;
       gy += w * (2 * dy) / pp.s2;
       gz += w * (2 * dz) / pp.s2;
+    }
+  } else if (MODE == PK_LJ || MODE == PK_HE) {
+    dx = pf_min_image(dx, pp.L[0]);
+    dy = pf_min_image(dy, pp.L[1]);
+    dz = pf_min_image(dz, pp.L[2]);
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (MODE == PK_LJ) {
+      // 4 (r^-12 - r^-6); force factor 48 r^-14 - 24 r^-8 (md_simulation_x.c:116-122,
+      // md_statistics_x.c:317-320)
+      const double i2 = 1.0 / (r * r), i6 = i2 * i2 * i2;
+      if (ENERGY) {
+        e += 4 * (i6 * i6 - i6);
+      } else {
+        const double w = (48 * i6 - 24) * i6 * i2;
+        gx += dx * w;
+        gy += dy * w;
+        gz += dz * w;
+      }
+    } else {
+      // Aziz-type helium potential with damping F(x) below x = D, x = r / r_m
+      // (md_simulation_x.c:385-399, md_statistics_x.c:496-506)
+      const double eps = pp.he[0], A = pp.he[1], al = pp.he[2], C6 = pp.he[3], C8 = pp.he[4], C10 = pp.he[5],
+                   D = pp.he[6], rm = pp.he[7];
+      const double x = r / rm, ix = 1.0 / x, ix2 = ix * ix;
+      const double ix6 = ix2 * ix2 * ix2, ix8 = ix6 * ix2, ix10 = ix8 * ix2;
+      const bool damp = x < D;
+      double F = 1.0;
+      if (damp) {
+        const double u = D * ix - 1;
+        F = exp(-u * u);
+      }
+      const double disp = C6 * ix6 + C8 * ix8 + C10 * ix10;
+      const double rep = A * exp(-al * x);
+      if (ENERGY) {
+        e += eps * (rep - disp * F);
+      } else {
+        double mult = al * rep - (6 * C6 * ix6 + 8 * C8 * ix8 + 10 * C10 * ix10) * ix * F;
+        if (damp) mult += disp * F * 2 * D * (D * ix - 1) * ix2;
+        const double w = eps * mult / (r * rm);
+        gx += dx * w;
+        gy += dy * w;
+        gz += dz * w;
+      }
     }
   } else if (NEAR) {
     ewald_real_near<!ENERGY, ENERGY>(dx, dy, dz, pp, MODE == PK_EWALD_R, MODE == PK_EWALD_NI, gx, gy, gz, e);
@@ -892,6 +936,8 @@ int pair_kind_of(md_pair_force_t_x pf) {
   if (pf == md_coulomb_3d_ewald_force_x) return PK_EWALD;
   if (pf == md_coulomb_3d_ewald_force_R_x) return PK_EWALD_R;
   if (pf == md_coulomb_3d_ewald_force_NI_x) return PK_EWALD_NI;
+  if (pf == md_LJ_periodic_force_x) return PK_LJ;
+  if (pf == md_periodic_helium_force_x) return PK_HE;
   return -1;
 }
 
@@ -899,6 +945,8 @@ int pair_energy_kind_of(md_pair_energy_t_x pe) {
   if (pe == md_gaussian_energy_x) return PK_GAUSS;
   if (pe == md_periodic_gaussian_energy_x) return PK_PGAUSS;
   if (pe == md_coulomb_3d_ewald_energy_x) return PK_EWALD;
+  if (pe == md_LJ_periodic_energy_x) return PK_LJ;
+  if (pe == md_periodic_helium_energy_x) return PK_HE;
   return -1;
 }
 
@@ -917,6 +965,9 @@ static PairParams make_params(DevPimd *p) {
   pp.rc2 = rc * rc;
   pp.nsum = md_coulomb_n_sum_x;
   pp.P = (double)p->P;
+  const double he[8] = {md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x,
+                        md_he_C8_x,  md_he_C10_x, md_he_D_x,   md_he_rm_x};
+  for (int k = 0; k < 8; ++k) pp.he[k] = he[k];
   return pp;
 }
 
@@ -1146,6 +1197,10 @@ int launch_pair_real(DevPimd *p, int kind, hipStream_t st) {
       return launch_pair_mode<PK_GAUSS, false>(p, pp, st, nullptr);
     case PK_PGAUSS:
       return launch_pair_mode<PK_PGAUSS, false>(p, pp, st, nullptr);
+    case PK_LJ:
+      return launch_pair_mode<PK_LJ, false>(p, pp, st, nullptr);
+    case PK_HE:
+      return launch_pair_mode<PK_HE, false>(p, pp, st, nullptr);
     case PK_EWALD:
     case PK_EWALD_R:
     case PK_EWALD_NI:
@@ -1193,6 +1248,10 @@ int pair_energy(DevPimd *p, int kind, double *out) {
     if (launch_pair_mode<PK_GAUSS, true>(p, pp, st, &e)) return -1;
   } else if (kind == PK_PGAUSS) {
     if (launch_pair_mode<PK_PGAUSS, true>(p, pp, st, &e)) return -1;
+  } else if (kind == PK_LJ) {
+    if (launch_pair_mode<PK_LJ, true>(p, pp, st, &e)) return -1;
+  } else if (kind == PK_HE) {
+    if (launch_pair_mode<PK_HE, true>(p, pp, st, &e)) return -1;
   } else if (kind == PK_EWALD) {
     EwaldPlan pl;
     pp = ewald_params(p, pl);
@@ -1280,14 +1339,33 @@ __global__ __launch_bounds__(256) void k_pair2_energy(const double *__restrict__
   if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
+static bool pair2_kind_ok(int kind) { return kind == PK_GAUSS || kind == PK_PGAUSS || kind == PK_LJ || kind == PK_HE; }
+
 static int pair2_check(md_pimd_t_x *a, md_pimd_t_x *b, int kind) {
   if (a->P != b->P) return -1;
-  if (kind != PK_GAUSS && kind != PK_PGAUSS) {
-    fprintf(stderr, "xmd: two-species pair kernels cover the Gaussian interactions only\n");
+  if (!pair2_kind_ok(kind)) {
+    fprintf(stderr, "xmd: two-species pair kernels cover the Gaussian, Lennard-Jones and helium interactions\n");
     return -1;
   }
   if (ensure_pimd_device(a) || ensure_pimd_device(b)) return -1;
   return 0;
+}
+
+template <int MODE>
+static void launch_pair2_force(DevPimd *a, DevPimd *b, const PairParams &pp, hipStream_t st) {
+  DevSim *da = a->ds, *db = b->ds;
+  const dim3 ga((a->N + 255) / 256, a->P), gb((b->N + 255) / 256, b->P);
+  hipLaunchKernelGGL((k_pair2_force<MODE, false>), ga, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb, db->X,
+                     db->M, db->F, b->N, db->nb, pp);
+  hipLaunchKernelGGL((k_pair2_force<MODE, true>), gb, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb, db->X,
+                     db->M, db->F, b->N, db->nb, pp);
+}
+
+template <int MODE>
+static void launch_pair2_energy(DevPimd *a, DevPimd *b, const PairParams &pp, hipStream_t st, double *part) {
+  DevSim *da = a->ds, *db = b->ds;
+  const dim3 g((a->N + 255) / 256, a->P);
+  hipLaunchKernelGGL(k_pair2_energy<MODE>, g, dim3(256), 0, st, da->X, a->N, da->nb, db->X, b->N, db->nb, pp, part);
 }
 
 }  // namespace xmd
@@ -1328,18 +1406,11 @@ int md_pimd_calc_pair_force_2_x(md_pimd_t_x *pimd, md_pimd_t_x *pimd2, md_pair_f
   DevPimd *a = pimd_state(pimd, false), *b = pimd_state(pimd2, false);
   PairParams pp = make_params(a);
   hipStream_t st = rt().s0;
-  DevSim *da = a->ds, *db = b->ds;
-  const dim3 ga((a->N + 255) / 256, a->P), gb((b->N + 255) / 256, b->P);
-  if (kind == PK_GAUSS) {
-    hipLaunchKernelGGL((k_pair2_force<PK_GAUSS, false>), ga, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb,
-                       db->X, db->M, db->F, b->N, db->nb, pp);
-    hipLaunchKernelGGL((k_pair2_force<PK_GAUSS, true>), gb, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb, db->X,
-                       db->M, db->F, b->N, db->nb, pp);
-  } else {
-    hipLaunchKernelGGL((k_pair2_force<PK_PGAUSS, false>), ga, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb,
-                       db->X, db->M, db->F, b->N, db->nb, pp);
-    hipLaunchKernelGGL((k_pair2_force<PK_PGAUSS, true>), gb, dim3(256), 0, st, da->X, da->M, da->F, a->N, da->nb,
-                       db->X, db->M, db->F, b->N, db->nb, pp);
+  switch (kind) {
+    case PK_GAUSS: launch_pair2_force<PK_GAUSS>(a, b, pp, st); break;
+    case PK_PGAUSS: launch_pair2_force<PK_PGAUSS>(a, b, pp, st); break;
+    case PK_LJ: launch_pair2_force<PK_LJ>(a, b, pp, st); break;
+    default: launch_pair2_force<PK_HE>(a, b, pp, st); break;
   }
   XMD_CHECK(hipGetLastError());
   return 0;
@@ -1352,17 +1423,15 @@ int md_pimd_calc_pair_energy_2_x(md_pimd_t_x *pimd, md_pimd_t_x *pimd2, md_pair_
   DevPimd *a = pimd_state(pimd, false), *b = pimd_state(pimd2, false);
   PairParams pp = make_params(a);
   hipStream_t st = rt().s0;
-  DevSim *da = a->ds, *db = b->ds;
-  const dim3 g((a->N + 255) / 256, a->P);
-  const int nblk = g.x * g.y;
+  const int nblk = ((a->N + 255) / 256) * a->P;
   double *part = nullptr;
   XMD_CHECK(hipMallocAsync((void **)&part, nblk * sizeof(double), st));
-  if (kind == PK_GAUSS)
-    hipLaunchKernelGGL(k_pair2_energy<PK_GAUSS>, g, dim3(256), 0, st, da->X, a->N, da->nb, db->X, b->N, db->nb, pp,
-                       part);
-  else
-    hipLaunchKernelGGL(k_pair2_energy<PK_PGAUSS>, g, dim3(256), 0, st, da->X, a->N, da->nb, db->X, b->N, db->nb, pp,
-                       part);
+  switch (kind) {
+    case PK_GAUSS: launch_pair2_energy<PK_GAUSS>(a, b, pp, st, part); break;
+    case PK_PGAUSS: launch_pair2_energy<PK_PGAUSS>(a, b, pp, st, part); break;
+    case PK_LJ: launch_pair2_energy<PK_LJ>(a, b, pp, st, part); break;
+    default: launch_pair2_energy<PK_HE>(a, b, pp, st, part); break;
+  }
   XMD_CHECK(hipGetLastError());
   double e = 0;
   const int rc = device_sum(part, nblk, &e);

@@ -134,7 +134,11 @@ int launch_pbc(DevSim *d, const double *L, hipStream_t st);
 int launch_nhc(DevSim *d, int half, double h, hipStream_t st);
 int launch_ring_springs(DevPimd *p, int image, const double *L, hipStream_t st);
 
-enum PairKind { PK_GAUSS = 0, PK_PGAUSS = 1, PK_EWALD = 2, PK_EWALD_R = 3, PK_EWALD_NI = 4 };
+enum PairKind { PK_GAUSS = 0, PK_PGAUSS = 1, PK_EWALD = 2, PK_EWALD_R = 3, PK_EWALD_NI = 4, PK_LJ = 5, PK_HE = 6 };
+
+// helium potential parameters (md_simulation_x.c:365-383), set by md_set_helium_parameters_x
+extern "C" md_num_t_x md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C8_x, md_he_C10_x, md_he_D_x,
+    md_he_rm_x;
 int pair_kind_of(md_pair_force_t_x pf);
 int pair_energy_kind_of(md_pair_energy_t_x pe);
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st);

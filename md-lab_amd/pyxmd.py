@@ -35,6 +35,10 @@ class Rect(C.Structure):
     _fields_ = [("L", C.c_double * 3)]
 
 
+class SkSphere(C.Structure):
+    _fields_ = [("N", C.c_int), ("qs", C.POINTER(C.c_double)), ("qis", C.POINTER(C.c_double))]
+
+
 class IndexPair(C.Structure):
     _fields_ = [("i", C.c_int), ("j", C.c_int)]
 
@@ -140,6 +144,36 @@ _SIGS = {
     "md_pimd_calc_pair_correlation_x": (C.c_int, [PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int, C.c_int]),
     "md_pimd_calc_Sk_structure_x": (C.c_int, [PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int]),
     "md_pimd_calc_pair_energy_2_x": (C.c_int, [PIMD_P, PIMD_P, C.c_void_p, STATS_P]),
+    "md_set_helium_parameters_x": (None, [C.c_double] * 8),
+    "md_pimd_calc_ITCF_x": (C.c_int, [PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int]),
+    "md_pimd_calc_ITCF_2_x": (C.c_int, [PIMD_P, PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int]),
+    "md_pimd_calc_pair_correlation_2_x": (C.c_int, [PIMD_P, PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int,
+                                                    C.c_int]),
+    "md_pimd_calc_pair_correlation_2d_x": (C.c_int, [PIMD_P, STATS_P, C.c_double, C.c_double, DP]),
+    "md_pimd_calc_2d_pair_correlation_x": (C.c_int, [PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int, C.c_int]),
+    "md_pimd_calc_2d_pair_correlation_2_x": (C.c_int, [PIMD_P, PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int,
+                                                       C.c_int]),
+    "md_pimd_calc_Sk_structure_2_x": (C.c_int, [PIMD_P, PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int]),
+    "md_pimd_calc_Sk_structure_full_2_x": (C.c_int, [PIMD_P, PIMD_P, STATS_P, C.c_double, C.c_double, C.c_int]),
+    "md_alloc_sk_sphere_3d_x": (C.POINTER(SkSphere), [C.c_double, C.c_double, C.c_double]),
+    "md_pimd_calc_sphere_Sk_structure_x": (C.c_int, [PIMD_P, STATS_P, C.c_int, C.POINTER(SkSphere)]),
+    "md_pimd_calc_sphere_Sk_structure_full_2_x": (C.c_int, [PIMD_P, PIMD_P, STATS_P, C.c_int,
+                                                            C.POINTER(SkSphere)]),
+    "md_pimd_calc_sphere_Sk_structure_3_x": (C.c_int, [PIMD_P, PIMD_P, PIMD_P, STATS_P, C.c_int,
+                                                       C.POINTER(SkSphere)]),
+    "md_pimd_calc_vi_virial_pressure_x": (C.c_int, [PIMD_P, STATS_P, C.c_int]),
+    "md_reset_stats_f_x": (C.c_int, [STATS_P, C.c_double]),
+    "md_pimd_ENk_x": (C.c_double, [PIMD_P, C.c_int, C.c_int, C.c_int]),
+    "md_pimd_ENk2_x": (C.c_double, [PIMD_P, C.c_int, C.c_int, C.c_int]),
+    "md_pimd_fast_ENk2_x": (C.c_double, [PIMD_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "md_pimd_xminE_x": (C.c_double, [PIMD_P, C.c_int]),
+    "md_pimd_xminE2_x": (C.c_double, [PIMD_P, C.c_int, DP, C.c_double]),
+    "md_pimd_fast_xminE_x": (C.c_double, [PIMD_P, C.c_int, DP]),
+    "md_pimd_dENk_x": (None, [PIMD_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, DP]),
+    "md_pimd_fast_dENk_x": (None, [PIMD_P, C.c_int, C.c_int, C.c_int, C.c_int, DP]),
+    "md_stats_add_to_f_x": (C.c_int, [STATS_P, STATS_P]),
+    "md_stats_copy_to_f_x": (C.c_int, [STATS_P, STATS_P]),
+    "md_stats_mult_to_f_x": (C.c_int, [STATS_P, STATS_P]),
     "md_checkpoint_read_x": (C.c_int, [C.c_void_p, SIM_P]),
     "md_pimd_get_backward_potentials_x": (C.c_int, [PIMD_P, DP]),
     "md_pimd_get_connection_matrix_x": (C.c_int, [PIMD_P, DP]),
@@ -151,7 +185,8 @@ _SIGS = {
 CALLBACKS = ["md_harmonic_trap_force_x", "md_gaussian_force_x", "md_periodic_gaussian_force_x",
              "md_coulomb_3d_ewald_force_x", "md_coulomb_3d_ewald_force_R_x", "md_coulomb_3d_ewald_force_NI_x",
              "md_harmonic_trap_energy_x", "md_gaussian_energy_x", "md_periodic_gaussian_energy_x",
-             "md_coulomb_3d_ewald_energy_x"]
+             "md_coulomb_3d_ewald_energy_x", "md_LJ_periodic_force_x", "md_LJ_periodic_energy_x",
+             "md_periodic_helium_force_x", "md_periodic_helium_energy_x"]
 
 
 class Lib:

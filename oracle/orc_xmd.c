@@ -126,6 +126,22 @@ void orc_pair_force_one(int kind, const orc_params_t *p, const double *p1, const
       for (int d = 0; d < 3; ++d) f[d] += orc_minimum_image(p1[d] - p2[d], L[d]) * w / m;
       break;
     }
+    case ORC_PAIR_LJ: { /* md_simulation_x.c:116-122 */
+      double r = mi_dist(p1, p2, L);
+      double mult = 48 * pow(r, -14.0) - 24 * pow(r, -8.0);
+      for (int i = 0; i < 3; ++i) f[i] += orc_minimum_image(p1[i] - p2[i], L[i]) * mult / m;
+      break;
+    }
+    case ORC_PAIR_HELIUM: { /* md_simulation_x.c:385-399 */
+      const double *h = p->he; /* eps, A, alpha, C6, C8, C10, D, rm */
+      double r = mi_dist(p1, p2, L);
+      double x = r / h[7];
+      double F = x < h[6] ? exp(-pow(h[6] / x - 1, 2)) : 1;
+      double mult = h[2] * h[1] * exp(-h[2] * x) - (6 * h[3] * pow(x, -7) + 8 * h[4] * pow(x, -9) + 10 * h[5] * pow(x, -11)) * F;
+      if (x < h[6]) mult += (h[3] * pow(x, -6) + h[4] * pow(x, -8) + h[5] * pow(x, -10)) * F * 2 * h[6] * (h[6] / x - 1) / (x * x);
+      for (int i = 0; i < 3; ++i) f[i] += h[0] * mult * orc_minimum_image(p1[i] - p2[i], L[i]) / (r * h[7] * m);
+      break;
+    }
     default:
       break;
   }
@@ -166,6 +182,16 @@ double orc_pair_energy_one(int kind, const orc_params_t *p, const double *p1, co
             res += erfc(al * R) / R;
           }
       return p->coul_q * res;
+    }
+    case ORC_PAIR_LJ: { /* md_statistics_x.c:317-320 */
+      double r = mi_dist(p1, p2, L);
+      return 4 * (pow(r, -12) - pow(r, -6));
+    }
+    case ORC_PAIR_HELIUM: { /* md_statistics_x.c:496-506 */
+      const double *h = p->he;
+      double x = mi_dist(p1, p2, L) / h[7];
+      double F = x < h[6] ? exp(-pow(h[6] / x - 1, 2)) : 1;
+      return h[0] * (h[1] * exp(-h[2] * x) - (h[3] * pow(x, -6) + h[4] * pow(x, -8) + h[5] * pow(x, -10)) * F);
     }
     default:
       return 0;
@@ -309,6 +335,145 @@ void orc_sk_structure(int N, int P, const orc_particle_t *b, double q0, double q
       res += sum * sum2 / N;
     }
     fx[j] += res / 3;
+  }
+}
+
+/* 2-D bin index with the reference's clamping (md_pimd_x.c:2739-2744). */
+static int clamp_bin(int i1, int i2, int n, int points) {
+  int idx = i1 * n + i2;
+  if (idx < 0) idx = 0;
+  if (idx >= points) idx = points - 1;
+  return idx;
+}
+
+/* md_pimd_x.c:2723-2747 (two = 0; a == b, every l, j incl. l == j) and
+ * md_pimd_x.c:2841-2872 (two = 1; a against b, both orders): 2-D histogram of
+ * (x_{l,1}, x_{j,pi}) along dimension 0, += 1/P per entry. */
+void orc_itcf(int NA, int NB, int P, const orc_particle_t *a, const orc_particle_t *b, double rmin, double rmax,
+              int pi, int points, int two, double *fx) {
+  int n = (int)sqrt(points);
+  double incre = (rmax - rmin) / n;
+  for (int l = 1; l <= NA; ++l)
+    for (int j = 1; j <= NB; ++j) {
+      int i1 = (int)((a[BEAD(l, 1, P)].x[0] - rmin) / incre);
+      int i2 = (int)((b[BEAD(j, pi, P)].x[0] - rmin) / incre);
+      fx[clamp_bin(i1, i2, n, points)] += 1.0 / P;
+      if (two) fx[clamp_bin(i2, i1, n, points)] += 1.0 / P;
+    }
+}
+
+/* md_pimd_x.c:3111-3138: bead 1 of every l of a against bead pi of every j of b. */
+void orc_pair_correlation_2(int NA, int NB, int P, const orc_particle_t *a, const orc_particle_t *b, const double *L,
+                            int image, double rmax, double norm, int pi, int points, double *fx) {
+  double incre = rmax / points;
+  for (int l = 1; l <= NA; ++l)
+    for (int j = 1; j <= NB; ++j) {
+      double r = obs_dist(a[BEAD(l, 1, P)].x, b[BEAD(j, pi, P)].x, L, image);
+      int k = (int)(r / incre);
+      if (k >= points) continue;
+      fx[k] += 1.0 / norm;
+    }
+}
+
+/* md_pimd_x.c:3248-3280: in-plane distances of bead pairs of one slice whose
+ * two beads both lie in the window box = {x0, x1, y0, y1}. */
+void orc_pair_correlation_2d(int N, int P, const orc_particle_t *b, double rmax, double norm, const double *box,
+                             int points, double *fx) {
+  double incre = rmax / points;
+  for (int pj = 1; pj <= P; ++pj)
+    for (int l = 1; l <= N; ++l)
+      for (int j = 1; j <= N; ++j) {
+        if (l == j) continue;
+        const double *u = b[BEAD(l, pj, P)].x, *w = b[BEAD(j, pj, P)].x;
+        if (u[0] < box[0] || u[0] > box[1] || u[1] < box[2] || u[1] > box[3]) continue;
+        if (w[0] < box[0] || w[0] > box[1] || w[1] < box[2] || w[1] > box[3]) continue;
+        double r = sqrt(pow(u[0] - w[0], 2) + pow(u[1] - w[1], 2));
+        int k = (int)(r / incre);
+        if (k >= points) continue;
+        fx[k] += 1.0 / norm;
+      }
+}
+
+/* md_pimd_x.c:4869-4911 (same = 1, a == b, l != j) and 4913-4956 (same = 0):
+ * 2-D histogram of |dx|, |dy| between bead 1 of l and bead pi of j. */
+void orc_2d_pair_correlation(int NA, int NB, int P, const orc_particle_t *a, const orc_particle_t *b,
+                             const double *L, int image, double rmax, double norm, int pi, int points, int same,
+                             double *fx) {
+  int n = (int)sqrt(points);
+  double incre = rmax / n;
+  for (int l = 1; l <= NA; ++l)
+    for (int j = 1; j <= NB; ++j) {
+      if (same && l == j) continue;
+      const double *u = a[BEAD(l, 1, P)].x, *w = b[BEAD(j, pi, P)].x;
+      double dx = u[0] - w[0], dy = u[1] - w[1];
+      if (image) {
+        dx = orc_minimum_image(dx, L[0]);
+        dy = orc_minimum_image(dy, L[1]);
+      }
+      int i1 = (int)(fabs(dx) / incre), i2 = (int)(fabs(dy) / incre);
+      fx[clamp_bin(i1, i2, n, points)] += 1.0 / norm;
+    }
+}
+
+/* Sum of cos / sin (q . x) over bead slot `bead` of every particle. */
+static void cs_sum(int N, int P, const orc_particle_t *b, int bead, const double *qv, int axis, double q,
+                   double *c, double *s) {
+  for (int l = 1; l <= N; ++l) {
+    const double *x = b[BEAD(l, bead, P)].x;
+    double t;
+    if (axis >= 0) {
+      t = q * x[axis];
+    } else {
+      t = 0;
+      for (int i = 0; i < 3; ++i) t += qv[i] * x[i];
+    }
+    *c += cos(t);
+    *s += sin(t);
+  }
+}
+
+/* md_pimd_x.c:5050-5083 (full = 0: bead 1 of a against bead pi of b, / N_a)
+ * and 5150-5203 (full = 1: a and b together on both sides, / (N_a + N_b)). */
+void orc_sk_structure_2(int NA, int NB, int P, const orc_particle_t *a, const orc_particle_t *b, double q0,
+                        double qincre, int pi, int points, int full, double *fx) {
+  for (int j = 0; j < points; ++j) {
+    double q = q0 + j * qincre, res = 0;
+    double norm = full ? NA + NB : NA;
+    for (int i = 0; i < 3; ++i) {
+      double c1 = 0, s1 = 0, c2 = 0, s2 = 0;
+      cs_sum(NA, P, a, 1, NULL, i, q, &c1, &s1);
+      if (full) cs_sum(NB, P, b, 1, NULL, i, q, &c1, &s1);
+      if (full) cs_sum(NA, P, a, pi, NULL, i, q, &c2, &s2);
+      cs_sum(NB, P, b, pi, NULL, i, q, &c2, &s2);
+      res += c1 * c2 / norm;
+      res += s1 * s2 / norm;
+    }
+    fx[j] += res / 3;
+  }
+}
+
+/* md_pimd_x.c:5404-5450 (variant 0: a alone), 5266-5340 (variant 1: a and b
+ * on both sides, / (N_a + N_b)) and 5342-5402 (variant 2: a and b against c,
+ * / sqrt((N_a + N_b) N_c)); q vectors qis[3 j .. 3 j + 2], second bead
+ * (pi - 1) % P + 1. */
+void orc_sphere_sk(int variant, int NA, int NB, int NC, int P, const orc_particle_t *a, const orc_particle_t *b,
+                   const orc_particle_t *c, int pi, const double *qis, int nq, double *fx) {
+  int b2 = (pi - 1) % P + 1;
+  for (int j = 0; j < nq; ++j) {
+    const double *qv = qis + 3 * j;
+    double c1 = 0, s1 = 0, c2 = 0, s2 = 0, norm;
+    cs_sum(NA, P, a, 1, qv, -1, 0, &c1, &s1);
+    if (variant >= 1) cs_sum(NB, P, b, 1, qv, -1, 0, &c1, &s1);
+    if (variant <= 1) cs_sum(NA, P, a, b2, qv, -1, 0, &c2, &s2);
+    if (variant == 1) cs_sum(NB, P, b, b2, qv, -1, 0, &c2, &s2);
+    if (variant == 2) cs_sum(NC, P, c, b2, qv, -1, 0, &c2, &s2);
+    if (variant == 0)
+      norm = NA;
+    else if (variant == 1)
+      norm = NA + NB;
+    else
+      norm = sqrt((double)((NA + NB) * NC));
+    fx[j] += c1 * c2 / norm + s1 * s2 / norm;
   }
 }
 
@@ -653,6 +818,50 @@ double orc_vi_virial_energy(int N, int P, double beta, double vi, int image, con
       }
     }
   return e + r2 / 2.0;
+}
+
+/* md_pimd_x.c:5452-5517 (pbc = 0): the virial recursion of the energy
+ * estimator enters as 2 res[N] / 3 + N / beta, plus the bead-1-relative and
+ * box-centre-relative virials, each / 3. */
+double orc_vi_virial_pressure(int N, int P, double beta, double vi, int image, const double *L,
+                              const double *ENk, const double *ENk2, const double *VBN,
+                              const orc_particle_t *b) {
+  double *res = (double *)malloc(sizeof(double) * (N + 1));
+  res[0] = 0;
+  for (int n = 1; n <= N; ++n) {
+    double t = xminE(n, beta, vi, ENk, VBN);
+    double s2 = 0, s = 0;
+    for (int k = 1; k <= n; ++k) {
+      if (vi == 0 && k != 1) continue;
+      s2 += xexp(k, ENk[TRI(n, k)] + VBN[n - k], t, beta, vi);
+    }
+    for (int k = 1; k <= n; ++k) {
+      if (vi == 0 && k != 1) continue;
+      s += (res[n - k] - ENk2[TRI(n, k)]) * xexp(k, ENk[TRI(n, k)] + VBN[n - k], t, beta, vi);
+    }
+    res[n] = s / s2;
+  }
+  double e = 2.0 * res[N] / 3;
+  e += N / beta;
+  free(res);
+  double r2 = 0;
+  for (int j = 2; j <= P; ++j)
+    for (int l = 1; l <= N; ++l) {
+      const orc_particle_t *q = &b[BEAD(l, j, P)], *q1 = &b[BEAD(l, 1, P)];
+      for (int i = 0; i < 3; ++i) {
+        double d = q->x[i] - q1->x[i];
+        if (image) d = orc_minimum_image(d, L[i]);
+        r2 += -d * q->f[i] * q->m;
+      }
+    }
+  e += r2 / 3;
+  r2 = 0;
+  for (int j = 1; j <= P; ++j)
+    for (int l = 1; l <= N; ++l) {
+      const orc_particle_t *q = &b[BEAD(l, j, P)];
+      for (int i = 0; i < 3; ++i) r2 += (q->x[i] - L[i] / 2.0) * q->f[i] * q->m;
+    }
+  return e + r2 / 3;
 }
 
 /* md_pimd_x.c:3914-3920 */

@@ -55,6 +55,7 @@ typedef struct {
   double gauss_g, gauss_s;
   double coul_q, ewald_alpha;
   int ewald_n;
+  double he[8]; /* helium: eps, A, alpha, C6, C8, C10, D, r_m (md_simulation_x.c:365-383) */
 } orc_params_t;
 
 enum {
@@ -62,7 +63,9 @@ enum {
   ORC_PAIR_PERIODIC_GAUSSIAN = 1,
   ORC_PAIR_EWALD = 2,
   ORC_PAIR_EWALD_R = 3,
-  ORC_PAIR_EWALD_NI = 4
+  ORC_PAIR_EWALD_NI = 4,
+  ORC_PAIR_LJ = 5,
+  ORC_PAIR_HELIUM = 6
 };
 
 /* Scalar helpers. */
@@ -90,6 +93,19 @@ void orc_density_distribution(int N, int P, const orc_particle_t *b, const doubl
                               const double *center, double rmax, int points, double *fx);
 void orc_pair_correlation(int N, int P, const orc_particle_t *b, const double *L, int image, double rmax,
                           double norm, int pi, int points, double *fx);
+void orc_itcf(int NA, int NB, int P, const orc_particle_t *a, const orc_particle_t *b, double rmin, double rmax,
+              int pi, int points, int two, double *fx);
+void orc_pair_correlation_2(int NA, int NB, int P, const orc_particle_t *a, const orc_particle_t *b, const double *L,
+                            int image, double rmax, double norm, int pi, int points, double *fx);
+void orc_pair_correlation_2d(int N, int P, const orc_particle_t *b, double rmax, double norm, const double *box,
+                             int points, double *fx);
+void orc_2d_pair_correlation(int NA, int NB, int P, const orc_particle_t *a, const orc_particle_t *b,
+                             const double *L, int image, double rmax, double norm, int pi, int points, int same,
+                             double *fx);
+void orc_sk_structure_2(int NA, int NB, int P, const orc_particle_t *a, const orc_particle_t *b, double q0,
+                        double qincre, int pi, int points, int full, double *fx);
+void orc_sphere_sk(int variant, int NA, int NB, int NC, int P, const orc_particle_t *a, const orc_particle_t *b,
+                   const orc_particle_t *c, int pi, const double *qis, int nq, double *fx);
 void orc_sk_structure(int N, int P, const orc_particle_t *b, double q0, double qincre, int pi, int points,
                       double *fx);
 
@@ -116,6 +132,9 @@ void orc_fill_ENk2_slow(int N, int P, double omegaP, int image, const double *L,
 double orc_vi_virial_energy(int N, int P, double beta, double vi, int image, const double *L,
                             const double *ENk, const double *ENk2, const double *VBN,
                             const orc_particle_t *b);
+double orc_vi_virial_pressure(int N, int P, double beta, double vi, int image, const double *L,
+                              const double *ENk, const double *ENk2, const double *VBN,
+                              const orc_particle_t *b);
 double orc_virial_energy(int N, int P, const double *L, const orc_particle_t *b);
 double orc_centroid_virial_energy(int N, int P, double beta, int fc, const orc_particle_t *b);
 /* Sign reweighting: returns the factor s the reference multiplies into the

@@ -193,6 +193,42 @@ static void run_case(const gcase *c) {
   md_reset_force_x(sim);
   md_pimd_fast_fill_force_VB_x(pimd, c->image);
   dump_forces(pf, "f_exch", sim);
+  {
+    /* single-value helpers: cycle energies E_n^(k) and link-virial terms for
+     * (n, k) in {(N, 1), (N, N), (N-1, 2), (1, 1)}, the shifts xminE(n) for
+     * every n, fast_xminE(u) for every u (V = VBN), xminE2 at xi' = 0.5 and 0,
+     * and spring gradients of beads (l, j) on cycle (N, k = N). */
+    int nk[4][2] = {{N, 1}, {N, N}, {N - 1, 2}, {1, 1}};
+    double h[8];
+    for (int q = 0; q < 4; ++q) {
+      h[2 * q] = md_pimd_ENk_x(pimd, nk[q][0], nk[q][1], c->image);
+      h[2 * q + 1] = md_pimd_ENk2_x(pimd, nk[q][0], nk[q][1], c->image);
+    }
+    dumpf(pf, "h_enk", h, 8);
+    double *xm = malloc(sizeof(double) * 4 * N);
+    for (int n = 1; n <= N; ++n) {
+      xm[n - 1] = md_pimd_xminE_x(pimd, n);
+      xm[N + n - 1] = md_pimd_fast_xminE_x(pimd, n, pimd->VBN);
+      xm[2 * N + n - 1] = md_pimd_xminE2_x(pimd, n, pimd->VBN, 0.5);
+      xm[3 * N + n - 1] = md_pimd_xminE2_x(pimd, n, pimd->VBN, 0.0);
+    }
+    dumpf(pf, "h_xmin", xm, 4 * N);
+    free(xm);
+    double gr[12];
+    int lj[2][2] = {{1, 1}, {N, P}};
+    for (int q = 0; q < 2; ++q) md_pimd_dENk_x(pimd, N, N, lj[q][0], lj[q][1], c->image, gr + 3 * q);
+    /* explicit triples: (bead (1,2); (1,3), (1,1)) and (bead (N,P); bead (1,1), (N,P-1)) */
+    md_pimd_fast_dENk_x(pimd, MD_PIMD_INDEX_X(1, 2, P), MD_PIMD_INDEX_X(1, 3 <= P ? 3 : 1, P),
+                        MD_PIMD_INDEX_X(1, 1, P), c->image, gr + 6);
+    md_pimd_fast_dENk_x(pimd, MD_PIMD_INDEX_X(N, P, P), MD_PIMD_INDEX_X(1, 1, P), MD_PIMD_INDEX_X(N, P - 1, P),
+                        c->image, gr + 9);
+    dumpf(pf, "h_grad", gr, 12);
+    double v2[2] = {md_pimd_fast_ENk2_x(pimd, MD_PIMD_INDEX_X(1, 1, P), MD_PIMD_INDEX_X(N, P, P),
+                                        MD_PIMD_INDEX_X(1, 1, P), c->image),
+                    md_pimd_fast_ENk2_x(pimd, MD_PIMD_INDEX_X(2, 1, P), MD_PIMD_INDEX_X(1, P, P),
+                                        MD_PIMD_INDEX_X(1, 1, P), c->image)};
+    dumpf(pf, "h_enk2", v2, 2);
+  }
   if (N <= 16) {
     md_reset_force_x(sim);
     md_pimd_fill_force_VB_x(pimd, c->image);
@@ -210,6 +246,9 @@ static void run_case(const gcase *c) {
   st->es[0] = 0;
   md_pimd_calc_vi_virial_energy_x(pimd, st, c->image);
   dumpf(pf, "e_vir", (double[]){stat_of(st)}, 1);
+  st->es[0] = 0;
+  md_pimd_calc_vi_virial_pressure_x(pimd, st, c->image);
+  dumpf(pf, "e_press", (double[]){stat_of(st)}, 1);
   md_pimd_fill_ENk2_x(pimd, c->image);
   dump_tri(pf, "ENk2_slow", pimd->ENk2, N);
   st->es[0] = 0;
@@ -373,6 +412,138 @@ static void observables_case(void) {
   }
 }
 
+/* Lennard-Jones and helium pair forces and energies, one species and two:
+ * 27 particles on a jittered 3 x 3 x 3 lattice (spacing a, jitter 0.1 a per
+ * bead coordinate, uniform from the seed), P = 4, cubic box 3 a.  Two-species
+ * records split the same beads into A = particles 1..15 (mass 1) and
+ * B = 16..27 (mass 2). */
+static void lattice_pos(md_simulation_t_x *sim, int N, int P, int first, int count, double a) {
+  for (int l = 1; l <= count; ++l) {
+    int site = first + l - 1;
+    int ix = site % 3, iy = (site / 3) % 3, iz = site / 9;
+    for (int j = 1; j <= P; ++j) {
+      md_particle_t_x *q = &sim->particles[MD_PIMD_INDEX_X(l, j, P)];
+      q->x[0] = (ix + 0.5) * a + md_random_uniform_x(-0.1 * a, 0.1 * a);
+      q->x[1] = (iy + 0.5) * a + md_random_uniform_x(-0.1 * a, 0.1 * a);
+      q->x[2] = (iz + 0.5) * a + md_random_uniform_x(-0.1 * a, 0.1 * a);
+    }
+  }
+  (void)N;
+}
+
+static void lj_he_case(void) {
+  const int N = 27, P = 4, NA = 15, NB = 12;
+  md_set_helium_parameters_x(10.8, 544850.4, 13.353384, 1.3732412, 0.4253785, 0.178100, 1.241314, 2.9673);
+  const double spacing[2] = {1.25, 3.1};
+  md_pair_force_t_x pfs[2] = {md_LJ_periodic_force_x, md_periodic_helium_force_x};
+  md_pair_energy_t_x pes[2] = {md_LJ_periodic_energy_x, md_periodic_helium_energy_x};
+  const char *pfx[2] = {"lj", "he"};
+  for (int k = 0; k < 2; ++k) {
+    const double a = spacing[k];
+    md_num_t_x L3[3] = {3 * a, 3 * a, 3 * a};
+    md_set_seed_x(41 + k);
+    md_simulation_t_x *sim = md_alloc_simulation_x(N * P, 1, 0, MD_ND_RECT_BOX_X);
+    md_init_nd_rect_box_x(sim->box, L3);
+    md_pimd_t_x *pimd = md_alloc_pimd_x(sim, N, P, 1.0, 1.0);
+    md_init_particle_mass_x(sim, 1.0, 0, N * P);
+    lattice_pos(sim, N, P, 0, N, a);
+    dump_beads(pfx[k], "0", sim);
+    dumpf(pfx[k], "L", L3, 3);
+    md_reset_force_x(sim);
+    md_pimd_calc_pair_force_x(pimd, pfs[k]);
+    dump_forces(pfx[k], "f_pair", sim);
+    md_stats_t_x *st = md_alloc_stats_x(1);
+    st->es[0] = 0;
+    md_pimd_calc_pair_energy_x(pimd, pes[k], st);
+    dumpf(pfx[k], "e_pair", (double[]){stat_of(st)}, 1);
+    /* two species from the same sites */
+    md_simulation_t_x *sa = md_alloc_simulation_x(NA * P, 1, 0, MD_ND_RECT_BOX_X);
+    md_simulation_t_x *sb = md_alloc_simulation_x(NB * P, 1, 0, MD_ND_RECT_BOX_X);
+    md_init_nd_rect_box_x(sa->box, L3);
+    md_init_nd_rect_box_x(sb->box, L3);
+    md_pimd_t_x *pa = md_alloc_pimd_x(sa, NA, P, 1.0, 1.0);
+    md_pimd_t_x *pb = md_alloc_pimd_x(sb, NB, P, 1.0, 1.0);
+    md_init_particle_mass_x(sa, 1.0, 0, NA * P);
+    md_init_particle_mass_x(sb, 2.0, 0, NB * P);
+    lattice_pos(sa, NA, P, 0, NA, a);
+    lattice_pos(sb, NB, P, NA, NB, a);
+    dump_beads(pfx[k], "A0", sa);
+    dump_beads(pfx[k], "B0", sb);
+    md_reset_force_x(sa);
+    md_reset_force_x(sb);
+    md_pimd_calc_pair_force_2_x(pa, pb, pfs[k]);
+    dump_forces(pfx[k], "fA", sa);
+    dump_forces(pfx[k], "fB", sb);
+    st->es[0] = 0;
+    md_pimd_calc_pair_energy_2_x(pa, pb, pes[k], st);
+    dumpf(pfx[k], "e2", (double[]){stat_of(st)}, 1);
+  }
+}
+
+/* The remaining observables on three seeded systems in one box (A: 7
+ * particles, B: 5, C: 4; P = 6; uniform positions in 5 x 5.5 x 6). */
+static void observables2_case(void) {
+  const int NA = 7, NB = 5, NC = 4, P = 6;
+  md_num_t_x L3[3] = {5.0, 5.5, 6.0}, ctr[3] = {2.5, 2.75, 3.0};
+  md_set_seed_x(57);
+  md_simulation_t_x *s[3];
+  md_pimd_t_x *pm[3];
+  const int Ns[3] = {NA, NB, NC};
+  const char *tag[3] = {"A0", "B0", "C0"};
+  for (int k = 0; k < 3; ++k) {
+    s[k] = md_alloc_simulation_x(Ns[k] * P, 1, 0, MD_ND_RECT_BOX_X);
+    md_init_nd_rect_box_x(s[k]->box, L3);
+    pm[k] = md_alloc_pimd_x(s[k], Ns[k], P, 1.0, 1.0);
+    md_init_particle_mass_x(s[k], 1.0, 0, Ns[k] * P);
+    md_pimd_init_particle_uniform_pos_x(pm[k], ctr, L3, 0.5);
+    md_periodic_boundary_x(s[k]);
+    dump_beads("obs2", tag[k], s[k]);
+  }
+  md_stats_t_x *h;
+  h = md_alloc_stats_x(36);
+  md_pimd_calc_ITCF_x(pm[0], h, 0.5, 4.5, 3);
+  dumpf("obs2", "itcf", h->fxs[0], 36);
+  h = md_alloc_stats_x(36);
+  md_pimd_calc_ITCF_2_x(pm[0], pm[1], h, 0.5, 4.5, 3);
+  dumpf("obs2", "itcf2", h->fxs[0], 36);
+  h = md_alloc_stats_x(20);
+  md_pimd_calc_pair_correlation_2_x(pm[0], pm[1], h, 3.5, 7.0, 2, 1);
+  dumpf("obs2", "paircorr2", h->fxs[0], 20);
+  h = md_alloc_stats_x(20);
+  md_pimd_calc_pair_correlation_2_x(pm[0], pm[1], h, 6.0, 3.0, 5, 0);
+  dumpf("obs2", "paircorr2_open", h->fxs[0], 20);
+  {
+    md_num_t_x box[4] = {0.5, 4.5, 0.5, 5.0};
+    h = md_alloc_stats_x(20);
+    md_pimd_calc_pair_correlation_2d_x(pm[0], h, 3.0, 5.0, box);
+    dumpf("obs2", "paircorr_2d", h->fxs[0], 20);
+  }
+  h = md_alloc_stats_x(25);
+  md_pimd_calc_2d_pair_correlation_x(pm[0], h, 3.0, 3.0, 4, 1);
+  dumpf("obs2", "pc2d", h->fxs[0], 25);
+  h = md_alloc_stats_x(25);
+  md_pimd_calc_2d_pair_correlation_2_x(pm[0], pm[1], h, 3.0, 3.0, 4, 0);
+  dumpf("obs2", "pc2d_2", h->fxs[0], 25);
+  h = md_alloc_stats_x(16);
+  md_pimd_calc_Sk_structure_2_x(pm[0], pm[1], h, 0.2, 0.35, 3);
+  dumpf("obs2", "sk2", h->fxs[0], 16);
+  h = md_alloc_stats_x(16);
+  md_pimd_calc_Sk_structure_full_2_x(pm[0], pm[1], h, 0.2, 0.35, 3);
+  dumpf("obs2", "sk_full2", h->fxs[0], 16);
+  md_sk_sphere_t_x *sk = md_alloc_sk_sphere_3d_x(0.0, 0.6, 2.0);
+  dumpf("obs2", "sphere_qs", sk->qs, sk->N);
+  dumpf("obs2", "sphere_qis", sk->qis, 3 * sk->N);
+  h = md_alloc_stats_x(sk->N + 3);
+  md_pimd_calc_sphere_Sk_structure_x(pm[0], h, 3, sk);
+  dumpf("obs2", "sphere_sk", h->fxs[0], sk->N + 3);
+  h = md_alloc_stats_x(sk->N);
+  md_pimd_calc_sphere_Sk_structure_full_2_x(pm[0], pm[1], h, 8, sk);
+  dumpf("obs2", "sphere_sk_full2", h->fxs[0], sk->N);
+  h = md_alloc_stats_x(sk->N - 2);
+  md_pimd_calc_sphere_Sk_structure_3_x(pm[0], pm[1], pm[2], h, 2, sk);
+  dumpf("obs2", "sphere_sk3", h->fxs[0], sk->N - 2);
+}
+
 int main(int argc, char **argv) {
   const char *path = argc > 1 ? argv[1] : "golden.bin";
   out = fopen(path, "wb");
@@ -401,6 +572,8 @@ int main(int argc, char **argv) {
   for (size_t i = 0; i < sizeof cases / sizeof cases[0]; ++i) run_case(&cases[i]);
   two_species_case();
   observables_case();
+  lj_he_case();
+  observables2_case();
   fclose(out);
   return 0;
 }

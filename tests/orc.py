@@ -44,10 +44,11 @@ class Nhc(C.Structure):
 class Params(C.Structure):
     _fields_ = [("trap_omega", C.c_double), ("trap_center", C.c_double * 3),
                 ("gauss_g", C.c_double), ("gauss_s", C.c_double),
-                ("coul_q", C.c_double), ("ewald_alpha", C.c_double), ("ewald_n", C.c_int)]
+                ("coul_q", C.c_double), ("ewald_alpha", C.c_double), ("ewald_n", C.c_int),
+                ("he", C.c_double * 8)]
 
 
-PAIR_KIND = {"gaussian": 0, "periodic_gaussian": 1, "ewald": 2, "ewald_R": 3, "ewald_NI": 4}
+PAIR_KIND = {"gaussian": 0, "periodic_gaussian": 1, "ewald": 2, "ewald_R": 3, "ewald_NI": 4, "lj": 5, "helium": 6}
 
 PARTICLE_DT = np.dtype([("m", "<f8"), ("x", "<f8", 3), ("v", "<f8", 3), ("f", "<f8", 3)])
 NHC_DT = np.dtype([("f", "<i4"), ("theta", "<f8", 8), ("vtheta", "<f8", 8), ("Q", "<f8", 8)], align=True)
@@ -105,6 +106,12 @@ class Oracle:
             "orc_density_distribution": (None, [i, i, vp, DP, i, DP, d, i, DP]),
             "orc_pair_correlation": (None, [i, i, vp, DP, i, d, d, i, i, DP]),
             "orc_sk_structure": (None, [i, i, vp, d, d, i, i, DP]),
+            "orc_itcf": (None, [i, i, i, vp, vp, d, d, i, i, i, DP]),
+            "orc_pair_correlation_2": (None, [i, i, i, vp, vp, DP, i, d, d, i, i, DP]),
+            "orc_pair_correlation_2d": (None, [i, i, vp, d, d, DP, i, DP]),
+            "orc_2d_pair_correlation": (None, [i, i, i, vp, vp, DP, i, d, d, i, i, i, DP]),
+            "orc_sk_structure_2": (None, [i, i, i, vp, vp, d, d, i, i, i, DP]),
+            "orc_sphere_sk": (None, [i, i, i, i, i, vp, vp, vp, i, DP, i, DP]),
             "orc_madelung": (d, [vp, DP]),
             "orc_fast_fill_ENk": (None, [i, i, d, i, DP, vp, DP, DP]),
             "orc_fill_ENk_slow": (None, [i, i, d, i, DP, vp, DP]),
@@ -115,6 +122,7 @@ class Oracle:
             "orc_fast_fill_ENk2": (None, [i, i, d, i, DP, vp, DP]),
             "orc_fill_ENk2_slow": (None, [i, i, d, i, DP, vp, DP]),
             "orc_vi_virial_energy": (d, [i, i, d, d, i, DP, DP, DP, DP, vp]),
+            "orc_vi_virial_pressure": (d, [i, i, d, d, i, DP, DP, DP, DP, vp]),
             "orc_virial_energy": (d, [i, i, DP, vp]),
             "orc_centroid_virial_energy": (d, [i, i, d, i, vp]),
             "orc_vi_sign": (d, [i, d, d, DP, DP]),
@@ -138,12 +146,18 @@ class Oracle:
         return getattr(self.lib, "orc_" + name)
 
 
-def make_params(trap_omega=1.0, center=(0, 0, 0), g=3.0, s=0.5, q=1.0, alpha=0.0, n=2):
+# Aziz HFDHE2 helium parameters in K and Angstrom (eps, A, alpha, C6, C8, C10, D, r_m)
+HE_AZIZ = (10.8, 544850.4, 13.353384, 1.3732412, 0.4253785, 0.178100, 1.241314, 2.9673)
+
+
+def make_params(trap_omega=1.0, center=(0, 0, 0), g=3.0, s=0.5, q=1.0, alpha=0.0, n=2, he=HE_AZIZ):
     p = Params()
     p.trap_omega = trap_omega
     for k in range(3):
         p.trap_center[k] = center[k]
     p.gauss_g, p.gauss_s, p.coul_q, p.ewald_alpha, p.ewald_n = g, s, q, alpha, n
+    for k in range(8):
+        p.he[k] = he[k]
     return p
 
 

@@ -133,6 +133,20 @@ def test_estimators(lib, golden, name):
     s.host_pimd()
     np.testing.assert_allclose(pyxmd.tri(s.pimd, "ENk2"), g["ENk2"], rtol=1e-12, atol=1e-11)
     assert approx(stat(lib, lib.md_pimd_calc_vi_virial_energy_x, s.pimd, m["image"]), g["e_vir"][0])
+    # The pressure's box-centred virial sum (x - L/2) . f m / 3 weighs every force by
+    # |x - L/2| ~ L/2, so force differences within the force-parity bar move it by up to
+    # tol_f * RMS(f) * sum |x - L/2| m / 3.
+    b = s.host_beads()
+    xc = np.abs(b["x"] - m["L"] / 2).reshape(-1) * np.repeat(b["m"], 3)
+    ptol = exch_tol(g) * np.sqrt(np.mean(g["f_total"] ** 2)) * xc.sum() / 3
+    press = stat(lib, lib.md_pimd_calc_vi_virial_pressure_x, s.pimd, m["image"])
+    print(f"{name}: pressure {press:.15g} vs {g['e_press'][0]:.15g} (tolerance {ptol:.1e})")
+    assert abs(press - g["e_press"][0]) <= ptol + 1e-12 * abs(g["e_press"][0])
+    # on the reference's own forces the estimator itself agrees at the 1e-12 bar
+    assert lib.md_simulation_sync_host_x(s.sim, 0)
+    b["f"] = g["f_total"].reshape(-1, 3)
+    pyxmd.set_beads(s.sim, b)
+    assert approx(stat(lib, lib.md_pimd_calc_vi_virial_pressure_x, s.pimd, m["image"]), g["e_press"][0])
     assert lib.md_pimd_fill_ENk2_x(s.pimd, m["image"]) == 0
     s.host_pimd()
     np.testing.assert_allclose(pyxmd.tri(s.pimd, "ENk2"), g["ENk2_slow"], rtol=1e-12, atol=1e-11)
@@ -153,6 +167,40 @@ def test_estimators(lib, golden, name):
         ss.contents.es[0] = 1
         assert lib.md_pimd_calc_vi_sign_x(s.pimd, es, ss, a) == 0
         assert ss.contents.es[0] == pytest.approx(sg, rel=1e-10, abs=1e-300)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_single_value_helpers(lib, golden, name):
+    """md_pimd_ENk_x / _ENk2_x / _fast_ENk2_x / _xminE_x / _xminE2_x /
+    _fast_xminE_x / _dENk_x / _fast_dENk_x on the device state against the
+    compiled reference's values on the same beads."""
+    s, g, m = system(lib, golden, name)
+    s.full_force()
+    N, P, im = m["N"], m["P"], m["image"]
+    pm = s.pimd
+    nk = [(N, 1), (N, N), (N - 1, 2), (1, 1)]
+    got = []
+    for n, k in nk:
+        got += [lib.md_pimd_ENk_x(pm, n, k, im), lib.md_pimd_ENk2_x(pm, n, k, im)]
+    np.testing.assert_allclose(got, g["h_enk"], rtol=1e-12, atol=1e-12 * np.abs(g["h_enk"]).max())
+    V = np.ascontiguousarray(g["VBN"])
+    xm = np.zeros(4 * N)
+    for n in range(1, N + 1):
+        xm[n - 1] = lib.md_pimd_xminE_x(pm, n)
+        xm[N + n - 1] = lib.md_pimd_fast_xminE_x(pm, n, dptr(V))
+        xm[2 * N + n - 1] = lib.md_pimd_xminE2_x(pm, n, dptr(V), 0.5)
+        xm[3 * N + n - 1] = lib.md_pimd_xminE2_x(pm, n, dptr(V), 0.0)
+    np.testing.assert_allclose(xm, g["h_xmin"], rtol=1e-12, atol=1e-12 * np.abs(g["h_xmin"]).max())
+    gr = np.zeros(12)
+    idx = lambda l, j: (l - 1) * P + (j - 1)
+    for q, (l, j) in enumerate([(1, 1), (N, P)]):
+        lib.md_pimd_dENk_x(pm, N, N, l, j, im, dptr(gr[3 * q:]))
+    lib.md_pimd_fast_dENk_x(pm, idx(1, 2), idx(1, 3 if P >= 3 else 1), idx(1, 1), im, dptr(gr[6:]))
+    lib.md_pimd_fast_dENk_x(pm, idx(N, P), idx(1, 1), idx(N, P - 1), im, dptr(gr[9:]))
+    np.testing.assert_allclose(gr, g["h_grad"], rtol=1e-12, atol=1e-12 * np.abs(g["h_grad"]).max())
+    v2 = [lib.md_pimd_fast_ENk2_x(pm, idx(1, 1), idx(N, P), idx(1, 1), im),
+          lib.md_pimd_fast_ENk2_x(pm, idx(2, 1), idx(1, P), idx(1, 1), im)]
+    np.testing.assert_allclose(v2, g["h_enk2"], rtol=1e-12, atol=1e-12 * np.abs(g["h_enk2"]).max())
 
 
 def trajectory_check(s, g, ev=None):

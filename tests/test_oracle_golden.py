@@ -99,6 +99,9 @@ def test_estimators(golden, oracle, name):
     e = oracle.vi_virial_energy(N, P, c["beta"], c["vi"], c["image"], dptr(c["Lv"]), dptr(g["ENk"]),
                                 dptr(g["ENk2"]), dptr(g["VBN"]), ptr(b))
     assert e == pytest.approx(g["e_vir"][0], rel=1e-13, abs=1e-13)
+    e = oracle.vi_virial_pressure(N, P, c["beta"], c["vi"], c["image"], dptr(c["Lv"]), dptr(g["ENk"]),
+                                  dptr(g["ENk2"]), dptr(g["VBN"]), ptr(b))
+    assert e == pytest.approx(g["e_press"][0], rel=1e-12, abs=1e-12)
     E2s = np.zeros(tri_size(N))
     oracle.fill_ENk2_slow(N, P, c["omegaP"], c["image"], dptr(c["Lv"]), ptr(b), dptr(E2s))
     np.testing.assert_allclose(E2s, g["ENk2_slow"], rtol=1e-13, atol=1e-12)
@@ -233,3 +236,91 @@ def test_observables(golden, oracle, name):
     fx = np.zeros(1)
     oracle.sk_structure(N, P, ptr(b), 0.0, 1.0, 1, 1, dptr(fx))
     assert fx[0] == pytest.approx(N / 3.0 * 3, rel=1e-15)
+
+
+@pytest.mark.parametrize("pot", ["lj", "he"])
+def test_lj_helium_pair(golden, oracle, pot):
+    """Lennard-Jones and helium pair forces and energies (one species and two)
+    against the compiled reference; momentum balance and a finite-difference
+    check of force against energy on one bead."""
+    g = {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(pot + "/")}
+    kind = PAIR_KIND["lj" if pot == "lj" else "helium"]
+    N, P, NA, NB = 27, 4, 15, 12
+    L = g["L"]
+    p = make_params()
+    b = beads_from(g["x0"], m=g["m0"])
+    oracle.pair_force(kind, C.byref(p), N, P, dptr(L), ptr(b))
+    assert rel_err(forces(b), g["f_pair"]) < 1e-13
+    assert np.abs((b["m"][:, None] * b["f"]).sum(0)).max() < 1e-12 * np.abs(b["f"]).max()
+    e = oracle.pair_energy(kind, C.byref(p), N, P, dptr(L), ptr(b))
+    assert e == pytest.approx(g["e_pair"][0], rel=1e-13)
+    # the energy is sum(e) / P and the acceleration F / (m P), so -dE/dx = m f for bead (1, 1)
+    h = 1e-6 * (1.0 if pot == "lj" else 3.0)
+    x = g["x0"].copy()
+    fd = np.zeros(3)
+    for d in range(3):
+        ep = []
+        for sgn in (1, -1):
+            x2 = x.copy()
+            x2[d] += sgn * h
+            bb = beads_from(x2, m=g["m0"])
+            ep.append(oracle.pair_energy(kind, C.byref(p), N, P, dptr(L), ptr(bb)))
+        fd[d] = -(ep[0] - ep[1]) / (2 * h)
+    np.testing.assert_allclose(b["f"][0] * b["m"][0], fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max())
+    a = beads_from(g["xA0"], m=g["mA0"])
+    c = beads_from(g["xB0"], m=g["mB0"])
+    oracle.pair_force_2(kind, C.byref(p), NA, NB, P, dptr(L), ptr(a), ptr(c))
+    assert rel_err(a["f"].reshape(-1), g["fA"]) < 1e-13
+    assert rel_err(c["f"].reshape(-1), g["fB"]) < 1e-13
+    e = oracle.pair_energy_2(kind, C.byref(p), NA, NB, P, dptr(L), ptr(a), ptr(c))
+    assert e == pytest.approx(g["e2"][0], rel=1e-13)
+
+
+def obs2_inputs(golden):
+    g = {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith("obs2/")}
+    return g, [beads_from(g[t], m=g["m" + t[1:]]) for t in ("xA0", "xB0", "xC0")]
+
+
+def test_observables_two_systems(golden, oracle):
+    """ITCF, two-system and 2-D pair correlations and the S(q) variants
+    against the compiled reference; histograms bit for bit."""
+    o = oracle
+    g, (a, b, c) = obs2_inputs(golden)
+    NA, NB, NC, P = 7, 5, 4, 6
+    L = np.array([5.0, 5.5, 6.0])
+
+    def hist(n, fn, *args):
+        fx = np.zeros(n)
+        fn(*args, dptr(fx))
+        return fx
+
+    assert np.array_equal(hist(36, o.itcf, NA, NA, P, ptr(a), ptr(a), 0.5, 4.5, 3, 36, 0), g["itcf"])
+    assert np.array_equal(hist(36, o.itcf, NA, NB, P, ptr(a), ptr(b), 0.5, 4.5, 3, 36, 1), g["itcf2"])
+    assert np.array_equal(hist(20, o.pair_correlation_2, NA, NB, P, ptr(a), ptr(b), dptr(L), 1, 3.5, 7.0, 2, 20),
+                          g["paircorr2"])
+    assert np.array_equal(hist(20, o.pair_correlation_2, NA, NB, P, ptr(a), ptr(b), dptr(L), 0, 6.0, 3.0, 5, 20),
+                          g["paircorr2_open"])
+    box = np.array([0.5, 4.5, 0.5, 5.0])
+    assert np.array_equal(hist(20, o.pair_correlation_2d, NA, P, ptr(a), 3.0, 5.0, dptr(box), 20), g["paircorr_2d"])
+    assert np.array_equal(hist(25, o.lib.orc_2d_pair_correlation, NA, NA, P, ptr(a), ptr(a), dptr(L), 1, 3.0, 3.0, 4,
+                               25, 1), g["pc2d"])
+    assert np.array_equal(hist(25, o.lib.orc_2d_pair_correlation, NA, NB, P, ptr(a), ptr(b), dptr(L), 0, 3.0, 3.0, 4,
+                               25, 0), g["pc2d_2"])
+    np.testing.assert_allclose(hist(16, o.sk_structure_2, NA, NB, P, ptr(a), ptr(b), 0.2, 0.35, 3, 16, 0), g["sk2"],
+                               rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(hist(16, o.sk_structure_2, NA, NB, P, ptr(a), ptr(b), 0.2, 0.35, 3, 16, 1),
+                               g["sk_full2"], rtol=1e-13, atol=1e-13)
+    qis = g["sphere_qis"]
+    nq = len(g["sphere_qs"])
+    np.testing.assert_allclose(np.linalg.norm(qis.reshape(-1, 3), axis=1), g["sphere_qs"], rtol=1e-15)
+    sk = hist(nq + 3, o.sphere_sk, 0, NA, 0, 0, P, ptr(a), None, None, 3, dptr(qis), nq)
+    np.testing.assert_allclose(sk, g["sphere_sk"], rtol=1e-13, atol=1e-13)
+    sk = hist(nq, o.sphere_sk, 1, NA, NB, 0, P, ptr(a), ptr(b), None, 8, dptr(qis), nq)
+    np.testing.assert_allclose(sk, g["sphere_sk_full2"], rtol=1e-13, atol=1e-13)
+    sk = hist(nq - 2, o.sphere_sk, 2, NA, NB, NC, P, ptr(a), ptr(b), ptr(c), 2, dptr(qis), nq - 2)
+    np.testing.assert_allclose(sk, g["sphere_sk3"], rtol=1e-13, atol=1e-13)
+    # S(q) of a bead set against itself at q -> 0 is the set size
+    z = hist(1, o.sk_structure_2, NA, NB, P, ptr(a), ptr(b), 0.0, 1.0, 1, 1, 1)
+    assert z[0] == pytest.approx(NA + NB, rel=1e-15)
+    # every ITCF entry is counted once (N^2 entries of weight 1/P)
+    assert hist(36, o.itcf, NA, NA, P, ptr(a), ptr(a), 0.5, 4.5, 3, 36, 0).sum() == pytest.approx(NA * NA / P)
