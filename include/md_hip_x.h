@@ -45,10 +45,21 @@ int md_pimd_get_ewald_plan_x(md_pimd_t_x *pimd, md_num_t_x *alpha, md_num_t_x *r
  * v, f and nhc blocks in the layout of md_fprint_particle_pos_x / _vel_x /
  * _force_x / md_fprint_nhcs_x (md_util_x.c:838-915), every number a C99
  * hexadecimal float ("%a"), so a run resumes bit for bit and the reference's
- * own md_fread_* readers parse the file.  Reading hands authority back to the
+ * own md_fread_* readers parse the file.  A last line "rng <seed> <draws>
+ * <cache flag> <cache>" records the state of md_random_uniform_x /
+ * md_random_gaussian_x (draws made through the library since the last
+ * md_set_seed_x); reading restores it.  Reading hands authority back to the
  * host mirror (the next device call uploads it). */
 int md_checkpoint_write_x(FILE *out, md_simulation_t_x *sim);
 int md_checkpoint_read_x(FILE *in, md_simulation_t_x *sim);
+
+/* Fermion-sign reweighting over a grid of exchange parameters xi' (the xi
+ * grid used to extrapolate to fermions): factor[i] = sign(N) exp(-beta
+ * V'(N) + beta V_B(N)) with V' the forward recursion at |xi'_i| and the sign
+ * tracked through it -- the factor md_pimd_calc_vi_sign_x (md_pimd_x.c:4665-4705)
+ * multiplies into its two statistics, for n <= 512 values in one launch
+ * (one workgroup per xi'). */
+int md_pimd_calc_vi_sign_grid_x(md_pimd_t_x *pimd, int n, const md_num_t_x *vi2, md_num_t_x *factor);
 
 /* Internal exchange quantities after md_pimd_fill_VB_x (tests, diagnostics):
  * backward potentials Vt^(0..N) and the connection-probability matrix
@@ -73,6 +84,15 @@ typedef struct {
 int md_pimd_compute_forces_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg);
 /* nsteps full steps (B1-B7); returns when the work is queued. */
 int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps);
+
+/* nsteps steps of n independent systems (replicas of uneven N, P, xi and
+ * interaction, each with its own config) in one hipGraph per batch step: the
+ * replicas' steps run concurrently on up to 8 stream lanes forked from the
+ * engine's stream.  Every replica's trajectory is bit-identical to running it
+ * alone through md_pimd_run_steps_x.  Interaction parameters are the process
+ * globals at the time of the call, as for every other entry point.  Returns
+ * when the work is done (-1 if any replica's recursion reported an error). */
+int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *cfgs, int n, int nsteps);
 
 /* Per-stage device time of nsteps eager steps, averaged per step (ms).  With
  * serial != 0 every stage runs alone on one stream (true kernel times);

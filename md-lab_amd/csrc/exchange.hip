@@ -838,7 +838,11 @@ __global__ __launch_bounds__(256) void k_box_virial(const double *__restrict__ X
 
 // Sign-tracked forward recursion at |xi'| (md_pimd_x.c:4675-4701): one
 // workgroup, one n at a time (a per-sample estimator, not per step).
-__global__ __launch_bounds__(256) void k_vi_sign(RecArgs a, double vi2, double *__restrict__ out) {
+// One workgroup per xi' of a grid: out[b] = sign * exp(-beta V'(N) + beta V_B(N)).
+__global__ __launch_bounds__(256) void k_vi_sign(RecArgs a, const double *__restrict__ vi2s,
+                                                 double *__restrict__ outs) {
+  const double vi2 = vi2s[blockIdx.x];
+  double *out = outs + blockIdx.x;
   __shared__ double red[256];
   extern __shared__ double lds[];
   const int N = a.N;
@@ -1357,21 +1361,42 @@ int md_pimd_calc_vi_virial_pressure_x(md_pimd_t_x *pimd, md_stats_t_x *stats, in
   return 0;
 }
 
-int md_pimd_calc_vi_sign_x(md_pimd_t_x *pimd, md_stats_t_x *e_s, md_stats_t_x *s_s, md_num_t_x vi2) {
-  if (ensure_pimd_device(pimd)) return -1;
-  DevPimd *p = pimd_state(pimd, false);
-  if (ensure_enk(p) || ensure_vbn(p)) return -1;
-  RecArgs a = rec_args(p);
-  double *d_out = p->work + 4 * (p->N + 1);
+// Sign-reweighting factors for n values of xi' in one launch (one workgroup
+// each): out[i] = sign_{xi'_i}(N) exp(-beta V_{|xi'_i|}(N) + beta V_B(N)).
+static int vi_sign_grid(DevPimd *p, int n, const double *vi2, double *out) {
+  if (n <= 0) return 0;
   if (2 * (p->N + 1) * sizeof(double) > 65536) {
     fprintf(stderr, "xmd: sign reweighting supports N <= 4094\n");
     return -1;
   }
-  hipLaunchKernelGGL(k_vi_sign, dim3(1), dim3(256), 2 * (p->N + 1) * sizeof(double), rt().s0, a, vi2, d_out);
+  if (n > 512) {
+    fprintf(stderr, "xmd: at most 512 xi' values per call\n");
+    return -1;
+  }
+  RecArgs a = rec_args(p);
+  double *d_vi = p->work + 4 * (p->N + 1) + 8, *d_out = d_vi + 512;
+  hipStream_t st = rt().s0;
+  XMD_CHECK(hipMemcpyAsync(d_vi, vi2, n * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_vi_sign, dim3(n), dim3(256), 2 * (p->N + 1) * sizeof(double), st, a, d_vi, d_out);
   XMD_CHECK(hipGetLastError());
+  XMD_CHECK(hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  XMD_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int md_pimd_calc_vi_sign_grid_x(md_pimd_t_x *pimd, int n, const md_num_t_x *vi2, md_num_t_x *factor) {
+  if (!vi2 || !factor || ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  if (ensure_enk(p) || ensure_vbn(p)) return -1;
+  return vi_sign_grid(p, n, vi2, factor);
+}
+
+int md_pimd_calc_vi_sign_x(md_pimd_t_x *pimd, md_stats_t_x *e_s, md_stats_t_x *s_s, md_num_t_x vi2) {
+  if (ensure_pimd_device(pimd)) return -1;
+  DevPimd *p = pimd_state(pimd, false);
+  if (ensure_enk(p) || ensure_vbn(p)) return -1;
   double s = 0;
-  XMD_CHECK(hipMemcpyAsync(&s, d_out, sizeof(double), hipMemcpyDeviceToHost, rt().s0));
-  XMD_CHECK(hipStreamSynchronize(rt().s0));
+  if (vi_sign_grid(p, 1, &vi2, &s)) return -1;
   e_s->es[0] *= s;
   s_s->es[0] *= s;
   return 0;

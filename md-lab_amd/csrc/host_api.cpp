@@ -276,29 +276,39 @@ md_num_t_x md_calc_3d_ueg_madelung_x(md_num_t_x *L) {
 
 // ---------------------------------------------------------------- random
 // glibc rand()/RAND_MAX and Box-Muller with a one-value cache, consumed in
-// the same order as md_util_x.c:671-693 so that seeded inputs match.
-void md_set_seed_x(unsigned int seed) { srand(seed); }
+// the same order as md_util_x.c:671-693 so that seeded inputs match.  The
+// library also keeps the seed, the number of rand() draws since it and the
+// cache, so that a checkpoint can restore the generator exactly.
+static unsigned g_rng_seed = 1;  // glibc's seed before any srand
+static unsigned long long g_rng_draws = 0;
+static double g_bm_cached = 0;
+static int g_bm_have = 0;
+
+void md_set_seed_x(unsigned int seed) {
+  srand(seed);
+  g_rng_seed = seed;
+  g_rng_draws = 0;
+}
 
 md_num_t_x md_random_uniform_x(md_num_t_x a, md_num_t_x b) {
   double r = ((double)rand()) / RAND_MAX;
+  ++g_rng_draws;
   return a + (b - a) * r;
 }
 
 md_num_t_x md_random_gaussian_x(void) {
-  static double cached;
-  static int have = 0;
-  if (have) {
-    have = 0;
-    return cached;
+  if (g_bm_have) {
+    g_bm_have = 0;
+    return g_bm_cached;
   }
-  have = 1;
+  g_bm_have = 1;
   double r = sqrt(-2 * log(md_random_uniform_x(0, 1)));
   double ang = 2 * M_PI * md_random_uniform_x(0, 1);
   // glibc's sincos: the reference build (gcc -O2) fuses its sin/cos pair into
   // this call, whose last bit can differ from separate sin and cos.
   double sn, cs;
   sincos(ang, &sn, &cs);
-  cached = r * sn;
+  g_bm_cached = r * sn;
   return r * cs;
 }
 
@@ -542,17 +552,33 @@ int md_fread_nhcs_x(FILE *in, md_simulation_t_x *sim) {
 // Lossless checkpoint: the x, v, f and nhc blocks of the reference's text
 // layout (md_util_x.c:838-915), every number a C99 hexadecimal float.  The
 // reference's readers (fscanf "%lf") parse it too.
+// A last line "rng <seed> <draws> <cache flag> <cache>" holds the library's
+// random-number state; the reader restores it by reseeding and replaying the
+// draws (checkpoints without the line leave the generator as it is).
 int md_checkpoint_write_x(FILE *out, md_simulation_t_x *sim) {
   if (!out || !sim) return -1;
   if (print_vec(out, sim, "x", 0, "%a ") || print_vec(out, sim, "v", 1, "%a ") || print_vec(out, sim, "f", 2, "%a ") ||
       print_nhcs(out, sim, "%a "))
     return -1;
-  return fflush(out) ? -1 : 0;
+  fprintf(out, "rng %u %llu %d %a\n", g_rng_seed, g_rng_draws, g_bm_have, g_bm_cached);
+  return fflush(out) || ferror(out) ? -1 : 0;
 }
 
 int md_checkpoint_read_x(FILE *in, md_simulation_t_x *sim) {
   if (!in || !sim) return -1;
   if (read_vec(in, sim, 0) || read_vec(in, sim, 1) || read_vec(in, sim, 2) || md_fread_nhcs_x(in, sim)) return -1;
+  char tag[8] = {0};
+  if (fscanf(in, "%7s", tag) != 1) return 0;  // no random-number state recorded
+  unsigned seed;
+  unsigned long long draws;
+  int have;
+  double cached;
+  if (strcmp(tag, "rng") != 0 || fscanf(in, "%u %llu %d %lf", &seed, &draws, &have, &cached) != 4) return -1;
+  md_set_seed_x(seed);
+  for (unsigned long long i = 0; i < draws; ++i) (void)rand();
+  g_rng_draws = draws;
+  g_bm_have = have;
+  g_bm_cached = cached;
   return 0;
 }
 

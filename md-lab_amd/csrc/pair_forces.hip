@@ -433,8 +433,27 @@ struct KSpace {
   int stamp = 0;
 };
 
-static KSpace g_ks[2];
-static int g_ks_clock = 0;
+// Tile-pair schedule and partial-sum scratch (grown on demand, kept).
+struct TileBufs {
+  int N = -1, T = 0, ntp = 0;
+  int2 *tiles = nullptr;
+  double *Fpart = nullptr, *Epart = nullptr;
+  size_t fcap = 0, ecap = 0;
+};
+
+// Per-object buffers of the pair kernels, so that independent systems can
+// run concurrently: the tile schedule and partial sums, and two k-space table
+// sets (most recently used kept).
+struct PairBufs {
+  TileBufs tb;
+  KSpace ks[2];
+  int ks_clock = 0;
+};
+
+static PairBufs *pair_bufs(DevPimd *p) {
+  if (!p->pair_bufs) p->pair_bufs = new PairBufs();
+  return (PairBufs *)p->pair_bufs;
+}
 
 static int grow(double2 *&p, size_t &cap, size_t need) {
   if (need <= cap) return 0;
@@ -452,14 +471,19 @@ static int table_mz(int M) {
   return -1;
 }
 
-static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, double gmax2, int nb, int P) {
+static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q, int n, double gmax2, int nb,
+                             int P) {
+  PairBufs *pb = pair_bufs(p);
+  KSpace *ks = pb->ks;
   KSpace *K = nullptr;
-  for (auto &k : g_ks)
+  for (int i = 0; i < 2; ++i) {
+    KSpace &k = ks[i];
     if (k.n == n && k.alpha == alpha && k.q == q && k.gmax2 == gmax2 && k.L[0] == L[0] && k.L[1] == L[1] &&
         k.L[2] == L[2])
       K = &k;
+  }
   if (!K) {
-    K = g_ks[0].stamp <= g_ks[1].stamp ? &g_ks[0] : &g_ks[1];
+    K = ks[0].stamp <= ks[1].stamp ? &ks[0] : &ks[1];
     int M = n;
     if (n < 0) {
       M = 0;
@@ -518,7 +542,7 @@ static KSpace *ensure_kspace(const double *L, double alpha, double q, int n, dou
     K->ncolp = ncolp;
     K->nk = nk;
   }
-  K->stamp = ++g_ks_clock;
+  K->stamp = ++pb->ks_clock;
   const size_t slots = (size_t)K->ncolp * K->ZP;
   if (grow(K->tx, K->tx_cap, (size_t)(K->MZ + 1) * nb) || grow(K->ty, K->ty_cap, (size_t)(2 * K->MZ + 1) * nb) ||
       grow(K->tz, K->tz_cap, (size_t)K->ZP * nb) || grow(K->S, K->s_cap, (size_t)P * slots) ||
@@ -971,17 +995,9 @@ static PairParams make_params(DevPimd *p) {
   return pp;
 }
 
-// Tile-pair schedule and partial-sum scratch (grown on demand, kept).
-struct TileBufs {
-  int N = -1, T = 0, ntp = 0;
-  int2 *tiles = nullptr;
-  double *Fpart = nullptr, *Epart = nullptr;
-  size_t fcap = 0, ecap = 0;
-};
-static TileBufs g_tb;
-
-static int ensure_tiles(int N, int P) {
-  TileBufs &B = g_tb;
+static int ensure_tiles(DevPimd *p) {
+  const int N = p->N, P = p->P;
+  TileBufs &B = pair_bufs(p)->tb;
   if (B.N != N) {
     int T = (N + 63) / 64;
     std::vector<int2> tl;
@@ -1020,8 +1036,8 @@ static int near_ok(const PairParams &pp) {
 template <int MODE, bool ENERGY>
 static int launch_pair_mode(DevPimd *p, const PairParams &pp, hipStream_t st, double *esum) {
   DevSim *d = p->ds;
-  if (ensure_tiles(p->N, p->P)) return -1;
-  TileBufs &B = g_tb;
+  if (ensure_tiles(p)) return -1;
+  TileBufs &B = pair_bufs(p)->tb;
   const bool near = (MODE == PK_EWALD_NI) || near_ok(pp);
   const int items = p->P * B.ntp;
   if (near)
@@ -1143,8 +1159,8 @@ static PairParams ewald_params(DevPimd *p, EwaldPlan &pl) {
 
 static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t st) {
   DevSim *d = p->ds;
-  if (ensure_tiles(p->N, p->P)) return -1;
-  TileBufs &B = g_tb;
+  if (ensure_tiles(p)) return -1;
+  TileBufs &B = pair_bufs(p)->tb;
   EwReal ep;
   for (int k = 0; k < 3; ++k) {
     ep.L[k] = pp.L[k];
@@ -1171,7 +1187,7 @@ static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t s
 static int launch_recip(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st, bool force,
                         KSpace **out) {
   DevSim *d = p->ds;
-  KSpace *K = ensure_kspace(pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P);
+  KSpace *K = ensure_kspace(p, pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P);
   if (!K) return -1;
   if (out) *out = K;
   hipLaunchKernelGGL(k_phase, dim3((3 * d->nb + 255) / 256), dim3(256), 0, st, d->X, d->nb, K->MZ, K->ZP, pp.L[0],
@@ -1393,7 +1409,7 @@ int md_pimd_get_ewald_plan_x(md_pimd_t_x *pimd, md_num_t_x *alpha, md_num_t_x *r
   if (alpha) *alpha = pl.alpha;
   if (r_cut) *r_cut = sqrt(pl.rc2);
   if (n_k) {
-    KSpace *K = ensure_kspace(pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, p->ds->nb, p->P);
+    KSpace *K = ensure_kspace(p, pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, p->ds->nb, p->P);
     if (!K) return -1;
     *n_k = K->nk;
   }

@@ -7,7 +7,13 @@
 //   s1: (after fork) forward + backward recursions --join-->
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces add into F after both branches are done.
+//
+// Batches of independent replicas (md_pimd_run_batch_x) run the same step
+// per replica on lanes (stream pairs) forked from one origin stream and
+// joined back, all captured into one hipGraph per step.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "md_hip_x.h"
 #include "xmd_internal.h"
@@ -46,10 +52,43 @@ static int plan_of(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, StepPlan
   return 0;
 }
 
+// A stream pair for one system's step: s0 carries the step, s1 the
+// recursion branch (fork / join events between them); start / done fork a
+// lane from a batch's origin stream and join it back.
+struct Lane {
+  hipStream_t s0 = nullptr, s1 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, start = nullptr, done = nullptr;
+};
+
+static Lane main_lane() {
+  Runtime &R = rt();
+  Lane L;
+  L.s0 = R.s0;
+  L.s1 = R.s1;
+  L.fork = R.fork;
+  L.join = R.join;
+  return L;
+}
+
+// Lanes of batched runs, created on first use and kept.
+static std::vector<Lane> g_lanes;
+static const int kBatchLanes = 8;
+
+static int ensure_lanes(int n) {
+  while ((int)g_lanes.size() < n) {
+    Lane L;
+    XMD_CHECK(hipStreamCreateWithFlags(&L.s0, hipStreamNonBlocking));
+    XMD_CHECK(hipStreamCreateWithFlags(&L.s1, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&L.fork, &L.join, &L.start, &L.done})
+      XMD_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    g_lanes.push_back(L);
+  }
+  return 0;
+}
+
 // ev: optional per-stage events (2 per stage: start, stop), may be null.
 // serial: run every stage on s0 (true per-stage device times for profiling).
-static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial = false) {
-  Runtime &R = rt();
+static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R) {
   DevPimd *p = sp.p;
   DevSim *d = p->ds;
   hipStream_t sx = serial ? R.s0 : R.s1;
@@ -86,17 +125,31 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial = false) {
   return 0;
 }
 
-static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial = false) {
-  Runtime &R = rt();
+static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R) {
   DevSim *d = sp.p->ds;
   if (ev) XMD_CHECK(hipEventRecord(ev[0], R.s0));
   if (launch_nhc(d, 1, sp.dt, R.s0)) return -1;
   if (sp.pbc && launch_pbc(d, sp.L, R.s0)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[1], R.s0));
-  if (enqueue_forces(sp, ev, serial)) return -1;
+  if (enqueue_forces(sp, ev, serial, R)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[14], R.s0));
   if (launch_nhc(d, 2, sp.dt, R.s0)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[15], R.s0));
+  return 0;
+}
+
+// One step of every replica: replica i on lane i % nl, the lanes forked from
+// and joined back into `origin`.
+static int enqueue_batch(std::vector<StepPlan> &plans, int nl, hipStream_t origin) {
+  hipEvent_t go = g_lanes[0].start;
+  XMD_CHECK(hipEventRecord(go, origin));
+  for (int k = 0; k < nl; ++k) XMD_CHECK(hipStreamWaitEvent(g_lanes[k].s0, go, 0));
+  for (size_t i = 0; i < plans.size(); ++i)
+    if (enqueue_step(plans[i], nullptr, false, g_lanes[i % nl])) return -1;
+  for (int k = 0; k < nl; ++k) {
+    XMD_CHECK(hipEventRecord(g_lanes[k].done, g_lanes[k].s0));
+    XMD_CHECK(hipStreamWaitEvent(origin, g_lanes[k].done, 0));
+  }
   return 0;
 }
 
@@ -149,27 +202,28 @@ int md_pimd_get_connection_matrix_x(md_pimd_t_x *pimd, md_num_t_x *G) {
 int md_pimd_compute_forces_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg) {
   StepPlan sp;
   if (plan_of(pimd, cfg, sp)) return -1;
-  return enqueue_forces(sp, nullptr);
+  return enqueue_forces(sp, nullptr, false, main_lane());
 }
 
 int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps) {
   StepPlan sp;
   if (plan_of(pimd, cfg, sp)) return -1;
   if (nsteps <= 0) return 0;
+  const Lane L = main_lane();
   if (!cfg->use_graph || nsteps == 1) {
     for (int s = 0; s < nsteps; ++s)
-      if (enqueue_step(sp, nullptr)) return -1;
+      if (enqueue_step(sp, nullptr, false, L)) return -1;
     return 0;
   }
   // The first step runs eagerly: it allocates every lazily created buffer,
   // so the captured graph contains kernel nodes only.
-  if (enqueue_step(sp, nullptr)) return -1;
+  if (enqueue_step(sp, nullptr, false, L)) return -1;
   Runtime &R = rt();
   XMD_CHECK(hipStreamSynchronize(R.s0));
   hipGraph_t graph;
   hipGraphExec_t exec;
   XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
-  int rc = enqueue_step(sp, nullptr);
+  int rc = enqueue_step(sp, nullptr, false, L);
   hipError_t ec = hipStreamEndCapture(R.s0, &graph);
   if (rc || ec != hipSuccess) {
     fprintf(stderr, "xmd: step capture failed\n");
@@ -194,7 +248,7 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
   // stage k <- events (a, b)
   const int pairs[8][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {10, 11}, {12, 13}, {14, 15}};
   for (int s = 0; s < nsteps; ++s) {
-    if (enqueue_step(sp, ev, serial != 0)) return -1;
+    if (enqueue_step(sp, ev, serial != 0, main_lane())) return -1;
     XMD_CHECK(hipStreamSynchronize(rt().s0));
     if (recursion_status(sp.p)) return -1;
     for (int k = 0; k < 8; ++k) {
@@ -209,6 +263,57 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
   for (int k = 0; k < MD_PIMD_NSTAGES_X; ++k) stage_ms[k] /= nsteps;
   for (int k = 0; k < NE; ++k) XMD_CHECK(hipEventDestroy(ev[k]));
   return 0;
+}
+
+int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *cfgs, int n, int nsteps) {
+  if (!pimds || !cfgs || n <= 0) return -1;
+  if (nsteps <= 0) return 0;
+  std::vector<StepPlan> plans(n);
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < i; ++j)
+      if (pimds[j] == pimds[i] || pimds[j]->sim == pimds[i]->sim) {
+        fprintf(stderr, "xmd: a batch lists one system twice\n");
+        return -1;
+      }
+    if (plan_of(pimds[i], &cfgs[i], plans[i])) return -1;
+  }
+  // Each replica's recursion is one persistent launch whose workgroups (one
+  // per CU, by their LDS) must all be resident together; lanes run
+  // concurrently, so their number is capped to keep the largest replicas'
+  // recursions within 192 of the device's CUs.
+  int rec_wg = 1;
+  for (int i = 0; i < n; ++i) {
+    const int nblk = (plans[i].p->N + 63) / 64;
+    rec_wg = std::max(rec_wg, 2 * (1 + std::max(0, nblk - 2)));
+  }
+  int nl = std::min(n, kBatchLanes);
+  nl = std::max(1, std::min(nl, 192 / rec_wg));
+  if (ensure_lanes(nl)) return -1;
+  Runtime &R = rt();
+  // The first step runs eagerly (lazily created buffers are allocated there),
+  // the rest replay one captured graph of a whole batch step.
+  if (enqueue_batch(plans, nl, R.s0)) return -1;
+  XMD_CHECK(hipStreamSynchronize(R.s0));
+  int rc = 0;
+  if (nsteps > 1) {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
+    rc = enqueue_batch(plans, nl, R.s0);
+    hipError_t ec = hipStreamEndCapture(R.s0, &graph);
+    if (rc || ec != hipSuccess) {
+      fprintf(stderr, "xmd: batch capture failed\n");
+      return -1;
+    }
+    XMD_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    for (int s = 1; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
+    XMD_CHECK(hipStreamSynchronize(R.s0));
+    XMD_CHECK(hipGraphExecDestroy(exec));
+    XMD_CHECK(hipGraphDestroy(graph));
+  }
+  for (int i = 0; i < n; ++i)
+    if (recursion_status(plans[i].p)) rc = -1;
+  return rc;
 }
 
 }  // extern "C"

@@ -102,6 +102,7 @@ struct DevPimd {
   double *work = nullptr;   // estimator scratch
   double *rec = nullptr;    // recursion state: values, accumulators, block coefficients
   int *recsync = nullptr;   // recursion hand-off flags: 4*nblk + 1
+  void *pair_bufs = nullptr;  // pair-kernel tile schedule, partial sums and k-space tables (pair_forces.hip)
   bool enk_dev = false, enk2_dev = false, vbn_dev = false, back_dev = false;
   bool enk_host_stale = false;
   unsigned enk_version = 0, back_version = 0;

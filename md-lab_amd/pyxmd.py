@@ -164,6 +164,7 @@ _SIGS = {
     "md_pimd_calc_vi_virial_pressure_x": (C.c_int, [PIMD_P, STATS_P, C.c_int]),
     "md_reset_stats_f_x": (C.c_int, [STATS_P, C.c_double]),
     "md_pimd_ENk_x": (C.c_double, [PIMD_P, C.c_int, C.c_int, C.c_int]),
+    "md_pimd_calc_vi_sign_grid_x": (C.c_int, [PIMD_P, C.c_int, DP, DP]),
     "md_pimd_ENk2_x": (C.c_double, [PIMD_P, C.c_int, C.c_int, C.c_int]),
     "md_pimd_fast_ENk2_x": (C.c_double, [PIMD_P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "md_pimd_xminE_x": (C.c_double, [PIMD_P, C.c_int]),
@@ -179,6 +180,7 @@ _SIGS = {
     "md_pimd_get_connection_matrix_x": (C.c_int, [PIMD_P, DP]),
     "md_pimd_compute_forces_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig)]),
     "md_pimd_run_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), C.c_int]),
+    "md_pimd_run_batch_x": (C.c_int, [C.POINTER(PIMD_P), C.POINTER(StepConfig), C.c_int, C.c_int]),
     "md_pimd_profile_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), C.c_int, C.c_int, DP]),
 }
 

@@ -86,3 +86,33 @@ def test_checkpoint_read_by_reference(tmp_path):
     ca, cr = pyxmd.nhcs(a), pyxmd.nhcs(r)
     for fld in ("theta", "vtheta", "Q"):
         assert np.array_equal(ca[fld], cr[fld])
+
+
+def test_checkpoint_restores_random_state(tmp_path):
+    """The checkpoint's rng line restores md_random_* exactly: after a read,
+    the next uniform and Gaussian draws (including a pending Box-Muller
+    partner) are the ones that followed the write."""
+    lib = pyxmd.Lib()
+    lib.md_random_uniform_x.restype = C.c_double
+    lib.md_random_uniform_x.argtypes = [C.c_double, C.c_double]
+    lib.md_random_gaussian_x.restype = C.c_double
+    lib.md_set_seed_x.argtypes = [C.c_uint]
+    path = str(tmp_path / "ckpt.txt").encode()
+    sim = filled_sim(lib, 5, 3)
+    lib.md_set_seed_x(91)
+    for _ in range(7):
+        lib.md_random_uniform_x(0.0, 1.0)
+    lib.md_random_gaussian_x()  # leaves the partner value cached
+    f = libc.fopen(path, b"w")
+    assert lib.md_checkpoint_write_x(f, sim) == 0
+    libc.fclose(f)
+    after = [lib.md_random_gaussian_x(), lib.md_random_uniform_x(-1.0, 1.0), lib.md_random_gaussian_x(),
+             lib.md_random_gaussian_x()]
+    lib.md_set_seed_x(5)  # disturb the generator
+    lib.md_random_gaussian_x()
+    f = libc.fopen(path, b"r")
+    assert lib.md_checkpoint_read_x(f, lib.md_alloc_simulation_x(5, 1, 0, 0)) == 0
+    libc.fclose(f)
+    again = [lib.md_random_gaussian_x(), lib.md_random_uniform_x(-1.0, 1.0), lib.md_random_gaussian_x(),
+             lib.md_random_gaussian_x()]
+    assert after == again

@@ -167,6 +167,11 @@ def test_estimators(lib, golden, name):
         ss.contents.es[0] = 1
         assert lib.md_pimd_calc_vi_sign_x(s.pimd, es, ss, a) == 0
         assert ss.contents.es[0] == pytest.approx(sg, rel=1e-10, abs=1e-300)
+    # the whole xi' grid in one launch, against the same per-value reference factors
+    vis = np.ascontiguousarray(g["vi_sign_args"])
+    fac = np.zeros(len(vis))
+    assert lib.md_pimd_calc_vi_sign_grid_x(s.pimd, len(vis), dptr(vis), dptr(fac)) == 0
+    np.testing.assert_allclose(fac, g["vi_sign"], rtol=1e-10, atol=1e-300)
 
 
 @pytest.mark.parametrize("name", CASES)
