@@ -87,6 +87,66 @@ def cpu_baseline(N, P, rs, theta, xi, nsum, pairs):
     }
 
 
+def replica_table(n=64, seed=2024):
+    """Batched-replica workload: n systems, N log-uniform over {8, 16, ..., 512},
+    P in {16, 32, 64}, half harmonic-trap Gaussian bosons (beta = 1), half UEG
+    at r_s in {1, 2, 4} (theta = 0.5), xi uniform over {0, 0.25, 0.5, 0.75, 1}."""
+    rng = np.random.default_rng(seed)
+    table = []
+    for _ in range(n):
+        N = int(8 * 2 ** rng.integers(0, 7))
+        P = int(rng.choice([16, 32, 64]))
+        xi = float(rng.choice([0.0, 0.25, 0.5, 0.75, 1.0]))
+        if rng.random() < 0.5:
+            table.append(dict(kind="trap", N=N, P=P, xi=xi, T=1.0, L=10.0, rs=None))
+        else:
+            rs = float(rng.choice([1.0, 2.0, 4.0]))
+            L = pyxmd.ueg_box(N, rs)
+            table.append(dict(kind="ueg", N=N, P=P, xi=xi, T=pyxmd.ueg_temperature(N, L, 0.5), L=L, rs=rs))
+    return table
+
+
+def batch_bench(lib, steps, dt=0.01):
+    """pbs/s of the replica table advanced by md_pimd_run_batch_x, and of the
+    same replicas advanced one after another (each its own captured graph)."""
+    table = replica_table()
+    # One converged Ewald parameter set for every UEG box of the table (the
+    # interaction parameters are process globals): alpha = 2 / bohr, |m| <= 200;
+    # each box is then evaluated with the engine's own split.
+    def build():
+        out = []
+        for i, r in enumerate(table):
+            s = pyxmd.System(lib, r["kind"], r["N"], r["P"], r["T"], r["xi"], seed=100 + i, L=r["L"], nsum=200,
+                             alpha_L=2.0 * r["L"])
+            out.append(s)
+        lib.md_set_coulomb_truc_x(2.0)
+        lib.md_set_coulomb_n_sum_x(200)
+        return out
+    pbs = sum(r["N"] * r["P"] for r in table)
+    res = {}
+    for mode in ("batched", "one_by_one"):
+        systems = build()
+        cfgs = (pyxmd.StepConfig * len(systems))(*[s.step_config(dt, use_graph=1) for s in systems])
+        ptrs = (pyxmd.PIMD_P * len(systems))(*[s.pimd for s in systems])
+        run = (lambda k: lib.md_pimd_run_batch_x(ptrs, cfgs, len(systems), k)) if mode == "batched" else \
+            (lambda k: sum(lib.md_pimd_run_steps_x(systems[i].pimd, C.byref(cfgs[i]), k)
+                           for i in range(len(systems))))
+        assert run(2) == 0 and lib.md_hip_synchronize_x() == 0
+        t0 = time.perf_counter()
+        assert run(steps) == 0 and lib.md_hip_synchronize_x() == 0
+        sec = time.perf_counter() - t0
+        res[mode] = pbs * steps / sec
+        res[mode + "_ms_per_step"] = 1e3 * sec / steps
+    kinds = {k: sum(1 for r in table if r["kind"] == k) for k in ("trap", "ueg")}
+    return {"workload": f"{len(table)} replicas (seeded table: N log-uniform 8..512, P in {{16,32,64}}, "
+                        f"{kinds['trap']} trap Gaussian beta=1, {kinds['ueg']} UEG r_s in {{1,2,4}} theta=0.5 "
+                        f"converged Ewald, xi in {{0,.25,.5,.75,1}}), NHC M=4, dt={dt}",
+            "particle_beads": pbs, "steps": steps, "unit": "particle-bead-steps/s",
+            "value": res["batched"], "ms_per_step": res["batched_ms_per_step"],
+            "one_by_one_value": res["one_by_one"], "one_by_one_ms_per_step": res["one_by_one_ms_per_step"],
+            "speedup": res["batched"] / res["one_by_one"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +156,8 @@ def main():
     ap.add_argument("--P", type=int, default=32)
     ap.add_argument("--cpu-pairs", type=int, default=25000, help="pairs in the reference CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="steps of the batched-replica measurement (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -149,6 +211,8 @@ def main():
     b = s.host_beads()
     finite = bool(np.all(np.isfinite(b["x"])) and np.all(np.isfinite(b["f"])))
 
+    batched = batch_bench(lib, args.batch_steps) if args.batch_steps > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N, P, rs, theta, xi, nsum, args.cpu_pairs)
@@ -170,6 +234,7 @@ def main():
                      "work_flop_eq": work},
         "stages_ms": {k: round(float(v), 5) for k, v in zip(names, stages)},
         "cpu_baseline": cpu,
+        "batched_replicas": batched,
     }
     if cpu:
         line["gpu_over_cpu"] = value / cpu["value"]

@@ -153,6 +153,24 @@ __global__ __launch_bounds__(1024) void k_sum(const double *__restrict__ a, int 
   if (threadIdx.x == 0) *out = red[0];
 }
 
+int batch_table_make(BatchTable &t, const void *reps, size_t rep_bytes, const std::vector<int> &blocks) {
+  t.nrep = (int)blocks.size();
+  std::vector<int> start(t.nrep + 1, 0);
+  for (int r = 0; r < t.nrep; ++r) start[r + 1] = start[r] + blocks[r];
+  t.total = start[t.nrep];
+  XMD_CHECK(hipMalloc(&t.reps, rep_bytes * (t.nrep > 0 ? t.nrep : 1)));
+  XMD_CHECK(hipMalloc(&t.start, start.size() * sizeof(int)));
+  if (t.nrep > 0) XMD_CHECK(hipMemcpy(t.reps, reps, rep_bytes * t.nrep, hipMemcpyHostToDevice));
+  XMD_CHECK(hipMemcpy(t.start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+void batch_table_free(BatchTable &t) {
+  if (t.reps) (void)hipFree(t.reps);
+  if (t.start) (void)hipFree(t.start);
+  t = BatchTable();
+}
+
 int device_sum(const double *src, int n, double *out) {
   static double *d_out = nullptr;
   if (!d_out) XMD_CHECK(hipMalloc(&d_out, sizeof(double)));

@@ -99,13 +99,13 @@ __device__ __forceinline__ double e2(const double *X, const double *M, int nb, i
 // Particle l (0-based): Eint_l, E_l^(1), link C_l.  A block holds 64
 // particles (lanes); its four waves split the P - 1 interior springs and
 // the partial sums are added in a fixed order.
-__global__ __launch_bounds__(256) void k_chain(const double *__restrict__ X, const double *__restrict__ M, int N,
-                                               int P, double om, Geo g, double *__restrict__ Eint,
-                                               double *__restrict__ Clink, double *__restrict__ ENk,
-                                               double *__restrict__ ET) {
+__device__ __forceinline__ void chain_body(int bx, const double *__restrict__ X, const double *__restrict__ M, int N,
+                                           int P, double om, Geo g, double *__restrict__ Eint,
+                                           double *__restrict__ Clink, double *__restrict__ ENk,
+                                           double *__restrict__ ET) {
   __shared__ double part[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int l = blockIdx.x * 64 + lane;
+  const int l = bx * 64 + lane;
   const int nb = N * P;
   double e = 0;
   if (l < N) {
@@ -124,8 +124,15 @@ __global__ __launch_bounds__(256) void k_chain(const double *__restrict__ X, con
   Clink[l] = (l < N - 1) ? e + spring_e(X, nb, sP, l + 1, M[sP], om, g) : 0.0;
 }
 
+__global__ __launch_bounds__(256) void k_chain(const double *__restrict__ X, const double *__restrict__ M, int N,
+                                               int P, double om, Geo g, double *__restrict__ Eint,
+                                               double *__restrict__ Clink, double *__restrict__ ENk,
+                                               double *__restrict__ ET) {
+  chain_body(blockIdx.x, X, M, N, P, om, g, Eint, Clink, ENk, ET);
+}
+
 // Exclusive double-double prefix sum of in[0..n-1] into pre[2*i] (hi, lo).
-__global__ __launch_bounds__(1024) void k_scan_dd(const double *__restrict__ in, int n, double *__restrict__ pre) {
+__device__ __forceinline__ void scan_dd_body(const double *__restrict__ in, int n, double *__restrict__ pre) {
   __shared__ double sh[1024], sl[1024];
   const int t = threadIdx.x;
   const int seg = (n + 1023) / 1024;
@@ -151,12 +158,16 @@ __global__ __launch_bounds__(1024) void k_scan_dd(const double *__restrict__ in,
   }
 }
 
+__global__ __launch_bounds__(1024) void k_scan_dd(const double *__restrict__ in, int n, double *__restrict__ pre) {
+  scan_dd_body(in, n, pre);
+}
+
 // Row n' (0-based) of the cycle-energy triangle, k >= 2, both layouts.
-__global__ __launch_bounds__(256) void k_enk_fill(const double *__restrict__ X, const double *__restrict__ M, int N,
-                                                  int P, double om, Geo g, const double *__restrict__ Eint,
-                                                  const double *__restrict__ pre, double *__restrict__ ENk,
-                                                  double *__restrict__ ET) {
-  const int r = blockIdx.x;  // 0-based last particle of the cycle
+__device__ __forceinline__ void enk_fill_body(int r, const double *__restrict__ X, const double *__restrict__ M,
+                                              int N, int P, double om, Geo g, const double *__restrict__ Eint,
+                                              const double *__restrict__ pre, double *__restrict__ ENk,
+                                              double *__restrict__ ET) {
+  // r: 0-based last particle of the cycle
   const int nb = N * P;
   const int sP = (P - 1) * N + r;
   const double mP = M[sP], ein = Eint[r];
@@ -168,6 +179,13 @@ __global__ __launch_bounds__(256) void k_enk_fill(const double *__restrict__ X, 
     ENk[tri_row(r + 1) + k - 1] = E;
     ET[col_off(m0, N) + k - 1] = E;
   }
+}
+
+__global__ __launch_bounds__(256) void k_enk_fill(const double *__restrict__ X, const double *__restrict__ M, int N,
+                                                  int P, double om, Geo g, const double *__restrict__ Eint,
+                                                  const double *__restrict__ pre, double *__restrict__ ENk,
+                                                  double *__restrict__ ET) {
+  enk_fill_body(blockIdx.x, X, M, N, P, om, g, Eint, pre, ENk, ET);
 }
 
 // Host-provided ENk (row layout) -> column layout.
@@ -297,14 +315,16 @@ __device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t) {
   }
 }
 
-__global__ void k_rec_prep(RecArgs a, int dir0) {
-  const int dir = dir0 + blockIdx.y;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.y == 0 && t < 4 * a.nblk + 1) a.sync[t] = 0;
+__device__ __forceinline__ void rec_prep_body(const RecArgs &a, int dir, int first, int t) {
+  if (first && t < 4 * a.nblk + 1) a.sync[t] = 0;
   if (dir == 0)
     rec_prep_one<true>(a, 0, t);
   else
     rec_prep_one<false>(a, 1, t);
+}
+
+__global__ void k_rec_prep(RecArgs a, int dir0) {
+  rec_prep_body(a, dir0 + blockIdx.y, blockIdx.y == 0, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // (M, S) <- log-sum-exp merge with (m, s).
@@ -367,11 +387,15 @@ __device__ __forceinline__ void rec_post(int *flag, int value) {
   if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <bool FWD>
-__device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[RB], double (*ps)[RB]) {
+// Folds solved blocks 0 .. G-2 into the accumulators of row block G (lane =
+// row, the four waves split each block's 64 columns), in block order.  WAIT:
+// another workgroup solves the blocks (wait for each); otherwise this
+// workgroup solved them itself.
+template <bool FWD, bool WAIT>
+__device__ __forceinline__ void rec_fold_rows(const RecArgs &a, int dir, int G, double (*pm)[RB], double (*ps)[RB]) {
   const int N = a.N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  int *solved = a.sync + dir * a.nblk, *folded = a.sync + (2 + dir) * a.nblk, *err = a.sync + 4 * a.nblk;
+  int *solved = a.sync + dir * a.nblk, *err = a.sync + 4 * a.nblk;
   const double *val = a.val2 + (size_t)dir * (N + 1);
   double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
   const int p = 1 + G * RB + lane;
@@ -382,7 +406,7 @@ __device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, do
     Sa = AS[p];
   }
   for (int b = 0; b <= G - 2; ++b) {
-    rec_wait(solved + b, 1, err);
+    if (WAIT) rec_wait(solved + b, 1, err);
     const int pprev = 1 + b * RB;
     double t[16];
 #pragma unroll
@@ -410,6 +434,12 @@ __device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, do
     AM[p] = Ma;
     AS[p] = Sa;
   }
+}
+
+template <bool FWD>
+__device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[RB], double (*ps)[RB]) {
+  int *folded = a.sync + (2 + dir) * a.nblk;
+  rec_fold_rows<FWD, true>(a, dir, G, pm, ps);
   rec_post(folded + G, G - 1);
 }
 
@@ -431,6 +461,7 @@ struct RecLds {
   double mo[RB], so[RB];    // row accumulators (max, scaled sum)
   double sg[RB];            // per-row shifts
   double pm[4][RB], ps[4][RB];
+  double fm[4][RB], fs[4][RB];  // partials of a self-fold (SOLO)
 };
 
 // The solver walks the blocks in order.  Per block b: stage its in-block
@@ -438,7 +469,9 @@ struct RecLds {
 // into the rows from the prefetched look-ahead coefficients and the values
 // kept in LDS, prefetch block b+1, wait for the updater of block b, merge,
 // then the (max,+) shift scan, scaled exps and the linear substitution.
-template <bool FWD>
+// SOLO: no updater workgroups; the solver folds blocks 0 .. b-2 into block b
+// itself (same order and arithmetic), so it waits on no other workgroup.
+template <bool FWD, bool SOLO = false>
 __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S) {
   const int N = a.N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -490,10 +523,14 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
 #pragma unroll
       for (int k = 0; k < LPT; ++k) S.la[cur ^ 1][tid + 256 * k] = lar[k];
     }
-    if (b >= 2)
-      rec_wait(folded + b, b - 1, err);
-    else
+    if (SOLO) {
       __syncthreads();
+      if (b >= 2) rec_fold_rows<FWD, false>(a, dir, b, S.fm, S.fs);
+    } else if (b >= 2) {
+      rec_wait(folded + b, b - 1, err);
+    } else {
+      __syncthreads();
+    }
     if (wv == 0) {
       double M = NEG, Sm = 0.0;
       if (lane < nrows) {
@@ -592,11 +629,11 @@ __global__ void k_vbn_to_L(const double *VBN, double *Lf, int N, double beta) {
 
 // Block per particle l (1-based l = blockIdx.x + 1): bead P against every
 // bead 1 (row l of G) and bead 1 against every bead P (column l of G).
-__global__ __launch_bounds__(256) void k_exch_force(const double *__restrict__ X, const double *__restrict__ M,
-                                                    double *__restrict__ F, int N, int P, double om, Geo g,
-                                                    RecArgs a) {
+__device__ __forceinline__ void exch_force_body(int bx, const double *__restrict__ X, const double *__restrict__ M,
+                                                double *__restrict__ F, int N, int P, double om, Geo g,
+                                                const RecArgs &a) {
   __shared__ double red[8][256];
-  const int l = blockIdx.x + 1;
+  const int l = bx + 1;
   const int nb = N * P;
   const int sP = (P - 1) * N + (l - 1), s1 = l - 1;
   const double LN = a.Lf[N];
@@ -660,6 +697,12 @@ __global__ __launch_bounds__(256) void k_exch_force(const double *__restrict__ X
       F[d * nb + s1] += -grad1 / m1;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_exch_force(const double *__restrict__ X, const double *__restrict__ M,
+                                                    double *__restrict__ F, int N, int P, double om, Geo g,
+                                                    RecArgs a) {
+  exch_force_body(blockIdx.x, X, M, F, N, P, om, g, a);
 }
 
 // Optional full connection matrix (tests): G[(l-1)*N + (j-1)].
@@ -892,6 +935,70 @@ __global__ __launch_bounds__(256) void k_vi_sign(RecArgs a, const double *__rest
   if (threadIdx.x == 0) *out = sg[N] * exp(-a.beta * V2[N] - a.Lf[N]);
 }
 
+// ------------------------------------------------- batched replicas
+//
+// One launch per stage over all replicas of a batch; the per-replica work of
+// each kernel is the single-system kernel's body with the same block
+// decomposition (virtual block index = global block - start[replica]).
+struct ExRep {
+  const double *X, *M;
+  double *F;
+  int N, P;
+  double om;
+  Geo g;
+  double *Eint, *clink, *Cpre, *ENk, *ET;
+  RecArgs a;
+  int prep_blocks;  // blocks per direction of the recursion set-up
+};
+
+__global__ __launch_bounds__(256) void kb_chain(const ExRep *__restrict__ reps, const int *__restrict__ start,
+                                                int nrep) {
+  const int r = batch_rep(start, nrep, blockIdx.x);
+  const ExRep &R = reps[r];
+  chain_body(blockIdx.x - start[r], R.X, R.M, R.N, R.P, R.om, R.g, R.Eint, R.clink, R.ENk, R.ET);
+}
+
+// one 1024-thread block per replica
+__global__ __launch_bounds__(1024) void kb_scan_dd(const ExRep *__restrict__ reps) {
+  const ExRep &R = reps[blockIdx.x];
+  scan_dd_body(R.clink, R.N, R.Cpre);
+}
+
+__global__ __launch_bounds__(256) void kb_enk_fill(const ExRep *__restrict__ reps, const int *__restrict__ start,
+                                                   int nrep) {
+  const int r = batch_rep(start, nrep, blockIdx.x);
+  const ExRep &R = reps[r];
+  enk_fill_body(blockIdx.x - start[r], R.X, R.M, R.N, R.P, R.om, R.g, R.Eint, R.Cpre, R.ENk, R.ET);
+}
+
+// both directions: blocks [0, prep_blocks) forward, [prep_blocks, 2 prep_blocks) backward
+__global__ __launch_bounds__(256) void kb_rec_prep(const ExRep *__restrict__ reps, const int *__restrict__ start,
+                                                   int nrep) {
+  const int r = batch_rep(start, nrep, blockIdx.x);
+  const ExRep &R = reps[r];
+  const int lb = blockIdx.x - start[r];
+  const int dir = lb >= R.prep_blocks ? 1 : 0, bx = lb - dir * R.prep_blocks;
+  rec_prep_body(R.a, dir, dir == 0, bx * 256 + threadIdx.x);
+}
+
+// one workgroup per (replica, direction), each solving its whole recursion
+// alone (no hand-off between workgroups)
+__global__ __launch_bounds__(256) void kb_rec_solo(const ExRep *__restrict__ reps) {
+  __shared__ RecLds S;
+  const ExRep &R = reps[blockIdx.x >> 1];
+  if ((blockIdx.x & 1) == 0)
+    rec_solver<true, true>(R.a, 0, S);
+  else
+    rec_solver<false, true>(R.a, 1, S);
+}
+
+__global__ __launch_bounds__(256) void kb_exch_force(const ExRep *__restrict__ reps, const int *__restrict__ start,
+                                                     int nrep) {
+  const int r = batch_rep(start, nrep, blockIdx.x);
+  const ExRep &R = reps[r];
+  exch_force_body(blockIdx.x - start[r], R.X, R.M, R.F, R.N, R.P, R.om, R.g, R.a);
+}
+
 // ------------------------------------------------------------ launchers
 
 static Geo geo_of(DevPimd *p, int image) {
@@ -995,6 +1102,113 @@ int launch_exchange_force(DevPimd *p, int image, hipStream_t st) {
   hipLaunchKernelGGL(k_exch_force, dim3(p->N), dim3(256), 0, st, d->X, d->M, d->F, p->N, p->P, p->host->omegaP, g, a);
   XMD_CHECK(hipGetLastError());
   return 0;
+}
+
+struct ExchBatch {
+  std::vector<DevPimd *> ps;
+  std::vector<int> image;
+  BatchTable chain, enk, prep, force;
+  ExRep *reps = nullptr;  // device table shared by the per-replica-grid kernels
+  int nrep = 0;
+};
+
+ExchBatch *exch_batch_create(const std::vector<BatchItem> &items) {
+  ExchBatch *b = new ExchBatch();
+  std::vector<ExRep> reps;
+  std::vector<int> bc, be, bp, bf;
+  for (const BatchItem &it : items) {
+    DevPimd *p = it.p;
+    DevSim *d = p->ds;
+    ExRep R;
+    R.X = d->X;
+    R.M = d->M;
+    R.F = d->F;
+    R.N = p->N;
+    R.P = p->P;
+    R.om = p->host->omegaP;
+    R.g = geo_of(p, it.image);
+    R.Eint = p->Eint;
+    R.clink = p->work + (p->N + 1);
+    R.Cpre = p->Cpre;
+    R.ENk = p->ENk;
+    R.ET = p->ET;
+    R.a = rec_args(p);
+    const int nprep = std::max(p->N + 1, R.a.nblk * RB * RB);
+    R.prep_blocks = (nprep + 255) / 256;
+    reps.push_back(R);
+    bc.push_back((p->N + 63) / 64);
+    be.push_back(p->N);
+    bp.push_back(2 * R.prep_blocks);
+    bf.push_back(p->N);
+    b->ps.push_back(p);
+    b->image.push_back(it.image);
+    // ln n is constant: written once here
+    hipLaunchKernelGGL(k_lnn, dim3((p->N + 256) / 256), dim3(256), 0, rt().s0, p->work, p->N);
+  }
+  b->nrep = (int)reps.size();
+  const size_t sz = sizeof(ExRep);
+  if (hipGetLastError() != hipSuccess || batch_table_make(b->chain, reps.data(), sz, bc) ||
+      batch_table_make(b->enk, reps.data(), sz, be) || batch_table_make(b->prep, reps.data(), sz, bp) ||
+      batch_table_make(b->force, reps.data(), sz, bf) || hipStreamSynchronize(rt().s0) != hipSuccess) {
+    exch_batch_free(b);
+    return nullptr;
+  }
+  b->reps = (ExRep *)b->chain.reps;
+  return b;
+}
+
+void exch_batch_free(ExchBatch *b) {
+  if (!b) return;
+  batch_table_free(b->chain);
+  batch_table_free(b->enk);
+  batch_table_free(b->prep);
+  batch_table_free(b->force);
+  delete b;
+}
+
+// cycle energies (k_chain, k_scan_dd, k_enk_fill of every replica)
+int exch_batch_enk(ExchBatch *b, hipStream_t st) {
+  hipLaunchKernelGGL(kb_chain, dim3(b->chain.total), dim3(256), 0, st, (const ExRep *)b->chain.reps, b->chain.start,
+                     b->nrep);
+  hipLaunchKernelGGL(kb_scan_dd, dim3(b->nrep), dim3(1024), 0, st, (const ExRep *)b->chain.reps);
+  hipLaunchKernelGGL(kb_enk_fill, dim3(b->enk.total), dim3(256), 0, st, (const ExRep *)b->enk.reps, b->enk.start,
+                     b->nrep);
+  XMD_CHECK(hipGetLastError());
+  for (size_t i = 0; i < b->ps.size(); ++i) {
+    DevPimd *p = b->ps[i];
+    p->enk_dev = true;
+    p->last_image = b->image[i];
+    p->enk_version++;
+  }
+  return 0;
+}
+
+// forward and backward recursions of every replica
+int exch_batch_recursion(ExchBatch *b, hipStream_t st) {
+  hipLaunchKernelGGL(kb_rec_prep, dim3(b->prep.total), dim3(256), 0, st, (const ExRep *)b->prep.reps, b->prep.start,
+                     b->nrep);
+  hipLaunchKernelGGL(kb_rec_solo, dim3(2 * b->nrep), dim3(256), 0, st, (const ExRep *)b->prep.reps);
+  XMD_CHECK(hipGetLastError());
+  for (DevPimd *p : b->ps) {
+    p->vbn_dev = true;
+    p->back_dev = true;
+    p->back_version = p->enk_version;
+  }
+  return 0;
+}
+
+int exch_batch_force(ExchBatch *b, hipStream_t st) {
+  hipLaunchKernelGGL(kb_exch_force, dim3(b->force.total), dim3(256), 0, st, (const ExRep *)b->force.reps,
+                     b->force.start, b->nrep);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+int exch_batch_status(ExchBatch *b) {
+  int rc = 0;
+  for (DevPimd *p : b->ps)
+    if (recursion_status(p)) rc = -1;
+  return rc;
 }
 
 // ENk filled on the host (driver-provided) -> device, both layouts.

@@ -8,6 +8,7 @@
 // (massive thermostats) or per chain, coalesced over the slice-major SoA.
 #include <cmath>
 
+#include "xmd_device.h"
 #include "xmd_internal.h"
 
 namespace xmd {
@@ -86,12 +87,12 @@ __global__ void k_ring_springs(const double *__restrict__ X, const double *__res
 // One chain per degree of freedom.  Thread e owns DOF e (device order), its
 // velocity, position, force and chain slots th/vth/Q[k*ndof + e].
 template <int HALF>
-__global__ void k_nhc_massive(double *__restrict__ X, double *__restrict__ V, const double *__restrict__ F,
-                              const double *__restrict__ M, double *__restrict__ th, double *__restrict__ vth,
-                              const double *__restrict__ Qa, int nb, int Mlen, double T, double h) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nhc_massive_body(double *__restrict__ X, double *__restrict__ V,
+                                                 const double *__restrict__ F, const double *__restrict__ M,
+                                                 double *__restrict__ th, double *__restrict__ vth,
+                                                 const double *__restrict__ Qa, int nb, int Mlen, double T, double h,
+                                                 int e) {
   int ndof = 3 * nb;
-  if (e >= ndof) return;
   double m = M[e % nb];
   double v = V[e], f = F[e];
   double t[MD_NHC_LENGTH_X], vt[MD_NHC_LENGTH_X], Q[MD_NHC_LENGTH_X], a[MD_NHC_LENGTH_X];
@@ -148,6 +149,15 @@ __global__ void k_nhc_massive(double *__restrict__ X, double *__restrict__ V, co
       th[k * ndof + e] = t[k];
       vth[k * ndof + e] = vt[k];
     }
+}
+
+template <int HALF>
+__global__ void k_nhc_massive(double *__restrict__ X, double *__restrict__ V, const double *__restrict__ F,
+                              const double *__restrict__ M, double *__restrict__ th, double *__restrict__ vth,
+                              const double *__restrict__ Qa, int nb, int Mlen, double T, double h) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * nb) return;
+  nhc_massive_body<HALF>(X, V, F, M, th, vth, Qa, nb, Mlen, T, h, e);
 }
 
 // General grouping: thread per chain, its DOF range [f0, f0+f2) in the
@@ -298,6 +308,144 @@ __global__ __launch_bounds__(256) void k_bead_sums(const double *__restrict__ X,
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------- batched replicas
+//
+// Per-replica arguments of the bead-local stages; every thread owns one
+// degree of freedom e = d * nb + s of one replica, with the same arithmetic
+// as the single-system kernels above.
+struct BeadRep {
+  double *X, *V, *F, *th, *vth;
+  const double *M, *Q;
+  int nb, N, P, Mlen;
+  double T, h;
+  int pbc, image, trap;
+  double L[3], w, c[3], om;
+};
+
+// NHC half-kick (+ drift for HALF 1), then the box wrap after HALF 1.
+template <int HALF>
+__global__ __launch_bounds__(256) void kb_nhc(const BeadRep *__restrict__ reps, const int *__restrict__ start,
+                                              int nrep) {
+  const int r = batch_rep(start, nrep, blockIdx.x);
+  const BeadRep &R = reps[r];
+  const int e = (blockIdx.x - start[r]) * 256 + threadIdx.x;
+  if (e >= 3 * R.nb) return;
+  nhc_massive_body<HALF>(R.X, R.V, R.F, R.M, R.th, R.vth, R.Q, R.nb, R.Mlen, R.T, R.h, e);
+  if (HALF == 1 && R.pbc) {
+    const int d = e / R.nb;
+    const double L = R.L[d];
+    double x = R.X[e];
+    if (x < 0)
+      x += L;
+    else if (x > L)
+      x -= L;
+    R.X[e] = x;
+  }
+}
+
+// Force reset, trap (md_pimd_x.c:1370-1389) and interior ring springs.
+__global__ __launch_bounds__(256) void kb_bead_forces(const BeadRep *__restrict__ reps, const int *__restrict__ start,
+                                                      int nrep) {
+  const int r = batch_rep(start, nrep, blockIdx.x);
+  const BeadRep &R = reps[r];
+  const int i = (blockIdx.x - start[r]) * 256 + threadIdx.x;
+  const int nb = R.nb;
+  if (i >= 3 * nb) return;
+  const int d = i / nb, s = i - d * nb;
+  double Fv = 0.0;
+  if (R.trap) {
+    double f = 0;
+    f += -R.w * R.w * (R.X[i] - R.c[d]);
+    Fv += f / R.P;
+  }
+  if (s >= R.N && s < nb - R.N) {
+    const double m = R.M[s], om = R.om;
+    const double x = R.X[i];
+    double a = x - R.X[i + R.N];
+    double c = x - R.X[i - R.N];
+    if (R.image) {
+      a = min_image(a, R.L[d]);
+      c = min_image(c, R.L[d]);
+    }
+    double g = 0;
+    g += m * om * om * a;
+    g += m * om * om * c;
+    Fv += -g / m;
+  }
+  R.F[i] = Fv;
+}
+
+struct BeadBatch {
+  BatchTable t;
+};
+
+BeadBatch *bead_batch_create(const std::vector<BatchItem> &items) {
+  std::vector<BeadRep> reps;
+  std::vector<int> blocks;
+  for (const BatchItem &it : items) {
+    DevPimd *p = it.p;
+    DevSim *d = p->ds;
+    if (!d->massive) {
+      fprintf(stderr, "xmd: batched steps need massive thermostats (one chain per degree of freedom)\n");
+      return nullptr;
+    }
+    BeadRep R;
+    R.X = d->X;
+    R.V = d->V;
+    R.F = d->F;
+    R.th = d->th;
+    R.vth = d->vth;
+    R.M = d->M;
+    R.Q = d->Q;
+    R.nb = d->nb;
+    R.N = p->N;
+    R.P = p->P;
+    R.Mlen = g_nhc_len;
+    R.T = d->host->T;
+    R.h = it.dt;
+    R.pbc = it.pbc;
+    R.image = it.image;
+    R.trap = it.trap;
+    for (int k = 0; k < 3; ++k) {
+      R.L[k] = it.L[k];
+      R.c[k] = md_trap_center_x[k];
+    }
+    R.w = md_trap_frequency_x;
+    R.om = p->host->omegaP;
+    reps.push_back(R);
+    blocks.push_back((3 * d->nb + 255) / 256);
+  }
+  BeadBatch *b = new BeadBatch();
+  if (batch_table_make(b->t, reps.data(), sizeof(BeadRep), blocks)) {
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
+void bead_batch_free(BeadBatch *b) {
+  if (!b) return;
+  batch_table_free(b->t);
+  delete b;
+}
+
+int bead_batch_nhc(BeadBatch *b, int half, hipStream_t st) {
+  const BeadRep *reps = (const BeadRep *)b->t.reps;
+  if (half == 1)
+    hipLaunchKernelGGL(kb_nhc<1>, dim3(b->t.total), dim3(256), 0, st, reps, b->t.start, b->t.nrep);
+  else
+    hipLaunchKernelGGL(kb_nhc<2>, dim3(b->t.total), dim3(256), 0, st, reps, b->t.start, b->t.nrep);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+int bead_batch_forces(BeadBatch *b, hipStream_t st) {
+  hipLaunchKernelGGL(kb_bead_forces, dim3(b->t.total), dim3(256), 0, st, (const BeadRep *)b->t.reps, b->t.start,
+                     b->t.nrep);
+  XMD_CHECK(hipGetLastError());
+  return 0;
 }
 
 // ------------------------------------------------------------ launchers

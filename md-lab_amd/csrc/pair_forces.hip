@@ -309,12 +309,12 @@ __device__ __forceinline__ void pair_term(double dx, double dy, double dz, const
 // (a wave writes 64 consecutive doubles) and k_reduce_tiles sums them in a
 // fixed order (deterministic forces).
 template <int MODE, bool ENERGY, bool NEAR>
-__global__ __launch_bounds__(64) void k_pair_tiles(const double *__restrict__ X, const double *__restrict__ M,
+__device__ __forceinline__ void k_pair_tiles_body(int bx, const double *__restrict__ X, const double *__restrict__ M,
                                                    double *__restrict__ Fpart, double *__restrict__ Epart,
                                                    const int2 *__restrict__ tiles, int N, int nb, int T, int ntp,
                                                    PairParams pp) {
   __shared__ double jx[64], jy[64], jz[64], jm[64], rx[64], ry[64], rz[64];
-  const int item = blockIdx.x;
+  const int item = bx;
   const int slice = item / ntp;
   const int2 IJ = tiles[item - slice * ntp];
   const int I = IJ.x, J = IJ.y;
@@ -379,13 +379,25 @@ __global__ __launch_bounds__(64) void k_pair_tiles(const double *__restrict__ X,
   }
 }
 
+template <int MODE, bool ENERGY, bool NEAR>
+__global__ __launch_bounds__(64) void k_pair_tiles(const double *__restrict__ X, const double *__restrict__ M,
+                                                   double *__restrict__ Fpart, double *__restrict__ Epart,
+                                                   const int2 *__restrict__ tiles, int N, int nb, int T, int ntp,
+                                                   PairParams pp) {
+  k_pair_tiles_body<MODE, ENERGY, NEAR>(blockIdx.x, X, M, Fpart, Epart, tiles, N, nb, T, ntp, pp);
+}
+
 // F[d*nb + s] += sum_slot Fpart / P, slots in a fixed order.
-__global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restrict__ F, int n3, int T, double P) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_reduce_tiles_body(int bx, const double *__restrict__ Fpart, double *__restrict__ F, int n3, int T, double P) {
+  const int k = bx * blockDim.x + threadIdx.x;
   if (k >= n3) return;
   double s = 0;
   for (int q = 0; q < T; ++q) s += Fpart[(size_t)q * n3 + k];
   F[k] += s / P;
+}
+
+__global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restrict__ F, int n3, int T, double P) {
+  k_reduce_tiles_body(blockIdx.x, Fpart, F, n3, T, P);
 }
 
 // ------------------------------------------------------ reciprocal space
@@ -553,9 +565,9 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
 
 // One thread per (dimension, bead): e^{i pi m u}, u = 2 x / L; negative m
 // are the conjugates.
-__global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZP, double L0, double L1, double L2,
+__device__ __forceinline__ void k_phase_body(int bx, const double *__restrict__ X, int nb, int MZ, int ZP, double L0, double L1, double L2,
                         double2 *__restrict__ tx, double2 *__restrict__ ty, double2 *__restrict__ tz) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = bx * blockDim.x + threadIdx.x;
   if (t >= 3 * nb) return;
   const int d = t / nb, s = t - d * nb;
   const double u = 2.0 * X[t] / (d == 0 ? L0 : (d == 1 ? L1 : L2));
@@ -588,6 +600,11 @@ __global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZP, do
   }
 }
 
+__global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZP, double L0, double L1, double L2,
+                        double2 *__restrict__ tx, double2 *__restrict__ ty, double2 *__restrict__ tz) {
+  k_phase_body(blockIdx.x, X, nb, MZ, ZP, L0, L1, L2, tx, ty, tz);
+}
+
 // Workgroups are numbered so that the tiles of one slice share an XCD
 // (consecutive workgroup ids go to consecutive XCDs) when the slice count
 // is a multiple of 8; otherwise the plain order.
@@ -606,7 +623,7 @@ __device__ __forceinline__ void xcd_tile(int w, int ntile, int P, int &slice, in
 // slice; the four waves split the beads and their partial sums are added in
 // a fixed order.  Writes S and W = coef conj(S).  Workgroup w covers
 // (slice, column tile, slot tile), slot tile fastest.
-__global__ __launch_bounds__(256) void k_sk_mfma(const double2 *__restrict__ tx, const double2 *__restrict__ ty,
+__device__ __forceinline__ void k_sk_mfma_body(int bx, const double2 *__restrict__ tx, const double2 *__restrict__ ty,
                                                  const double2 *__restrict__ tz, const int2 *__restrict__ cols,
                                                  const double *__restrict__ coef, int ncolp, int MZ, int ZP, int N,
                                                  int nb, int P, double2 *__restrict__ S, double2 *__restrict__ W) {
@@ -614,7 +631,7 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2 *__restrict__ tx,
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nzt = ZP / 16;
   int slice, tile;
-  xcd_tile(blockIdx.x, (ncolp / 16) * nzt, P, slice, tile);
+  xcd_tile(bx, (ncolp / 16) * nzt, P, slice, tile);
   const int c0 = (tile / nzt) * 16, z0 = (tile % nzt) * 16;
   const int2 cl = cols[c0 + (lane & 15)];
   const double2 *px = tx + (size_t)cl.x * nb;
@@ -666,6 +683,13 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2 *__restrict__ tx,
   W[(size_t)slice * ncolp * ZP + k] = make_double2(c * re, -c * im);
 }
 
+__global__ __launch_bounds__(256) void k_sk_mfma(const double2 *__restrict__ tx, const double2 *__restrict__ ty,
+                                                 const double2 *__restrict__ tz, const int2 *__restrict__ cols,
+                                                 const double *__restrict__ coef, int ncolp, int MZ, int ZP, int N,
+                                                 int nb, int P, double2 *__restrict__ S, double2 *__restrict__ W) {
+  k_sk_mfma_body(blockIdx.x, tx, ty, tz, cols, coef, ncolp, MZ, ZP, N, nb, P, S, W);
+}
+
 // Forces on 16 beads of one slice.  Work units are (column tile, slot
 // tile) pairs, dealt round-robin to the four waves; per unit
 //   U[s][c] = sum_z Z[s][z] W[c][z],  V[s][c] = sum_z Z[s][z] m_z W[c][z]
@@ -674,7 +698,7 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2 *__restrict__ tx,
 // tiles add up).  For bead i this is sum_k coef_k g_k Im(e^{i k.x_i} S_k*) /
 // (m_i P), the reference's per-pair sum (md_simulation_x.c:266-281) since
 // sum_{l != i} sin(k.(x_i - x_l)) = Im(e^{i k.x_i} S_k*).
-__global__ __launch_bounds__(256) void k_recip_force_mfma(
+__device__ __forceinline__ void k_recip_force_mfma_body(int bx, 
     const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
     const int2 *__restrict__ cols, int ncolp, int MZ, int ZP, const double2 *__restrict__ W,
     const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
@@ -683,7 +707,7 @@ __global__ __launch_bounds__(256) void k_recip_force_mfma(
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nbt = (N + 15) / 16;
   int slice, bt;
-  xcd_tile(blockIdx.x, nbt, P, slice, bt);
+  xcd_tile(bx, nbt, P, slice, bt);
   const int b0 = bt * 16;
   const int abead = b0 + (lane & 15);  // A operand row of this lane
   const double2 *pza = tz + (size_t)(slice * N + (abead < N ? abead : 0)) * ZP + (lane >> 4);
@@ -775,6 +799,14 @@ __global__ __launch_bounds__(256) void k_recip_force_mfma(
       F[2 * nb + s] += gz / L2 / m / Pd;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_recip_force_mfma(
+    const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
+    const int2 *__restrict__ cols, int ncolp, int MZ, int ZP, const double2 *__restrict__ W,
+    const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
+    double Pd) {
+  k_recip_force_mfma_body(blockIdx.x, tx, ty, tz, cols, ncolp, MZ, ZP, W, Mass, F, N, nb, P, L0, L1, L2, Pd);
 }
 
 // Reciprocal energy of every slice: sum_{k in half} A_k (|S_k|^2 - N), into partials.
@@ -882,7 +914,7 @@ __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, 
 // Four tile pairs per workgroup (one per wave) sharing the LDS table; the
 // per-wave schedule is that of k_pair_tiles.
 template <bool CIMG>
-__global__ __launch_bounds__(256) void k_ewald_real(const double *__restrict__ X, const double *__restrict__ M,
+__device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restrict__ X, const double *__restrict__ M,
                                                     double *__restrict__ Fpart, const int2 *__restrict__ tiles, int N,
                                                     int nb, int T, int ntp, int nitems, EwReal ep) {
   __shared__ double2 tab[kEwTabIntervals * kEwTabStride / 2];
@@ -890,7 +922,7 @@ __global__ __launch_bounds__(256) void k_ewald_real(const double *__restrict__ X
   const int tid = threadIdx.x, t = tid & 63, w = tid >> 6;
   for (int k = tid; k < kEwTabIntervals * kEwTabStride / 2; k += 256)
     tab[k] = make_double2(kEwTab[2 * k], kEwTab[2 * k + 1]);
-  const int item = blockIdx.x * 4 + w;
+  const int item = bx * 4 + w;
   const bool valid = item < nitems;
   int slice = 0, I = 0, J = 0;
   if (valid) {
@@ -950,6 +982,13 @@ __global__ __launch_bounds__(256) void k_ewald_real(const double *__restrict__ X
     if (act) put(i, J, fx, fy, fz);
     if (jact) put(jl, I, rx[w][t], ry[w][t], rz[w][t]);
   }
+}
+
+template <bool CIMG>
+__global__ __launch_bounds__(256) void k_ewald_real(const double *__restrict__ X, const double *__restrict__ M,
+                                                    double *__restrict__ Fpart, const int2 *__restrict__ tiles, int N,
+                                                    int nb, int T, int ntp, int nitems, EwReal ep) {
+  k_ewald_real_body<CIMG>(blockIdx.x, X, M, Fpart, tiles, N, nb, T, ntp, nitems, ep);
 }
 
 // ------------------------------------------------------------ launchers
@@ -1382,6 +1421,191 @@ static void launch_pair2_energy(DevPimd *a, DevPimd *b, const PairParams &pp, hi
   DevSim *da = a->ds, *db = b->ds;
   const dim3 g((a->N + 255) / 256, a->P);
   hipLaunchKernelGGL(k_pair2_energy<MODE>, g, dim3(256), 0, st, da->X, a->N, da->nb, db->X, b->N, db->nb, pp, part);
+}
+
+// ------------------------------------------------- batched replicas
+//
+// One launch per kernel over every replica that uses it: the Gaussian tile
+// kernels (per mode), the tabulated Ewald real space (per image variant),
+// the tile reduction and the three reciprocal-space kernels.  Each
+// replica's blocks run the single-system body with its own arguments;
+// replicas whose interaction has no batched kernel are launched one by one.
+struct PairRep {
+  const double *X, *M;
+  double *F, *Fpart;
+  const int2 *tiles;
+  int N, nb, T, ntp, P, items;
+  PairParams pp;
+  EwReal ep;
+  double2 *tx, *ty, *tz, *S, *W;
+  const int2 *cols;
+  const double *coef;
+  int ncolp, MZ, ZP;
+};
+
+#define PAIR_REP const PairRep *__restrict__ reps, const int *__restrict__ start, int nrep
+#define PAIR_R                                    \
+  const int r = batch_rep(start, nrep, blockIdx.x); \
+  const PairRep &R = reps[r];                     \
+  const int bx = blockIdx.x - start[r];
+
+template <int MODE>
+__global__ __launch_bounds__(64) void kb_pair_tiles(PAIR_REP) {
+  PAIR_R
+  k_pair_tiles_body<MODE, false, true>(bx, R.X, R.M, R.Fpart, nullptr, R.tiles, R.N, R.nb, R.T, R.ntp, R.pp);
+}
+
+template <bool CIMG>
+__global__ __launch_bounds__(256) void kb_ewald_real(PAIR_REP) {
+  PAIR_R
+  k_ewald_real_body<CIMG>(bx, R.X, R.M, R.Fpart, R.tiles, R.N, R.nb, R.T, R.ntp, R.items, R.ep);
+}
+
+__global__ __launch_bounds__(256) void kb_reduce_tiles(PAIR_REP) {
+  PAIR_R
+  k_reduce_tiles_body(bx, R.Fpart, R.F, 3 * R.nb, R.T, R.pp.P);
+}
+
+__global__ __launch_bounds__(256) void kb_phase(PAIR_REP) {
+  PAIR_R
+  k_phase_body(bx, R.X, R.nb, R.MZ, R.ZP, R.pp.L[0], R.pp.L[1], R.pp.L[2], R.tx, R.ty, R.tz);
+}
+
+__global__ __launch_bounds__(256) void kb_sk_mfma(PAIR_REP) {
+  PAIR_R
+  k_sk_mfma_body(bx, R.tx, R.ty, R.tz, R.cols, R.coef, R.ncolp, R.MZ, R.ZP, R.N, R.nb, R.P, R.S, R.W);
+}
+
+__global__ __launch_bounds__(256) void kb_recip_force_mfma(PAIR_REP) {
+  PAIR_R
+  k_recip_force_mfma_body(bx, R.tx, R.ty, R.tz, R.cols, R.ncolp, R.MZ, R.ZP, R.W, R.M, R.F, R.N, R.nb, R.P,
+                          R.pp.L[0], R.pp.L[1], R.pp.L[2], R.pp.P);
+}
+
+#undef PAIR_R
+#undef PAIR_REP
+
+// Batched kernel groups: G_GAUSS, G_PGAUSS (tiles), G_EW0 / G_EW1 (Ewald real
+// space without / with third image copies), G_RED (tile reduction of all of
+// them, in that order), G_PHASE / G_SK / G_FRC (reciprocal space).
+enum { G_GAUSS, G_PGAUSS, G_EW0, G_EW1, G_RED, G_PHASE, G_SK, G_FRC, G_COUNT };
+
+struct PairBatch {
+  BatchTable g[G_COUNT];
+  std::vector<std::pair<DevPimd *, int>> solo;  // (replica, pair kind) launched one by one
+};
+
+PairBatch *pair_batch_create(const std::vector<BatchItem> &items) {
+  std::vector<PairRep> reps[G_COUNT];
+  std::vector<int> blocks[G_COUNT];
+  PairBatch *b = new PairBatch();
+  auto fail = [&]() -> PairBatch * {
+    pair_batch_free(b);
+    return nullptr;
+  };
+  for (const BatchItem &it : items) {
+    DevPimd *p = it.p;
+    const int kind = it.pair_kind;
+    if (kind < 0) continue;
+    DevSim *d = p->ds;
+    PairRep R{};
+    R.X = d->X;
+    R.M = d->M;
+    R.F = d->F;
+    R.N = p->N;
+    R.nb = d->nb;
+    R.P = p->P;
+    PairParams pp = make_params(p);
+    bool ew = false;
+    if (kind == PK_EWALD && d->uniform_mass) {
+      EwaldPlan pl;
+      pp = ewald_params(p, pl);
+      if (near_ok(pp)) {
+        ew = true;
+        KSpace *K = ensure_kspace(p, pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, d->nb, p->P);
+        if (!K) return fail();
+        R.tx = K->tx;
+        R.ty = K->ty;
+        R.tz = K->tz;
+        R.S = K->S;
+        R.W = K->W;
+        R.cols = K->cols;
+        R.coef = K->coef;
+        R.ncolp = K->ncolp;
+        R.MZ = K->MZ;
+        R.ZP = K->ZP;
+      }
+    }
+    if (!(ew || kind == PK_GAUSS || kind == PK_PGAUSS)) {
+      b->solo.push_back({p, kind});
+      continue;
+    }
+    if (ensure_tiles(p)) return fail();
+    TileBufs &B = pair_bufs(p)->tb;
+    R.Fpart = B.Fpart;
+    R.tiles = B.tiles;
+    R.T = B.T;
+    R.ntp = B.ntp;
+    R.items = p->P * B.ntp;
+    R.pp = pp;
+    for (int k = 0; k < 3; ++k) {
+      R.ep.L[k] = pp.L[k];
+      R.ep.iL[k] = 1.0 / pp.L[k];
+    }
+    R.ep.q = pp.q;
+    R.ep.al8 = kEwTabInvWidth * pp.alpha;
+    R.ep.rc2 = pp.rc2;
+    int grp;
+    if (ew) {
+      const double Lmin = lmin3(pp.L);
+      grp = pp.rc2 >= Lmin * Lmin ? G_EW1 : G_EW0;
+      blocks[grp].push_back((R.items + 3) / 4);
+    } else {
+      grp = kind == PK_GAUSS ? G_GAUSS : G_PGAUSS;
+      blocks[grp].push_back(R.items);
+    }
+    reps[grp].push_back(R);
+    reps[G_RED].push_back(R);
+    blocks[G_RED].push_back((3 * R.nb + 255) / 256);
+    if (ew) {
+      const int nzt = R.ZP / 16;
+      reps[G_PHASE].push_back(R);
+      blocks[G_PHASE].push_back((3 * R.nb + 255) / 256);
+      reps[G_SK].push_back(R);
+      blocks[G_SK].push_back((R.ncolp / 16) * nzt * R.P);
+      reps[G_FRC].push_back(R);
+      blocks[G_FRC].push_back(((R.N + 15) / 16) * R.P);
+    }
+  }
+  for (int k = 0; k < G_COUNT; ++k)
+    if (batch_table_make(b->g[k], reps[k].data(), sizeof(PairRep), blocks[k])) return fail();
+  return b;
+}
+
+void pair_batch_free(PairBatch *b) {
+  if (!b) return;
+  for (auto &t : b->g) batch_table_free(t);
+  delete b;
+}
+
+int pair_batch_forces(PairBatch *b, hipStream_t st) {
+  auto go = [&](int k, void (*kern)(const PairRep *, const int *, int), int threads) {
+    const BatchTable &t = b->g[k];
+    if (t.nrep == 0) return;
+    hipLaunchKernelGGL(kern, dim3(t.total), dim3(threads), 0, st, (const PairRep *)t.reps, t.start, t.nrep);
+  };
+  go(G_GAUSS, kb_pair_tiles<PK_GAUSS>, 64);
+  go(G_PGAUSS, kb_pair_tiles<PK_PGAUSS>, 64);
+  go(G_EW0, kb_ewald_real<false>, 256);
+  go(G_EW1, kb_ewald_real<true>, 256);
+  go(G_RED, kb_reduce_tiles, 256);
+  go(G_PHASE, kb_phase, 256);
+  go(G_SK, kb_sk_mfma, 256);
+  go(G_FRC, kb_recip_force_mfma, 256);
+  XMD_CHECK(hipGetLastError());
+  for (auto &pk : b->solo)
+    if (launch_pair_force(pk.first, pk.second, st)) return -1;
+  return 0;
 }
 
 }  // namespace xmd

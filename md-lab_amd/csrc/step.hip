@@ -8,9 +8,9 @@
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces add into F after both branches are done.
 //
-// Batches of independent replicas (md_pimd_run_batch_x) run the same step
-// per replica on lanes (stream pairs) forked from one origin stream and
-// joined back, all captured into one hipGraph per step.
+// Batches of independent replicas (md_pimd_run_batch_x) run the same stages
+// with one launch per stage over a packed table of all replicas, captured
+// into one hipGraph per batch step.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -53,11 +53,10 @@ static int plan_of(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, StepPlan
 }
 
 // A stream pair for one system's step: s0 carries the step, s1 the
-// recursion branch (fork / join events between them); start / done fork a
-// lane from a batch's origin stream and join it back.
+// recursion branch (fork / join events between them).
 struct Lane {
   hipStream_t s0 = nullptr, s1 = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, start = nullptr, done = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 static Lane main_lane() {
@@ -68,22 +67,6 @@ static Lane main_lane() {
   L.fork = R.fork;
   L.join = R.join;
   return L;
-}
-
-// Lanes of batched runs, created on first use and kept.
-static std::vector<Lane> g_lanes;
-static const int kBatchLanes = 8;
-
-static int ensure_lanes(int n) {
-  while ((int)g_lanes.size() < n) {
-    Lane L;
-    XMD_CHECK(hipStreamCreateWithFlags(&L.s0, hipStreamNonBlocking));
-    XMD_CHECK(hipStreamCreateWithFlags(&L.s1, hipStreamNonBlocking));
-    for (hipEvent_t *e : {&L.fork, &L.join, &L.start, &L.done})
-      XMD_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
-    g_lanes.push_back(L);
-  }
-  return 0;
 }
 
 // ev: optional per-stage events (2 per stage: start, stop), may be null.
@@ -138,18 +121,29 @@ static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R
   return 0;
 }
 
-// One step of every replica: replica i on lane i % nl, the lanes forked from
-// and joined back into `origin`.
-static int enqueue_batch(std::vector<StepPlan> &plans, int nl, hipStream_t origin) {
-  hipEvent_t go = g_lanes[0].start;
-  XMD_CHECK(hipEventRecord(go, origin));
-  for (int k = 0; k < nl; ++k) XMD_CHECK(hipStreamWaitEvent(g_lanes[k].s0, go, 0));
-  for (size_t i = 0; i < plans.size(); ++i)
-    if (enqueue_step(plans[i], nullptr, false, g_lanes[i % nl])) return -1;
-  for (int k = 0; k < nl; ++k) {
-    XMD_CHECK(hipEventRecord(g_lanes[k].done, g_lanes[k].s0));
-    XMD_CHECK(hipStreamWaitEvent(origin, g_lanes[k].done, 0));
+// One step of a whole batch, the single-system step's stage order with one
+// launch per stage over all replicas; the recursions fork onto s1 as in the
+// single-system step.
+struct Batch {
+  BeadBatch *bead = nullptr;
+  ExchBatch *exch = nullptr;
+  PairBatch *pair = nullptr;
+  ~Batch() {
+    bead_batch_free(bead);
+    exch_batch_free(exch);
+    pair_batch_free(pair);
   }
+};
+
+static int enqueue_batch(Batch &B, const Lane &L) {
+  if (bead_batch_nhc(B.bead, 1, L.s0) || exch_batch_enk(B.exch, L.s0)) return -1;
+  XMD_CHECK(hipEventRecord(L.fork, L.s0));
+  XMD_CHECK(hipStreamWaitEvent(L.s1, L.fork, 0));
+  if (exch_batch_recursion(B.exch, L.s1)) return -1;
+  XMD_CHECK(hipEventRecord(L.join, L.s1));
+  if (bead_batch_forces(B.bead, L.s0) || pair_batch_forces(B.pair, L.s0)) return -1;
+  XMD_CHECK(hipStreamWaitEvent(L.s0, L.join, 0));
+  if (exch_batch_force(B.exch, L.s0) || bead_batch_nhc(B.bead, 2, L.s0)) return -1;
   return 0;
 }
 
@@ -268,38 +262,42 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
 int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *cfgs, int n, int nsteps) {
   if (!pimds || !cfgs || n <= 0) return -1;
   if (nsteps <= 0) return 0;
-  std::vector<StepPlan> plans(n);
+  std::vector<BatchItem> items(n);
   for (int i = 0; i < n; ++i) {
     for (int j = 0; j < i; ++j)
       if (pimds[j] == pimds[i] || pimds[j]->sim == pimds[i]->sim) {
         fprintf(stderr, "xmd: a batch lists one system twice\n");
         return -1;
       }
-    if (plan_of(pimds[i], &cfgs[i], plans[i])) return -1;
+    StepPlan sp;
+    if (plan_of(pimds[i], &cfgs[i], sp)) return -1;
+    items[i].p = sp.p;
+    items[i].dt = sp.dt;
+    items[i].pbc = sp.pbc;
+    items[i].image = sp.image;
+    items[i].trap = sp.trap;
+    items[i].pair_kind = sp.pair_kind;
+    for (int d = 0; d < 3; ++d) items[i].L[d] = sp.L[d];
   }
-  // Each replica's recursion is one persistent launch whose workgroups (one
-  // per CU, by their LDS) must all be resident together; lanes run
-  // concurrently, so their number is capped to keep the largest replicas'
-  // recursions within 192 of the device's CUs.
-  int rec_wg = 1;
-  for (int i = 0; i < n; ++i) {
-    const int nblk = (plans[i].p->N + 63) / 64;
-    rec_wg = std::max(rec_wg, 2 * (1 + std::max(0, nblk - 2)));
-  }
-  int nl = std::min(n, kBatchLanes);
-  nl = std::max(1, std::min(nl, 192 / rec_wg));
-  if (ensure_lanes(nl)) return -1;
+  Batch B;
+  B.bead = bead_batch_create(items);
+  B.exch = exch_batch_create(items);
+  B.pair = pair_batch_create(items);
+  if (!B.bead || !B.exch || !B.pair) return -1;
+  const Lane L = main_lane();
   Runtime &R = rt();
-  // The first step runs eagerly (lazily created buffers are allocated there),
-  // the rest replay one captured graph of a whole batch step.
-  if (enqueue_batch(plans, nl, R.s0)) return -1;
+  // The first step runs eagerly; the rest replay one captured graph of a
+  // whole batch step (or run eagerly when the configs ask for no graph).
+  if (enqueue_batch(B, L)) return -1;
   XMD_CHECK(hipStreamSynchronize(R.s0));
-  int rc = 0;
-  if (nsteps > 1) {
+  if (nsteps > 1 && !cfgs[0].use_graph) {
+    for (int s = 1; s < nsteps; ++s)
+      if (enqueue_batch(B, L)) return -1;
+  } else if (nsteps > 1) {
     hipGraph_t graph;
     hipGraphExec_t exec;
     XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
-    rc = enqueue_batch(plans, nl, R.s0);
+    int rc = enqueue_batch(B, L);
     hipError_t ec = hipStreamEndCapture(R.s0, &graph);
     if (rc || ec != hipSuccess) {
       fprintf(stderr, "xmd: batch capture failed\n");
@@ -311,9 +309,8 @@ int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *
     XMD_CHECK(hipGraphExecDestroy(exec));
     XMD_CHECK(hipGraphDestroy(graph));
   }
-  for (int i = 0; i < n; ++i)
-    if (recursion_status(plans[i].p)) rc = -1;
-  return rc;
+  XMD_CHECK(hipStreamSynchronize(R.s0));
+  return exch_batch_status(B.exch);
 }
 
 }  // extern "C"

@@ -20,6 +20,21 @@ __device__ __forceinline__ double dv_min_image(double a, double L) {
   return s - L / 2;
 }
 
+// Batched launches cover many independent systems ("replicas") with one
+// grid: replica r owns the blocks start[r] <= b < start[r+1] (start holds
+// nrep + 1 prefix sums).  Returns r for block b < start[nrep].
+__device__ __forceinline__ int batch_rep(const int *__restrict__ start, int nrep, int b) {
+  int lo = 0, hi = nrep - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // Broadcast lane `lane` (wave-uniform) of a double through two scalar reads.
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);

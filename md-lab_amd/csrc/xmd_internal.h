@@ -159,5 +159,43 @@ extern double g_ewald_split_aL;
 extern DevPimd *g_rec_last;
 int recursion_status(DevPimd *p);
 
+// ------------------------------------------------------ batched replicas
+// One launch per step stage covers every replica of a batch: each stage has
+// a device table of per-replica arguments and a prefix table of per-replica
+// block counts (batch_rep in xmd_device.h maps a block to its replica).
+struct BatchItem {
+  DevPimd *p = nullptr;
+  double dt = 0;
+  int pbc = 0, image = 0, trap = 0, pair_kind = -1;
+  double L[3] = {0, 0, 0};
+};
+
+// Device copy of a host table plus its block prefix (grid = start[nrep]).
+struct BatchTable {
+  void *reps = nullptr;
+  int *start = nullptr;
+  int nrep = 0, total = 0;
+};
+int batch_table_make(BatchTable &t, const void *reps, size_t rep_bytes, const std::vector<int> &blocks);
+void batch_table_free(BatchTable &t);
+
+struct BeadBatch;  // integrator stages
+BeadBatch *bead_batch_create(const std::vector<BatchItem> &items);
+void bead_batch_free(BeadBatch *b);
+int bead_batch_nhc(BeadBatch *b, int half, hipStream_t st);     // NHC half (+ wrap after half 1)
+int bead_batch_forces(BeadBatch *b, hipStream_t st);            // reset, trap, interior springs
+
+struct ExchBatch;  // cycle energies, recursions, exchange forces
+ExchBatch *exch_batch_create(const std::vector<BatchItem> &items);
+void exch_batch_free(ExchBatch *b);
+int exch_batch_enk(ExchBatch *b, hipStream_t st);
+int exch_batch_recursion(ExchBatch *b, hipStream_t st);
+int exch_batch_force(ExchBatch *b, hipStream_t st);
+int exch_batch_status(ExchBatch *b);
+
+struct PairBatch;  // pair forces
+PairBatch *pair_batch_create(const std::vector<BatchItem> &items);
+void pair_batch_free(PairBatch *b);
+int pair_batch_forces(PairBatch *b, hipStream_t st);
 
 }  // namespace xmd
