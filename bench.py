@@ -106,7 +106,7 @@ def replica_table(n=64, seed=2024):
     return table
 
 
-def batch_bench(lib, steps, dt=0.01):
+def batch_bench(lib, steps, dt=0.01, one_by_one=True):
     """pbs/s of the replica table advanced by md_pimd_run_batch_x, and of the
     same replicas advanced one after another (each its own captured graph)."""
     table = replica_table()
@@ -124,7 +124,7 @@ def batch_bench(lib, steps, dt=0.01):
         return out
     pbs = sum(r["N"] * r["P"] for r in table)
     res = {}
-    for mode in ("batched", "one_by_one"):
+    for mode in ("batched", "one_by_one") if one_by_one else ("batched",):
         systems = build()
         cfgs = (pyxmd.StepConfig * len(systems))(*[s.step_config(dt, use_graph=1) for s in systems])
         ptrs = (pyxmd.PIMD_P * len(systems))(*[s.pimd for s in systems])
@@ -143,8 +143,8 @@ def batch_bench(lib, steps, dt=0.01):
                         f"converged Ewald, xi in {{0,.25,.5,.75,1}}), NHC M=4, dt={dt}",
             "particle_beads": pbs, "steps": steps, "unit": "particle-bead-steps/s",
             "value": res["batched"], "ms_per_step": res["batched_ms_per_step"],
-            "one_by_one_value": res["one_by_one"], "one_by_one_ms_per_step": res["one_by_one_ms_per_step"],
-            "speedup": res["batched"] / res["one_by_one"]}
+            "one_by_one_value": res.get("one_by_one"), "one_by_one_ms_per_step": res.get("one_by_one_ms_per_step"),
+            "speedup": res["batched"] / res["one_by_one"] if one_by_one else None}
 
 
 def main():
@@ -158,6 +158,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="steps of the batched-replica measurement (0 skips it)")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batched-replica measurement (for profiling it)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -180,6 +182,9 @@ def main():
     N, P, rs, theta, xi, nsum, dt = args.N, args.P, 2.0, 0.5, 1.0, 8, 0.05
     lib = pyxmd.Lib()
     lib.md_set_nhc_length_x(4)
+    if args.only_batch:
+        print(json.dumps({"batched_replicas": batch_bench(lib, args.batch_steps, one_by_one=False)}))
+        return
     L = pyxmd.ueg_box(N, rs)
     T = pyxmd.ueg_temperature(N, L, theta)
     s = pyxmd.System(lib, "ueg", N, P, T, xi, seed=1, L=L, nsum=nsum)

@@ -324,14 +324,10 @@ struct BeadRep {
   double L[3], w, c[3], om;
 };
 
-// NHC half-kick (+ drift for HALF 1), then the box wrap after HALF 1.
+// NHC half-kick (+ drift for HALF 1), then the box wrap after HALF 1, for
+// degree of freedom e < 3 nb.
 template <int HALF>
-__global__ __launch_bounds__(256) void kb_nhc(const BeadRep *__restrict__ reps, const int *__restrict__ start,
-                                              int nrep) {
-  const int r = batch_rep(start, nrep, blockIdx.x);
-  const BeadRep &R = reps[r];
-  const int e = (blockIdx.x - start[r]) * 256 + threadIdx.x;
-  if (e >= 3 * R.nb) return;
+__device__ __forceinline__ void nhc_wrap_body(const BeadRep &R, int e) {
   nhc_massive_body<HALF>(R.X, R.V, R.F, R.M, R.th, R.vth, R.Q, R.nb, R.Mlen, R.T, R.h, e);
   if (HALF == 1 && R.pbc) {
     const int d = e / R.nb;
@@ -345,14 +341,28 @@ __global__ __launch_bounds__(256) void kb_nhc(const BeadRep *__restrict__ reps, 
   }
 }
 
-// Force reset, trap (md_pimd_x.c:1370-1389) and interior ring springs.
-__global__ __launch_bounds__(256) void kb_bead_forces(const BeadRep *__restrict__ reps, const int *__restrict__ start,
-                                                      int nrep) {
+template <int HALF>
+__global__ __launch_bounds__(256) void kb_nhc(const BeadRep *__restrict__ reps, const int *__restrict__ start,
+                                              int nrep) {
   const int r = batch_rep(start, nrep, blockIdx.x);
   const BeadRep &R = reps[r];
-  const int i = (blockIdx.x - start[r]) * 256 + threadIdx.x;
+  const int e = (blockIdx.x - start[r]) * 256 + threadIdx.x;
+  if (e >= 3 * R.nb) return;
+  nhc_wrap_body<HALF>(R, e);
+}
+
+template <int HALF>
+__global__ __launch_bounds__(256) void k_nhc_wrap(BeadRep R) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= 3 * R.nb) return;
+  nhc_wrap_body<HALF>(R, e);
+}
+
+// Force reset, trap (md_pimd_x.c:1370-1389) and interior ring springs of
+// degree of freedom i < 3 nb: the arithmetic of k_fill, k_trap and
+// k_ring_springs in one pass.
+__device__ __forceinline__ void bead_forces_body(const BeadRep &R, int i) {
   const int nb = R.nb;
-  if (i >= 3 * nb) return;
   const int d = i / nb, s = i - d * nb;
   double Fv = 0.0;
   if (R.trap) {
@@ -377,6 +387,70 @@ __global__ __launch_bounds__(256) void kb_bead_forces(const BeadRep *__restrict_
   R.F[i] = Fv;
 }
 
+__global__ __launch_bounds__(256) void kb_bead_forces(const BeadRep *__restrict__ reps, const int *__restrict__ start,
+                                                      int nrep) {
+  const int r = batch_rep(start, nrep, blockIdx.x);
+  const BeadRep &R = reps[r];
+  const int i = (blockIdx.x - start[r]) * 256 + threadIdx.x;
+  if (i >= 3 * R.nb) return;
+  bead_forces_body(R, i);
+}
+
+__global__ __launch_bounds__(256) void k_bead_forces(BeadRep R) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= 3 * R.nb) return;
+  bead_forces_body(R, i);
+}
+
+static BeadRep bead_rep(const BatchItem &it) {
+  DevPimd *p = it.p;
+  DevSim *d = p->ds;
+  BeadRep R;
+  R.X = d->X;
+  R.V = d->V;
+  R.F = d->F;
+  R.th = d->th;
+  R.vth = d->vth;
+  R.M = d->M;
+  R.Q = d->Q;
+  R.nb = d->nb;
+  R.N = p->N;
+  R.P = p->P;
+  R.Mlen = g_nhc_len;
+  R.T = d->host->T;
+  R.h = it.dt;
+  R.pbc = it.pbc;
+  R.image = it.image;
+  R.trap = it.trap;
+  for (int k = 0; k < 3; ++k) {
+    R.L[k] = it.L[k];
+    R.c[k] = md_trap_center_x[k];
+  }
+  R.w = md_trap_frequency_x;
+  R.om = p->host->omegaP;
+  return R;
+}
+
+// Fused bead-local stages of one system's step (massive thermostats):
+// NHC half (+ wrap after half 1), or reset + trap + interior springs.
+int launch_step_nhc(const BatchItem &it, int half, hipStream_t st) {
+  const BeadRep R = bead_rep(it);
+  const dim3 g((3 * R.nb + 255) / 256);
+  if (half == 1)
+    hipLaunchKernelGGL(k_nhc_wrap<1>, g, dim3(256), 0, st, R);
+  else
+    hipLaunchKernelGGL(k_nhc_wrap<2>, g, dim3(256), 0, st, R);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_step_bead_forces(const BatchItem &it, hipStream_t st) {
+  const BeadRep R = bead_rep(it);
+  hipLaunchKernelGGL(k_bead_forces, dim3((3 * R.nb + 255) / 256), dim3(256), 0, st, R);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
 struct BeadBatch {
   BatchTable t;
 };
@@ -385,37 +459,12 @@ BeadBatch *bead_batch_create(const std::vector<BatchItem> &items) {
   std::vector<BeadRep> reps;
   std::vector<int> blocks;
   for (const BatchItem &it : items) {
-    DevPimd *p = it.p;
-    DevSim *d = p->ds;
-    if (!d->massive) {
+    if (!it.p->ds->massive) {
       fprintf(stderr, "xmd: batched steps need massive thermostats (one chain per degree of freedom)\n");
       return nullptr;
     }
-    BeadRep R;
-    R.X = d->X;
-    R.V = d->V;
-    R.F = d->F;
-    R.th = d->th;
-    R.vth = d->vth;
-    R.M = d->M;
-    R.Q = d->Q;
-    R.nb = d->nb;
-    R.N = p->N;
-    R.P = p->P;
-    R.Mlen = g_nhc_len;
-    R.T = d->host->T;
-    R.h = it.dt;
-    R.pbc = it.pbc;
-    R.image = it.image;
-    R.trap = it.trap;
-    for (int k = 0; k < 3; ++k) {
-      R.L[k] = it.L[k];
-      R.c[k] = md_trap_center_x[k];
-    }
-    R.w = md_trap_frequency_x;
-    R.om = p->host->omegaP;
-    reps.push_back(R);
-    blocks.push_back((3 * d->nb + 255) / 256);
+    reps.push_back(bead_rep(it));
+    blocks.push_back((3 * it.p->ds->nb + 255) / 256);
   }
   BeadBatch *b = new BeadBatch();
   if (batch_table_make(b->t, reps.data(), sizeof(BeadRep), blocks)) {

@@ -4,6 +4,8 @@
 //   s0: NHC half 1 (+ wrap), cycle energies --fork--> reset F, trap, ring
 //       springs, pair real, pair reciprocal --wait join--> exchange end-bead
 //       forces, NHC half 2
+// With massive thermostats NHC half 1 and the wrap are one kernel, and so are
+// the force reset, trap and interior springs.
 //   s1: (after fork) forward + backward recursions --join-->
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces add into F after both branches are done.
@@ -52,6 +54,18 @@ static int plan_of(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, StepPlan
   return 0;
 }
 
+static BatchItem item_of(const StepPlan &sp) {
+  BatchItem it;
+  it.p = sp.p;
+  it.dt = sp.dt;
+  it.pbc = sp.pbc;
+  it.image = sp.image;
+  it.trap = sp.trap;
+  it.pair_kind = sp.pair_kind;
+  for (int d = 0; d < 3; ++d) it.L[d] = sp.L[d];
+  return it;
+}
+
 // A stream pair for one system's step: s0 carries the step, s1 the
 // recursion branch (fork / join events between them).
 struct Lane {
@@ -90,9 +104,15 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   if (rec(4, sx) || launch_recursion(p, true, true, sx) || rec(5, sx)) return -1;
   if (!serial) XMD_CHECK(hipEventRecord(R.join, R.s1));
   // bead-local and pair forces
-  if (rec(6, R.s0) || launch_reset_force(d, R.s0)) return -1;
-  if (sp.trap && launch_trap_force(d, md_trap_frequency_x, md_trap_center_x, (double)p->P, R.s0)) return -1;
-  if (launch_ring_springs(p, sp.image, sp.L, R.s0) || rec(7, R.s0)) return -1;
+  if (rec(6, R.s0)) return -1;
+  if (d->massive) {
+    if (launch_step_bead_forces(item_of(sp), R.s0)) return -1;
+  } else {
+    if (launch_reset_force(d, R.s0)) return -1;
+    if (sp.trap && launch_trap_force(d, md_trap_frequency_x, md_trap_center_x, (double)p->P, R.s0)) return -1;
+    if (launch_ring_springs(p, sp.image, sp.L, R.s0)) return -1;
+  }
+  if (rec(7, R.s0)) return -1;
   if (sp.pair_kind >= 0) {
     if (rec(8, R.s0) || launch_pair_real(p, sp.pair_kind, R.s0) || rec(9, R.s0)) return -1;
     if (pair_has_recip(sp.pair_kind)) {
@@ -111,12 +131,16 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
 static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R) {
   DevSim *d = sp.p->ds;
   if (ev) XMD_CHECK(hipEventRecord(ev[0], R.s0));
-  if (launch_nhc(d, 1, sp.dt, R.s0)) return -1;
-  if (sp.pbc && launch_pbc(d, sp.L, R.s0)) return -1;
+  if (d->massive) {
+    if (launch_step_nhc(item_of(sp), 1, R.s0)) return -1;
+  } else {
+    if (launch_nhc(d, 1, sp.dt, R.s0)) return -1;
+    if (sp.pbc && launch_pbc(d, sp.L, R.s0)) return -1;
+  }
   if (ev) XMD_CHECK(hipEventRecord(ev[1], R.s0));
   if (enqueue_forces(sp, ev, serial, R)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[14], R.s0));
-  if (launch_nhc(d, 2, sp.dt, R.s0)) return -1;
+  if (d->massive ? launch_step_nhc(item_of(sp), 2, R.s0) : launch_nhc(d, 2, sp.dt, R.s0)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[15], R.s0));
   return 0;
 }
@@ -271,13 +295,7 @@ int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *
       }
     StepPlan sp;
     if (plan_of(pimds[i], &cfgs[i], sp)) return -1;
-    items[i].p = sp.p;
-    items[i].dt = sp.dt;
-    items[i].pbc = sp.pbc;
-    items[i].image = sp.image;
-    items[i].trap = sp.trap;
-    items[i].pair_kind = sp.pair_kind;
-    for (int d = 0; d < 3; ++d) items[i].L[d] = sp.L[d];
+    items[i] = item_of(sp);
   }
   Batch B;
   B.bead = bead_batch_create(items);

@@ -179,6 +179,10 @@ struct BatchTable {
 int batch_table_make(BatchTable &t, const void *reps, size_t rep_bytes, const std::vector<int> &blocks);
 void batch_table_free(BatchTable &t);
 
+// Fused bead-local stages of one system's step (massive thermostats only).
+int launch_step_nhc(const BatchItem &it, int half, hipStream_t st);
+int launch_step_bead_forces(const BatchItem &it, hipStream_t st);
+
 struct BeadBatch;  // integrator stages
 BeadBatch *bead_batch_create(const std::vector<BatchItem> &items);
 void bead_batch_free(BeadBatch *b);

@@ -71,3 +71,31 @@ def test_batch_matches_individual_runs():
     cf2 = (pyxmd.StepConfig * 2)(cfgs[0], cfgs[0])
     assert lib.md_pimd_run_batch_x(dup, cf2, 2, 1) == -1
     lib.md_set_nhc_length_x(8)
+
+
+def test_batch_with_unbatched_interaction():
+    """A replica whose interaction has no batched kernel (Lennard-Jones) is
+    launched on its own inside the batch step; its trajectory and the others'
+    still match their solo runs bit for bit."""
+    from orc import read_golden
+    lib = pyxmd.Lib()
+    lib.md_set_nhc_length_x(4)
+    g = read_golden()
+    x_lj, L_lj = g["lj/x0"], float(g["lj/L"][0])
+
+    def build():
+        lj = pyxmd.System(lib, "trap", 27, 4, 1.0, 1.0, seed=3, L=L_lj, positions=x_lj)
+        ga = pyxmd.System(lib, "trap", 16, 8, 1.0, 0.5, seed=4)
+        cl = pyxmd.StepConfig(None, lib.cb["md_LJ_periodic_force_x"], 1e-4, 1, 1, 1)
+        return [lj, ga], [cl, ga.step_config(0.01, use_graph=1)]
+
+    alone, cfg_a = build()
+    batch, cfg_b = build()
+    for s, c in zip(alone, cfg_a):
+        assert lib.md_pimd_run_steps_x(s.pimd, C.byref(c), 4) == 0
+    cfgs = (pyxmd.StepConfig * 2)(*cfg_b)
+    ptrs = (pyxmd.PIMD_P * 2)(*[s.pimd for s in batch])
+    assert lib.md_pimd_run_batch_x(ptrs, cfgs, 2, 4) == 0
+    for a, b in zip(alone, batch):
+        assert all(np.array_equal(u, w) for u, w in zip(state(a), state(b)))
+    lib.md_set_nhc_length_x(8)
