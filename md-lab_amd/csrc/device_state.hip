@@ -287,7 +287,8 @@ static int alloc_pimd(DevPimd *p) {
   XMD_CHECK(hipMalloc(&p->rec, (6 * (size_t)(N + 1) + 2 * nblk * (64 * 63 / 2 + 64 * 64)) * sizeof(double)));
   XMD_CHECK(hipMalloc(&p->recsync, (4 * nblk + 1) * sizeof(int)));
   XMD_CHECK(hipMemset(p->recsync, 0, (4 * nblk + 1) * sizeof(int)));
-  return 0;
+  // ln n, n = 0..N, in the first N+1 doubles of the scratch (constant)
+  return init_lnn(p);
 }
 
 int ensure_pimd_device(md_pimd_t_x *pimd) {

@@ -1036,6 +1036,12 @@ static RecArgs rec_args(DevPimd *p) {
   return a;
 }
 
+int init_lnn(DevPimd *p) {
+  hipLaunchKernelGGL(k_lnn, dim3((p->N + 256) / 256), dim3(256), 0, rt().s0, p->work, p->N);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
 int launch_enk(DevPimd *p, int image, hipStream_t st) {
   DevSim *d = p->ds;
   const int N = p->N, P = p->P;
@@ -1048,8 +1054,6 @@ int launch_enk(DevPimd *p, int image, hipStream_t st) {
   hipLaunchKernelGGL(k_scan_dd, dim3(1), dim3(1024), 0, st, clink, N, p->Cpre);
   XMD_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_enk_fill, dim3(N), dim3(256), 0, st, d->X, d->M, N, P, om, g, p->Eint, p->Cpre, p->ENk, p->ET);
-  XMD_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_lnn, dim3((N + 256) / 256), dim3(256), 0, st, p->work, N);
   XMD_CHECK(hipGetLastError());
   p->enk_dev = true;
   p->last_image = image;
@@ -1142,8 +1146,6 @@ ExchBatch *exch_batch_create(const std::vector<BatchItem> &items) {
     bf.push_back(p->N);
     b->ps.push_back(p);
     b->image.push_back(it.image);
-    // ln n is constant: written once here
-    hipLaunchKernelGGL(k_lnn, dim3((p->N + 256) / 256), dim3(256), 0, rt().s0, p->work, p->N);
   }
   b->nrep = (int)reps.size();
   const size_t sz = sizeof(ExRep);
@@ -1218,8 +1220,6 @@ static int ensure_enk(DevPimd *p) {
   hipStream_t st = rt().s0;
   XMD_CHECK(hipMemcpyAsync(p->ENk, p->host->ENk[0], tri * sizeof(double), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_enk_transpose, dim3(p->N), dim3(256), 0, st, p->ENk, p->ET, p->N);
-  XMD_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_lnn, dim3((p->N + 256) / 256), dim3(256), 0, st, p->work, p->N);
   XMD_CHECK(hipGetLastError());
   XMD_CHECK(hipStreamSynchronize(st));
   p->enk_dev = true;

@@ -893,17 +893,18 @@ __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, 
     const double t8 = (R2 * ir) * ep.al8;
     const int iv = min((int)t8, kEwTabIntervals - 1);
     const double y = fma(2.0, t8, -(double)(2 * iv + 1));
-    const double2 *c = tab + iv * (kEwTabStride / 2);
-    const double2 c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], c4 = c[4];
-    double h = fma(c4.y, y, c4.x);
-    h = fma(h, y, c3.y);
-    h = fma(h, y, c3.x);
-    h = fma(h, y, c2.y);
-    h = fma(h, y, c2.x);
-    h = fma(h, y, c1.y);
-    h = fma(h, y, c1.x);
-    h = fma(h, y, c0.y);
-    h = fma(h, y, c0.x);
+    // coefficients c_0 .. c_DEG of the interval as (c_2k, c_2k+1) pairs; Horner in y
+    constexpr int NC2 = kEwTabStride / 2;
+    const double2 *c = tab + iv * NC2;
+    double2 cc[NC2];
+#pragma unroll
+    for (int k = 0; k < NC2; ++k) cc[k] = c[k];
+    double h = fma(cc[NC2 - 1].y, y, cc[NC2 - 1].x);
+#pragma unroll
+    for (int k = NC2 - 2; k >= 0; --k) {
+      h = fma(h, y, cc[k].y);
+      h = fma(h, y, cc[k].x);
+    }
     const double w = ep.q * h * (ir * ir * ir);
     gx = fma(w, rx, gx);
     gy = fma(w, ry, gy);

@@ -149,6 +149,7 @@ bool pair_has_recip(int kind);
 int pair_energy(DevPimd *p, int kind, double *out);
 
 int launch_enk(DevPimd *p, int image, hipStream_t st);
+int init_lnn(DevPimd *p);  // ln n table at p->work[0..N], once per object
 int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st);
 int launch_exchange_force(DevPimd *p, int image, hipStream_t st);
 int connection_matrix(DevPimd *p, double *host_G);
