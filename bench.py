@@ -60,6 +60,25 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def counter_flops(kernel, workload):
+    """FP64 FLOPs per launch of `kernel` counted by the hardware
+    (SQ_INSTS_VALU_FLOPS_FP64[_TRANS] x 64 + SQ_INSTS_VALU_MFMA_MOPS_F64 x 512)
+    from the committed rocprofv3 --pmc summary (profiles/fp64_counters.json),
+    or None when it holds no entry for this workload."""
+    path = os.path.join(ROOT, "profiles", "fp64_counters.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if table.get("workload") != workload:
+        return None
+    for name, e in table.get("kernels", {}).items():
+        if kernel in name:
+            return e["valu_fp64_flop"] + e["mfma_fp64_flop"]
+    return None
+
+
 def cpu_baseline(N, P, rs, theta, xi, nsum, pairs):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
     if not os.path.exists(exe):
@@ -224,6 +243,7 @@ def main():
 
     traffic_key = f"UEG N={N} P={P} alpha'L={alpha * L:.2f}"
     traffic = measured_traffic("k_ewald_real", traffic_key)
+    hw_flop = counter_flops("k_ewald_real", traffic_key)
     line = {
         "metric": METRIC, "value": value, "unit": "particle-bead-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -236,7 +256,9 @@ def main():
         "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "k_ewald_real (Ewald real space)", "avg_ms": real_ms,
-                     "work_flop_eq": work},
+                     "work_flop_eq": work,
+                     "counted_fp64_flop": hw_flop,
+                     "counted_tflops": hw_flop / (real_ms * 1e-3) / 1e12 if hw_flop else None},
         "stages_ms": {k: round(float(v), 5) for k, v in zip(names, stages)},
         "cpu_baseline": cpu,
         "batched_replicas": batched,
