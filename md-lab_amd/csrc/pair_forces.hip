@@ -422,10 +422,12 @@ __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restr
 //                      (M = beads, N = columns, K = m_z), W = coef conj(S),
 // then f_s = sum_c (mx, my) Im(A U) / L + (0, 0, Im(A V)).
 //
-// Phase tables (k_phase), table half-width MZ:
-//   tx[m * nb + s]        e^{i 2 pi m x / L_x},  m = 0..MZ    (row-major)
-//   ty[(m + MZ) * nb + s] e^{i 2 pi m y / L_y},  m = -MZ..MZ  (row-major)
-//   tz[s * ZP + m + MZ]   e^{i 2 pi m z / L_z},  m = -MZ..MZ  (bead-major, slot ZP-1 zero)
+// Phase tables (k_phase), table half-width MZ, all bead-major so that the 16
+// columns of a tile (consecutive m_y, mostly one m_x) read one bead's
+// contiguous run:
+//   tx[s * (MZ + 1) + m]        e^{i 2 pi m x / L_x},  m = 0..MZ
+//   ty[s * (2 MZ + 1) + m + MZ] e^{i 2 pi m y / L_y},  m = -MZ..MZ
+//   tz[s * ZP + m + MZ]         e^{i 2 pi m z / L_z},  m = -MZ..MZ  (slot ZP-1 zero)
 constexpr int MAX_KM = 31;     // largest table half-width of the k kernels
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -572,20 +574,21 @@ __device__ __forceinline__ void k_phase_body(int bx, const double *__restrict__ 
   const int d = t / nb, s = t - d * nb;
   const double u = 2.0 * X[t] / (d == 0 ? L0 : (d == 1 ? L1 : L2));
   if (d == 0) {
-    tx[s] = make_double2(1.0, 0.0);
+    double2 *r = tx + (size_t)s * (MZ + 1);
+    r[0] = make_double2(1.0, 0.0);
     for (int m = 1; m <= MZ; ++m) {
       double sn, cs;
       sincospi(m * u, &sn, &cs);
-      tx[(size_t)m * nb + s] = make_double2(cs, sn);
+      r[m] = make_double2(cs, sn);
     }
   } else if (d == 1) {
-    double2 *r0 = ty + (size_t)MZ * nb + s;
+    double2 *r0 = ty + (size_t)s * (2 * MZ + 1) + MZ;
     r0[0] = make_double2(1.0, 0.0);
     for (int m = 1; m <= MZ; ++m) {
       double sn, cs;
       sincospi(m * u, &sn, &cs);
-      r0[(ptrdiff_t)m * nb] = make_double2(cs, sn);
-      r0[-(ptrdiff_t)m * nb] = make_double2(cs, -sn);
+      r0[m] = make_double2(cs, sn);
+      r0[-m] = make_double2(cs, -sn);
     }
   } else {
     double2 *r = tz + (size_t)s * ZP + MZ;
@@ -634,8 +637,6 @@ __device__ __forceinline__ void k_sk_mfma_body(int bx, const double2 *__restrict
   xcd_tile(bx, (ncolp / 16) * nzt, P, slice, tile);
   const int c0 = (tile / nzt) * 16, z0 = (tile % nzt) * 16;
   const int2 cl = cols[c0 + (lane & 15)];
-  const double2 *px = tx + (size_t)cl.x * nb;
-  const double2 *py = ty + (size_t)(MZ + cl.y) * nb;
   const int kb = lane >> 4;  // bead of this lane within a K step
   v4d sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
   const int q0 = (N * wv) / 4, q1 = (N * (wv + 1)) / 4;
@@ -645,8 +646,8 @@ __device__ __forceinline__ void k_sk_mfma_body(int bx, const double2 *__restrict
     const int bead = b + kb;
     ok = bead < q1;
     const int s = slice * N + (ok ? bead : q0);
-    ex = px[s];
-    ey = py[s];
+    ex = tx[(size_t)s * (MZ + 1) + cl.x];
+    ey = ty[(size_t)s * (2 * MZ + 1) + MZ + cl.y];
     z = tz[(size_t)s * ZP + z0 + (lane & 15)];
   };
   double2 ex, ey, z;
@@ -735,13 +736,11 @@ __device__ __forceinline__ void k_recip_force_mfma_body(int bx,
       wb[ks] = Ws[(size_t)c * ZP + z0 + 4 * ks + (lane >> 4)];
     }
     const int2 cl = cols[c];
-    const double2 *px = tx + (size_t)cl.x * nb;
-    const double2 *py = ty + (size_t)(MZ + cl.y) * nb;
     double2 exs[4], eys[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      exs[r] = px[sb[r]];
-      eys[r] = py[sb[r]];
+      exs[r] = tx[(size_t)sb[r] * (MZ + 1) + cl.x];
+      eys[r] = ty[(size_t)sb[r] * (2 * MZ + 1) + MZ + cl.y];
     }
     v4d ur = {0, 0, 0, 0}, ui = {0, 0, 0, 0}, vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0};
 #pragma unroll

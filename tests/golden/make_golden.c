@@ -544,6 +544,108 @@ static void observables2_case(void) {
   dumpf("obs2", "sphere_sk3", h->fxs[0], sk->N - 2);
 }
 
+/* Full-size parity cases at the benchmark configurations: trapped bosons
+ * N=256, P=64, beta=2 (g=3, s=0.5) at xi in {0, 0.25, 0.5, 0.75, 1}, and the
+ * spin-polarised UEG N=128, P=64, r_s=2, theta=0.5, alpha=5/L, n=8 at xi in
+ * {0, 0.5, 1}.  One seeded initial state per system (x, v stored whole); per xi the
+ * total force at that state (every `stride`-th bead), V_B^(N) and the
+ * thermodynamic and virial estimators; for the last xi `steps` MD steps
+ * (dt, NHC M=8) and the final positions and velocities (every stride-th bead). */
+static void dump_sub(const char *prefix, const char *what, md_simulation_t_x *sim, int field, int stride) {
+  int nb = sim->N, n = 0;
+  double *a = malloc(sizeof(double) * 3 * nb);
+  for (int i = 0; i < nb; i += stride)
+    for (int d = 0; d < 3; ++d) {
+      const md_particle_t_x *q = &sim->particles[i];
+      a[n++] = field == 0 ? q->x[d] : (field == 1 ? q->v[d] : q->f[d]);
+    }
+  dumpf(prefix, what, a, n);
+  free(a);
+}
+
+static void config_case(const gcase *c, const double *xis, int nxi, int stride) {
+  const char *pf = c->name;
+  int N = c->N, P = c->P, nb = N * P;
+  set_params(c);
+  md_set_seed_x(c->seed);
+  md_simulation_t_x *sim = md_alloc_simulation_x(nb, 1, 0, MD_ND_RECT_BOX_X);
+  md_num_t_x L3[3] = {c->L, c->L, c->L};
+  md_init_nd_rect_box_x(sim->box, L3);
+  md_pimd_t_x *pimd = md_alloc_pimd_x(sim, N, P, xis[0], c->T);
+  md_init_particle_mass_x(sim, 1.0, 0, nb);
+  if (c->kind == 0) {
+    for (int l = 1; l <= N; ++l)
+      for (int j = 1; j <= P; ++j)
+        for (int d = 0; d < 3; ++d) sim->particles[MD_PIMD_INDEX_X(l, j, P)].x[d] = md_random_gaussian_x();
+  } else {
+    md_num_t_x ctr[3] = {c->L / 2, c->L / 2, c->L / 2};
+    md_pimd_init_particle_uniform_pos_x(pimd, ctr, L3, 0.4);
+    md_periodic_boundary_x(sim);
+  }
+  md_init_maxwell_vel_x(sim, c->T);
+  md_init_nhc_x(sim, md_get_quick_Q_x(c->T, pimd->omegaP), 0, sim->Nf, 0);
+  double meta[10] = {N, P, c->T, 0, c->kind, c->image, c->L, c->nsum, stride, c->dt};
+  dumpf(pf, "meta", meta, 10);
+  dumpf(pf, "xis", xis, nxi);
+  /* positions and velocities of every bead; the chains start from
+   * md_init_nhc_x as above (deterministic) */
+  dump_sub(pf, "x0", sim, 0, 1);
+  dump_sub(pf, "v0", sim, 1, 1);
+  /* the pair (and trap) force is xi-independent: once */
+  md_reset_force_x(sim);
+  if (c->kind == 0) md_pimd_calc_trap_force_x(pimd, md_harmonic_trap_force_x);
+  md_pimd_calc_pair_force_x(pimd, pair_force_of(c));
+  double *f0 = malloc(sizeof(double) * 3 * nb);
+  for (int i = 0; i < nb; ++i)
+    for (int d = 0; d < 3; ++d) f0[3 * i + d] = sim->particles[i].f[d];
+  dump_sub(pf, "f_ext", sim, 2, stride);
+  md_stats_t_x *st = md_alloc_stats_x(1);
+  for (int q = 0; q < nxi; ++q) {
+    char w[32];
+    pimd->vi = xis[q];
+    for (int i = 0; i < nb; ++i)
+      for (int d = 0; d < 3; ++d) sim->particles[i].f[d] = f0[3 * i + d];
+    md_pimd_fast_fill_ENk_x(pimd, c->image);
+    md_pimd_fill_VB_x(pimd);
+    md_pimd_fast_fill_force_VB_x(pimd, c->image);
+    snprintf(w, sizeof w, "f_total_%d", q);
+    dump_sub(pf, w, sim, 2, stride);
+    double e[4];
+    e[0] = pimd->VBN[N];
+    st->es[0] = 0;
+    md_pimd_calc_VBN_energy_x(pimd, st);
+    e[1] = stat_of(st);
+    md_pimd_fast_fill_ENk2_x(pimd, c->image);
+    st->es[0] = 0;
+    md_pimd_calc_vi_virial_energy_x(pimd, st, c->image);
+    e[2] = stat_of(st);
+    st->es[0] = 0;
+    md_pimd_calc_pair_energy_x(pimd, pair_energy_of(c), st);
+    e[3] = stat_of(st);
+    snprintf(w, sizeof w, "e_%d", q);
+    dumpf(pf, w, e, 4);
+  }
+  /* trajectory at the last xi, starting from the total force above */
+  double *ev = malloc(sizeof(double) * (c->steps > 0 ? c->steps : 1));
+  for (int s = 0; s < c->steps; ++s) {
+    md_update_nhc_VV3_1_x(sim, c->dt);
+    if (c->kind == 1) md_periodic_boundary_x(sim);
+    full_force(c, pimd);
+    md_update_nhc_VV3_2_x(sim, c->dt);
+    md_pimd_fast_fill_ENk2_x(pimd, c->image);
+    st->es[0] = 0;
+    md_pimd_calc_vi_virial_energy_x(pimd, st, c->image);
+    ev[s] = stat_of(st);
+  }
+  if (c->steps > 0) {
+    dump_sub(pf, "x_end", sim, 0, stride);
+    dump_sub(pf, "v_end", sim, 1, stride);
+    dumpf(pf, "e_vir_series", ev, c->steps);
+  }
+  free(ev);
+  free(f0);
+}
+
 int main(int argc, char **argv) {
   const char *path = argc > 1 ? argv[1] : "golden.bin";
   out = fopen(path, "wb");
@@ -551,6 +653,18 @@ int main(int argc, char **argv) {
   md_init_config_x cfg = {1, 0, 0, 1, ""};
   md_init_setup_x(NULL, &cfg);
   md_set_pimd_mode_x(1);
+  if (argc > 2 && strcmp(argv[2], "configs") == 0) {
+    double rs = 2.0, L128 = pow(4 * M_PI * 128 / 3.0, 1.0 / 3.0) * rs;
+    double kF = pow(6 * M_PI * M_PI * 128 / (L128 * L128 * L128), 1.0 / 3.0);
+    /*            name       N    P   T                  vi kind img L     nsum fgrp steps dt    seed */
+    gcase trap = {"cfg_trap256", 256, 64, 0.5, 0, 0, 0, 10.0, 2, 1, 3, 0.01, 12};
+    gcase ueg = {"cfg_ueg128", 128, 64, 0.25 * kF * kF, 0, 1, 1, L128, 8, 1, 2, 0.05, 13};
+    const double xt[5] = {0.0, 0.25, 0.5, 0.75, 1.0}, xu[3] = {0.0, 0.5, 1.0};
+    config_case(&trap, xt, 5, 13);
+    config_case(&ueg, xu, 3, 7);
+    fclose(out);
+    return 0;
+  }
   rng_case();
   double rs = 2.0;
   double L8 = pow(4 * M_PI * 8 / 3.0, 1.0 / 3.0) * rs;

@@ -1,0 +1,73 @@
+"""Full-size parity at the benchmark configurations against the compiled
+reference (tests/golden/configs.bin, written by tests/golden/make_golden.c in
+its `configs` mode from the reference's own nocl build):
+
+* trapped bosons N=256, P=64, beta=2, Gaussian g=3 s=0.5, at the five xi of
+  the fermion-extrapolation grid {0, 0.25, 0.5, 0.75, 1};
+* spin-polarised UEG N=128, P=64, r_s=2, theta=0.5, Ewald alpha=5/L, |m|<=8
+  (converged: evaluated with the engine's own split), xi in {0, 0.5, 1}.
+
+Same initial state on both sides (positions and velocities stored whole,
+chains from md_init_nhc_x).  Per xi: the total
+force (trap + pair + exchange; every stride-th bead), V_B^(N) and the
+thermodynamic, virial and pair estimators; at the last xi a short fused-step
+trajectory.  Bars: forces max|dF|/RMS <= 1e-11, energies 1e-12 relative,
+trajectories 1e-10 (x) / 1e-9 (v)."""
+import os
+
+import numpy as np
+import pytest
+
+import pyxmd
+from orc import read_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "configs.bin")
+
+
+def stat(lib, fn, *args, pos=1):
+    st = lib.md_alloc_stats_x(1)
+    a = list(args)
+    a.insert(pos, st)
+    assert fn(*a) == 0
+    return st.contents.es[0]
+
+
+@pytest.mark.parametrize("name", ["cfg_trap256", "cfg_ueg128"])
+def test_config_parity(name):
+    gold = read_golden(CONFIGS)
+    g = {k.split("/", 1)[1]: v for k, v in gold.items() if k.startswith(name + "/")}
+    N, P, T, _, kind, image, L, nsum, stride, dt = g["meta"]
+    N, P, image, nsum, stride = int(N), int(P), int(image), int(nsum), int(stride)
+    kind = "trap" if kind == 0 else "ueg"
+    xis = g["xis"]
+    lib = pyxmd.Lib()
+    lib.md_set_nhc_length_x(8)
+    s = pyxmd.System(lib, kind, N, P, T, float(xis[0]), L=L, nsum=nsum, positions=g["x0"])
+    pyxmd.load_state(s, g["x0"], g["v0"])  # chains: md_init_nhc_x, as on the reference side
+    sub = slice(0, None, stride)
+    for q, xi in enumerate(xis):
+        s.pimd.contents.vi = float(xi)
+        s.full_force()
+        f = s.host_beads()["f"][sub].reshape(-1)
+        err = rel_err(f, g[f"f_total_{q}"])
+        s.host_pimd()
+        e = g[f"e_{q}"]
+        vbn = s.pimd.contents.VBN[N]
+        e_vbn = stat(lib, lib.md_pimd_calc_VBN_energy_x, s.pimd)
+        assert lib.md_pimd_fast_fill_ENk2_x(s.pimd, image) == 0
+        e_vir = stat(lib, lib.md_pimd_calc_vi_virial_energy_x, s.pimd, image)
+        e_pair = stat(lib, lib.md_pimd_calc_pair_energy_x, s.pimd, s.pair_energy_cb(), pos=2)
+        print(f"{name} xi={xi}: force rel err {err:.2e}; V_B {vbn:.12g} vs {e[0]:.12g}")
+        assert err < 1e-11
+        for got, ref in zip((vbn, e_vbn, e_vir, e_pair), e):
+            assert got == pytest.approx(ref, rel=1e-12, abs=1e-12)
+    # trajectory at the last xi, fused and graph-replayed, from the state above
+    cfg = s.step_config(dt, use_graph=1)
+    steps = len(g["e_vir_series"])
+    assert lib.md_pimd_run_steps_x(s.pimd, pyxmd.C.byref(cfg), steps) == 0
+    b = s.host_beads()
+    ex, ev = rel_err(b["x"][sub].reshape(-1), g["x_end"]), rel_err(b["v"][sub].reshape(-1), g["v_end"])
+    print(f"{name}: {steps}-step trajectory rel err x {ex:.2e}, v {ev:.2e}")
+    assert ex < 1e-10 and ev < 1e-9
