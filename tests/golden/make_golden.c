@@ -548,7 +548,8 @@ static void observables2_case(void) {
  * N=256, P=64, beta=2 (g=3, s=0.5) at xi in {0, 0.25, 0.5, 0.75, 1}, and the
  * spin-polarised UEG N=128, P=64, r_s=2, theta=0.5, alpha=5/L, n=8 at xi in
  * {0, 0.5, 1}.  One seeded initial state per system (x, v stored whole); per xi the
- * total force at that state (every `stride`-th bead), V_B^(N) and the
+ * total force at that state (every `stride`-th bead) with the fast and with
+ * the cycle-walk exchange algorithms, V_B^(N) and the
  * thermodynamic and virial estimators; for the last xi `steps` MD steps
  * (dt, NHC M=8) and the final positions and velocities (every stride-th bead). */
 static void dump_sub(const char *prefix, const char *what, md_simulation_t_x *sim, int field, int stride) {
@@ -600,9 +601,21 @@ static void config_case(const gcase *c, const double *xis, int nxi, int stride) 
     for (int d = 0; d < 3; ++d) f0[3 * i + d] = sim->particles[i].f[d];
   dump_sub(pf, "f_ext", sim, 2, stride);
   md_stats_t_x *st = md_alloc_stats_x(1);
+  st->es[0] = 0;
+  md_pimd_calc_pair_energy_x(pimd, pair_energy_of(c), st); /* xi-independent */
+  const double e_pair = stat_of(st);
   for (int q = 0; q < nxi; ++q) {
     char w[32];
     pimd->vi = xis[q];
+    /* the cycle-walk algorithm first (its disagreement with the fast one
+     * measures the reference's own reproducibility on this input) */
+    for (int i = 0; i < nb; ++i)
+      for (int d = 0; d < 3; ++d) sim->particles[i].f[d] = f0[3 * i + d];
+    md_pimd_fill_ENk_x(pimd, c->image);
+    md_pimd_fill_VB_x(pimd);
+    md_pimd_fill_force_VB_x(pimd, c->image);
+    snprintf(w, sizeof w, "f_total_slow_%d", q);
+    dump_sub(pf, w, sim, 2, stride);
     for (int i = 0; i < nb; ++i)
       for (int d = 0; d < 3; ++d) sim->particles[i].f[d] = f0[3 * i + d];
     md_pimd_fast_fill_ENk_x(pimd, c->image);
@@ -619,9 +632,7 @@ static void config_case(const gcase *c, const double *xis, int nxi, int stride) 
     st->es[0] = 0;
     md_pimd_calc_vi_virial_energy_x(pimd, st, c->image);
     e[2] = stat_of(st);
-    st->es[0] = 0;
-    md_pimd_calc_pair_energy_x(pimd, pair_energy_of(c), st);
-    e[3] = stat_of(st);
+    e[3] = e_pair;
     snprintf(w, sizeof w, "e_%d", q);
     dumpf(pf, w, e, 4);
   }

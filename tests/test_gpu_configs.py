@@ -11,8 +11,12 @@ Same initial state on both sides (positions and velocities stored whole,
 chains from md_init_nhc_x).  Per xi: the total
 force (trap + pair + exchange; every stride-th bead), V_B^(N) and the
 thermodynamic, virial and pair estimators; at the last xi a short fused-step
-trajectory.  Bars: forces max|dF|/RMS <= 1e-11, energies 1e-12 relative,
-trajectories 1e-10 (x) / 1e-9 (v)."""
+trajectory.  Bars: forces max|dF|/RMS <= 1e-11, or twice the reference's own
+disagreement between its fast and cycle-walk exchange algorithms on the same
+input when that is larger (at beta=2 with N(0,1) beads, beta V_B ~ 1.6e6, so
+the 1 - exp(-x) entries of the connection matrix that should vanish carry the
+round-off of x, ~1e-10, in both codes); energies 1e-12 relative; trajectories
+1e-10 (x) / 1e-9 (v)."""
 import os
 
 import numpy as np
@@ -59,10 +63,24 @@ def test_config_parity(name):
         assert lib.md_pimd_fast_fill_ENk2_x(s.pimd, image) == 0
         e_vir = stat(lib, lib.md_pimd_calc_vi_virial_energy_x, s.pimd, image)
         e_pair = stat(lib, lib.md_pimd_calc_pair_energy_x, s.pimd, s.pair_energy_cb(), pos=2)
-        print(f"{name} xi={xi}: force rel err {err:.2e}; V_B {vbn:.12g} vs {e[0]:.12g}")
-        assert err < 1e-11
-        for got, ref in zip((vbn, e_vbn, e_vir, e_pair), e):
+        # the reference's own floor: its fast and cycle-walk exchange algorithms on this input
+        floor = rel_err(g[f"f_total_{q}"], g[f"f_total_slow_{q}"])
+        tol = max(1e-11, 2 * floor)
+        print(f"{name} xi={xi}: force rel err {err:.2e} (reference fast vs slow {floor:.2e}); "
+              f"V_B {vbn:.12g} vs {e[0]:.12g}")
+        assert err < tol
+        for got, ref in zip((vbn, e_vbn, e_pair), (e[0], e[1], e[3])):
             assert got == pytest.approx(ref, rel=1e-12, abs=1e-12)
+        # the virial estimator adds sum_j>1 (x_j - x_1) . f_j m / 2, so the force bar
+        # carries into it: tol * RMS(f) * sum |x_j - x_1| m / 2
+        b = s.host_beads()
+        x = b["x"].reshape(N, P, 3)
+        m = b["m"].reshape(N, P)
+        d = np.abs(x[:, 1:] - x[:, :1])
+        if image:
+            d = np.minimum(d, L - d)
+        vtol = tol * np.sqrt(np.mean(g[f"f_total_{q}"] ** 2)) * float((d.sum(-1) * m[:, 1:]).sum()) / 2
+        assert abs(e_vir - e[2]) <= vtol + 1e-12 * abs(e[2])
     # trajectory at the last xi, fused and graph-replayed, from the state above
     cfg = s.step_config(dt, use_graph=1)
     steps = len(g["e_vir_series"])
