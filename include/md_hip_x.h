@@ -23,7 +23,10 @@ int md_get_nhc_length_x(void);
 /* Waits for all queued device work of the engine. */
 int md_hip_synchronize_x(void);
 
-/* Real-space Ewald cutoff in units of 1/alpha (default 6: erfc(6) ~ 2e-17). */
+/* Cutoff x_cut of both Ewald series in units of the split alpha: real space
+ * within x_cut / alpha, k space within pi |G| / alpha <= x_cut (default 5.25:
+ * omitted terms below erfc(5.25) = 1.4e-13 and exp(-5.25^2) = 1.1e-12 of the
+ * leading ones; 6 brings the sum to the rounding floor for 20 % more work). */
 int md_set_ewald_real_cutoff_x(md_num_t_x x_cut);
 
 /* Ewald split used for md_coulomb_3d_ewald_force_x / _energy_x.  The Ewald

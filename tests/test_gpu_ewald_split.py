@@ -1,8 +1,13 @@
 """The Ewald sum evaluated with the engine's own splitting parameter equals
 the reference's split (alpha = 5/L, |m| <= 8 cube) once both series are
 converged: forces and energies agree to round-off for every alpha' the
-engine may pick, and the unconverged reference parameters are left alone.
+engine may pick when both series run to x_cut = 6, and the unconverged
+reference parameters are left alone.  At the default force cutoff
+x_cut = 5.25 the omitted force terms are exp(-x_cut^2) = 1.1e-12 of the
+leading ones, so the forces are held to 5 exp(-x_cut^2) of the RMS force;
+energies always sum to x_cut >= 6.
 """
+import math
 import ctypes as C
 
 import numpy as np
@@ -19,6 +24,10 @@ def lib():
     lib = pyxmd.Lib()
     yield lib
     lib.md_set_ewald_split_x(-1.0)
+    lib.md_set_ewald_real_cutoff_x(DEFAULT_XCUT)
+
+
+DEFAULT_XCUT = 5.25
 
 
 def ueg(lib, N, P, nsum=8, seed=5):
@@ -42,8 +51,11 @@ def plan(lib, s):
     return a.value, rc.value, nk.value
 
 
+@pytest.mark.parametrize("xcut", [6.0, DEFAULT_XCUT])
 @pytest.mark.parametrize("N,P", [(64, 4), (300, 2)])
-def test_split_independence(lib, N, P):
+def test_split_independence(lib, N, P, xcut):
+    assert lib.md_set_ewald_real_cutoff_x(xcut) == 0
+    ftol = max(1e-12, 5.0 * math.exp(-xcut * xcut))
     s = ueg(lib, N, P)
     lib.md_set_ewald_split_x(0.0)
     a0, _, nk0 = plan(lib, s)
@@ -56,11 +68,12 @@ def test_split_independence(lib, N, P):
             assert a * s.L == pytest.approx(aL, rel=1e-14)
         f, e = force_energy(lib, s)
         err = rel_err(f, f0)
-        print(f"N={N} P={P} alpha L={a * s.L:.2f} r_c={rc:.2f} n_k={nk}: force rel err {err:.2e}, "
-              f"energy {e:.15f} vs {e0:.15f}")
-        assert err < 1e-12
+        print(f"N={N} P={P} x_cut={xcut} alpha L={a * s.L:.2f} r_c={rc:.2f} n_k={nk}: "
+              f"force rel err {err:.2e}, energy {e:.15f} vs {e0:.15f}")
+        assert err < ftol
         assert e == pytest.approx(e0, rel=1e-12, abs=1e-12)
     lib.md_set_ewald_split_x(-1.0)
+    assert lib.md_set_ewald_real_cutoff_x(DEFAULT_XCUT) == 0
 
 
 def test_unconverged_reference_split_is_kept(lib):
