@@ -20,6 +20,7 @@
 //    k_sk_fin, k_recip_force, k_recip_energy).
 //  * When the reference's own Ewald parameters are converged, the same sum is
 //    evaluated with the engine's split (plan_ewald).
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -559,6 +560,10 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
     K->nk = nk;
   }
   K->stamp = ++pb->ks_clock;
+  if ((long long)nb * (3 * K->MZ + 2 + K->ZP) > INT_MAX) {
+    fprintf(stderr, "xmd: %d beads exceed the Ewald phase-table index range\n", nb);
+    return nullptr;
+  }
   const size_t slots = (size_t)K->ncolp * K->ZP;
   if (grow(K->tx, K->tx_cap, (size_t)(K->MZ + 1) * nb) || grow(K->ty, K->ty_cap, (size_t)(2 * K->MZ + 1) * nb) ||
       grow(K->tz, K->tz_cap, (size_t)K->ZP * nb) || grow(K->S, K->s_cap, (size_t)P * slots) ||
@@ -567,42 +572,38 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
   return K;
 }
 
-// One thread per (dimension, bead): e^{i pi m u}, u = 2 x / L; negative m
-// are the conjugates.
+// e^{i pi m u}, u = 2 x / L, one thread per table entry: threads run over
+// tx, then ty, then tz, each in storage order, so the stores of a wave are
+// consecutive.  Negative m take the conjugate of the |m| entry.
+__host__ __device__ inline int phase_entries(int nb, int MZ, int ZP) { return nb * (3 * MZ + 2 + ZP); }
+
 __device__ __forceinline__ void k_phase_body(int bx, const double *__restrict__ X, int nb, int MZ, int ZP, double L0, double L1, double L2,
                         double2 *__restrict__ tx, double2 *__restrict__ ty, double2 *__restrict__ tz) {
-  const int t = bx * blockDim.x + threadIdx.x;
-  if (t >= 3 * nb) return;
-  const int d = t / nb, s = t - d * nb;
-  const double u = 2.0 * X[t] / (d == 0 ? L0 : (d == 1 ? L1 : L2));
-  if (d == 0) {
-    double2 *r = tx + (size_t)s * (MZ + 1);
-    r[0] = make_double2(1.0, 0.0);
-    for (int m = 1; m <= MZ; ++m) {
-      double sn, cs;
-      sincospi(m * u, &sn, &cs);
-      r[m] = make_double2(cs, sn);
-    }
-  } else if (d == 1) {
-    double2 *r0 = ty + (size_t)s * (2 * MZ + 1) + MZ;
-    r0[0] = make_double2(1.0, 0.0);
-    for (int m = 1; m <= MZ; ++m) {
-      double sn, cs;
-      sincospi(m * u, &sn, &cs);
-      r0[m] = make_double2(cs, sn);
-      r0[-m] = make_double2(cs, -sn);
+  int t = bx * blockDim.x + threadIdx.x;
+  const int nx = nb * (MZ + 1), ny = nb * (2 * MZ + 1), nz = nb * ZP;
+  int d, s, m;
+  double2 *out;
+  if (t < nx) {
+    d = 0, s = t / (MZ + 1), m = t - s * (MZ + 1), out = tx + t;
+  } else if ((t -= nx) < ny) {
+    d = 1, s = t / (2 * MZ + 1), m = t - s * (2 * MZ + 1) - MZ, out = ty + t;
+  } else if ((t -= ny) < nz) {
+    d = 2, s = t / ZP, m = t - s * ZP - MZ, out = tz + t;
+    if (m > MZ) {
+      *out = make_double2(0.0, 0.0);
+      return;
     }
   } else {
-    double2 *r = tz + (size_t)s * ZP + MZ;
-    r[0] = make_double2(1.0, 0.0);
-    for (int m = 1; m <= MZ; ++m) {
-      double sn, cs;
-      sincospi(m * u, &sn, &cs);
-      r[m] = make_double2(cs, sn);
-      r[-m] = make_double2(cs, -sn);
-    }
-    r[MZ + 1] = make_double2(0.0, 0.0);
+    return;
   }
+  if (m == 0) {
+    *out = make_double2(1.0, 0.0);
+    return;
+  }
+  const double u = 2.0 * X[d * nb + s] / (d == 0 ? L0 : (d == 1 ? L1 : L2));
+  double sn, cs;
+  sincospi((m < 0 ? -m : m) * u, &sn, &cs);
+  *out = make_double2(cs, m < 0 ? -sn : sn);
 }
 
 __global__ void k_phase(const double *__restrict__ X, int nb, int MZ, int ZP, double L0, double L1, double L2,
@@ -1241,7 +1242,8 @@ static int launch_recip(DevPimd *p, const PairParams &pp, int cube_n, double gma
   KSpace *K = ensure_kspace(p, pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P);
   if (!K) return -1;
   if (out) *out = K;
-  hipLaunchKernelGGL(k_phase, dim3((3 * d->nb + 255) / 256), dim3(256), 0, st, d->X, d->nb, K->MZ, K->ZP, pp.L[0],
+  hipLaunchKernelGGL(k_phase, dim3((phase_entries(d->nb, K->MZ, K->ZP) + 255) / 256), dim3(256), 0, st, d->X, d->nb,
+                     K->MZ, K->ZP, pp.L[0],
                      pp.L[1], pp.L[2], K->tx, K->ty, K->tz);
   XMD_CHECK(hipGetLastError());
   const int nzt = K->ZP / 16;
@@ -1582,7 +1584,7 @@ PairBatch *pair_batch_create(const std::vector<BatchItem> &items) {
     if (ew) {
       const int nzt = R.ZP / 16;
       reps[G_PHASE].push_back(R);
-      blocks[G_PHASE].push_back((3 * R.nb + 255) / 256);
+      blocks[G_PHASE].push_back((phase_entries(R.nb, R.MZ, R.ZP) + 255) / 256);
       reps[G_SK].push_back(R);
       blocks[G_SK].push_back((R.ncolp / 16) * nzt * R.P);
       reps[G_FRC].push_back(R);
