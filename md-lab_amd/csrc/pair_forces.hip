@@ -625,97 +625,153 @@ __device__ __forceinline__ void xcd_tile(int w, int ntile, int P, int &slice, in
   }
 }
 
-// Structure factors of one 16 x 16 tile (16 columns x 16 m_z slots) of one
-// slice; the four waves split the beads and their partial sums are added in
-// a fixed order.  Writes S and W = coef conj(S).  Workgroup w covers
-// (slice, column tile, slot tile), slot tile fastest.
-__device__ __forceinline__ void k_sk_mfma_body(int bx, const double2 *__restrict__ tx, const double2 *__restrict__ ty,
-                                                 const double2 *__restrict__ tz, const int2 *__restrict__ cols,
-                                                 const double *__restrict__ coef, int ncolp, int MZ, int ZP, int N,
-                                                 int nb, int P, double2 *__restrict__ S, double2 *__restrict__ W) {
-  __shared__ double red[4][2][256];
+// Complex matrix products on the matrix cores use Gauss's three-product form:
+// for C = sum_k (a_k)(b_k) with a = ar + i ai, b = br + i bi,
+//   T1 = sum (ar + ai) br,  T2 = sum ar (bi - br),  T3 = sum ai (br + bi),
+//   Re C = T1 - T3,  Im C = T1 + T2,
+// three real MFMA chains per complex product instead of four.
+
+// Structure factors of one 16-column tile of one slice over all NZT x 16
+// m_z slots: S[c][z] = sum_s A[c][s] Z[s][z].  The four waves split the
+// beads into quarters; their partial sums are added in a fixed order through
+// `red` (2 x 4 x 256 doubles).  Writes S and W = coef conj(S).
+template <int NZT>
+__device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const double2 *__restrict__ tx,
+                                        const double2 *__restrict__ ty, const double2 *__restrict__ tz, int2 cl,
+                                        const double *__restrict__ coef, int ncolp, int MZ, int N,
+                                        double2 *__restrict__ S, double2 *__restrict__ W) {
+  constexpr int ZP = 16 * NZT;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nzt = ZP / 16;
-  int slice, tile;
-  xcd_tile(bx, (ncolp / 16) * nzt, P, slice, tile);
-  const int c0 = (tile / nzt) * 16, z0 = (tile % nzt) * 16;
-  const int2 cl = cols[c0 + (lane & 15)];
   const int kb = lane >> 4;  // bead of this lane within a K step
-  v4d sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
   const int q0 = (N * wv) / 4, q1 = (N * (wv + 1)) / 4;
-  // software pipeline: the loads of K step b + 4 are in flight while the
-  // matrix cores work on step b
-  auto fetch = [&](int b, double2 &ex, double2 &ey, double2 &z, bool &ok) {
-    const int bead = b + kb;
-    ok = bead < q1;
-    const int s = slice * N + (ok ? bead : q0);
-    ex = tx[(size_t)s * (MZ + 1) + cl.x];
-    ey = ty[(size_t)s * (2 * MZ + 1) + MZ + cl.y];
-    z = tz[(size_t)s * ZP + z0 + (lane & 15)];
-  };
-  double2 ex, ey, z;
-  bool ok;
-  if (q0 < q1) fetch(q0, ex, ey, z, ok);
-  for (int b = q0; b < q1; b += 4) {
-    const double2 cex = ex, cey = ey, cz = z;
-    const bool cok = ok;
-    if (b + 4 < q1) fetch(b + 4, ex, ey, z, ok);
-    double ar = 0, ai = 0;
-    if (cok) {
-      ar = cex.x * cey.x - cex.y * cey.y;
-      ai = cex.x * cey.y + cex.y * cey.x;
+  const int nfull = (q1 - q0) >> 2;
+  const int sx = 4 * (MZ + 1), sy = 4 * (2 * MZ + 1);
+  const size_t s0 = (size_t)slice * N + q0 + kb;
+  const double2 *px = tx + s0 * (MZ + 1) + cl.x;
+  const double2 *py = ty + s0 * (2 * MZ + 1) + MZ + cl.y;
+  const double2 *pz = tz + s0 * ZP + (lane & 15);
+  v4d t1[NZT], t2[NZT], t3[NZT];
+#pragma unroll
+  for (int t = 0; t < NZT; ++t) t1[t] = t2[t] = t3[t] = v4d{0, 0, 0, 0};
+  auto step = [&](double2 ex, double2 ey, const double2 *z, bool keep) {
+    double ar = ex.x * ey.x - ex.y * ey.y, ai = ex.x * ey.y + ex.y * ey.x;
+    if (!keep) ar = ai = 0.0;
+    const double as = ar + ai;
+#pragma unroll
+    for (int t = 0; t < NZT; ++t) {
+      t1[t] = mfma4(as, z[t].x, t1[t]);
+      t2[t] = mfma4(ar, z[t].y - z[t].x, t2[t]);
+      t3[t] = mfma4(ai, z[t].x + z[t].y, t3[t]);
     }
-    sr = mfma4(ar, cz.x, sr);
-    si = mfma4(ar, cz.y, si);
-    sr = mfma4(-ai, cz.y, sr);
-    si = mfma4(ai, cz.x, si);
+  };
+  int k = 0;
+  for (; k + 4 <= nfull; k += 4) {
+    double2 ex[4], ey[4], z[4][NZT];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ex[u] = px[u * sx];
+      ey[u] = py[u * sy];
+#pragma unroll
+      for (int t = 0; t < NZT; ++t) z[u][t] = pz[u * 4 * ZP + 16 * t];
+    }
+    px += 4 * sx;
+    py += 4 * sy;
+    pz += 16 * ZP;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) step(ex[u], ey[u], z[u], true);
+  }
+  for (; k < nfull; ++k) {
+    double2 z[NZT];
+#pragma unroll
+    for (int t = 0; t < NZT; ++t) z[t] = pz[16 * t];
+    step(px[0], py[0], z, true);
+    px += sx;
+    py += sy;
+    pz += 4 * ZP;
+  }
+  if ((q1 - q0) & 3) {
+    // last partial K step: lanes whose bead is past the quarter read the
+    // quarter's first bead and contribute zero
+    const bool ok = q0 + 4 * nfull + kb < q1;
+    const int back = ok ? 0 : 4 * nfull + kb;
+    double2 z[NZT];
+#pragma unroll
+    for (int t = 0; t < NZT; ++t) z[t] = pz[16 * t - back * ZP];
+    step(px[-back * (MZ + 1)], py[-back * (2 * MZ + 1)], z, ok);
   }
   // D[i = column][j = slot]: lane j + 16 (i%4), register i/4
+  const size_t base = (size_t)slice * ncolp * ZP;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int e = (4 * r + (lane >> 4)) * 16 + (lane & 15);
-    red[wv][0][e] = sr[r];
-    red[wv][1][e] = si[r];
+  for (int t = 0; t < NZT; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = (4 * r + (lane >> 4)) * 16 + (lane & 15);
+      red[(2 * wv) * 256 + e] = t1[t][r] - t3[t][r];
+      red[(2 * wv + 1) * 256 + e] = t1[t][r] + t2[t][r];
+    }
+    __syncthreads();
+    const int e = threadIdx.x, i = e >> 4, j = e & 15;
+    const double re = ((red[e] + red[2 * 256 + e]) + red[4 * 256 + e]) + red[6 * 256 + e];
+    const double im = ((red[256 + e] + red[3 * 256 + e]) + red[5 * 256 + e]) + red[7 * 256 + e];
+    const size_t kk = (size_t)(c0 + i) * ZP + 16 * t + j;
+    const double c = coef[kk];
+    S[base + kk] = make_double2(re, im);
+    W[base + kk] = make_double2(c * re, -c * im);
+    __syncthreads();
   }
-  __syncthreads();
-  const int e = threadIdx.x, i = e >> 4, j = e & 15;
-  const double re = ((red[0][0][e] + red[1][0][e]) + red[2][0][e]) + red[3][0][e];
-  const double im = ((red[0][1][e] + red[1][1][e]) + red[2][1][e]) + red[3][1][e];
-  const size_t k = (size_t)(c0 + i) * ZP + z0 + j;
-  const double c = coef[k];
-  S[(size_t)slice * ncolp * ZP + k] = make_double2(re, im);
-  W[(size_t)slice * ncolp * ZP + k] = make_double2(c * re, -c * im);
 }
 
+// Workgroup w covers (slice, column tile) for every slot; NZT = ZP / 16.
+template <int NZT>
+__device__ __forceinline__ void k_sk_mfma_body(int bx, const double2 *__restrict__ tx, const double2 *__restrict__ ty,
+                                                 const double2 *__restrict__ tz, const int2 *__restrict__ cols,
+                                                 const double *__restrict__ coef, int ncolp, int MZ, int N, int P,
+                                                 double2 *__restrict__ S, double2 *__restrict__ W) {
+  __shared__ double red[8 * 256];
+  int slice, tile;
+  xcd_tile(bx, ncolp / 16, P, slice, tile);
+  const int c0 = tile * 16;
+  const int2 cl = cols[c0 + (threadIdx.x & 15)];
+  sk_tile<NZT>(red, slice, c0, tx, ty, tz, cl, coef, ncolp, MZ, N, S, W);
+}
+
+template <int NZT>
 __global__ __launch_bounds__(256) void k_sk_mfma(const double2 *__restrict__ tx, const double2 *__restrict__ ty,
                                                  const double2 *__restrict__ tz, const int2 *__restrict__ cols,
-                                                 const double *__restrict__ coef, int ncolp, int MZ, int ZP, int N,
-                                                 int nb, int P, double2 *__restrict__ S, double2 *__restrict__ W) {
-  k_sk_mfma_body(blockIdx.x, tx, ty, tz, cols, coef, ncolp, MZ, ZP, N, nb, P, S, W);
+                                                 const double *__restrict__ coef, int ncolp, int MZ, int N, int P,
+                                                 double2 *__restrict__ S, double2 *__restrict__ W) {
+  k_sk_mfma_body<NZT>(blockIdx.x, tx, ty, tz, cols, coef, ncolp, MZ, N, P, S, W);
 }
 
-// Forces on 16 beads of one slice.  Work units are (column tile, slot
-// tile) pairs, dealt round-robin to the four waves; per unit
-//   U[s][c] = sum_z Z[s][z] W[c][z],  V[s][c] = sum_z Z[s][z] m_z W[c][z]
-// over the unit's 16 slots, and f_s += sum_c (mx Im(A U) / L_x,
-// my Im(A U) / L_y, Im(A V) / L_z) / (m P) (linear in U, V, so the slot
-// tiles add up).  For bead i this is sum_k coef_k g_k Im(e^{i k.x_i} S_k*) /
-// (m_i P), the reference's per-pair sum (md_simulation_x.c:266-281) since
-// sum_{l != i} sin(k.(x_i - x_l)) = Im(e^{i k.x_i} S_k*).
-__device__ __forceinline__ void k_recip_force_mfma_body(int bx, 
-    const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
-    const int2 *__restrict__ cols, int ncolp, int MZ, int ZP, const double2 *__restrict__ W,
-    const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
-    double Pd) {
-  __shared__ double red[3][4][16];
+// Forces on 16 beads of one slice.  The column tiles are dealt round-robin
+// to the four waves; per tile, over all NZT x 16 slots,
+//   U[s][c] = sum_z Z[s][z] W[c][z],  V[s][c] = sum_z m_z Z[s][z] W[c][z],
+// and f_s += sum_c (mx Im(A U) / L_x, my Im(A U) / L_y, Im(A V) / L_z) / (m P).
+// For bead i this is sum_k coef_k g_k Im(e^{i k.x_i} S_k*) / (m_i P), the
+// reference's per-pair sum (md_simulation_x.c:266-281) since
+// sum_{l != i} sin(k.(x_i - x_l)) = Im(e^{i k.x_i} S_k*).  The Z operands of
+// the 16 beads are the same for every column tile and stay in registers.
+template <int NZT>
+__device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0, const double2 *__restrict__ tx,
+                                                 const double2 *__restrict__ ty, const double2 *__restrict__ tz,
+                                                 const int2 *__restrict__ cols, int ncolp, int MZ,
+                                                 const double2 *__restrict__ W, const double *__restrict__ Mass,
+                                                 double *__restrict__ F, int N, int nb, double L0, double L1,
+                                                 double L2, double Pd) {
+  constexpr int ZP = 16 * NZT, KS = 4 * NZT;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nbt = (N + 15) / 16;
-  int slice, bt;
-  xcd_tile(bx, nbt, P, slice, bt);
-  const int b0 = bt * 16;
   const int abead = b0 + (lane & 15);  // A operand row of this lane
-  const double2 *pza = tz + (size_t)(slice * N + (abead < N ? abead : 0)) * ZP + (lane >> 4);
   const bool aok = abead < N;
+  const double2 *pza = tz + (size_t)(slice * N + (aok ? abead : 0)) * ZP + (lane >> 4);
+  double zs[KS], zr[KS], zi[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const double2 z = aok ? pza[4 * k] : make_double2(0.0, 0.0);
+    zs[k] = z.x + z.y;
+    zr[k] = z.x;
+    zi[k] = z.y;
+  }
+  const double mz0 = (double)((lane >> 4) - MZ);  // m_z of this lane's K row at k = 0
   // D[i = bead][j = column]: lane j + 16 (i%4), register i/4
   const int beadq = lane >> 4;  // this lane's beads: b0 + 4 r + beadq
   int sb[4];
@@ -726,18 +782,13 @@ __device__ __forceinline__ void k_recip_force_mfma_body(int bx,
   }
   double fx[4] = {0, 0, 0, 0}, fy[4] = {0, 0, 0, 0}, fz[4] = {0, 0, 0, 0};
   const double2 *Ws = W + (size_t)slice * ncolp * ZP;
-  const int nzt = ZP / 16, nunit = (ncolp / 16) * nzt;
-  for (int u = wv; u < nunit; u += 4) {
-    const int ct = u / nzt, z0 = (u - ct * nzt) * 16;
+  const int nct = ncolp / 16;
+  for (int ct = wv; ct < nct; ct += 4) {
     const int c = ct * 16 + (lane & 15);
-    // operands: Z rows of the beads, W rows of the columns, and the
-    // epilogue's phase factors, all issued before the matrix-core chain
-    double2 za[4], wb[4];
+    const double2 *pw = Ws + (size_t)c * ZP + (lane >> 4);
+    double2 wb[KS];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      za[ks] = aok ? pza[z0 + 4 * ks] : make_double2(0.0, 0.0);
-      wb[ks] = Ws[(size_t)c * ZP + z0 + 4 * ks + (lane >> 4)];
-    }
+    for (int k = 0; k < KS; ++k) wb[k] = pw[4 * k];
     const int2 cl = cols[c];
     double2 exs[4], eys[4];
 #pragma unroll
@@ -745,29 +796,29 @@ __device__ __forceinline__ void k_recip_force_mfma_body(int bx,
       exs[r] = tx[(size_t)sb[r] * (MZ + 1) + cl.x];
       eys[r] = ty[(size_t)sb[r] * (2 * MZ + 1) + MZ + cl.y];
     }
-    v4d ur = {0, 0, 0, 0}, ui = {0, 0, 0, 0}, vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0};
+    v4d u1 = {0, 0, 0, 0}, u2 = u1, u3 = u1, v1 = u1, v2 = u1, v3 = u1;
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const double2 w = wb[ks];
-      const double mz = (double)(z0 + 4 * ks + (lane >> 4) - MZ);
-      const double wzr = mz * w.x, wzi = mz * w.y;
-      ur = mfma4(za[ks].x, w.x, ur);
-      ui = mfma4(za[ks].x, w.y, ui);
-      vr = mfma4(za[ks].x, wzr, vr);
-      vi = mfma4(za[ks].x, wzi, vi);
-      ur = mfma4(-za[ks].y, w.y, ur);
-      ui = mfma4(za[ks].y, w.x, ui);
-      vr = mfma4(-za[ks].y, wzi, vr);
-      vi = mfma4(za[ks].y, wzr, vi);
+    for (int k = 0; k < KS; ++k) {
+      // B operand: W[c][z] with z = 4 k + lane / 16; V takes m_z W
+      const double wr = wb[k].x, wd = wb[k].y - wb[k].x, ws = wb[k].x + wb[k].y;
+      const double m = mz0 + 4 * k;
+      u1 = mfma4(zs[k], wr, u1);
+      u2 = mfma4(zr[k], wd, u2);
+      u3 = mfma4(zi[k], ws, u3);
+      v1 = mfma4(zs[k], m * wr, v1);
+      v2 = mfma4(zr[k], m * wd, v2);
+      v3 = mfma4(zi[k], m * ws, v3);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
+      const double ur = u1[r] - u3[r], ui = u1[r] + u2[r];
+      const double vr = v1[r] - v3[r], vi = v1[r] + v2[r];
       const double2 ex = exs[r], ey = eys[r];
       const double ar = ex.x * ey.x - ex.y * ey.y, ai = ex.x * ey.y + ex.y * ey.x;
-      const double g = fma(ar, ui[r], ai * ur[r]);
+      const double g = fma(ar, ui, ai * ur);
       fx[r] = fma((double)cl.x, g, fx[r]);
       fy[r] = fma((double)cl.y, g, fy[r]);
-      fz[r] = fma(ar, vi[r], fma(ai, vr[r], fz[r]));
+      fz[r] = fma(ar, vi, fma(ai, vr, fz[r]));
     }
   }
   // sum over the 16 columns held by the lanes of each 16-lane row
@@ -779,12 +830,13 @@ __device__ __forceinline__ void k_recip_force_mfma_body(int bx,
       fy[r] += __shfl_xor(fy[r], o, 64);
       fz[r] += __shfl_xor(fz[r], o, 64);
     }
+  // red[d][wave][bead]
   if ((lane & 15) == 0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      red[0][wv][4 * r + beadq] = fx[r];
-      red[1][wv][4 * r + beadq] = fy[r];
-      red[2][wv][4 * r + beadq] = fz[r];
+      red[(0 * 4 + wv) * 16 + 4 * r + beadq] = fx[r];
+      red[(1 * 4 + wv) * 16 + 4 * r + beadq] = fy[r];
+      red[(2 * 4 + wv) * 16 + 4 * r + beadq] = fz[r];
     }
   }
   __syncthreads();
@@ -792,23 +844,38 @@ __device__ __forceinline__ void k_recip_force_mfma_body(int bx,
     const int t = threadIdx.x, bead = b0 + t;
     if (bead < N) {
       const int s = slice * N + bead;
-      const double gx = ((red[0][0][t] + red[0][1][t]) + red[0][2][t]) + red[0][3][t];
-      const double gy = ((red[1][0][t] + red[1][1][t]) + red[1][2][t]) + red[1][3][t];
-      const double gz = ((red[2][0][t] + red[2][1][t]) + red[2][2][t]) + red[2][3][t];
+      double g[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        g[d] = ((red[(d * 4 + 0) * 16 + t] + red[(d * 4 + 1) * 16 + t]) + red[(d * 4 + 2) * 16 + t]) +
+               red[(d * 4 + 3) * 16 + t];
       const double m = Mass[s];
-      F[s] += gx / L0 / m / Pd;
-      F[nb + s] += gy / L1 / m / Pd;
-      F[2 * nb + s] += gz / L2 / m / Pd;
+      F[s] += g[0] / L0 / m / Pd;
+      F[nb + s] += g[1] / L1 / m / Pd;
+      F[2 * nb + s] += g[2] / L2 / m / Pd;
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_recip_force_mfma(
+template <int NZT>
+__device__ __forceinline__ void k_recip_force_mfma_body(int bx,
     const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
-    const int2 *__restrict__ cols, int ncolp, int MZ, int ZP, const double2 *__restrict__ W,
+    const int2 *__restrict__ cols, int ncolp, int MZ, const double2 *__restrict__ W,
     const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
     double Pd) {
-  k_recip_force_mfma_body(blockIdx.x, tx, ty, tz, cols, ncolp, MZ, ZP, W, Mass, F, N, nb, P, L0, L1, L2, Pd);
+  __shared__ double red[3 * 4 * 16];
+  int slice, bt;
+  xcd_tile(bx, (N + 15) / 16, P, slice, bt);
+  recip_force_tile<NZT>(red, slice, bt * 16, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, L0, L1, L2, Pd);
+}
+
+template <int NZT>
+__global__ __launch_bounds__(256) void k_recip_force_mfma(
+    const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
+    const int2 *__restrict__ cols, int ncolp, int MZ, const double2 *__restrict__ W,
+    const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
+    double Pd) {
+  k_recip_force_mfma_body<NZT>(blockIdx.x, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, P, L0, L1, L2, Pd);
 }
 
 // Reciprocal energy of every slice: sum_{k in half} A_k (|S_k|^2 - N), into partials.
@@ -1247,13 +1314,19 @@ static int launch_recip(DevPimd *p, const PairParams &pp, int cube_n, double gma
                      pp.L[1], pp.L[2], K->tx, K->ty, K->tz);
   XMD_CHECK(hipGetLastError());
   const int nzt = K->ZP / 16;
-  const dim3 gf(((p->N + 15) / 16) * p->P);
-  hipLaunchKernelGGL(k_sk_mfma, dim3((K->ncolp / 16) * nzt * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz, K->cols,
-                     K->coef, K->ncolp, K->MZ, K->ZP, p->N, d->nb, p->P, K->S, K->W);
+  static void (*const sk[4])(const double2 *, const double2 *, const double2 *, const int2 *, const double *, int,
+                             int, int, int, double2 *, double2 *) = {k_sk_mfma<1>, k_sk_mfma<2>, k_sk_mfma<3>,
+                                                                      k_sk_mfma<4>};
+  static void (*const frc[4])(const double2 *, const double2 *, const double2 *, const int2 *, int, int,
+                              const double2 *, const double *, double *, int, int, int, double, double, double,
+                              double) = {k_recip_force_mfma<1>, k_recip_force_mfma<2>, k_recip_force_mfma<3>,
+                                         k_recip_force_mfma<4>};
+  hipLaunchKernelGGL(sk[nzt - 1], dim3((K->ncolp / 16) * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz, K->cols,
+                     K->coef, K->ncolp, K->MZ, p->N, p->P, K->S, K->W);
   XMD_CHECK(hipGetLastError());
   if (force) {
-    hipLaunchKernelGGL(k_recip_force_mfma, gf, dim3(256), 0, st, K->tx, K->ty, K->tz, K->cols, K->ncolp, K->MZ, K->ZP,
-                       K->W, d->M, d->F, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P);
+    hipLaunchKernelGGL(frc[nzt - 1], dim3(((p->N + 15) / 16) * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz, K->cols,
+                       K->ncolp, K->MZ, K->W, d->M, d->F, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P);
     XMD_CHECK(hipGetLastError());
   }
   return 0;
@@ -1485,15 +1558,17 @@ __global__ __launch_bounds__(256) void kb_phase(PAIR_REP) {
   k_phase_body(bx, R.X, R.nb, R.MZ, R.ZP, R.pp.L[0], R.pp.L[1], R.pp.L[2], R.tx, R.ty, R.tz);
 }
 
+template <int NZT>
 __global__ __launch_bounds__(256) void kb_sk_mfma(PAIR_REP) {
   PAIR_R
-  k_sk_mfma_body(bx, R.tx, R.ty, R.tz, R.cols, R.coef, R.ncolp, R.MZ, R.ZP, R.N, R.nb, R.P, R.S, R.W);
+  k_sk_mfma_body<NZT>(bx, R.tx, R.ty, R.tz, R.cols, R.coef, R.ncolp, R.MZ, R.N, R.P, R.S, R.W);
 }
 
+template <int NZT>
 __global__ __launch_bounds__(256) void kb_recip_force_mfma(PAIR_REP) {
   PAIR_R
-  k_recip_force_mfma_body(bx, R.tx, R.ty, R.tz, R.cols, R.ncolp, R.MZ, R.ZP, R.W, R.M, R.F, R.N, R.nb, R.P,
-                          R.pp.L[0], R.pp.L[1], R.pp.L[2], R.pp.P);
+  k_recip_force_mfma_body<NZT>(bx, R.tx, R.ty, R.tz, R.cols, R.ncolp, R.MZ, R.W, R.M, R.F, R.N, R.nb, R.P,
+                               R.pp.L[0], R.pp.L[1], R.pp.L[2], R.pp.P);
 }
 
 #undef PAIR_R
@@ -1501,8 +1576,9 @@ __global__ __launch_bounds__(256) void kb_recip_force_mfma(PAIR_REP) {
 
 // Batched kernel groups: G_GAUSS, G_PGAUSS (tiles), G_EW0 / G_EW1 (Ewald real
 // space without / with third image copies), G_RED (tile reduction of all of
-// them, in that order), G_PHASE / G_SK / G_FRC (reciprocal space).
-enum { G_GAUSS, G_PGAUSS, G_EW0, G_EW1, G_RED, G_PHASE, G_SK, G_FRC, G_COUNT };
+// them, in that order), G_PHASE / G_SK / G_FRC (reciprocal space; one group
+// per slot-tile count ZP / 16 = 1..4).
+enum { G_GAUSS, G_PGAUSS, G_EW0, G_EW1, G_RED, G_PHASE, G_SK, G_FRC = G_SK + 4, G_COUNT = G_FRC + 4 };
 
 struct PairBatch {
   BatchTable g[G_COUNT];
@@ -1582,13 +1658,13 @@ PairBatch *pair_batch_create(const std::vector<BatchItem> &items) {
     reps[G_RED].push_back(R);
     blocks[G_RED].push_back((3 * R.nb + 255) / 256);
     if (ew) {
-      const int nzt = R.ZP / 16;
       reps[G_PHASE].push_back(R);
       blocks[G_PHASE].push_back((phase_entries(R.nb, R.MZ, R.ZP) + 255) / 256);
-      reps[G_SK].push_back(R);
-      blocks[G_SK].push_back((R.ncolp / 16) * nzt * R.P);
-      reps[G_FRC].push_back(R);
-      blocks[G_FRC].push_back(((R.N + 15) / 16) * R.P);
+      const int nzt = R.ZP / 16;
+      reps[G_SK + nzt - 1].push_back(R);
+      blocks[G_SK + nzt - 1].push_back((R.ncolp / 16) * R.P);
+      reps[G_FRC + nzt - 1].push_back(R);
+      blocks[G_FRC + nzt - 1].push_back(((R.N + 15) / 16) * R.P);
     }
   }
   for (int k = 0; k < G_COUNT; ++k)
@@ -1614,8 +1690,14 @@ int pair_batch_forces(PairBatch *b, hipStream_t st) {
   go(G_EW1, kb_ewald_real<true>, 256);
   go(G_RED, kb_reduce_tiles, 256);
   go(G_PHASE, kb_phase, 256);
-  go(G_SK, kb_sk_mfma, 256);
-  go(G_FRC, kb_recip_force_mfma, 256);
+  go(G_SK, kb_sk_mfma<1>, 256);
+  go(G_SK + 1, kb_sk_mfma<2>, 256);
+  go(G_SK + 2, kb_sk_mfma<3>, 256);
+  go(G_SK + 3, kb_sk_mfma<4>, 256);
+  go(G_FRC, kb_recip_force_mfma<1>, 256);
+  go(G_FRC + 1, kb_recip_force_mfma<2>, 256);
+  go(G_FRC + 2, kb_recip_force_mfma<3>, 256);
+  go(G_FRC + 3, kb_recip_force_mfma<4>, 256);
   XMD_CHECK(hipGetLastError());
   for (auto &pk : b->solo)
     if (launch_pair_force(pk.first, pk.second, st)) return -1;
