@@ -28,8 +28,11 @@ int ensure_runtime() {
   XMD_CHECK(hipSetDevice(g_rt.device));
   XMD_CHECK(hipStreamCreateWithFlags(&g_rt.s0, hipStreamNonBlocking));
   XMD_CHECK(hipStreamCreateWithFlags(&g_rt.s1, hipStreamNonBlocking));
+  XMD_CHECK(hipStreamCreateWithFlags(&g_rt.s2, hipStreamNonBlocking));
   XMD_CHECK(hipEventCreateWithFlags(&g_rt.fork, hipEventDisableTiming));
   XMD_CHECK(hipEventCreateWithFlags(&g_rt.join, hipEventDisableTiming));
+  XMD_CHECK(hipEventCreateWithFlags(&g_rt.fork2, hipEventDisableTiming));
+  XMD_CHECK(hipEventCreateWithFlags(&g_rt.join2, hipEventDisableTiming));
   hipDeviceProp_t prop;
   XMD_CHECK(hipGetDeviceProperties(&prop, g_rt.device));
   g_rt.num_cus = prop.multiProcessorCount;

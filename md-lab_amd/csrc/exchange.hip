@@ -365,10 +365,13 @@ __device__ __forceinline__ void lse16(const double *t, double &M, double &S) {
 // grid always drains.
 constexpr int REC_SPIN_MAX = 1 << 22;
 
+// The poll itself is relaxed; the acquire fence follows it once per wait
+// (in every wave, after the barrier), so the caches are invalidated once per
+// hand-off instead of once per poll.
 __device__ __forceinline__ void rec_wait(int *flag, int target, int *err) {
   if (threadIdx.x == 0) {
     int it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
       if (++it > REC_SPIN_MAX) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -378,11 +381,13 @@ __device__ __forceinline__ void rec_wait(int *flag, int target, int *err) {
     }
   }
   __syncthreads();
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// Every wave publishes its own stores (release fence: write-back, no
+// invalidate), then thread 0 raises the flag.
 __device__ __forceinline__ void rec_post(int *flag, int value) {
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -464,11 +469,12 @@ struct RecLds {
   double fm[4][RB], fs[4][RB];  // partials of a self-fold (SOLO)
 };
 
-// The solver walks the blocks in order.  Per block b: stage its in-block
-// coefficients (prefetched into registers during block b-1), fold block b-1
-// into the rows from the prefetched look-ahead coefficients and the values
-// kept in LDS, prefetch block b+1, wait for the updater of block b, merge,
-// then the (max,+) shift scan, scaled exps and the linear substitution.
+// The solver walks the blocks in order.  Per block b: issue the loads of
+// block b+1's coefficients, wait for the updater of block b and issue the
+// loads of its row accumulators, fold block b-1 into the rows from the
+// look-ahead coefficients and the values kept in LDS, merge, then the (max,+)
+// shift scan, scaled exps and the linear substitution; block b+1's
+// coefficients are staged into LDS last, so their loads overlap the block.
 // SOLO: no updater workgroups; the solver folds blocks 0 .. b-2 into block b
 // itself (same order and arithmetic), so it waits on no other workgroup.
 template <bool FWD, bool SOLO = false>
@@ -486,11 +492,21 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
   for (int w = tid; w < RTRI; w += 256) S.dg[0][w] = DGd[w];
   for (int w = tid; w < RB * RB; w += 256) S.la[0][w] = LAd[w];
   const int rowq = lane * (lane - 1) / 2;
+  // (q, r) of this thread's in-block coefficients, the same for every block
+  int tq[DPT], tr[DPT];
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const int w = tid + 256 * k;
+    const int qr = w < RTRI ? kTri.v[w] : 0;
+    tq[k] = qr >> 8;
+    tr[k] = qr & 255;
+  }
   for (int b = 0; b < a.nblk; ++b) {
     const int cur = b & 1;
     const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
     __syncthreads();
-    // prefetch block b+1: loads in flight during the look-ahead
+    // prefetch block b+1: loads in flight during the whole block, staged into
+    // LDS at its end
     double dgr[DPT], lar[LPT];
     const bool more = b + 1 < a.nblk;
     if (more) {
@@ -501,6 +517,18 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
       }
 #pragma unroll
       for (int k = 0; k < LPT; ++k) lar[k] = LAd[(size_t)(b + 1) * (RB * RB) + tid + 256 * k];
+    }
+    // rows' accumulators over blocks 0 .. b-2 (updater or self-fold); their
+    // loads are in flight during the look-ahead
+    if (SOLO) {
+      if (b >= 2) rec_fold_rows<FWD, false>(a, dir, b, S.fm, S.fs);
+    } else if (b >= 2) {
+      rec_wait(folded + b, b - 1, err);
+    }
+    double Mrow = NEG, Srow = 0.0;
+    if (wv == 0 && lane < nrows) {
+      Mrow = AM[p0 + lane];
+      Srow = AS[p0 + lane];
     }
     {
       double t[16], M, Sm;
@@ -514,33 +542,14 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
       S.pm[wv][lane] = M;
       S.ps[wv][lane] = Sm;
     }
-    if (more) {
-#pragma unroll
-      for (int k = 0; k < DPT; ++k) {
-        const int w = tid + 256 * k;
-        if (w < RTRI) S.dg[cur ^ 1][w] = dgr[k];
-      }
-#pragma unroll
-      for (int k = 0; k < LPT; ++k) S.la[cur ^ 1][tid + 256 * k] = lar[k];
-    }
-    if (SOLO) {
-      __syncthreads();
-      if (b >= 2) rec_fold_rows<FWD, false>(a, dir, b, S.fm, S.fs);
-    } else if (b >= 2) {
-      rec_wait(folded + b, b - 1, err);
-    } else {
-      __syncthreads();
-    }
+    __syncthreads();
     if (wv == 0) {
-      double M = NEG, Sm = 0.0;
       if (lane < nrows) {
-        M = AM[p0 + lane];
-        Sm = AS[p0 + lane];
 #pragma unroll
-        for (int w = 0; w < 4; ++w) lse_pair(M, Sm, S.pm[w][lane], S.ps[w][lane]);
+        for (int w = 0; w < 4; ++w) lse_pair(Mrow, Srow, S.pm[w][lane], S.ps[w][lane]);
       }
-      S.mo[lane] = M;
-      S.so[lane] = Sm;
+      S.mo[lane] = Mrow;
+      S.so[lane] = Srow;
     }
     __syncthreads();
     double *dg = S.dg[cur];
@@ -549,9 +558,14 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
     // row of c in registers.  Every term obeys c(q,r) + L(r) <= L(q) <=
     // sigma_q + (rows) ln N, so the scaled values below stay in FP64 range.
     if (wv == 0) {
+      // row q's coefficients c(q, 0 .. q-1) start at rowq; the reads past
+      // them stay inside dg (rowq + RB - 2 < RTRI) and are masked
       double c[RB - 1];
 #pragma unroll
-      for (int r = 0; r < RB - 1; ++r) c[r] = r < lane ? dg[rowq + r] : NEG;
+      for (int r = 0; r < RB - 1; ++r) {
+        const double x = dg[rowq + r];
+        c[r] = r < lane ? x : NEG;
+      }
       double sig = lane < nrows ? S.mo[lane] : NEG;
 #pragma unroll
       for (int r = 0; r < RB - 1; ++r) sig = fmax(sig, c[r] + readlane_d(sig, r));
@@ -564,7 +578,7 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
     for (int k = 0; k < DPT; ++k) {
       const int w = tid + 256 * k;
       if (w < RTRI) {
-        const int qr = kTri.v[w], q = qr >> 8, r = qr & 255;
+        const int q = tq[k], r = tr[k];
         const double c = dg[w];
         dg[w] = (q < nrows && c > NEG) ? exp(c + S.sg[r] - S.sg[q]) : 0.0;
       }
@@ -575,7 +589,10 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
     if (wv == 0) {
       double av[RB - 1];
 #pragma unroll
-      for (int r = 0; r < RB - 1; ++r) av[r] = r < lane ? dg[rowq + r] : 0.0;
+      for (int r = 0; r < RB - 1; ++r) {
+        const double x = dg[rowq + r];
+        av[r] = r < lane ? x : 0.0;
+      }
       double W = 0.0;
       if (lane < nrows && S.mo[lane] > NEG && S.mo[lane] > S.sg[lane] - 745.0)
         W = S.so[lane] * exp(S.mo[lane] - S.sg[lane]);
@@ -593,6 +610,15 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
           a.Rb[N - p] = v;
         }
       }
+    }
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const int w = tid + 256 * k;
+        if (w < RTRI) S.dg[cur ^ 1][w] = dgr[k];
+      }
+#pragma unroll
+      for (int k = 0; k < LPT; ++k) S.la[cur ^ 1][tid + 256 * k] = lar[k];
     }
     rec_post(solved + b, 1);
   }

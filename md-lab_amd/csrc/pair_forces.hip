@@ -907,16 +907,62 @@ __global__ __launch_bounds__(256) void k_recip_energy(const double2 *__restrict_
 // Images: with the minimum image a_d of the separation (|a_d| <= L_d/2),
 // b_d = a_d - sign(a_d) L_d and c_d = a_d + sign(a_d) L_d, every image within
 // r_c < sqrt(2) L takes per dimension a, b or c, and at most one dimension
-// takes c (two would put it beyond sqrt(2) L).  Each lane builds the bit
-// mask of its candidates inside r_c at bit kx | ky << 2 | kz << 4 with
-// k = 0 (a), 1 (b), 3 (c); the image is then a_d - sext(k_d) s_d.  CIMG =
-// false when r_c < min L_d (no c candidates).
+// takes c (two would put it beyond sqrt(2) L).  Three enumerations, by r_c:
+//   IMG_FACE   r_c^2 < min L_d^2 / 2: at most one image besides a, and it
+//              takes b in one dimension only (two b's, or b in two
+//              dimensions, are at least min L_d / sqrt(2) away), so each pair
+//              evaluates exactly two candidates, a and the face image whose
+//              distance is smallest, with no per-lane loop;
+//   IMG_NEAR   r_c < min L_d: candidates a/b per dimension;
+//   IMG_CORNER r_c < sqrt(2) min L_d: also the c candidates.
+// IMG_NEAR and IMG_CORNER build the bit mask of the candidates inside r_c at
+// bit kx | ky << 2 | kz << 4 with k = 0 (a), 1 (b), 3 (c), and walk it; the
+// image is a_d - sext(k_d) s_d.  All three take the images in increasing bit
+// order, so they agree to the last bit where more than one applies.
+enum { IMG_FACE, IMG_NEAR, IMG_CORNER };
+
 struct EwReal {
   double L[3], iL[3];
   double q, al8, rc2;  // al8 = 8 alpha: table intervals of width 1/8 in x = alpha r
 };
 
-template <bool CIMG>
+// 1/sqrt(x) for finite x > 0 (no zero, infinity or subnormal handling): the
+// hardware estimate y refined by one second-order Newton-Raphson step,
+// y (1 + e/2 + 3e^2/8) with e = 1 - x y^2.
+__device__ __forceinline__ double rsqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(y * -x, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// Force of one image r of the pair, added to g when `in` holds.  r is never
+// zero: distinct beads of a periodic system.
+__device__ __forceinline__ void ewald_image(double rx, double ry, double rz, bool in, const EwReal &ep,
+                                            const double2 *__restrict__ tab, double &gx, double &gy, double &gz) {
+  const double R2 = fma(rx, rx, fma(ry, ry, rz * rz));
+  const double ir = rsqrt_pos(R2);
+  const double t8 = (R2 * ir) * ep.al8;
+  const int iv = min((int)t8, kEwTabIntervals - 1);
+  const double y = fma(2.0, t8, -(double)(2 * iv + 1));
+  // coefficients c_0 .. c_DEG of the interval as (c_2k, c_2k+1) pairs; Horner in y
+  constexpr int NC2 = kEwTabStride / 2;
+  const double2 *c = tab + iv * NC2;
+  double2 cc[NC2];
+#pragma unroll
+  for (int k = 0; k < NC2; ++k) cc[k] = c[k];
+  double h = fma(cc[NC2 - 1].y, y, cc[NC2 - 1].x);
+#pragma unroll
+  for (int k = NC2 - 2; k >= 0; --k) {
+    h = fma(h, y, cc[k].y);
+    h = fma(h, y, cc[k].x);
+  }
+  const double w = in ? ep.q * h * (ir * ir * ir) : 0.0;
+  gx = fma(w, rx, gx);
+  gy = fma(w, ry, gy);
+  gz = fma(w, rz, gz);
+}
+
+template <int IMG>
 __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, const EwReal &ep,
                                                const double2 *__restrict__ tab, double &gx, double &gy,
                                                double &gz) {
@@ -929,6 +975,12 @@ __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, 
   const double bx2 = bx * bx, by2 = by * by, bz2 = bz * bz;
   const double rc2 = ep.rc2;
   const double xy[4] = {ax2 + ay2, bx2 + ay2, ax2 + by2, bx2 + by2};
+  if (IMG == IMG_FACE) {
+    const bool kx = xy[1] + az2 <= rc2, ky = xy[2] + az2 <= rc2, kz = xy[0] + bz2 <= rc2;
+    ewald_image(ax, ay, az, xy[0] + az2 <= rc2, ep, tab, gx, gy, gz);
+    ewald_image(kx ? bx : ax, ky ? by : ay, kz ? bz : az, kx || ky || kz, ep, tab, gx, gy, gz);
+    return;
+  }
   unsigned lo = 0, hi = 0;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -936,7 +988,7 @@ __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, 
     const int pos = (q & 1) | ((q & 2) << 1) | ((q & 4) << 2);
     lo |= (R2 <= rc2) ? (1u << pos) : 0u;
   }
-  if (CIMG) {
+  if (IMG == IMG_CORNER) {
     const double cx = ax + sx, cy = ay + sy, cz = az + sz;
     const double cx2 = cx * cx, cy2 = cy * cy, cz2 = cz * cz;
 #pragma unroll
@@ -956,39 +1008,18 @@ __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, 
     const int pos = __builtin_ctzll(m);
     m &= m - 1;
     const double kx = (double)((pos << 30) >> 30), ky = (double)((pos << 28) >> 30), kz = (double)((pos << 26) >> 30);
-    const double rx = fma(-kx, sx, ax), ry = fma(-ky, sy, ay), rz = fma(-kz, sz, az);
-    const double R2 = fma(rx, rx, fma(ry, ry, rz * rz));
-    const double ir = rsqrt(R2);
-    const double t8 = (R2 * ir) * ep.al8;
-    const int iv = min((int)t8, kEwTabIntervals - 1);
-    const double y = fma(2.0, t8, -(double)(2 * iv + 1));
-    // coefficients c_0 .. c_DEG of the interval as (c_2k, c_2k+1) pairs; Horner in y
-    constexpr int NC2 = kEwTabStride / 2;
-    const double2 *c = tab + iv * NC2;
-    double2 cc[NC2];
-#pragma unroll
-    for (int k = 0; k < NC2; ++k) cc[k] = c[k];
-    double h = fma(cc[NC2 - 1].y, y, cc[NC2 - 1].x);
-#pragma unroll
-    for (int k = NC2 - 2; k >= 0; --k) {
-      h = fma(h, y, cc[k].y);
-      h = fma(h, y, cc[k].x);
-    }
-    const double w = ep.q * h * (ir * ir * ir);
-    gx = fma(w, rx, gx);
-    gy = fma(w, ry, gy);
-    gz = fma(w, rz, gz);
+    ewald_image(fma(-kx, sx, ax), fma(-ky, sy, ay), fma(-kz, sz, az), true, ep, tab, gx, gy, gz);
   }
 }
 
 // Four tile pairs per workgroup (one per wave) sharing the LDS table; the
 // per-wave schedule is that of k_pair_tiles.
-template <bool CIMG>
+template <int IMG>
 __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restrict__ X, const double *__restrict__ M,
                                                     double *__restrict__ Fpart, const int2 *__restrict__ tiles, int N,
                                                     int nb, int T, int ntp, int nitems, EwReal ep) {
   __shared__ double2 tab[kEwTabIntervals * kEwTabStride / 2];
-  __shared__ double jx[4][64], jy[4][64], jz[4][64], jw[4][64], rx[4][64], ry[4][64], rz[4][64];
+  __shared__ double jx[4][64], jy[4][64], jz[4][64], rx[4][64], ry[4][64], rz[4][64];
   const int tid = threadIdx.x, t = tid & 63, w = tid >> 6;
   for (int k = tid; k < kEwTabIntervals * kEwTabStride / 2; k += 256)
     tab[k] = make_double2(kEwTab[2 * k], kEwTab[2 * k + 1]);
@@ -1014,7 +1045,6 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
   jx[w][t] = jact ? X[base + jl] : 0.0;
   jy[w][t] = jact ? X[nb + base + jl] : 0.0;
   jz[w][t] = jact ? X[2 * nb + base + jl] : 0.0;
-  jw[w][t] = jact ? 1.0 / M[base + jl] : 1.0;
   rx[w][t] = 0;
   ry[w][t] = 0;
   rz[w][t] = 0;
@@ -1028,8 +1058,10 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
       const int jg = J * 64 + j;
       if (jg >= N || (diag && s == 32 && t >= 32)) continue;
       double gx = 0, gy = 0, gz = 0;
-      ewald_pair_tab<CIMG>(xi - jx[w][j], yi - jy[w][j], zi - jz[w][j], ep, tab, gx, gy, gz);
-      const double iw = jg > i ? jw[w][j] : wi;
+      ewald_pair_tab<IMG>(xi - jx[w][j], yi - jy[w][j], zi - jz[w][j], ep, tab, gx, gy, gz);
+      // the pair's force over the mass of its higher-index bead; the table
+      // kernels run only for equal masses, so that is this lane's 1 / m_i
+      const double iw = wi;
       fx = fma(gx, iw, fx);
       fy = fma(gy, iw, fy);
       fz = fma(gz, iw, fz);
@@ -1054,11 +1086,11 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
   }
 }
 
-template <bool CIMG>
+template <int IMG>
 __global__ __launch_bounds__(256) void k_ewald_real(const double *__restrict__ X, const double *__restrict__ M,
                                                     double *__restrict__ Fpart, const int2 *__restrict__ tiles, int N,
                                                     int nb, int T, int ntp, int nitems, EwReal ep) {
-  k_ewald_real_body<CIMG>(blockIdx.x, X, M, Fpart, tiles, N, nb, T, ntp, nitems, ep);
+  k_ewald_real_body<IMG>(blockIdx.x, X, M, Fpart, tiles, N, nb, T, ntp, nitems, ep);
 }
 
 // ------------------------------------------------------------ launchers
@@ -1204,15 +1236,25 @@ static bool ref_converged(const PairParams &pp) {
 }
 
 // Modelled device time (ps) of a split alpha L_min = aL, fitted to the
-// kernels' measured times on MI355X at N = 1024, P = 32: real space 4 ps per
-// pair plus 3.7 ps per image inside r_c (11 + 2.1 when r_c >= L admits third
-// copies); reciprocal space 0.55 ps per (bead, padded k slot) plus a fixed
-// 30 us per 32 slices.
+// kernels' measured times on MI355X (UEG, N = 256 .. 2048, P = 16 .. 32).
+// Real space per pair: 6.5 ps with the face enumeration (two candidate images
+// whatever r_c), else 1.0 + 5.2 ps per image inside r_c (11 + 2.1 when
+// r_c >= L admits third copies); never below one wave's walk over its tile
+// pair, about 5.9e6 pair costs (small systems do not fill the device).
+// Reciprocal space 0.42 ps
+// per (bead, padded k slot) plus 12 us per 32 slices.
 static double split_cost(const PairParams &pp, int N, int P, double aL) {
   const double Lmin = lmin3(pp.L), V = pp.L[0] * pp.L[1] * pp.L[2];
   const double alpha = aL / Lmin, rc = pp.xcut / alpha;
   const double images = 4.0 / 3.0 * M_PI * rc * rc * rc / V;
-  const double per_pair = rc >= Lmin ? 11.0 + 2.1 * images : 4.0 + 3.7 * images;
+  double per_pair;
+  if (rc * rc < 0.5 * Lmin * Lmin * (1.0 - 1e-6))
+    per_pair = 6.5;
+  else if (rc >= Lmin)
+    per_pair = 11.0 + 2.1 * images;
+  else
+    per_pair = 1.0 + 5.2 * images;
+  const double real = std::max(P * 0.5 * N * (N - 1.0), 5.9e6) * per_pair;
   const double gmax = pp.xcut * alpha / M_PI;
   int M = 0;
   for (int d = 0; d < 3; ++d) M = std::max(M, (int)floor(gmax * pp.L[d]));
@@ -1224,7 +1266,7 @@ static double split_cost(const PairParams &pp, int N, int P, double aL) {
       if ((mx > 0 || my >= 0) && (mx / pp.L[0]) * (mx / pp.L[0]) + (my / pp.L[1]) * (my / pp.L[1]) <= gmax * gmax)
         ++ncol;
   const double slots = (double)((ncol + 15) / 16 * 16) * (2 * MZ + 2);
-  return P * (0.5 * N * (N - 1.0) * per_pair + 0.55 * N * slots) + 30e6 * P / 32.0;
+  return real + 0.42 * N * P * slots + 12e6 * P / 32.0;
 }
 
 static bool split_ok(const PairParams &pp, double aL) {
@@ -1276,7 +1318,17 @@ static PairParams ewald_params(DevPimd *p, EwaldPlan &pl, double xcut = g_ewald_
   return pp;
 }
 
-static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t st) {
+// Image enumeration of the real-space kernel for the cutoff of pp (the face
+// bound keeps a relative margin of 1e-6 against rounding of the distances).
+static int image_mode(const PairParams &pp) {
+  const double L2 = lmin3(pp.L) * lmin3(pp.L);
+  if (pp.rc2 < 0.5 * L2 * (1.0 - 1e-6)) return IMG_FACE;
+  return pp.rc2 >= L2 ? IMG_CORNER : IMG_NEAR;
+}
+
+// Table-driven real space: per-tile partial forces (k_ewald_real), then
+// their sum into F (k_reduce_tiles) unless `reduce` is false.
+static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t st, bool reduce = true) {
   DevSim *d = p->ds;
   if (ensure_tiles(p)) return -1;
   TileBufs &B = pair_bufs(p)->tb;
@@ -1289,47 +1341,57 @@ static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t s
   ep.al8 = kEwTabInvWidth * pp.alpha;
   ep.rc2 = pp.rc2;
   const int items = p->P * B.ntp;
-  const double Lmin = lmin3(pp.L);
-  if (pp.rc2 >= Lmin * Lmin)
-    hipLaunchKernelGGL(k_ewald_real<true>, dim3((items + 3) / 4), dim3(256), 0, st, d->X, d->M, B.Fpart, B.tiles,
-                       p->N, d->nb, B.T, B.ntp, items, ep);
-  else
-    hipLaunchKernelGGL(k_ewald_real<false>, dim3((items + 3) / 4), dim3(256), 0, st, d->X, d->M, B.Fpart, B.tiles,
-                       p->N, d->nb, B.T, B.ntp, items, ep);
+  static void (*const kern[3])(const double *, const double *, double *, const int2 *, int, int, int, int, int,
+                               EwReal) = {k_ewald_real<IMG_FACE>, k_ewald_real<IMG_NEAR>, k_ewald_real<IMG_CORNER>};
+  hipLaunchKernelGGL(kern[image_mode(pp)], dim3((items + 3) / 4), dim3(256), 0, st, d->X, d->M, B.Fpart, B.tiles,
+                     p->N, d->nb, B.T, B.ntp, items, ep);
   XMD_CHECK(hipGetLastError());
+  if (!reduce) return 0;
   const int n3 = 3 * d->nb;
   hipLaunchKernelGGL(k_reduce_tiles, dim3((n3 + 255) / 256), dim3(256), 0, st, B.Fpart, d->F, n3, B.T, pp.P);
   XMD_CHECK(hipGetLastError());
   return 0;
 }
 
-static int launch_recip(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st, bool force,
-                        KSpace **out) {
+static void (*const k_sk_of[4])(const double2 *, const double2 *, const double2 *, const int2 *, const double *, int,
+                                int, int, int, double2 *, double2 *) = {k_sk_mfma<1>, k_sk_mfma<2>, k_sk_mfma<3>,
+                                                                         k_sk_mfma<4>};
+static void (*const k_frc_of[4])(const double2 *, const double2 *, const double2 *, const int2 *, int, int,
+                                 const double2 *, const double *, double *, int, int, int, double, double, double,
+                                 double) = {k_recip_force_mfma<1>, k_recip_force_mfma<2>, k_recip_force_mfma<3>,
+                                            k_recip_force_mfma<4>};
+
+// Phase tables and structure factors S, W of every slice (reads X only).
+static int launch_recip_sk(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st,
+                           KSpace **out) {
   DevSim *d = p->ds;
   KSpace *K = ensure_kspace(p, pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P);
   if (!K) return -1;
-  if (out) *out = K;
+  *out = K;
   hipLaunchKernelGGL(k_phase, dim3((phase_entries(d->nb, K->MZ, K->ZP) + 255) / 256), dim3(256), 0, st, d->X, d->nb,
-                     K->MZ, K->ZP, pp.L[0],
-                     pp.L[1], pp.L[2], K->tx, K->ty, K->tz);
+                     K->MZ, K->ZP, pp.L[0], pp.L[1], pp.L[2], K->tx, K->ty, K->tz);
   XMD_CHECK(hipGetLastError());
-  const int nzt = K->ZP / 16;
-  static void (*const sk[4])(const double2 *, const double2 *, const double2 *, const int2 *, const double *, int,
-                             int, int, int, double2 *, double2 *) = {k_sk_mfma<1>, k_sk_mfma<2>, k_sk_mfma<3>,
-                                                                      k_sk_mfma<4>};
-  static void (*const frc[4])(const double2 *, const double2 *, const double2 *, const int2 *, int, int,
-                              const double2 *, const double *, double *, int, int, int, double, double, double,
-                              double) = {k_recip_force_mfma<1>, k_recip_force_mfma<2>, k_recip_force_mfma<3>,
-                                         k_recip_force_mfma<4>};
-  hipLaunchKernelGGL(sk[nzt - 1], dim3((K->ncolp / 16) * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz, K->cols,
-                     K->coef, K->ncolp, K->MZ, p->N, p->P, K->S, K->W);
+  hipLaunchKernelGGL(k_sk_of[K->ZP / 16 - 1], dim3((K->ncolp / 16) * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz,
+                     K->cols, K->coef, K->ncolp, K->MZ, p->N, p->P, K->S, K->W);
   XMD_CHECK(hipGetLastError());
-  if (force) {
-    hipLaunchKernelGGL(frc[nzt - 1], dim3(((p->N + 15) / 16) * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz, K->cols,
-                       K->ncolp, K->MZ, K->W, d->M, d->F, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P);
-    XMD_CHECK(hipGetLastError());
-  }
   return 0;
+}
+
+// Reciprocal forces from the tables and W of launch_recip_sk, added into F.
+static int launch_recip_force(DevPimd *p, const PairParams &pp, const KSpace *K, hipStream_t st) {
+  DevSim *d = p->ds;
+  hipLaunchKernelGGL(k_frc_of[K->ZP / 16 - 1], dim3(((p->N + 15) / 16) * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz,
+                     K->cols, K->ncolp, K->MZ, K->W, d->M, d->F, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+static int launch_recip(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st, bool force,
+                        KSpace **out) {
+  KSpace *K = nullptr;
+  if (launch_recip_sk(p, pp, cube_n, gmax2, st, &K)) return -1;
+  if (out) *out = K;
+  return force ? launch_recip_force(p, pp, K, st) : 0;
 }
 
 int launch_pair_real(DevPimd *p, int kind, hipStream_t st) {
@@ -1373,6 +1435,27 @@ int launch_pair_recip(DevPimd *p, int kind, hipStream_t st) {
   }
   PairParams pp = make_params(p);
   return launch_recip(p, pp, pp.nsum, 0.0, st, true, nullptr);
+}
+
+int launch_pair_force_overlap(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork,
+                              hipEvent_t join) {
+  if (kind != PK_EWALD || !p->ds->uniform_mass) return 0;
+  EwaldPlan pl;
+  PairParams pp = ewald_params(p, pl);
+  if (!near_ok(pp)) return 0;
+  XMD_CHECK(hipEventRecord(fork, st));
+  XMD_CHECK(hipStreamWaitEvent(s2, fork, 0));
+  KSpace *K = nullptr;
+  if (launch_recip_sk(p, pp, pl.cube_n, pl.gmax2, s2, &K)) return -1;
+  XMD_CHECK(hipEventRecord(join, s2));
+  if (launch_ewald_real_tab(p, pp, st, false)) return -1;
+  XMD_CHECK(hipStreamWaitEvent(st, join, 0));
+  TileBufs &B = pair_bufs(p)->tb;
+  const int n3 = 3 * p->ds->nb;
+  hipLaunchKernelGGL(k_reduce_tiles, dim3((n3 + 255) / 256), dim3(256), 0, st, B.Fpart, p->ds->F, n3, B.T, pp.P);
+  XMD_CHECK(hipGetLastError());
+  if (launch_recip_force(p, pp, K, st)) return -1;
+  return 1;
 }
 
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st) {
@@ -1542,10 +1625,10 @@ __global__ __launch_bounds__(64) void kb_pair_tiles(PAIR_REP) {
   k_pair_tiles_body<MODE, false, true>(bx, R.X, R.M, R.Fpart, nullptr, R.tiles, R.N, R.nb, R.T, R.ntp, R.pp);
 }
 
-template <bool CIMG>
+template <int IMG>
 __global__ __launch_bounds__(256) void kb_ewald_real(PAIR_REP) {
   PAIR_R
-  k_ewald_real_body<CIMG>(bx, R.X, R.M, R.Fpart, R.tiles, R.N, R.nb, R.T, R.ntp, R.items, R.ep);
+  k_ewald_real_body<IMG>(bx, R.X, R.M, R.Fpart, R.tiles, R.N, R.nb, R.T, R.ntp, R.items, R.ep);
 }
 
 __global__ __launch_bounds__(256) void kb_reduce_tiles(PAIR_REP) {
@@ -1574,11 +1657,11 @@ __global__ __launch_bounds__(256) void kb_recip_force_mfma(PAIR_REP) {
 #undef PAIR_R
 #undef PAIR_REP
 
-// Batched kernel groups: G_GAUSS, G_PGAUSS (tiles), G_EW0 / G_EW1 (Ewald real
-// space without / with third image copies), G_RED (tile reduction of all of
-// them, in that order), G_PHASE / G_SK / G_FRC (reciprocal space; one group
-// per slot-tile count ZP / 16 = 1..4).
-enum { G_GAUSS, G_PGAUSS, G_EW0, G_EW1, G_RED, G_PHASE, G_SK, G_FRC = G_SK + 4, G_COUNT = G_FRC + 4 };
+// Batched kernel groups: G_GAUSS, G_PGAUSS (tiles), G_EW + image mode (Ewald
+// real space: face, near, corner enumeration), G_RED (tile reduction of all
+// of them, in that order), G_PHASE / G_SK / G_FRC (reciprocal space; one
+// group per slot-tile count ZP / 16 = 1..4).
+enum { G_GAUSS, G_PGAUSS, G_EW, G_RED = G_EW + 3, G_PHASE, G_SK, G_FRC = G_SK + 4, G_COUNT = G_FRC + 4 };
 
 struct PairBatch {
   BatchTable g[G_COUNT];
@@ -1647,8 +1730,7 @@ PairBatch *pair_batch_create(const std::vector<BatchItem> &items) {
     R.ep.rc2 = pp.rc2;
     int grp;
     if (ew) {
-      const double Lmin = lmin3(pp.L);
-      grp = pp.rc2 >= Lmin * Lmin ? G_EW1 : G_EW0;
+      grp = G_EW + image_mode(pp);
       blocks[grp].push_back((R.items + 3) / 4);
     } else {
       grp = kind == PK_GAUSS ? G_GAUSS : G_PGAUSS;
@@ -1686,8 +1768,9 @@ int pair_batch_forces(PairBatch *b, hipStream_t st) {
   };
   go(G_GAUSS, kb_pair_tiles<PK_GAUSS>, 64);
   go(G_PGAUSS, kb_pair_tiles<PK_PGAUSS>, 64);
-  go(G_EW0, kb_ewald_real<false>, 256);
-  go(G_EW1, kb_ewald_real<true>, 256);
+  go(G_EW + IMG_FACE, kb_ewald_real<IMG_FACE>, 256);
+  go(G_EW + IMG_NEAR, kb_ewald_real<IMG_NEAR>, 256);
+  go(G_EW + IMG_CORNER, kb_ewald_real<IMG_CORNER>, 256);
   go(G_RED, kb_reduce_tiles, 256);
   go(G_PHASE, kb_phase, 256);
   go(G_SK, kb_sk_mfma<1>, 256);

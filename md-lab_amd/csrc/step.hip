@@ -7,6 +7,8 @@
 // With massive thermostats NHC half 1 and the wrap are one kernel, and so are
 // the force reset, trap and interior springs.
 //   s1: (after fork) forward + backward recursions --join-->
+//   s2: (Ewald) phase tables and structure factors, forked from s0 before
+//       the real-space kernel and joined before the real-space sum into F
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces add into F after both branches are done.
 //
@@ -69,8 +71,8 @@ static BatchItem item_of(const StepPlan &sp) {
 // A stream pair for one system's step: s0 carries the step, s1 the
 // recursion branch (fork / join events between them).
 struct Lane {
-  hipStream_t s0 = nullptr, s1 = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, fork2 = nullptr, join2 = nullptr;
 };
 
 static Lane main_lane() {
@@ -78,8 +80,11 @@ static Lane main_lane() {
   Lane L;
   L.s0 = R.s0;
   L.s1 = R.s1;
+  L.s2 = R.s2;
   L.fork = R.fork;
   L.join = R.join;
+  L.fork2 = R.fork2;
+  L.join2 = R.join2;
   return L;
 }
 
@@ -113,7 +118,14 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     if (launch_ring_springs(p, sp.image, sp.L, R.s0)) return -1;
   }
   if (rec(7, R.s0)) return -1;
-  if (sp.pair_kind >= 0) {
+  // The reciprocal structure factors overlap the real-space kernel on s2
+  // (one uses the matrix cores, the other the vector ALUs).
+  const int overlapped =
+      (!serial && sp.pair_kind >= 0) ? launch_pair_force_overlap(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
+  if (overlapped < 0) return -1;
+  if (overlapped) {
+    if (rec(8, R.s0) || rec(9, R.s0) || rec(10, R.s0) || rec(11, R.s0)) return -1;
+  } else if (sp.pair_kind >= 0) {
     if (rec(8, R.s0) || launch_pair_real(p, sp.pair_kind, R.s0) || rec(9, R.s0)) return -1;
     if (pair_has_recip(sp.pair_kind)) {
       if (rec(10, R.s0) || launch_pair_recip(p, sp.pair_kind, R.s0) || rec(11, R.s0)) return -1;

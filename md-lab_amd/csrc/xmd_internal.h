@@ -54,7 +54,8 @@ struct Runtime {
   int device = 0;
   hipStream_t s0 = nullptr;   // main stream (all API work is ordered on it)
   hipStream_t s1 = nullptr;   // exchange branch of the fused step
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipStream_t s2 = nullptr;   // reciprocal-space branch of the fused step
+  hipEvent_t fork = nullptr, join = nullptr, fork2 = nullptr, join2 = nullptr;
   int num_cus = 0;
 };
 Runtime &rt();
@@ -143,6 +144,13 @@ extern "C" md_num_t_x md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C
 int pair_kind_of(md_pair_force_t_x pf);
 int pair_energy_kind_of(md_pair_energy_t_x pe);
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st);
+// Pair forces with the reciprocal structure factors on s2 (forked from st
+// through `fork`, joined back through `join`) while the real-space kernel runs
+// on st; F receives the real-space then the reciprocal part, as in
+// launch_pair_force.  Returns 1 when done, 0 when the kind has no such split
+// (nothing launched), -1 on error.
+int launch_pair_force_overlap(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork,
+                              hipEvent_t join);
 int launch_pair_real(DevPimd *p, int kind, hipStream_t st);
 int launch_pair_recip(DevPimd *p, int kind, hipStream_t st);
 bool pair_has_recip(int kind);
