@@ -1437,8 +1437,7 @@ int launch_pair_recip(DevPimd *p, int kind, hipStream_t st) {
   return launch_recip(p, pp, pp.nsum, 0.0, st, true, nullptr);
 }
 
-int launch_pair_force_overlap(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork,
-                              hipEvent_t join) {
+int launch_pair_kspace_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join) {
   if (kind != PK_EWALD || !p->ds->uniform_mass) return 0;
   EwaldPlan pl;
   PairParams pp = ewald_params(p, pl);
@@ -1448,14 +1447,21 @@ int launch_pair_force_overlap(DevPimd *p, int kind, hipStream_t st, hipStream_t 
   KSpace *K = nullptr;
   if (launch_recip_sk(p, pp, pl.cube_n, pl.gmax2, s2, &K)) return -1;
   XMD_CHECK(hipEventRecord(join, s2));
+  return 1;
+}
+
+int launch_pair_force_joined(DevPimd *p, int kind, hipStream_t st, hipEvent_t join) {
+  EwaldPlan pl;
+  PairParams pp = ewald_params(p, pl);
   if (launch_ewald_real_tab(p, pp, st, false)) return -1;
   XMD_CHECK(hipStreamWaitEvent(st, join, 0));
   TileBufs &B = pair_bufs(p)->tb;
   const int n3 = 3 * p->ds->nb;
   hipLaunchKernelGGL(k_reduce_tiles, dim3((n3 + 255) / 256), dim3(256), 0, st, B.Fpart, p->ds->F, n3, B.T, pp.P);
   XMD_CHECK(hipGetLastError());
-  if (launch_recip_force(p, pp, K, st)) return -1;
-  return 1;
+  KSpace *K = ensure_kspace(p, pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, p->ds->nb, p->P);
+  if (!K) return -1;
+  return launch_recip_force(p, pp, K, st);
 }
 
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st) {

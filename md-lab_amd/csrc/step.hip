@@ -7,8 +7,8 @@
 // With massive thermostats NHC half 1 and the wrap are one kernel, and so are
 // the force reset, trap and interior springs.
 //   s1: (after fork) forward + backward recursions --join-->
-//   s2: (Ewald) phase tables and structure factors, forked from s0 before
-//       the real-space kernel and joined before the real-space sum into F
+//   s2: (Ewald) phase tables and structure factors, forked from s0 after
+//       NHC half 1 and joined before the real-space sum into F
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces add into F after both branches are done.
 //
@@ -98,9 +98,14 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     if (ev) XMD_CHECK(hipEventRecord(ev[k], st));
     return 0;
   };
-  // Cycle energies (short, position-only) first; then the two recursions are
-  // forked onto s1 before the pair kernels are queued, so their two
-  // workgroups are resident before the pair grid fills the device.
+  // The reciprocal structure factors (positions only) fork onto s2 first and
+  // overlap the cycle energies and the real-space kernel.
+  const int kfork =
+      (!serial && sp.pair_kind >= 0) ? launch_pair_kspace_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
+  if (kfork < 0) return -1;
+  // Cycle energies (short, position-only); then the two recursions are
+  // forked onto s1 before the pair kernels are queued, so their workgroups
+  // are resident before the pair grid fills the device.
   if (rec(2, R.s0) || launch_enk(p, sp.image, R.s0) || rec(3, R.s0)) return -1;
   if (!serial) {
     XMD_CHECK(hipEventRecord(R.fork, R.s0));
@@ -118,12 +123,8 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     if (launch_ring_springs(p, sp.image, sp.L, R.s0)) return -1;
   }
   if (rec(7, R.s0)) return -1;
-  // The reciprocal structure factors overlap the real-space kernel on s2
-  // (one uses the matrix cores, the other the vector ALUs).
-  const int overlapped =
-      (!serial && sp.pair_kind >= 0) ? launch_pair_force_overlap(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
-  if (overlapped < 0) return -1;
-  if (overlapped) {
+  if (kfork) {
+    if (launch_pair_force_joined(p, sp.pair_kind, R.s0, R.join2)) return -1;
     if (rec(8, R.s0) || rec(9, R.s0) || rec(10, R.s0) || rec(11, R.s0)) return -1;
   } else if (sp.pair_kind >= 0) {
     if (rec(8, R.s0) || launch_pair_real(p, sp.pair_kind, R.s0) || rec(9, R.s0)) return -1;

@@ -144,13 +144,16 @@ extern "C" md_num_t_x md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C
 int pair_kind_of(md_pair_force_t_x pf);
 int pair_energy_kind_of(md_pair_energy_t_x pe);
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st);
-// Pair forces with the reciprocal structure factors on s2 (forked from st
-// through `fork`, joined back through `join`) while the real-space kernel runs
-// on st; F receives the real-space then the reciprocal part, as in
-// launch_pair_force.  Returns 1 when done, 0 when the kind has no such split
-// (nothing launched), -1 on error.
-int launch_pair_force_overlap(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork,
-                              hipEvent_t join);
+// Pair forces in two parts, with the reciprocal structure factors on a side
+// stream: launch_pair_kspace_fork forks s2 from st (through `fork`) and
+// queues the phase tables and structure factors there (they read positions
+// only), recording `join`; launch_pair_force_joined later queues the
+// real-space kernel on st, waits for `join`, and adds the real-space then the
+// reciprocal forces into F, as launch_pair_force does.  The fork returns 1
+// when it applies (Ewald with the table kernels), 0 when nothing was launched
+// (use launch_pair_force), -1 on error.
+int launch_pair_kspace_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
+int launch_pair_force_joined(DevPimd *p, int kind, hipStream_t st, hipEvent_t join);
 int launch_pair_real(DevPimd *p, int kind, hipStream_t st);
 int launch_pair_recip(DevPimd *p, int kind, hipStream_t st);
 bool pair_has_recip(int kind);
