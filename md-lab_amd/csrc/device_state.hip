@@ -284,6 +284,7 @@ static int alloc_pimd(DevPimd *p) {
   XMD_CHECK(hipMalloc(&p->Lf, (N + 1) * sizeof(double)));
   XMD_CHECK(hipMalloc(&p->Rb, (N + 1) * sizeof(double)));
   XMD_CHECK(hipMalloc(&p->work, 8 * (N + 1) * sizeof(double) + 64 * 1024 * sizeof(double)));
+  XMD_CHECK(hipMalloc(&p->Fx, 6 * N * sizeof(double)));
   // recursion state: values and accumulators 6(N+1), then per direction and
   // 64-row block the in-block (64*63/2) and previous-block (64*64) coefficients
   const size_t nblk = (N + 63) / 64;

@@ -655,9 +655,11 @@ __global__ void k_vbn_to_L(const double *VBN, double *Lf, int N, double beta) {
 
 // Block per particle l (1-based l = blockIdx.x + 1): bead P against every
 // bead 1 (row l of G) and bead 1 against every bead P (column l of G).
+// Fx null: the forces are added into F.  Otherwise they are stored in Fx
+// ([slice P-1, slice 0][dimension][particle]) for k_add_exch to add later.
 __device__ __forceinline__ void exch_force_body(int bx, const double *__restrict__ X, const double *__restrict__ M,
                                                 double *__restrict__ F, int N, int P, double om, Geo g,
-                                                const RecArgs &a) {
+                                                const RecArgs &a, double *__restrict__ Fx = nullptr) {
   __shared__ double red[8][256];
   const int l = bx + 1;
   const int nb = N * P;
@@ -717,18 +719,33 @@ __device__ __forceinline__ void exch_force_body(int bx, const double *__restrict
     for (int d = 0; d < 3; ++d) {
       double aP = comp(xP[d] - X[d * nb + sPm], Ls[d], g.image);
       double gradP = (mP * om * om) * (red[0][0] * aP) + (mP * om * om) * red[1 + d][0];
-      F[d * nb + sP] += -gradP / mP;
       double a1 = comp(x1[d] - X[d * nb + s2], Ls[d], g.image);
       double grad1 = (m1 * om * om) * (red[4][0] * a1) + (m1 * om * om) * red[5 + d][0];
-      F[d * nb + s1] += -grad1 / m1;
+      if (Fx) {
+        Fx[d * N + (l - 1)] = -gradP / mP;
+        Fx[(3 + d) * N + (l - 1)] = -grad1 / m1;
+      } else {
+        F[d * nb + sP] += -gradP / mP;
+        F[d * nb + s1] += -grad1 / m1;
+      }
     }
   }
 }
 
 __global__ __launch_bounds__(256) void k_exch_force(const double *__restrict__ X, const double *__restrict__ M,
                                                     double *__restrict__ F, int N, int P, double om, Geo g,
-                                                    RecArgs a) {
-  exch_force_body(blockIdx.x, X, M, F, N, P, om, g, a);
+                                                    RecArgs a, double *__restrict__ Fx) {
+  exch_force_body(blockIdx.x, X, M, F, N, P, om, g, a, Fx);
+}
+
+// F of the end beads (slices P-1 and 0, P >= 2) += the stored exchange forces.
+__global__ __launch_bounds__(256) void k_add_exch(double *__restrict__ F, const double *__restrict__ Fx, int N, int P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * N) return;
+  const int d = t / N, l = t - d * N;
+  const size_t nb = (size_t)N * P;
+  F[d * nb + (size_t)(P - 1) * N + l] += Fx[t];
+  F[d * nb + l] += Fx[3 * N + t];
 }
 
 // Optional full connection matrix (tests): G[(l-1)*N + (j-1)].
@@ -1125,11 +1142,18 @@ int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
   return 0;
 }
 
-int launch_exchange_force(DevPimd *p, int image, hipStream_t st) {
+int launch_exchange_force(DevPimd *p, int image, hipStream_t st, bool stored) {
   DevSim *d = p->ds;
   Geo g = geo_of(p, image);
   RecArgs a = rec_args(p);
-  hipLaunchKernelGGL(k_exch_force, dim3(p->N), dim3(256), 0, st, d->X, d->M, d->F, p->N, p->P, p->host->omegaP, g, a);
+  hipLaunchKernelGGL(k_exch_force, dim3(p->N), dim3(256), 0, st, d->X, d->M, d->F, p->N, p->P, p->host->omegaP, g, a,
+                     stored ? p->Fx : nullptr);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_add_exchange_force(DevPimd *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_exch, dim3((3 * p->N + 255) / 256), dim3(256), 0, st, p->ds->F, p->Fx, p->N, p->P);
   XMD_CHECK(hipGetLastError());
   return 0;
 }
@@ -1509,7 +1533,7 @@ int md_pimd_fast_fill_force_VB_x(md_pimd_t_x *pimd, int image) {
   md_simulation_t_x *sim = pimd->sim;
   double L[3] = {box_L(sim, 0), box_L(sim, 1), box_L(sim, 2)};
   if (launch_ring_springs(p, image, L, rt().s0)) return -1;
-  return launch_exchange_force(p, image, rt().s0);
+  return launch_exchange_force(p, image, rt().s0, false);
 }
 
 // The forward-mode gradient of md_pimd_x.c:997-1109 gives the same forces

@@ -2,15 +2,17 @@
 //
 // Stream layout of one step (the reference driver's order of stages):
 //   s0: NHC half 1 (+ wrap), cycle energies --fork--> reset F, trap, ring
-//       springs, pair real, pair reciprocal --wait join--> exchange end-bead
-//       forces, NHC half 2
+//       springs, pair real, pair reciprocal --wait join--> add the exchange
+//       end-bead forces, NHC half 2
 // With massive thermostats NHC half 1 and the wrap are one kernel, and so are
 // the force reset, trap and interior springs.
-//   s1: (after fork) forward + backward recursions --join-->
+//   s1: (after fork) forward + backward recursions, end-bead exchange forces
+//       into a side buffer --join--> (added into F on s0)
 //   s2: (Ewald) phase tables and structure factors, forked from s0 after
 //       NHC half 1 and joined before the real-space sum into F
 // The recursions only read cycle energies, so they overlap the pair forces;
-// the end-bead exchange forces add into F after both branches are done.
+// the end-bead exchange forces are added into F after both branches are done
+// (the serial profiling order computes and adds them there in one kernel).
 //
 // Batches of independent replicas (md_pimd_run_batch_x) run the same stages
 // with one launch per stage over a packed table of all replicas, captured
@@ -112,6 +114,9 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     XMD_CHECK(hipStreamWaitEvent(R.s1, R.fork, 0));
   }
   if (rec(4, sx) || launch_recursion(p, true, true, sx) || rec(5, sx)) return -1;
+  // overlapped: the end-bead exchange forces follow the recursions on s1
+  // into a side buffer, added into F after the join
+  if (!serial && launch_exchange_force(p, sp.image, R.s1, true)) return -1;
   if (!serial) XMD_CHECK(hipEventRecord(R.join, R.s1));
   // bead-local and pair forces
   if (rec(6, R.s0)) return -1;
@@ -136,8 +141,12 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   } else if (rec(8, R.s0) || rec(9, R.s0) || rec(10, R.s0) || rec(11, R.s0)) {
     return -1;
   }
-  if (!serial) XMD_CHECK(hipStreamWaitEvent(R.s0, R.join, 0));
-  if (rec(12, R.s0) || launch_exchange_force(p, sp.image, R.s0) || rec(13, R.s0)) return -1;
+  if (!serial) {
+    XMD_CHECK(hipStreamWaitEvent(R.s0, R.join, 0));
+    if (rec(12, R.s0) || launch_add_exchange_force(p, R.s0) || rec(13, R.s0)) return -1;
+  } else if (rec(12, R.s0) || launch_exchange_force(p, sp.image, R.s0, false) || rec(13, R.s0)) {
+    return -1;
+  }
   return 0;
 }
 

@@ -101,6 +101,7 @@ struct DevPimd {
   double *Rb = nullptr;     // backward R(m) = -beta Vt^(m),  N+1
   double *Gm = nullptr;     // optional connection matrix (debug/tests), N*N
   double *work = nullptr;   // estimator scratch
+  double *Fx = nullptr;     // exchange forces of the end beads, [slice P-1, slice 0][dim][particle] (6N)
   double *rec = nullptr;    // recursion state: values, accumulators, block coefficients
   int *recsync = nullptr;   // recursion hand-off flags: 4*nblk + 1
   void *pair_bufs = nullptr;  // pair-kernel tile schedule, partial sums and k-space tables (pair_forces.hip)
@@ -162,7 +163,10 @@ int pair_energy(DevPimd *p, int kind, double *out);
 int launch_enk(DevPimd *p, int image, hipStream_t st);
 int init_lnn(DevPimd *p);  // ln n table at p->work[0..N], once per object
 int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st);
-int launch_exchange_force(DevPimd *p, int image, hipStream_t st);
+// stored: write the end-bead forces to p->Fx instead of adding them to F;
+// launch_add_exchange_force adds them later (both on the caller's ordering).
+int launch_exchange_force(DevPimd *p, int image, hipStream_t st, bool stored);
+int launch_add_exchange_force(DevPimd *p, hipStream_t st);
 int connection_matrix(DevPimd *p, double *host_G);
 int backward_potentials(DevPimd *p, double *host_Vt);
 int bead_sum(DevSim *d, int mode, const double *trap_c, double w, const double *L, double *out);
