@@ -114,14 +114,15 @@ __device__ __forceinline__ void chain_body(int bx, const double *__restrict__ X,
   }
   part[wv][lane] = e;
   __syncthreads();
-  if (wv != 0 || l >= N) return;
-  e = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-  Eint[l] = e;
-  const int sP = (P - 1) * N + l;
-  const double e1 = e + spring_e(X, nb, sP, l, M[sP], om, g);
-  ENk[tri_row(l + 1)] = e1;
-  ET[col_off(l, N)] = e1;
-  Clink[l] = (l < N - 1) ? e + spring_e(X, nb, sP, l + 1, M[sP], om, g) : 0.0;
+  if (wv == 0 && l < N) {
+    e = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    Eint[l] = e;
+    const int sP = (P - 1) * N + l;
+    const double e1 = e + spring_e(X, nb, sP, l, M[sP], om, g);
+    ENk[tri_row(l + 1)] = e1;
+    ET[col_off(l, N)] = e1;
+    Clink[l] = (l < N - 1) ? e + spring_e(X, nb, sP, l + 1, M[sP], om, g) : 0.0;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_chain(const double *__restrict__ X, const double *__restrict__ M, int N,
@@ -131,18 +132,20 @@ __global__ __launch_bounds__(256) void k_chain(const double *__restrict__ X, con
   chain_body(blockIdx.x, X, M, N, P, om, g, Eint, Clink, ENk, ET);
 }
 
-// Exclusive double-double prefix sum of in[0..n-1] into pre[2*i] (hi, lo).
+// Exclusive double-double prefix sum of in[0..n-1] into pre[2*i] (hi, lo)
+// by one workgroup of NT threads.
+template <int NT = 256>
 __device__ __forceinline__ void scan_dd_body(const double *__restrict__ in, int n, double *__restrict__ pre) {
-  __shared__ double sh[1024], sl[1024];
+  __shared__ double sh[NT], sl[NT];
   const int t = threadIdx.x;
-  const int seg = (n + 1023) / 1024;
+  const int seg = (n + NT - 1) / NT;
   const int a = min(n, t * seg), b = min(n, a + seg);
   dd tot{0, 0};
   for (int i = a; i < b; ++i) tot = dd_add(tot, dd{in[i], 0});
   sh[t] = tot.hi;
   sl[t] = tot.lo;
   __syncthreads();
-  for (int w = 1; w < 1024; w <<= 1) {
+  for (int w = 1; w < NT; w <<= 1) {
     dd v{sh[t], sl[t]};
     if (t >= w) v = dd_add(v, dd{sh[t - w], sl[t - w]});
     __syncthreads();
@@ -158,8 +161,28 @@ __device__ __forceinline__ void scan_dd_body(const double *__restrict__ in, int 
   }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_dd(const double *__restrict__ in, int n, double *__restrict__ pre) {
+__global__ __launch_bounds__(256) void k_scan_dd(const double *__restrict__ in, int n, double *__restrict__ pre) {
   scan_dd_body(in, n, pre);
+}
+
+// k_chain and k_scan_dd in one launch: the last workgroup to finish its
+// particles (counted in *done, which it resets) scans the links.
+__global__ __launch_bounds__(256) void k_chain_scan(const double *__restrict__ X, const double *__restrict__ M, int N,
+                                                    int P, double om, Geo g, double *__restrict__ Eint,
+                                                    double *__restrict__ Clink, double *__restrict__ ENk,
+                                                    double *__restrict__ ET, double *__restrict__ pre,
+                                                    int *__restrict__ done) {
+  __shared__ int last;
+  chain_body(blockIdx.x, X, M, N, P, om, g, Eint, Clink, ENk, ET);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    last = __hip_atomic_fetch_add(done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  scan_dd_body(Clink, N, pre);
+  if (threadIdx.x == 0) __hip_atomic_store(done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Row n' (0-based) of the cycle-energy triangle, k >= 2, both layouts.
@@ -186,6 +209,25 @@ __global__ __launch_bounds__(256) void k_enk_fill(const double *__restrict__ X, 
                                                   const double *__restrict__ pre, double *__restrict__ ENk,
                                                   double *__restrict__ ET) {
   enk_fill_body(blockIdx.x, X, M, N, P, om, g, Eint, pre, ENk, ET);
+}
+
+// Positions, masses and prefix sums from which a cycle energy is computed
+// directly, with the arithmetic of k_chain (k = 1) and k_enk_fill (k >= 2).
+struct EnkArgs {
+  const double *X, *M, *Eint, *pre;
+  double om;
+  Geo g;
+  int N, P;
+};
+
+// E of the cycle of particles m0 .. r (0-based, m0 <= r).
+__device__ __forceinline__ double E_cycle(const EnkArgs &e, int m0, int r) {
+  const int nb = e.N * e.P;
+  const int sP = (e.P - 1) * e.N + r;
+  const double mP = e.M[sP], ein = e.Eint[r];
+  if (m0 == r) return ein + spring_e(e.X, nb, sP, r, mP, e.om, e.g);
+  const double chain = dd_sub_to_double(dd{e.pre[2 * r], e.pre[2 * r + 1]}, dd{e.pre[2 * m0], e.pre[2 * m0 + 1]});
+  return chain + ein + spring_e(e.X, nb, sP, m0, mP, e.om, e.g);
 }
 
 // Host-provided ENk (row layout) -> column layout.
@@ -235,6 +277,17 @@ __device__ __forceinline__ double rec_coef(const RecArgs &a, int p, int pp) {
   return (k - 1) * a.lnxi - a.beta * E - ln_n;
 }
 
+// rec_coef with E computed from positions and prefix sums (the value the
+// tables hold) instead of read from them.
+template <bool FWD>
+__device__ __forceinline__ double rec_coef_direct(const RecArgs &a, const EnkArgs &e, int p, int pp) {
+  const int k = p - pp;
+  if (a.xi0 && k != 1) return NEG;
+  const double E = FWD ? E_cycle(e, pp, p - 1) : E_cycle(e, a.N - p, a.N - pp - 1);
+  const double ln_n = FWD ? a.lnn[p] : a.lnn[a.N - pp];
+  return (k - 1) * a.lnxi - a.beta * E - ln_n;
+}
+
 // ---- blocked right-looking solve (positions in blocks of RB rows) ----
 //
 // Direction dir (0 forward, 1 backward) keeps, per position p, the solved
@@ -279,8 +332,15 @@ __device__ __forceinline__ void tri_decode(int w, int &q, int &r) {
   r = w - q * (q - 1) / 2;
 }
 
-template <bool FWD>
-__device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t) {
+// DIRECT: coefficients from positions and prefix sums (rec_coef_direct, e
+// used) instead of the cycle-energy tables, so this may run beside the
+// kernel that fills them.
+template <bool FWD, bool DIRECT = false>
+__device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t, const EnkArgs &e = EnkArgs{}) {
+  auto coef = [&](int p, int pp) { return DIRECT ? rec_coef_direct<FWD>(a, e, p, pp) : rec_coef<FWD>(a, p, pp); };
+  auto coef_col = [&](int p, int pp) {
+    return DIRECT ? rec_coef_direct<FWD>(a, e, p, pp) : rec_coef_col<FWD>(a, p, pp);
+  };
   const int N = a.N;
   if (t <= N) {
     double *val = a.val2 + (size_t)dir * (N + 1);
@@ -294,7 +354,7 @@ __device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t) {
         a.Rb[N] = 0.0;
       }
     } else {
-      const double c = rec_coef<FWD>(a, t, 0);
+      const double c = coef(t, 0);
       AM[t] = c > NEG ? c : NEG;
       AS[t] = c > NEG ? 1.0 : 0.0;
     }
@@ -305,13 +365,13 @@ __device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t) {
     int q, r;
     tri_decode(w, q, r);
     const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
-    a.dg[(size_t)dir * ndg + t] = q < nrows ? rec_coef<FWD>(a, p0 + q, p0 + r) : NEG;
+    a.dg[(size_t)dir * ndg + t] = q < nrows ? coef(p0 + q, p0 + r) : NEG;
   }
   const int nla = a.nblk * RB * RB;
   if (t < nla) {  // transposed: [b][r][q]
     const int b = t / (RB * RB), w = t - b * (RB * RB), r = w / RB, q = w - r * RB;
     const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
-    a.la[(size_t)dir * nla + t] = (b > 0 && q < nrows) ? rec_coef_col<FWD>(a, p0 + q, p0 - RB + r) : NEG;
+    a.la[(size_t)dir * nla + t] = (b > 0 && q < nrows) ? coef_col(p0 + q, p0 - RB + r) : NEG;
   }
 }
 
@@ -325,6 +385,24 @@ __device__ __forceinline__ void rec_prep_body(const RecArgs &a, int dir, int fir
 
 __global__ void k_rec_prep(RecArgs a, int dir0) {
   rec_prep_body(a, dir0 + blockIdx.y, blockIdx.y == 0, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// k_enk_fill (workgroups 0 .. N-1) and k_rec_prep of both directions (the
+// next 2 * nprepb workgroups, coefficients computed directly) in one launch.
+__global__ __launch_bounds__(256) void k_enk_prep(EnkArgs e, double *__restrict__ ENk, double *__restrict__ ET,
+                                                  RecArgs a, int nprepb) {
+  const int bx = blockIdx.x;
+  if (bx < e.N) {
+    enk_fill_body(bx, e.X, e.M, e.N, e.P, e.om, e.g, e.Eint, e.pre, ENk, ET);
+    return;
+  }
+  const int w = bx - e.N, dir = w / nprepb;
+  const int t = (w - dir * nprepb) * blockDim.x + threadIdx.x;
+  if (dir == 0 && t < 4 * a.nblk + 1) a.sync[t] = 0;
+  if (dir == 0)
+    rec_prep_one<true, true>(a, 0, t, e);
+  else
+    rec_prep_one<false, true>(a, 1, t, e);
 }
 
 // (M, S) <- log-sum-exp merge with (m, s).
@@ -1002,7 +1080,7 @@ __global__ __launch_bounds__(256) void kb_chain(const ExRep *__restrict__ reps, 
 }
 
 // one 1024-thread block per replica
-__global__ __launch_bounds__(1024) void kb_scan_dd(const ExRep *__restrict__ reps) {
+__global__ __launch_bounds__(256) void kb_scan_dd(const ExRep *__restrict__ reps) {
   const ExRep &R = reps[blockIdx.x];
   scan_dd_body(R.clink, R.N, R.Cpre);
 }
@@ -1094,7 +1172,7 @@ int launch_enk(DevPimd *p, int image, hipStream_t st) {
   hipLaunchKernelGGL(k_chain, dim3((N + 63) / 64), dim3(256), 0, st, d->X, d->M, N, P, om, g, p->Eint, clink,
                      p->ENk, p->ET);
   XMD_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_dd, dim3(1), dim3(1024), 0, st, clink, N, p->Cpre);
+  hipLaunchKernelGGL(k_scan_dd, dim3(1), dim3(256), 0, st, clink, N, p->Cpre);
   XMD_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_enk_fill, dim3(N), dim3(256), 0, st, d->X, d->M, N, P, om, g, p->Eint, p->Cpre, p->ENk, p->ET);
   XMD_CHECK(hipGetLastError());
@@ -1118,6 +1196,44 @@ int recursion_status(DevPimd *p) {
     fprintf(stderr, "xmd: exchange recursion hand-off timed out\n");
     return -1;
   }
+  return 0;
+}
+
+int launch_enk_prep(DevPimd *p, int image, hipStream_t st) {
+  DevSim *d = p->ds;
+  const int N = p->N, P = p->P;
+  EnkArgs e;
+  e.X = d->X;
+  e.M = d->M;
+  e.Eint = p->Eint;
+  e.pre = p->Cpre;
+  e.om = p->host->omegaP;
+  e.g = geo_of(p, image);
+  e.N = N;
+  e.P = P;
+  double *clink = p->work + (N + 1);
+  hipLaunchKernelGGL(k_chain_scan, dim3((N + 63) / 64), dim3(256), 0, st, d->X, d->M, N, P, e.om, e.g, p->Eint, clink,
+                     p->ENk, p->ET, p->Cpre, p->done);
+  XMD_CHECK(hipGetLastError());
+  RecArgs a = rec_args(p);
+  const int nprepb = (std::max(N + 1, a.nblk * RB * RB) + 255) / 256;
+  hipLaunchKernelGGL(k_enk_prep, dim3(N + 2 * nprepb), dim3(256), 0, st, e, p->ENk, p->ET, a, nprepb);
+  XMD_CHECK(hipGetLastError());
+  p->enk_dev = true;
+  p->last_image = image;
+  p->enk_version++;
+  return 0;
+}
+
+int launch_recursion_solve(DevPimd *p, hipStream_t st) {
+  RecArgs a = rec_args(p);
+  const int per = 1 + std::max(0, a.nblk - 2);
+  hipLaunchKernelGGL(k_rec_persist, dim3(2 * per), dim3(256), 0, st, a, 0);
+  XMD_CHECK(hipGetLastError());
+  g_rec_last = p;
+  p->vbn_dev = true;
+  p->back_dev = true;
+  p->back_version = p->enk_version;
   return 0;
 }
 
@@ -1222,7 +1338,7 @@ void exch_batch_free(ExchBatch *b) {
 int exch_batch_enk(ExchBatch *b, hipStream_t st) {
   hipLaunchKernelGGL(kb_chain, dim3(b->chain.total), dim3(256), 0, st, (const ExRep *)b->chain.reps, b->chain.start,
                      b->nrep);
-  hipLaunchKernelGGL(kb_scan_dd, dim3(b->nrep), dim3(1024), 0, st, (const ExRep *)b->chain.reps);
+  hipLaunchKernelGGL(kb_scan_dd, dim3(b->nrep), dim3(256), 0, st, (const ExRep *)b->chain.reps);
   hipLaunchKernelGGL(kb_enk_fill, dim3(b->enk.total), dim3(256), 0, st, (const ExRep *)b->enk.reps, b->enk.start,
                      b->nrep);
   XMD_CHECK(hipGetLastError());
@@ -1313,7 +1429,7 @@ static int fill_enk2(DevPimd *p, int image, int forward) {
   double *link = p->work + 2 * (N + 1);
   hipLaunchKernelGGL(k_chain2, dim3((N + 255) / 256), dim3(256), 0, st, d->X, d->M, N, P, om, g, forward, link);
   XMD_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_dd, dim3(1), dim3(1024), 0, st, link, N, p->C2pre);
+  hipLaunchKernelGGL(k_scan_dd, dim3(1), dim3(256), 0, st, link, N, p->C2pre);
   XMD_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_enk2_fill, dim3(N), dim3(256), 0, st, d->X, d->M, N, P, om, g, forward, p->C2pre, p->ENk2);
   XMD_CHECK(hipGetLastError());

@@ -108,12 +108,15 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   // Cycle energies (short, position-only); then the two recursions are
   // forked onto s1 before the pair kernels are queued, so their workgroups
   // are resident before the pair grid fills the device.
-  if (rec(2, R.s0) || launch_enk(p, sp.image, R.s0) || rec(3, R.s0)) return -1;
+  if (rec(2, R.s0) || (serial ? launch_enk(p, sp.image, R.s0) : launch_enk_prep(p, sp.image, R.s0)) ||
+      rec(3, R.s0))
+    return -1;
   if (!serial) {
     XMD_CHECK(hipEventRecord(R.fork, R.s0));
     XMD_CHECK(hipStreamWaitEvent(R.s1, R.fork, 0));
   }
-  if (rec(4, sx) || launch_recursion(p, true, true, sx) || rec(5, sx)) return -1;
+  if (rec(4, sx) || (serial ? launch_recursion(p, true, true, sx) : launch_recursion_solve(p, sx)) || rec(5, sx))
+    return -1;
   // overlapped: the end-bead exchange forces follow the recursions on s1
   // into a side buffer, added into F after the join
   if (!serial && launch_exchange_force(p, sp.image, R.s1, true)) return -1;

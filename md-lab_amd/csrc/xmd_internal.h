@@ -104,6 +104,7 @@ struct DevPimd {
   double *Fx = nullptr;     // exchange forces of the end beads, [slice P-1, slice 0][dim][particle] (6N)
   double *rec = nullptr;    // recursion state: values, accumulators, block coefficients
   int *recsync = nullptr;   // recursion hand-off flags: 4*nblk + 1
+  int *done = nullptr;      // finished-workgroup count of k_chain_scan (zero between launches)
   void *pair_bufs = nullptr;  // pair-kernel tile schedule, partial sums and k-space tables (pair_forces.hip)
   bool enk_dev = false, enk2_dev = false, vbn_dev = false, back_dev = false;
   bool enk_host_stale = false;
@@ -161,6 +162,11 @@ bool pair_has_recip(int kind);
 int pair_energy(DevPimd *p, int kind, double *out);
 
 int launch_enk(DevPimd *p, int image, hipStream_t st);
+// launch_enk plus the recursion set-up of both directions in two kernels;
+// launch_recursion_solve then runs both recursions (together: launch_enk
+// followed by launch_recursion(p, true, true)).
+int launch_enk_prep(DevPimd *p, int image, hipStream_t st);
+int launch_recursion_solve(DevPimd *p, hipStream_t st);
 int init_lnn(DevPimd *p);  // ln n table at p->work[0..N], once per object
 int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st);
 // stored: write the end-bead forces to p->Fx instead of adding them to F;
