@@ -179,6 +179,8 @@ def main():
                     help="steps of the batched-replica measurement (0 skips it)")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched-replica measurement (for profiling it)")
+    ap.add_argument("--serial-only", action="store_true",
+                    help="only run --steps steps with every stage alone on one stream (for per-kernel counters)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -208,6 +210,14 @@ def main():
     T = pyxmd.ueg_temperature(N, L, theta)
     s = pyxmd.System(lib, "ueg", N, P, T, xi, seed=1, L=L, nsum=nsum)
     cfg = s.step_config(dt, use_graph=1)
+    names = ["nhc1_wrap", "cycle_energies", "vb_recursions", "bead_forces", "ewald_real", "ewald_recip",
+             "exchange_forces", "nhc2", "step"]
+    if args.serial_only:
+        stages = np.zeros(9)
+        assert lib.md_pimd_profile_steps_x(s.pimd, C.byref(cfg), args.steps, 1, pyxmd.dptr(stages)) == 0
+        print(json.dumps({"serial_only": True, "steps": args.steps,
+                          "stages_ms": {k: round(float(v), 5) for k, v in zip(names, stages)}}))
+        return
     assert lib.md_pimd_compute_forces_x(s.pimd, C.byref(cfg)) == 0
     assert lib.md_pimd_run_steps_x(s.pimd, C.byref(cfg), args.warmup) == 0
     assert lib.md_hip_synchronize_x() == 0
@@ -222,8 +232,6 @@ def main():
     # per-stage device times, every stage alone on one stream (HIP events)
     stages = np.zeros(9)
     assert lib.md_pimd_profile_steps_x(s.pimd, C.byref(cfg), 5, 1, pyxmd.dptr(stages)) == 0
-    names = ["nhc1_wrap", "cycle_energies", "vb_recursions", "bead_forces", "ewald_real", "ewald_recip",
-             "exchange_forces", "nhc2", "step"]
     real_ms = stages[4]
     a_, rc_, nk_ = C.c_double(), C.c_double(), C.c_int()
     assert lib.md_pimd_get_ewald_plan_x(s.pimd, C.byref(a_), C.byref(rc_), C.byref(nk_)) == 0
