@@ -20,6 +20,7 @@
 //    k_sk_fin, k_recip_force, k_recip_energy).
 //  * When the reference's own Ewald parameters are converged, the same sum is
 //    evaluated with the engine's split (plan_ewald).
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <vector>
@@ -443,7 +444,7 @@ struct KSpace {
   double L[3] = {0, 0, 0}, alpha = 0, q = 0, gmax2 = 0;
   int n = -2;  // cube half-width, or -1 for the cylinder of radius^2 gmax2
   int MZ = 0, ZP = 0, ncol = 0, ncolp = 0, nk = 0;  // nk: k vectors with nonzero weight
-  int2 *cols = nullptr;    // [ncolp] (mx, my)
+  int2 *cols = nullptr;    // [ncolp] (mx, my), then [ncolp / 16] K-step range of each column tile
   double *coef = nullptr;  // [ncolp][ZP]
   double2 *tx = nullptr, *ty = nullptr, *tz = nullptr, *S = nullptr, *W = nullptr;
   size_t tx_cap = 0, ty_cap = 0, tz_cap = 0, s_cap = 0, w_cap = 0;
@@ -512,8 +513,17 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
       return nullptr;
     }
     const int ZP = 2 * MZ + 2;
-    std::vector<int2> cols;
-    std::vector<double> coef;
+    // Columns (mx, my) of the half space, each with its m_z slots: the cube
+    // |m| <= n, or the ball |G|^2 <= gmax2.  Columns are ordered by |G_xy| so
+    // that a tile of 16 has a narrow m_z range; the slots outside it hold
+    // zero weights, and the force kernel skips the K steps that are zero for
+    // the whole tile.
+    struct Col {
+      int mx, my, zlo, zhi;
+      double gxy2;
+      std::vector<double> c;
+    };
+    std::vector<Col> cl;
     const double V = L[0] * L[1] * L[2];
     int nk = 0;
     for (int mx = 0; mx <= M; ++mx)
@@ -521,25 +531,42 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
         if (mx == 0 && my < 0) continue;
         const double gxy2 = (mx / L[0]) * (mx / L[0]) + (my / L[1]) * (my / L[1]);
         if (n < 0 && gxy2 > gmax2) continue;
-        cols.push_back(make_int2(mx, my));
+        Col col{mx, my, ZP, -1, gxy2, std::vector<double>(ZP, 0.0)};
         for (int z = 0; z < ZP; ++z) {
           const int mz = z - MZ;
-          bool in = mz <= MZ && (n < 0 || (mz >= -n && mz <= n));
+          const double G2 = gxy2 + (mz / L[2]) * (mz / L[2]);
+          bool in = mz <= MZ && (n < 0 ? G2 <= gmax2 : (mz >= -n && mz <= n));
           if (mx == 0 && my == 0 && mz <= 0) in = false;
-          double c = 0;
           if (in) {
-            const double G2 = gxy2 + (mz / L[2]) * (mz / L[2]);
-            c = 4 * M_PI * q * exp(-M_PI * M_PI * G2 / (alpha * alpha)) / (G2 * V * M_PI);
+            col.c[z] = 4 * M_PI * q * exp(-M_PI * M_PI * G2 / (alpha * alpha)) / (G2 * V * M_PI);
+            col.zlo = std::min(col.zlo, z);
+            col.zhi = std::max(col.zhi, z);
             ++nk;
           }
-          coef.push_back(c);
         }
+        cl.push_back(std::move(col));
       }
-    const int ncol = (int)cols.size(), ncolp = (ncol + 15) / 16 * 16;
-    while ((int)cols.size() < ncolp) {
-      cols.push_back(make_int2(0, 0));
-      for (int z = 0; z < ZP; ++z) coef.push_back(0.0);
+    std::stable_sort(cl.begin(), cl.end(), [](const Col &a, const Col &b) { return a.gxy2 < b.gxy2; });
+    const int ncol = (int)cl.size(), ncolp = (ncol + 15) / 16 * 16;
+    // cols: [ncolp] (mx, my), then per tile of 16 columns the first and last
+    // K step (4 slots) holding a nonzero weight (first > last: none)
+    std::vector<int2> cols;
+    std::vector<double> coef;
+    std::vector<int2> kr(ncolp / 16, make_int2(ZP, -1));
+    for (int c = 0; c < ncolp; ++c) {
+      if (c < ncol) {
+        cols.push_back(make_int2(cl[c].mx, cl[c].my));
+        coef.insert(coef.end(), cl[c].c.begin(), cl[c].c.end());
+        if (cl[c].zlo <= cl[c].zhi) {
+          kr[c / 16].x = std::min(kr[c / 16].x, cl[c].zlo / 4);
+          kr[c / 16].y = std::max(kr[c / 16].y, cl[c].zhi / 4);
+        }
+      } else {
+        cols.push_back(make_int2(0, 0));
+        coef.insert(coef.end(), ZP, 0.0);
+      }
     }
+    cols.insert(cols.end(), kr.begin(), kr.end());
     if (K->cols) {
       (void)hipFree(K->cols);
       (void)hipFree(K->coef);
@@ -785,10 +812,13 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
   const int nct = ncolp / 16;
   for (int ct = wv; ct < nct; ct += 4) {
     const int c = ct * 16 + (lane & 15);
+    // K steps outside [kr.x, kr.y] carry zero weights for every column of
+    // the tile: skipping them leaves U and V unchanged
+    const int2 kr = cols[ncolp + ct];
     const double2 *pw = Ws + (size_t)c * ZP + (lane >> 4);
     double2 wb[KS];
 #pragma unroll
-    for (int k = 0; k < KS; ++k) wb[k] = pw[4 * k];
+    for (int k = 0; k < KS; ++k) wb[k] = (k >= kr.x && k <= kr.y) ? pw[4 * k] : make_double2(0.0, 0.0);
     const int2 cl = cols[c];
     double2 exs[4], eys[4];
 #pragma unroll
@@ -799,6 +829,7 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
     v4d u1 = {0, 0, 0, 0}, u2 = u1, u3 = u1, v1 = u1, v2 = u1, v3 = u1;
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
+      if (k < kr.x || k > kr.y) continue;
       // B operand: W[c][z] with z = 4 k + lane / 16; V takes m_z W
       const double wr = wb[k].x, wd = wb[k].y - wb[k].x, ws = wb[k].x + wb[k].y;
       const double m = mz0 + 4 * k;
