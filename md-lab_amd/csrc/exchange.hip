@@ -538,23 +538,52 @@ struct TriTable {
 __constant__ TriTable kTri = TriTable();
 
 struct RecLds {
-  double dg[2][RTRI];       // in-block coefficients, then scaled exps (ping-pong)
-  double la[2][RB * RB];    // look-ahead coefficients [r][q] (ping-pong)
-  double vlast[RB];         // values of the block solved last
-  double mo[RB], so[RB];    // row accumulators (max, scaled sum)
-  double sg[RB];            // per-row shifts
-  double pm[4][RB], ps[4][RB];
+  double dg[2][RTRI];           // in-block coefficients, then scaled exps (ping-pong)
+  double mo[RB], so[RB];        // row accumulators (max, scaled sum)
+  double sg[RB];                // per-row shifts
+  double wl[RB];                // scaled values W of the block solved last
+  double tmax[3][RB], tacc[3][RB];  // look-ahead partials of waves 1..3 (shift, scaled sum)
+  double pm[4][RB], ps[4][RB];  // updater partials
   double fm[4][RB], fs[4][RB];  // partials of a self-fold (SOLO)
 };
 
-// The solver walks the blocks in order.  Per block b: issue the loads of
-// block b+1's coefficients, wait for the updater of block b and issue the
-// loads of its row accumulators, fold block b-1 into the rows from the
-// look-ahead coefficients and the values kept in LDS, merge, then the (max,+)
-// shift scan, scaled exps and the linear substitution; block b+1's
-// coefficients are staged into LDS last, so their loads overlap the block.
-// SOLO: no updater workgroups; the solver folds blocks 0 .. b-2 into block b
-// itself (same order and arithmetic), so it waits on no other workgroup.
+// Wave-level hand-off wait (lane 0 polls, the whole wave then acquires),
+// bounded like rec_wait.
+__device__ __forceinline__ void rec_wait_wave(int *flag, int target, int *err) {
+  if ((threadIdx.x & 63) == 0) {
+    int it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (++it > REC_SPIN_MAX) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// The solver walks the blocks in order; wave 0 carries the serial chains.
+// Per block b (its rows' accumulators over all earlier blocks already merged
+// into mo, so):
+//   issue the loads of block b+1's coefficients (in-block: all waves; on
+//   block b: waves 1..3, columns r = wave - 1 mod 3);
+//   wave 0: the (max,+) scan for the per-row shifts sigma;
+//   all: the scaled in-block exps;
+//   wave 0: the linear substitution, values of block b; waves 1..3
+//   meanwhile: tau_q = max_r c(q, r) + sigma_r over their columns and
+//   f(q, r) = exp(c(q, r) + sigma_r - tau_q) in registers for block b+1's
+//   rows q;
+//   publish block b; waves 1..3: the look-ahead sums sum_r f(q, r) W_r;
+//   wave 0 meanwhile: wait for block b+1's updater, load its accumulators;
+//   wave 0: merge the three (tau + ln sum) into them.
+// W_r = exp(L_r - sigma_r) is the substitution's own unknown, so the
+// look-ahead is the log-sum-exp of c(q, r) + L_r without a log/exp round trip.
+// SOLO: no updater workgroups; the solver folds blocks 0 .. b-1 into block
+// b+1 itself (same order and arithmetic as an updater), so it waits on no
+// other workgroup.
 template <bool FWD, bool SOLO = false>
 __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S) {
   const int N = a.N;
@@ -565,10 +594,14 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
   const double *DGd = a.dg + (size_t)dir * a.nblk * RTRI;
   const double *LAd = a.la + (size_t)dir * a.nblk * (RB * RB);
   constexpr int DPT = (RTRI + 255) / 256;  // in-block coefficients per thread
-  constexpr int LPT = RB * RB / 256;       // look-ahead coefficients per thread
-  // block 0's coefficients
+  constexpr int LPW = (RB + 2) / 3;        // look-ahead coefficients per lane of waves 1..3
+  // block 0: coefficients and accumulators (column 0 only)
   for (int w = tid; w < RTRI; w += 256) S.dg[0][w] = DGd[w];
-  for (int w = tid; w < RB * RB; w += 256) S.la[0][w] = LAd[w];
+  if (wv == 0) {
+    const int nr0 = min(RB, N);
+    S.mo[lane] = lane < nr0 ? AM[1 + lane] : NEG;
+    S.so[lane] = lane < nr0 ? AS[1 + lane] : 0.0;
+  }
   const int rowq = lane * (lane - 1) / 2;
   // (q, r) of this thread's in-block coefficients, the same for every block
   int tq[DPT], tr[DPT];
@@ -582,54 +615,25 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
   for (int b = 0; b < a.nblk; ++b) {
     const int cur = b & 1;
     const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
-    __syncthreads();
-    // prefetch block b+1: loads in flight during the whole block, staged into
-    // LDS at its end
-    double dgr[DPT], lar[LPT];
     const bool more = b + 1 < a.nblk;
+    const int p1 = p0 + RB, nrows1 = more ? min(RB, N - p1 + 1) : 0;
+    __syncthreads();
+    // loads of block b+1's coefficients, in flight during the scan
+    double dgr[DPT], lar[LPW];
     if (more) {
 #pragma unroll
       for (int k = 0; k < DPT; ++k) {
         const int w = tid + 256 * k;
         dgr[k] = w < RTRI ? DGd[(size_t)(b + 1) * RTRI + w] : NEG;
       }
+      if (wv > 0) {
 #pragma unroll
-      for (int k = 0; k < LPT; ++k) lar[k] = LAd[(size_t)(b + 1) * (RB * RB) + tid + 256 * k];
-    }
-    // rows' accumulators over blocks 0 .. b-2 (updater or self-fold); their
-    // loads are in flight during the look-ahead
-    if (SOLO) {
-      if (b >= 2) rec_fold_rows<FWD, false>(a, dir, b, S.fm, S.fs);
-    } else if (b >= 2) {
-      rec_wait(folded + b, b - 1, err);
-    }
-    double Mrow = NEG, Srow = 0.0;
-    if (wv == 0 && lane < nrows) {
-      Mrow = AM[p0 + lane];
-      Srow = AS[p0 + lane];
-    }
-    {
-      double t[16], M, Sm;
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int r = 16 * wv + j;
-        const double c = S.la[cur][r * RB + lane];
-        t[j] = (b > 0 && c > NEG) ? c + S.vlast[r] : NEG;
+        for (int k = 0; k < LPW; ++k) {
+          const int r = wv - 1 + 3 * k;  // [r][q] layout: row q = lane of block b+1 on column r of block b
+          lar[k] = r < RB ? LAd[(size_t)(b + 1) * (RB * RB) + r * RB + lane] : NEG;
+        }
       }
-      lse16(t, M, Sm);
-      S.pm[wv][lane] = M;
-      S.ps[wv][lane] = Sm;
     }
-    __syncthreads();
-    if (wv == 0) {
-      if (lane < nrows) {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) lse_pair(Mrow, Srow, S.pm[w][lane], S.ps[w][lane]);
-      }
-      S.mo[lane] = Mrow;
-      S.so[lane] = Srow;
-    }
-    __syncthreads();
     double *dg = S.dg[cur];
     // per-row shift sigma_q = max(M_q, max_{r<q} c(q,r) + sigma_r): a (max,+)
     // scan with no transcendental on its serial chain; each lane holds its
@@ -662,9 +666,9 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
       }
     }
     __syncthreads();
-    // linear forward substitution W_q = b_q + sum_{r<q} a(q,r) W_r, with
-    // W_q = exp(L(q) - sigma_q): one FMA per step (a(q,r) = 0 for r >= q)
     if (wv == 0) {
+      // linear forward substitution W_q = b_q + sum_{r<q} a(q,r) W_r, with
+      // W_q = exp(L(q) - sigma_q): one FMA per step (a(q,r) = 0 for r >= q)
       double av[RB - 1];
 #pragma unroll
       for (int r = 0; r < RB - 1; ++r) {
@@ -676,8 +680,8 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
         W = S.so[lane] * exp(S.mo[lane] - S.sg[lane]);
 #pragma unroll
       for (int r = 0; r < RB - 1; ++r) W = fma(av[r], readlane_d(W, r), W);
+      S.wl[lane] = W;
       const double v = S.sg[lane] + log(W);
-      S.vlast[lane] = v;
       if (lane < nrows) {
         const int p = p0 + lane;
         val[p] = v;
@@ -688,6 +692,38 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
           a.Rb[N - p] = v;
         }
       }
+    } else if (more) {
+      // look-ahead weights of block b+1's row q = lane over this wave's
+      // columns r: f = exp(c(q, r) + sigma_r - tau), tau their maximum
+      double tau = NEG;
+#pragma unroll
+      for (int k = 0; k < LPW; ++k) {
+        const int r = wv - 1 + 3 * k;
+        if (r < RB && lar[k] > NEG) tau = fmax(tau, lar[k] + S.sg[r]);
+      }
+#pragma unroll
+      for (int k = 0; k < LPW; ++k) {
+        const int r = wv - 1 + 3 * k;
+        lar[k] = (r < RB && lar[k] > NEG && tau > NEG) ? exp(lar[k] + S.sg[r] - tau) : 0.0;
+      }
+      S.tmax[wv - 1][lane] = tau;
+    }
+    // wave 0 wrote every global result of the block: it publishes them
+    // itself (updaters only need these values), then waits for block b+1's
+    // updater and issues the loads of its accumulators, while waves 1..3
+    // finish their look-ahead weights
+    double M = NEG, Sm = 0.0;
+    if (wv == 0) {
+      if (!SOLO) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) __hip_atomic_store(solved + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (more && b + 1 >= 2) rec_wait_wave(folded + b + 1, b, err);
+      }
+      if (!SOLO && lane < nrows1) {
+        M = AM[p1 + lane];
+        Sm = AS[p1 + lane];
+      }
     }
     if (more) {
 #pragma unroll
@@ -695,10 +731,44 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
         const int w = tid + 256 * k;
         if (w < RTRI) S.dg[cur ^ 1][w] = dgr[k];
       }
-#pragma unroll
-      for (int k = 0; k < LPT; ++k) S.la[cur ^ 1][tid + 256 * k] = lar[k];
     }
-    rec_post(solved + b, 1);
+    if (!more) break;
+    __syncthreads();
+    // waves 1..3: the look-ahead sums over their columns
+    if (wv > 0) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < LPW; ++k) {
+        const int r = wv - 1 + 3 * k;
+        if (r < RB) acc = fma(lar[k], S.wl[r], acc);
+      }
+      S.tacc[wv - 1][lane] = acc;
+    }
+    if (SOLO) {
+      // block b+1's accumulators over blocks 0 .. b-1, folded here
+      if (b + 1 >= 2) rec_fold_rows<FWD, false>(a, dir, b + 1, S.fm, S.fs);
+      if (wv == 0 && lane < nrows1) {
+        M = AM[p1 + lane];
+        Sm = AS[p1 + lane];
+      }
+    }
+    __syncthreads();
+    if (wv == 0) {
+      if (lane < nrows1) {
+        // wave w's look-ahead sum is exp(tmax_w) tacc_w with tacc_w >= 1
+        // (its largest weight is 1 and W >= 1); their total in log form
+        const double t = fmax(fmax(S.tmax[0][lane], S.tmax[1][lane]), S.tmax[2][lane]);
+        if (t > NEG) {
+          double sum = 0.0;
+#pragma unroll
+          for (int w = 0; w < 3; ++w)
+            if (S.tmax[w][lane] > NEG) sum += S.tacc[w][lane] * exp(S.tmax[w][lane] - t);
+          if (sum > 0.0) lse_pair(M, Sm, t + log(sum), 1.0);
+        }
+      }
+      S.mo[lane] = M;
+      S.so[lane] = Sm;
+    }
   }
 }
 
@@ -738,7 +808,7 @@ __global__ void k_vbn_to_L(const double *VBN, double *Lf, int N, double beta) {
 __device__ __forceinline__ void exch_force_body(int bx, const double *__restrict__ X, const double *__restrict__ M,
                                                 double *__restrict__ F, int N, int P, double om, Geo g,
                                                 const RecArgs &a, double *__restrict__ Fx = nullptr) {
-  __shared__ double red[8][256];
+  __shared__ double red[8][4];
   const int l = bx + 1;
   const int nb = N * P;
   const int sP = (P - 1) * N + (l - 1), s1 = l - 1;
@@ -781,17 +851,21 @@ __device__ __forceinline__ void exch_force_body(int bx, const double *__restrict
 #pragma unroll
     for (int d = 0; d < 3; ++d) hc[d] += G * comp(x1[d] - X[d * nb + sj], Ls[d], g.image);
   }
+  // sums over the workgroup: butterfly within each wave, then the four wave
+  // totals in a fixed order
   double v[8] = {gs, gc[0], gc[1], gc[2], hs, hc[0], hc[1], hc[2]};
 #pragma unroll
-  for (int q = 0; q < 8; ++q) red[q][threadIdx.x] = v[q];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
+  for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-    __syncthreads();
-  }
+    for (int q = 0; q < 8; ++q) v[q] += __shfl_xor(v[q], o, 64);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) red[q][wv] = v[q];
+  __syncthreads();
   if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) red[q][0] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
     const int sPm = (P - 2) * N + (l - 1), s2 = N + (l - 1);
     const double mP = M[sP], m1 = M[s1];
     for (int d = 0; d < 3; ++d) {

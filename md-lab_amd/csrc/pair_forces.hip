@@ -411,8 +411,9 @@ __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restr
 // S(-k) = S(k)*.  coef = 4 pi A_k, A_k = q exp(-pi^2 G^2/alpha^2)/(pi V G^2),
 // G_d = m_d / L_d, is zero on the slots outside the set in use:
 //  * the reference's cube |m_d| <= n (md_simulation_x.c:266-281), or
-//  * the cylinder of a converged split: (mx, my) in the disc |G_xy| <= G_max,
-//    |m_z| <= MZ (all terms beyond G_max are below exp(-x_cut^2)).
+//  * the ball |G| <= G_max of a converged split (all terms beyond G_max are
+//    below exp(-x_cut^2)), stored as the columns of the disc |G_xy| <= G_max
+//    with their m_z slots, columns ordered by |G_xy|.
 // Column (0, 0) keeps m_z > 0 only; padding columns and the slot z = 2MZ+1
 // carry zero weight.  MZ is 7, 15, 23 or 31, so ZP = 2 MZ + 2 is a multiple
 // of the 16-wide matrix-core tiles.
@@ -442,7 +443,7 @@ __device__ __forceinline__ v4d mfma4(double a, double b, v4d c) {
 
 struct KSpace {
   double L[3] = {0, 0, 0}, alpha = 0, q = 0, gmax2 = 0;
-  int n = -2;  // cube half-width, or -1 for the cylinder of radius^2 gmax2
+  int n = -2;  // cube half-width, or -1 for the ball of radius^2 gmax2
   int MZ = 0, ZP = 0, ncol = 0, ncolp = 0, nk = 0;  // nk: k vectors with nonzero weight
   int2 *cols = nullptr;    // [ncolp] (mx, my), then [ncolp / 16] K-step range of each column tile
   double *coef = nullptr;  // [ncolp][ZP]
