@@ -259,12 +259,15 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
     return 0;
   }
   // The first step runs eagerly: it allocates every lazily created buffer,
-  // so the captured graph contains kernel nodes only.
+  // so the captured graph contains kernel nodes only.  The capture runs on
+  // the host while the device works on that step.  Every call captures the
+  // step afresh (so the graph holds this call's parameters and buffers); an
+  // executable graph kept from the previous call on the same object is
+  // updated in place when the new graph has its topology, which saves the
+  // instantiation.
   if (enqueue_step(sp, nullptr, false, L)) return -1;
   Runtime &R = rt();
-  XMD_CHECK(hipStreamSynchronize(R.s0));
   hipGraph_t graph;
-  hipGraphExec_t exec;
   XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
   int rc = enqueue_step(sp, nullptr, false, L);
   hipError_t ec = hipStreamEndCapture(R.s0, &graph);
@@ -272,11 +275,24 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
     fprintf(stderr, "xmd: step capture failed\n");
     return -1;
   }
-  XMD_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  DevPimd *p = sp.p;
+  hipGraphExec_t exec = (hipGraphExec_t)p->step_exec;
+  bool updated = false;
+  if (exec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    updated = hipGraphExecUpdate(exec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    if (!updated) {
+      (void)hipGetLastError();
+      XMD_CHECK(hipGraphExecDestroy(exec));
+      exec = nullptr;
+    }
+  }
+  if (!updated) XMD_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  p->step_exec = exec;
+  XMD_CHECK(hipGraphDestroy(graph));
   for (int s = 1; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
   XMD_CHECK(hipStreamSynchronize(R.s0));
-  XMD_CHECK(hipGraphExecDestroy(exec));
-  XMD_CHECK(hipGraphDestroy(graph));
   return recursion_status(sp.p);
 }
 

@@ -106,6 +106,7 @@ struct DevPimd {
   int *recsync = nullptr;   // recursion hand-off flags: 4*nblk + 1
   int *done = nullptr;      // finished-workgroup count of k_chain_scan (zero between launches)
   void *pair_bufs = nullptr;  // pair-kernel tile schedule, partial sums and k-space tables (pair_forces.hip)
+  void *step_exec = nullptr;  // executable step graph of the last md_pimd_run_steps_x (hipGraphExec_t)
   bool enk_dev = false, enk2_dev = false, vbn_dev = false, back_dev = false;
   bool enk_host_stale = false;
   unsigned enk_version = 0, back_version = 0;
