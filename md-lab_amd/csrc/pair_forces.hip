@@ -457,7 +457,8 @@ struct TileBufs {
   int N = -1, T = 0, ntp = 0;
   int2 *tiles = nullptr;
   double *Fpart = nullptr, *Epart = nullptr;
-  size_t fcap = 0, ecap = 0;
+  double *Frec = nullptr;  // reciprocal forces kept apart for k_combine_forces (3 N P)
+  size_t fcap = 0, ecap = 0, rcap = 0;
 };
 
 // Per-object buffers of the pair kernels, so that independent systems can
@@ -785,7 +786,7 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
                                                  const int2 *__restrict__ cols, int ncolp, int MZ,
                                                  const double2 *__restrict__ W, const double *__restrict__ Mass,
                                                  double *__restrict__ F, int N, int nb, double L0, double L1,
-                                                 double L2, double Pd) {
+                                                 double L2, double Pd, double *__restrict__ Frec) {
   constexpr int ZP = 16 * NZT, KS = 4 * NZT;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int abead = b0 + (lane & 15);  // A operand row of this lane
@@ -882,9 +883,15 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
         g[d] = ((red[(d * 4 + 0) * 16 + t] + red[(d * 4 + 1) * 16 + t]) + red[(d * 4 + 2) * 16 + t]) +
                red[(d * 4 + 3) * 16 + t];
       const double m = Mass[s];
-      F[s] += g[0] / L0 / m / Pd;
-      F[nb + s] += g[1] / L1 / m / Pd;
-      F[2 * nb + s] += g[2] / L2 / m / Pd;
+      if (Frec) {  // kept apart, added into F later in the same order
+        Frec[s] = g[0] / L0 / m / Pd;
+        Frec[nb + s] = g[1] / L1 / m / Pd;
+        Frec[2 * nb + s] = g[2] / L2 / m / Pd;
+      } else {
+        F[s] += g[0] / L0 / m / Pd;
+        F[nb + s] += g[1] / L1 / m / Pd;
+        F[2 * nb + s] += g[2] / L2 / m / Pd;
+      }
     }
   }
 }
@@ -894,11 +901,11 @@ __device__ __forceinline__ void k_recip_force_mfma_body(int bx,
     const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
     const int2 *__restrict__ cols, int ncolp, int MZ, const double2 *__restrict__ W,
     const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
-    double Pd) {
+    double Pd, double *__restrict__ Frec = nullptr) {
   __shared__ double red[3 * 4 * 16];
   int slice, bt;
   xcd_tile(bx, (N + 15) / 16, P, slice, bt);
-  recip_force_tile<NZT>(red, slice, bt * 16, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, L0, L1, L2, Pd);
+  recip_force_tile<NZT>(red, slice, bt * 16, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, L0, L1, L2, Pd, Frec);
 }
 
 template <int NZT>
@@ -906,8 +913,8 @@ __global__ __launch_bounds__(256) void k_recip_force_mfma(
     const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
     const int2 *__restrict__ cols, int ncolp, int MZ, const double2 *__restrict__ W,
     const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
-    double Pd) {
-  k_recip_force_mfma_body<NZT>(blockIdx.x, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, P, L0, L1, L2, Pd);
+    double Pd, double *__restrict__ Frec) {
+  k_recip_force_mfma_body<NZT>(blockIdx.x, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, P, L0, L1, L2, Pd, Frec);
 }
 
 // Reciprocal energy of every slice: sum_{k in half} A_k (|S_k|^2 - N), into partials.
@@ -1204,6 +1211,12 @@ static int ensure_tiles(DevPimd *p) {
     XMD_CHECK(hipMalloc(&B.Epart, eneed * sizeof(double)));
     B.ecap = eneed;
   }
+  const size_t rneed = (size_t)3 * N * P;
+  if (rneed > B.rcap) {
+    if (B.Frec) (void)hipFree(B.Frec);
+    XMD_CHECK(hipMalloc(&B.Frec, rneed * sizeof(double)));
+    B.rcap = rneed;
+  }
   return 0;
 }
 
@@ -1390,7 +1403,7 @@ static void (*const k_sk_of[4])(const double2 *, const double2 *, const double2 
                                                                          k_sk_mfma<4>};
 static void (*const k_frc_of[4])(const double2 *, const double2 *, const double2 *, const int2 *, int, int,
                                  const double2 *, const double *, double *, int, int, int, double, double, double,
-                                 double) = {k_recip_force_mfma<1>, k_recip_force_mfma<2>, k_recip_force_mfma<3>,
+                                 double, double *) = {k_recip_force_mfma<1>, k_recip_force_mfma<2>, k_recip_force_mfma<3>,
                                             k_recip_force_mfma<4>};
 
 // Phase tables and structure factors S, W of every slice (reads X only).
@@ -1409,11 +1422,14 @@ static int launch_recip_sk(DevPimd *p, const PairParams &pp, int cube_n, double 
   return 0;
 }
 
-// Reciprocal forces from the tables and W of launch_recip_sk, added into F.
-static int launch_recip_force(DevPimd *p, const PairParams &pp, const KSpace *K, hipStream_t st) {
+// Reciprocal forces from the tables and W of launch_recip_sk, added into F
+// (Frec null) or stored in Frec.
+static int launch_recip_force(DevPimd *p, const PairParams &pp, const KSpace *K, hipStream_t st,
+                              double *Frec = nullptr) {
   DevSim *d = p->ds;
   hipLaunchKernelGGL(k_frc_of[K->ZP / 16 - 1], dim3(((p->N + 15) / 16) * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz,
-                     K->cols, K->ncolp, K->MZ, K->W, d->M, d->F, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P);
+                     K->cols, K->ncolp, K->MZ, K->W, d->M, d->F, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P,
+                     Frec);
   XMD_CHECK(hipGetLastError());
   return 0;
 }
@@ -1476,24 +1492,47 @@ int launch_pair_kspace_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2
   if (!near_ok(pp)) return 0;
   XMD_CHECK(hipEventRecord(fork, st));
   XMD_CHECK(hipStreamWaitEvent(s2, fork, 0));
+  if (ensure_tiles(p)) return -1;
   KSpace *K = nullptr;
   if (launch_recip_sk(p, pp, pl.cube_n, pl.gmax2, s2, &K)) return -1;
+  if (launch_recip_force(p, pp, K, s2, pair_bufs(p)->tb.Frec)) return -1;
   XMD_CHECK(hipEventRecord(join, s2));
   return 1;
 }
 
-int launch_pair_force_joined(DevPimd *p, int kind, hipStream_t st, hipEvent_t join) {
+int launch_pair_real_partial(DevPimd *p, int kind, hipStream_t st) {
   EwaldPlan pl;
   PairParams pp = ewald_params(p, pl);
-  if (launch_ewald_real_tab(p, pp, st, false)) return -1;
-  XMD_CHECK(hipStreamWaitEvent(st, join, 0));
+  return launch_ewald_real_tab(p, pp, st, false);
+}
+
+// F += real-space tile sums / P, then the reciprocal forces, then (when Fx
+// is given) the exchange forces of the end beads: the order, and so the
+// rounding, of k_reduce_tiles, k_recip_force_mfma and k_add_exch in turn.
+__global__ __launch_bounds__(256) void k_combine_forces(double *__restrict__ F, const double *__restrict__ Fpart,
+                                                        int n3, int T, double Pd, const double *__restrict__ Frec,
+                                                        const double *__restrict__ Fx, int N, int P) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n3) return;
+  double s = 0;
+  for (int q = 0; q < T; ++q) s += Fpart[(size_t)q * n3 + k];
+  double f = F[k] + s / Pd;
+  f += Frec[k];
+  if (Fx) {
+    const int nb = N * P, d = k / nb, i = k - d * nb;
+    if (i >= (P - 1) * N) f += Fx[d * N + (i - (P - 1) * N)];
+    if (i < N) f += Fx[(3 + d) * N + i];
+  }
+  F[k] = f;
+}
+
+int launch_combine_forces(DevPimd *p, const double *Fx, hipStream_t st) {
   TileBufs &B = pair_bufs(p)->tb;
   const int n3 = 3 * p->ds->nb;
-  hipLaunchKernelGGL(k_reduce_tiles, dim3((n3 + 255) / 256), dim3(256), 0, st, B.Fpart, p->ds->F, n3, B.T, pp.P);
+  hipLaunchKernelGGL(k_combine_forces, dim3((n3 + 255) / 256), dim3(256), 0, st, p->ds->F, B.Fpart, n3, B.T,
+                     (double)p->P, B.Frec, Fx, p->N, p->P);
   XMD_CHECK(hipGetLastError());
-  KSpace *K = ensure_kspace(p, pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, p->ds->nb, p->P);
-  if (!K) return -1;
-  return launch_recip_force(p, pp, K, st);
+  return 0;
 }
 
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st) {

@@ -8,8 +8,9 @@
 // the force reset, trap and interior springs.
 //   s1: (after fork) forward + backward recursions, end-bead exchange forces
 //       into a side buffer --join--> (added into F on s0)
-//   s2: (Ewald) phase tables and structure factors, forked from s0 after
-//       NHC half 1 and joined before the real-space sum into F
+//   s2: (Ewald) phase tables, structure factors and reciprocal forces into a
+//       side buffer, forked from s0 after the recursions --join2--> one
+//       kernel on s0 then adds the real-space, reciprocal and exchange forces
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces are added into F after both branches are done
 // (the serial profiling order computes and adds them there in one kernel).
@@ -100,14 +101,9 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     if (ev) XMD_CHECK(hipEventRecord(ev[k], st));
     return 0;
   };
-  // The reciprocal structure factors (positions only) fork onto s2 first and
-  // overlap the cycle energies and the real-space kernel.
-  const int kfork =
-      (!serial && sp.pair_kind >= 0) ? launch_pair_kspace_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
-  if (kfork < 0) return -1;
   // Cycle energies (short, position-only); then the two recursions are
   // forked onto s1 before the pair kernels are queued, so their workgroups
-  // are resident before the pair grid fills the device.
+  // are resident before the pair grids fill the device.
   if (rec(2, R.s0) || (serial ? launch_enk(p, sp.image, R.s0) : launch_enk_prep(p, sp.image, R.s0)) ||
       rec(3, R.s0))
     return -1;
@@ -121,6 +117,10 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   // into a side buffer, added into F after the join
   if (!serial && launch_exchange_force(p, sp.image, R.s1, true)) return -1;
   if (!serial) XMD_CHECK(hipEventRecord(R.join, R.s1));
+  // Reciprocal space (positions only) on s2 beside the real-space kernel.
+  const int kfork =
+      (!serial && sp.pair_kind >= 0) ? launch_pair_kspace_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
+  if (kfork < 0) return -1;
   // bead-local and pair forces
   if (rec(6, R.s0)) return -1;
   if (d->massive) {
@@ -132,7 +132,7 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   }
   if (rec(7, R.s0)) return -1;
   if (kfork) {
-    if (launch_pair_force_joined(p, sp.pair_kind, R.s0, R.join2)) return -1;
+    if (launch_pair_real_partial(p, sp.pair_kind, R.s0)) return -1;
     if (rec(8, R.s0) || rec(9, R.s0) || rec(10, R.s0) || rec(11, R.s0)) return -1;
   } else if (sp.pair_kind >= 0) {
     if (rec(8, R.s0) || launch_pair_real(p, sp.pair_kind, R.s0) || rec(9, R.s0)) return -1;
@@ -146,7 +146,13 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   }
   if (!serial) {
     XMD_CHECK(hipStreamWaitEvent(R.s0, R.join, 0));
-    if (rec(12, R.s0) || launch_add_exchange_force(p, R.s0) || rec(13, R.s0)) return -1;
+    if (kfork) {
+      // one kernel adds the real-space, reciprocal and exchange forces
+      XMD_CHECK(hipStreamWaitEvent(R.s0, R.join2, 0));
+      if (rec(12, R.s0) || launch_combine_forces(p, p->Fx, R.s0) || rec(13, R.s0)) return -1;
+    } else if (rec(12, R.s0) || launch_add_exchange_force(p, R.s0) || rec(13, R.s0)) {
+      return -1;
+    }
   } else if (rec(12, R.s0) || launch_exchange_force(p, sp.image, R.s0, false) || rec(13, R.s0)) {
     return -1;
   }

@@ -147,16 +147,19 @@ extern "C" md_num_t_x md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C
 int pair_kind_of(md_pair_force_t_x pf);
 int pair_energy_kind_of(md_pair_energy_t_x pe);
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st);
-// Pair forces in two parts, with the reciprocal structure factors on a side
-// stream: launch_pair_kspace_fork forks s2 from st (through `fork`) and
-// queues the phase tables and structure factors there (they read positions
-// only), recording `join`; launch_pair_force_joined later queues the
-// real-space kernel on st, waits for `join`, and adds the real-space then the
-// reciprocal forces into F, as launch_pair_force does.  The fork returns 1
-// when it applies (Ewald with the table kernels), 0 when nothing was launched
-// (use launch_pair_force), -1 on error.
+// Pair forces in parts, with reciprocal space on a side stream:
+// launch_pair_kspace_fork forks s2 from st (through `fork`) and queues the
+// phase tables, structure factors and reciprocal forces there (they read
+// positions only; the forces go to a side buffer), recording `join`;
+// launch_pair_real_partial queues the real-space tile kernel on st; after
+// both (and the exchange branch) launch_combine_forces adds the real-space,
+// reciprocal and, when Fx is given, exchange end-bead forces into F in the
+// order of the unsplit kernels.  The fork returns 1 when it applies (Ewald
+// with the table kernels), 0 when nothing was launched (use
+// launch_pair_force), -1 on error.
 int launch_pair_kspace_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
-int launch_pair_force_joined(DevPimd *p, int kind, hipStream_t st, hipEvent_t join);
+int launch_pair_real_partial(DevPimd *p, int kind, hipStream_t st);
+int launch_combine_forces(DevPimd *p, const double *Fx, hipStream_t st);
 int launch_pair_real(DevPimd *p, int kind, hipStream_t st);
 int launch_pair_recip(DevPimd *p, int kind, hipStream_t st);
 bool pair_has_recip(int kind);
