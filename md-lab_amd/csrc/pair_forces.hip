@@ -475,8 +475,12 @@ static PairBufs *pair_bufs(DevPimd *p) {
   return (PairBufs *)p->pair_bufs;
 }
 
+// Device allocations made by the pair buffers (pair_prepare reports them).
+static unsigned g_pair_allocs = 0;
+
 static int grow(double2 *&p, size_t &cap, size_t need) {
   if (need <= cap) return 0;
+  ++g_pair_allocs;
   if (p) (void)hipFree(p);
   p = nullptr;
   XMD_CHECK(hipMalloc(&p, need * sizeof(double2)));
@@ -503,6 +507,7 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
       K = &k;
   }
   if (!K) {
+    ++g_pair_allocs;
     K = ks[0].stamp <= ks[1].stamp ? &ks[0] : &ks[1];
     int M = n;
     if (n < 0) {
@@ -1189,6 +1194,7 @@ static int ensure_tiles(DevPimd *p) {
   const int N = p->N, P = p->P;
   TileBufs &B = pair_bufs(p)->tb;
   if (B.N != N) {
+    ++g_pair_allocs;
     int T = (N + 63) / 64;
     std::vector<int2> tl;
     for (int I = 0; I < T; ++I)
@@ -1202,17 +1208,20 @@ static int ensure_tiles(DevPimd *p) {
   }
   size_t fneed = (size_t)3 * N * P * B.T, eneed = (size_t)P * B.ntp;
   if (fneed > B.fcap) {
+    ++g_pair_allocs;
     if (B.Fpart) (void)hipFree(B.Fpart);
     XMD_CHECK(hipMalloc(&B.Fpart, fneed * sizeof(double)));
     B.fcap = fneed;
   }
   if (eneed > B.ecap) {
+    ++g_pair_allocs;
     if (B.Epart) (void)hipFree(B.Epart);
     XMD_CHECK(hipMalloc(&B.Epart, eneed * sizeof(double)));
     B.ecap = eneed;
   }
   const size_t rneed = (size_t)3 * N * P;
   if (rneed > B.rcap) {
+    ++g_pair_allocs;
     if (B.Frec) (void)hipFree(B.Frec);
     XMD_CHECK(hipMalloc(&B.Frec, rneed * sizeof(double)));
     B.rcap = rneed;
@@ -1533,6 +1542,21 @@ int launch_combine_forces(DevPimd *p, const double *Fx, hipStream_t st) {
                      (double)p->P, B.Frec, Fx, p->N, p->P);
   XMD_CHECK(hipGetLastError());
   return 0;
+}
+
+int pair_prepare(DevPimd *p, int kind) {
+  const unsigned a0 = g_pair_allocs;
+  if (kind < 0) return 0;
+  if (ensure_tiles(p)) return -1;
+  if (kind == PK_EWALD && p->ds->uniform_mass) {
+    EwaldPlan pl;
+    PairParams pp = ewald_params(p, pl);
+    if (!ensure_kspace(p, pp.L, pp.alpha, pp.q, pl.cube_n, pl.gmax2, p->ds->nb, p->P)) return -1;
+  } else if (kind == PK_EWALD_R) {
+    PairParams pp = make_params(p);
+    if (!ensure_kspace(p, pp.L, pp.alpha, pp.q, pp.nsum, 0.0, p->ds->nb, p->P)) return -1;
+  }
+  return g_pair_allocs != a0 ? 1 : 0;
 }
 
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st) {

@@ -264,24 +264,28 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
       if (enqueue_step(sp, nullptr, false, L)) return -1;
     return 0;
   }
-  // The first step runs eagerly: it allocates every lazily created buffer,
-  // so the captured graph contains kernel nodes only.  The capture runs on
-  // the host while the device works on that step.  Every call captures the
-  // step afresh (so the graph holds this call's parameters and buffers); an
-  // executable graph kept from the previous call on the same object is
-  // updated in place when the new graph has its topology, which saves the
-  // instantiation.
-  if (enqueue_step(sp, nullptr, false, L)) return -1;
+  // Every call captures the step afresh, so the graph holds this call's
+  // parameters and buffers.  The step's lazily created buffers are made
+  // first, on the host; when that allocated anything, or on the first call
+  // on an object, the first step runs eagerly (so allocation and the
+  // capture's kernel-only graph cannot interfere) and the capture runs on the
+  // host while the device works on that step.  An executable graph kept from
+  // the previous call on the same object is updated in place when the new
+  // graph has its topology, which saves the instantiation.
   Runtime &R = rt();
-  hipGraph_t graph;
+  DevPimd *p = sp.p;
+  const int fresh = pair_prepare(p, sp.pair_kind);
+  if (fresh < 0) return -1;
+  const int eager = (fresh || !p->step_exec) ? 1 : 0;
+  if (eager && enqueue_step(sp, nullptr, false, L)) return -1;
+  hipGraph_t graph = nullptr;
   XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
-  int rc = enqueue_step(sp, nullptr, false, L);
-  hipError_t ec = hipStreamEndCapture(R.s0, &graph);
-  if (rc || ec != hipSuccess) {
+  const int rc = enqueue_step(sp, nullptr, false, L);
+  const hipError_t ec = hipStreamEndCapture(R.s0, &graph);
+  if (rc || ec != hipSuccess || !graph) {
     fprintf(stderr, "xmd: step capture failed\n");
     return -1;
   }
-  DevPimd *p = sp.p;
   hipGraphExec_t exec = (hipGraphExec_t)p->step_exec;
   bool updated = false;
   if (exec) {
@@ -297,7 +301,7 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   if (!updated) XMD_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   p->step_exec = exec;
   XMD_CHECK(hipGraphDestroy(graph));
-  for (int s = 1; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
+  for (int s = eager; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
   XMD_CHECK(hipStreamSynchronize(R.s0));
   return recursion_status(sp.p);
 }

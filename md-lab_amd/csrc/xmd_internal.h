@@ -147,6 +147,10 @@ extern "C" md_num_t_x md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C
 int pair_kind_of(md_pair_force_t_x pf);
 int pair_energy_kind_of(md_pair_energy_t_x pe);
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st);
+// Makes (host side, nothing launched) every buffer the step's pair kernels of
+// this kind need; 1 when anything was allocated, 0 when all existed, -1 on
+// error.
+int pair_prepare(DevPimd *p, int kind);
 // Pair forces in parts, with reciprocal space on a side stream:
 // launch_pair_kspace_fork forks s2 from st (through `fork`) and queues the
 // phase tables, structure factors and reciprocal forces there (they read
