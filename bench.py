@@ -169,8 +169,8 @@ def batch_bench(lib, steps, dt=0.01, one_by_one=True):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--P", type=int, default=32)
     ap.add_argument("--cpu-pairs", type=int, default=25000, help="pairs in the reference CPU sample")
