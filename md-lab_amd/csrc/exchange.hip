@@ -489,18 +489,18 @@ __device__ __forceinline__ void rec_fold_rows(const RecArgs &a, int dir, int G, 
     Sa = AS[p];
   }
   for (int b = 0; b <= G - 2; ++b) {
-    if (WAIT) rec_wait(solved + b, 1, err);
     const int pprev = 1 + b * RB;
+    // the coefficients do not depend on block b's values: their loads are
+    // in flight while this workgroup waits for the block
+    double c[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) c[j] = act ? rec_coef_col<FWD>(a, p, pprev + 16 * wv + j) : NEG;
+    if (WAIT) rec_wait(solved + b, 1, err);
     double t[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int pp = pprev + 16 * wv + j;
-      double v = NEG;
-      if (act) {
-        const double c = rec_coef_col<FWD>(a, p, pp);
-        if (c > NEG) v = c + val[pp];
-      }
-      t[j] = v;
+      t[j] = c[j] > NEG ? c[j] + val[pp] : NEG;
     }
     double M, S;
     lse16(t, M, S);
