@@ -85,21 +85,27 @@ typedef struct {
 
 /* Forces at the current positions (step stages B3-B6). */
 int md_pimd_compute_forces_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg);
-/* nsteps full steps (B1-B7); returns when the work is queued. */
+/* nsteps full steps (B1-B7).  Eager (use_graph == 0 or nsteps == 1): returns
+ * when the work is queued.  Graph (use_graph != 0): the step is captured once
+ * per call and replayed; the executable graph is kept with the object and
+ * updated in place by its next call (instantiated again only when the step's
+ * shape changed); returns when the steps are done, -1 if the exchange
+ * recursion reported an error. */
 int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps);
 
 /* nsteps steps of n independent systems (replicas of uneven N, P, xi and
- * interaction, each with its own config) in one hipGraph per batch step: the
- * replicas' steps run concurrently on up to 8 stream lanes forked from the
- * engine's stream.  Every replica's trajectory is bit-identical to running it
- * alone through md_pimd_run_steps_x.  Interaction parameters are the process
+ * interaction, each with its own config), one launch per step stage over a
+ * packed table of all replicas, captured into one hipGraph per batch step.
+ * Every replica's trajectory is bit-identical to running it alone through
+ * md_pimd_run_steps_x.  Interaction parameters are the process
  * globals at the time of the call, as for every other entry point.  Returns
  * when the work is done (-1 if any replica's recursion reported an error). */
 int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *cfgs, int n, int nsteps);
 
 /* Per-stage device time of nsteps eager steps, averaged per step (ms).  With
  * serial != 0 every stage runs alone on one stream (true kernel times);
- * otherwise the two branches overlap as in md_pimd_run_steps_x:
+ * otherwise the branches overlap as in md_pimd_run_steps_x (stage times are
+ * then bracketed only where a stage still runs on the main stream):
  *   [0] NHC first half + wrap   [1] cycle energies   [2] V_B recursions
  *   [3] reset/trap/ring springs [4] pair real space  [5] pair reciprocal
  *   [6] exchange end-bead forces [7] NHC second half [8] whole step (wall)
