@@ -293,18 +293,19 @@ __device__ __forceinline__ double rec_coef_direct(const RecArgs &a, const EnkArg
 // Direction dir (0 forward, 1 backward) keeps, per position p, the solved
 // value val[p] and a running log-sum-exp (accM[p], accS[p]) of the terms
 // coef(p, p') + val[p'] over solved positions p' outside p's own block.
-//   k_rec_prep   (one launch) initial accumulators from position 0, and the
+//   k_rec_prep   (one launch; in the fused step its work is part of
+//                k_enk_prep) initial accumulators from position 0, and the
 //                coefficients of every diagonal block and of every block on
 //                its predecessor, so the serial kernels read them with
 //                independent, coalesced loads;
-//   k_rec_persist (one launch) per direction a solver workgroup that folds
-//                block b-1 into block b's accumulators (look-ahead) and
-//                solves block b, and one updater workgroup per later row
-//                block that folds each finished block into its rows.
+//   k_rec_persist (one launch) per direction a solver workgroup that solves
+//                block b and folds it into block b+1's rows (look-ahead),
+//                and one updater workgroup per later row block that folds
+//                each finished block into its rows.
 // Every row of block B thus receives column 0 (prep), blocks 0..B-2 (its
 // updater) and block B-1 (the solver's look-ahead).
-// Lanes are rows throughout: log-sum-exps run along each lane, so no
-// cross-lane reductions are needed.
+// Lanes are rows throughout in the updaters: log-sum-exps run along each
+// lane, so no cross-lane reductions are needed.
 constexpr int RTRI = RB * (RB - 1) / 2;
 
 // coef(p, p') read through the other layout (column p' of the forward
