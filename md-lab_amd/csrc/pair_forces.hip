@@ -961,8 +961,9 @@ __global__ __launch_bounds__(256) void k_recip_energy(const double2 *__restrict_
 //   IMG_CORNER r_c < sqrt(2) min L_d: also the c candidates.
 // IMG_NEAR and IMG_CORNER build the bit mask of the candidates inside r_c at
 // bit kx | ky << 2 | kz << 4 with k = 0 (a), 1 (b), 3 (c), and walk it; the
-// image is a_d - sext(k_d) s_d.  All three take the images in increasing bit
-// order, so they agree to the last bit where more than one applies.
+// image is a_d - sext(k_d) s_d.  All three evaluate an image with the same
+// arithmetic (ewald_image) and take the images in increasing bit order; the
+// face enumeration's out-of-range candidate adds an exact zero.
 enum { IMG_FACE, IMG_NEAR, IMG_CORNER };
 
 struct EwReal {
