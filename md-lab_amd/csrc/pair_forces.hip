@@ -407,7 +407,10 @@ __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restr
 // ------------------------------------------------------ reciprocal space
 //
 // k set: columns (mx, my) of the half space (mx > 0, or mx == 0 and my >= 0)
-// times m_z = z - MZ, z in [0, ZP); the other half follows from
+// times m_z = slot_mz(z), z in [0, ZP), slots in order of |m_z| (0, 1, -1,
+// 2, -2, ...), so a column tile whose m_z range is small needs only its first
+// slots (whole 16-slot tiles and 4-slot K steps beyond it are skipped); the
+// other half follows from
 // S(-k) = S(k)*.  coef = 4 pi A_k, A_k = q exp(-pi^2 G^2/alpha^2)/(pi V G^2),
 // G_d = m_d / L_d, is zero on the slots outside the set in use:
 //  * the reference's cube |m_d| <= n (md_simulation_x.c:266-281), or
@@ -415,7 +418,7 @@ __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restr
 //    below exp(-x_cut^2)), stored as the columns of the disc |G_xy| <= G_max
 //    with their m_z slots, columns ordered by |G_xy|.
 // Column (0, 0) keeps m_z > 0 only; padding columns and the slot z = 2MZ+1
-// carry zero weight.  MZ is 7, 15, 23 or 31, so ZP = 2 MZ + 2 is a multiple
+// (m_z = MZ + 1 by slot_mz) carry zero weight.  MZ is 7, 15, 23 or 31, so ZP = 2 MZ + 2 is a multiple
 // of the 16-wide matrix-core tiles.
 //
 // Both k kernels are complex GEMMs on the FP64 matrix cores
@@ -432,8 +435,11 @@ __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restr
 // contiguous run:
 //   tx[s * (MZ + 1) + m]        e^{i 2 pi m x / L_x},  m = 0..MZ
 //   ty[s * (2 MZ + 1) + m + MZ] e^{i 2 pi m y / L_y},  m = -MZ..MZ
-//   tz[s * ZP + m + MZ]         e^{i 2 pi m z / L_z},  m = -MZ..MZ  (slot ZP-1 zero)
+//   tz[s * ZP + z]              e^{i 2 pi m z / L_z},  m = slot_mz(z)  (slot ZP-1 zero)
 constexpr int MAX_KM = 31;     // largest table half-width of the k kernels
+
+// m_z of slot z: 0, 1, -1, 2, -2, ...
+__host__ __device__ inline int slot_mz(int z) { return (z & 1) ? (z + 1) >> 1 : -(z >> 1); }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -540,7 +546,7 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
         if (n < 0 && gxy2 > gmax2) continue;
         Col col{mx, my, ZP, -1, gxy2, std::vector<double>(ZP, 0.0)};
         for (int z = 0; z < ZP; ++z) {
-          const int mz = z - MZ;
+          const int mz = slot_mz(z);
           const double G2 = gxy2 + (mz / L[2]) * (mz / L[2]);
           bool in = mz <= MZ && (n < 0 ? G2 <= gmax2 : (mz >= -n && mz <= n));
           if (mx == 0 && my == 0 && mz <= 0) in = false;
@@ -622,7 +628,7 @@ __device__ __forceinline__ void k_phase_body(int bx, const double *__restrict__ 
   } else if ((t -= nx) < ny) {
     d = 1, s = t / (2 * MZ + 1), m = t - s * (2 * MZ + 1) - MZ, out = ty + t;
   } else if ((t -= ny) < nz) {
-    d = 2, s = t / ZP, m = t - s * ZP - MZ, out = tz + t;
+    d = 2, s = t / ZP, m = slot_mz(t - s * ZP), out = tz + t;
     if (m > MZ) {
       *out = make_double2(0.0, 0.0);
       return;
@@ -672,7 +678,7 @@ __device__ __forceinline__ void xcd_tile(int w, int ntile, int P, int &slice, in
 template <int NZT>
 __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const double2 *__restrict__ tx,
                                         const double2 *__restrict__ ty, const double2 *__restrict__ tz, int2 cl,
-                                        const double *__restrict__ coef, int ncolp, int MZ, int N,
+                                        int2 kr, const double *__restrict__ coef, int ncolp, int MZ, int N,
                                         double2 *__restrict__ S, double2 *__restrict__ W) {
   constexpr int ZP = 16 * NZT;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -684,6 +690,9 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
   const double2 *px = tx + s0 * (MZ + 1) + cl.x;
   const double2 *py = ty + s0 * (2 * MZ + 1) + MZ + cl.y;
   const double2 *pz = tz + s0 * ZP + (lane & 15);
+  // slot tiles beyond the column tile's last nonzero K step (kr.y) hold
+  // zero weights: their sums are not formed (S and W stay zero there)
+  const int ntl = kr.x <= kr.y ? min(NZT, kr.y / 4 + 1) : 0;
   v4d t1[NZT], t2[NZT], t3[NZT];
 #pragma unroll
   for (int t = 0; t < NZT; ++t) t1[t] = t2[t] = t3[t] = v4d{0, 0, 0, 0};
@@ -693,6 +702,7 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
     const double as = ar + ai;
 #pragma unroll
     for (int t = 0; t < NZT; ++t) {
+      if (t >= ntl) continue;
       t1[t] = mfma4(as, z[t].x, t1[t]);
       t2[t] = mfma4(ar, z[t].y - z[t].x, t2[t]);
       t3[t] = mfma4(ai, z[t].x + z[t].y, t3[t]);
@@ -706,7 +716,7 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
       ex[u] = px[u * sx];
       ey[u] = py[u * sy];
 #pragma unroll
-      for (int t = 0; t < NZT; ++t) z[u][t] = pz[u * 4 * ZP + 16 * t];
+      for (int t = 0; t < NZT; ++t) z[u][t] = t < ntl ? pz[u * 4 * ZP + 16 * t] : make_double2(0.0, 0.0);
     }
     px += 4 * sx;
     py += 4 * sy;
@@ -717,7 +727,7 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
   for (; k < nfull; ++k) {
     double2 z[NZT];
 #pragma unroll
-    for (int t = 0; t < NZT; ++t) z[t] = pz[16 * t];
+    for (int t = 0; t < NZT; ++t) z[t] = t < ntl ? pz[16 * t] : make_double2(0.0, 0.0);
     step(px[0], py[0], z, true);
     px += sx;
     py += sy;
@@ -730,7 +740,7 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
     const int back = ok ? 0 : 4 * nfull + kb;
     double2 z[NZT];
 #pragma unroll
-    for (int t = 0; t < NZT; ++t) z[t] = pz[16 * t - back * ZP];
+    for (int t = 0; t < NZT; ++t) z[t] = t < ntl ? pz[16 * t - back * ZP] : make_double2(0.0, 0.0);
     step(px[-back * (MZ + 1)], py[-back * (2 * MZ + 1)], z, ok);
   }
   // D[i = column][j = slot]: lane j + 16 (i%4), register i/4
@@ -766,7 +776,7 @@ __device__ __forceinline__ void k_sk_mfma_body(int bx, const double2 *__restrict
   xcd_tile(bx, ncolp / 16, P, slice, tile);
   const int c0 = tile * 16;
   const int2 cl = cols[c0 + (threadIdx.x & 15)];
-  sk_tile<NZT>(red, slice, c0, tx, ty, tz, cl, coef, ncolp, MZ, N, S, W);
+  sk_tile<NZT>(red, slice, c0, tx, ty, tz, cl, cols[ncolp + tile], coef, ncolp, MZ, N, S, W);
 }
 
 template <int NZT>
@@ -805,7 +815,7 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
     zr[k] = z.x;
     zi[k] = z.y;
   }
-  const double mz0 = (double)((lane >> 4) - MZ);  // m_z of this lane's K row at k = 0
+  const int zl = lane >> 4;  // this lane's K row within a K step
   // D[i = bead][j = column]: lane j + 16 (i%4), register i/4
   const int beadq = lane >> 4;  // this lane's beads: b0 + 4 r + beadq
   int sb[4];
@@ -839,7 +849,7 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
       if (k < kr.x || k > kr.y) continue;
       // B operand: W[c][z] with z = 4 k + lane / 16; V takes m_z W
       const double wr = wb[k].x, wd = wb[k].y - wb[k].x, ws = wb[k].x + wb[k].y;
-      const double m = mz0 + 4 * k;
+      const double m = (double)slot_mz(4 * k + zl);
       u1 = mfma4(zs[k], wr, u1);
       u2 = mfma4(zr[k], wd, u2);
       u3 = mfma4(zi[k], ws, u3);
