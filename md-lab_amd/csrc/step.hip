@@ -101,9 +101,14 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     if (ev) XMD_CHECK(hipEventRecord(ev[k], st));
     return 0;
   };
+  // Reciprocal space (positions only) forks onto s2 first: the phase tables
+  // and structure factors overlap the cycle energies.
+  const int kfork =
+      (!serial && sp.pair_kind >= 0) ? launch_pair_kspace_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
+  if (kfork < 0) return -1;
   // Cycle energies (short, position-only); then the two recursions are
-  // forked onto s1 before the pair kernels are queued, so their workgroups
-  // are resident before the pair grids fill the device.
+  // forked onto s1 before the real-space grid is queued, so their
+  // workgroups are resident before it fills the device.
   if (rec(2, R.s0) || (serial ? launch_enk(p, sp.image, R.s0) : launch_enk_prep(p, sp.image, R.s0)) ||
       rec(3, R.s0))
     return -1;
@@ -117,10 +122,6 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   // into a side buffer, added into F after the join
   if (!serial && launch_exchange_force(p, sp.image, R.s1, true)) return -1;
   if (!serial) XMD_CHECK(hipEventRecord(R.join, R.s1));
-  // Reciprocal space (positions only) on s2 beside the real-space kernel.
-  const int kfork =
-      (!serial && sp.pair_kind >= 0) ? launch_pair_kspace_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
-  if (kfork < 0) return -1;
   // bead-local and pair forces
   if (rec(6, R.s0)) return -1;
   if (d->massive) {
