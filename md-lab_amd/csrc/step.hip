@@ -1,15 +1,16 @@
 // The fused MD step and its hipGraph capture, plus run-time knobs.
 //
 // Stream layout of one step (the reference driver's order of stages):
-//   s0: NHC half 1 (+ wrap), cycle energies --fork--> reset F, trap, ring
-//       springs, pair real, pair reciprocal --wait join--> add the exchange
-//       end-bead forces, NHC half 2
+//   s0: NHC half 1 (+ wrap) --fork2--> cycle energies --fork--> reset F,
+//       trap, ring springs, pair real --wait join, join2--> add the pair
+//       and exchange end-bead forces, NHC half 2
 // With massive thermostats NHC half 1 and the wrap are one kernel, and so are
 // the force reset, trap and interior springs.
 //   s1: (after fork) forward + backward recursions, end-bead exchange forces
 //       into a side buffer --join--> (added into F on s0)
 //   s2: (Ewald) phase tables, structure factors and reciprocal forces into a
-//       side buffer, forked from s0 after the recursions --join2--> one
+//       side buffer, forked from s0 right after NHC half 1 (they read
+//       positions only, so they overlap the cycle energies) --join2--> one
 //       kernel on s0 then adds the real-space, reciprocal and exchange forces
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces are added into F after both branches are done
