@@ -1075,7 +1075,9 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
                                                     int nb, int T, int ntp, int nitems, EwReal ep) {
   __shared__ double2 tab[kEwTabIntervals * kEwTabStride / 2];
   __shared__ double jx[4][64], jy[4][64], jz[4][64], rx[4][64], ry[4][64], rz[4][64];
-  const int tid = threadIdx.x, t = tid & 63, w = tid >> 6;
+  // the wave index and everything derived from it are wave-uniform: read
+  // them as such, so the step loop below runs on scalar registers
+  const int tid = threadIdx.x, t = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int k = tid; k < kEwTabIntervals * kEwTabStride / 2; k += 256)
     tab[k] = make_double2(kEwTab[2 * k], kEwTab[2 * k + 1]);
   const int item = bx * 4 + w;
@@ -1084,8 +1086,8 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
   if (valid) {
     slice = item / ntp;
     const int2 IJ = tiles[item - slice * ntp];
-    I = IJ.x;
-    J = IJ.y;
+    I = __builtin_amdgcn_readfirstlane(IJ.x);
+    J = __builtin_amdgcn_readfirstlane(IJ.y);
   }
   const int base = slice * N;
   const int i = I * 64 + t, jl = J * 64 + t;
@@ -1106,24 +1108,32 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
   __syncthreads();
   const bool diag = I == J;
   double fx = 0, fy = 0, fz = 0;
+  // lane t meets bead (t + s) mod 64 of tile J at step s; a diagonal tile
+  // takes s = 1..32 and only lanes t < 32 at s = 32 (each pair once)
+  auto pair = [&](int j) {
+    double gx = 0, gy = 0, gz = 0;
+    ewald_pair_tab<IMG>(xi - jx[w][j], yi - jy[w][j], zi - jz[w][j], ep, tab, gx, gy, gz);
+    // the pair's force over the mass of its higher-index bead; the table
+    // kernels run only for equal masses, so that is this lane's 1 / m_i
+    const double iw = wi;
+    fx = fma(gx, iw, fx);
+    fy = fma(gy, iw, fy);
+    fz = fma(gz, iw, fz);
+    rx[w][j] -= gx * iw;
+    ry[w][j] -= gy * iw;
+    rz[w][j] -= gz * iw;
+  };
   if (act) {
-    const int s_lo = diag ? 1 : 0, s_hi = diag ? 32 : 63;
-    for (int s = s_lo; s <= s_hi; ++s) {
-      const int j = (t + s) & 63;
-      const int jg = J * 64 + j;
-      if (jg >= N || (diag && s == 32 && t >= 32)) continue;
-      double gx = 0, gy = 0, gz = 0;
-      ewald_pair_tab<IMG>(xi - jx[w][j], yi - jy[w][j], zi - jz[w][j], ep, tab, gx, gy, gz);
-      // the pair's force over the mass of its higher-index bead; the table
-      // kernels run only for equal masses, so that is this lane's 1 / m_i
-      const double iw = wi;
-      fx = fma(gx, iw, fx);
-      fy = fma(gy, iw, fy);
-      fz = fma(gz, iw, fz);
-      rx[w][j] -= gx * iw;
-      ry[w][j] -= gy * iw;
-      rz[w][j] -= gz * iw;
+    const int s_lo = diag ? 1 : 0, s_end = diag ? 31 : 63;
+    if (J * 64 + 64 <= N) {
+      for (int s = s_lo; s <= s_end; ++s) pair((t + s) & 63);
+    } else {
+      for (int s = s_lo; s <= s_end; ++s) {
+        const int j = (t + s) & 63;
+        if (J * 64 + j < N) pair(j);
+      }
     }
+    if (diag && t < 32 && J * 64 + t + 32 < N) pair(t + 32);
   }
   __syncthreads();
   if (!valid) return;
