@@ -87,14 +87,14 @@ __global__ void k_ring_springs(const double *__restrict__ X, const double *__res
 // One chain per degree of freedom.  Thread e owns DOF e (device order), its
 // velocity, position, force and chain slots th/vth/Q[k*ndof + e].
 template <int HALF>
-__device__ __forceinline__ void nhc_massive_body(double *__restrict__ X, double *__restrict__ V,
-                                                 const double *__restrict__ F, const double *__restrict__ M,
+__device__ __forceinline__ void nhc_massive_body(double *__restrict__ X, double *__restrict__ V, double f,
+                                                 const double *__restrict__ M,
                                                  double *__restrict__ th, double *__restrict__ vth,
                                                  const double *__restrict__ Qa, int nb, int Mlen, double T, double h,
                                                  int e) {
   int ndof = 3 * nb;
   double m = M[e % nb];
-  double v = V[e], f = F[e];
+  double v = V[e];
   double t[MD_NHC_LENGTH_X], vt[MD_NHC_LENGTH_X], Q[MD_NHC_LENGTH_X], a[MD_NHC_LENGTH_X];
 #pragma unroll
   for (int k = 0; k < MD_NHC_LENGTH_X; ++k)
@@ -157,7 +157,7 @@ __global__ void k_nhc_massive(double *__restrict__ X, double *__restrict__ V, co
                               const double *__restrict__ Qa, int nb, int Mlen, double T, double h) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * nb) return;
-  nhc_massive_body<HALF>(X, V, F, M, th, vth, Qa, nb, Mlen, T, h, e);
+  nhc_massive_body<HALF>(X, V, F[e], M, th, vth, Qa, nb, Mlen, T, h, e);
 }
 
 // General grouping: thread per chain, its DOF range [f0, f0+f2) in the
@@ -322,13 +322,21 @@ struct BeadRep {
   double T, h;
   int pbc, image, trap;
   double L[3], w, c[3], om;
+  ForceSums cs;  // half 2: parts added into F first (cs.Fpart set)
 };
 
 // NHC half-kick (+ drift for HALF 1), then the box wrap after HALF 1, for
 // degree of freedom e < 3 nb.
 template <int HALF>
 __device__ __forceinline__ void nhc_wrap_body(const BeadRep &R, int e) {
-  nhc_massive_body<HALF>(R.X, R.V, R.F, R.M, R.th, R.vth, R.Q, R.nb, R.Mlen, R.T, R.h, e);
+  double f;
+  if (HALF == 2 && R.cs.Fpart) {
+    f = combined_force(R.F, R.cs, 3 * R.nb, e);
+    R.F[e] = f;
+  } else {
+    f = R.F[e];
+  }
+  nhc_massive_body<HALF>(R.X, R.V, f, R.M, R.th, R.vth, R.Q, R.nb, R.Mlen, R.T, R.h, e);
   if (HALF == 1 && R.pbc) {
     const int d = e / R.nb;
     const double L = R.L[d];
@@ -433,8 +441,9 @@ static BeadRep bead_rep(const BatchItem &it) {
 
 // Fused bead-local stages of one system's step (massive thermostats):
 // NHC half (+ wrap after half 1), or reset + trap + interior springs.
-int launch_step_nhc(const BatchItem &it, int half, hipStream_t st) {
-  const BeadRep R = bead_rep(it);
+int launch_step_nhc(const BatchItem &it, int half, hipStream_t st, const ForceSums *cs) {
+  BeadRep R = bead_rep(it);
+  if (half == 2 && cs) R.cs = *cs;
   const dim3 g((3 * R.nb + 255) / 256);
   if (half == 1)
     hipLaunchKernelGGL(k_nhc_wrap<1>, g, dim3(256), 0, st, R);

@@ -1537,30 +1537,31 @@ int launch_pair_real_partial(DevPimd *p, int kind, hipStream_t st) {
 }
 
 // F += real-space tile sums / P, then the reciprocal forces, then (when Fx
-// is given) the exchange forces of the end beads: the order, and so the
-// rounding, of k_reduce_tiles, k_recip_force_mfma and k_add_exch in turn.
-__global__ __launch_bounds__(256) void k_combine_forces(double *__restrict__ F, const double *__restrict__ Fpart,
-                                                        int n3, int T, double Pd, const double *__restrict__ Frec,
-                                                        const double *__restrict__ Fx, int N, int P) {
+// is given) the exchange forces of the end beads.
+__global__ __launch_bounds__(256) void k_combine_forces(double *__restrict__ F, ForceSums c, int n3) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n3) return;
-  double s = 0;
-  for (int q = 0; q < T; ++q) s += Fpart[(size_t)q * n3 + k];
-  double f = F[k] + s / Pd;
-  f += Frec[k];
-  if (Fx) {
-    const int nb = N * P, d = k / nb, i = k - d * nb;
-    if (i >= (P - 1) * N) f += Fx[d * N + (i - (P - 1) * N)];
-    if (i < N) f += Fx[(3 + d) * N + i];
-  }
-  F[k] = f;
+  F[k] = combined_force(F, c, n3, k);
+}
+
+int force_sums(DevPimd *p, const double *Fx, ForceSums *out) {
+  TileBufs &B = pair_bufs(p)->tb;
+  ForceSums c;
+  c.Fpart = B.Fpart;
+  c.Frec = B.Frec;
+  c.Fx = Fx;
+  c.T = B.T;
+  c.N = p->N;
+  c.P = p->P;
+  *out = c;
+  return 0;
 }
 
 int launch_combine_forces(DevPimd *p, const double *Fx, hipStream_t st) {
-  TileBufs &B = pair_bufs(p)->tb;
+  ForceSums c;
+  if (force_sums(p, Fx, &c)) return -1;
   const int n3 = 3 * p->ds->nb;
-  hipLaunchKernelGGL(k_combine_forces, dim3((n3 + 255) / 256), dim3(256), 0, st, p->ds->F, B.Fpart, n3, B.T,
-                     (double)p->P, B.Frec, Fx, p->N, p->P);
+  hipLaunchKernelGGL(k_combine_forces, dim3((n3 + 255) / 256), dim3(256), 0, st, p->ds->F, c, n3);
   XMD_CHECK(hipGetLastError());
   return 0;
 }

@@ -3,15 +3,16 @@
 // Stream layout of one step (the reference driver's order of stages):
 //   s0: NHC half 1 (+ wrap) --fork2--> cycle energies --fork--> reset F,
 //       trap, ring springs, pair real --wait join, join2--> add the pair
-//       and exchange end-bead forces, NHC half 2
+//       and exchange end-bead forces, NHC half 2 (one kernel with massive
+//       thermostats)
 // With massive thermostats NHC half 1 and the wrap are one kernel, and so are
 // the force reset, trap and interior springs.
 //   s1: (after fork) forward + backward recursions, end-bead exchange forces
 //       into a side buffer --join--> (added into F on s0)
 //   s2: (Ewald) phase tables, structure factors and reciprocal forces into a
 //       side buffer, forked from s0 right after NHC half 1 (they read
-//       positions only, so they overlap the cycle energies) --join2--> one
-//       kernel on s0 then adds the real-space, reciprocal and exchange forces
+//       positions only, so they overlap the cycle energies) --join2--> s0
+//       then adds the real-space, reciprocal and exchange forces
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces are added into F after both branches are done
 // (the serial profiling order computes and adds them there in one kernel).
@@ -94,7 +95,10 @@ static Lane main_lane() {
 
 // ev: optional per-stage events (2 per stage: start, stop), may be null.
 // serial: run every stage on s0 (true per-stage device times for profiling).
-static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R) {
+// fused (may be null: F complete on return): set (Fpart non-null) when the
+// pair and exchange parts are left for NHC half 2 to add into F (massive
+// thermostats), instead of a combine kernel.
+static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R, ForceSums *fused) {
   DevPimd *p = sp.p;
   DevSim *d = p->ds;
   hipStream_t sx = serial ? R.s0 : R.s1;
@@ -151,7 +155,11 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     if (kfork) {
       // one kernel adds the real-space, reciprocal and exchange forces
       XMD_CHECK(hipStreamWaitEvent(R.s0, R.join2, 0));
-      if (rec(12, R.s0) || launch_combine_forces(p, p->Fx, R.s0) || rec(13, R.s0)) return -1;
+      if (fused && d->massive) {
+        if (rec(12, R.s0) || force_sums(p, p->Fx, fused) || rec(13, R.s0)) return -1;
+      } else if (rec(12, R.s0) || launch_combine_forces(p, p->Fx, R.s0) || rec(13, R.s0)) {
+        return -1;
+      }
     } else if (rec(12, R.s0) || launch_add_exchange_force(p, R.s0) || rec(13, R.s0)) {
       return -1;
     }
@@ -171,9 +179,12 @@ static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R
     if (sp.pbc && launch_pbc(d, sp.L, R.s0)) return -1;
   }
   if (ev) XMD_CHECK(hipEventRecord(ev[1], R.s0));
-  if (enqueue_forces(sp, ev, serial, R)) return -1;
+  ForceSums fused;
+  if (enqueue_forces(sp, ev, serial, R, &fused)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[14], R.s0));
-  if (d->massive ? launch_step_nhc(item_of(sp), 2, R.s0) : launch_nhc(d, 2, sp.dt, R.s0)) return -1;
+  if (d->massive ? launch_step_nhc(item_of(sp), 2, R.s0, fused.Fpart ? &fused : nullptr)
+                 : launch_nhc(d, 2, sp.dt, R.s0))
+    return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[15], R.s0));
   return 0;
 }
@@ -253,7 +264,7 @@ int md_pimd_get_connection_matrix_x(md_pimd_t_x *pimd, md_num_t_x *G) {
 int md_pimd_compute_forces_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg) {
   StepPlan sp;
   if (plan_of(pimd, cfg, sp)) return -1;
-  return enqueue_forces(sp, nullptr, false, main_lane());
+  return enqueue_forces(sp, nullptr, false, main_lane(), nullptr);
 }
 
 int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps) {

@@ -147,6 +147,30 @@ extern "C" md_num_t_x md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C
 int pair_kind_of(md_pair_force_t_x pf);
 int pair_energy_kind_of(md_pair_energy_t_x pe);
 int launch_pair_force(DevPimd *p, int kind, hipStream_t st);
+// Pair and exchange forces computed in parts, summed into F: the real-space
+// tile sums over P, then the reciprocal forces, then (Fx given) the stored
+// exchange forces of the end beads (slices P-1 and 0).  Fpart == nullptr:
+// nothing to add.
+struct ForceSums {
+  const double *Fpart = nullptr, *Frec = nullptr, *Fx = nullptr;
+  int T = 0, N = 0, P = 0;
+};
+
+// F[k] plus the parts, in the order above (so the rounding is that of the
+// unsplit kernels: k_reduce_tiles, k_recip_force_mfma, k_add_exch in turn).
+__device__ __forceinline__ double combined_force(const double *__restrict__ F, const ForceSums &c, int n3, int k) {
+  double s = 0;
+  for (int q = 0; q < c.T; ++q) s += c.Fpart[(size_t)q * n3 + k];
+  double f = F[k] + s / (double)c.P;
+  f += c.Frec[k];
+  if (c.Fx) {
+    const int nb = c.N * c.P, d = k / nb, i = k - d * nb;
+    if (i >= (c.P - 1) * c.N) f += c.Fx[d * c.N + (i - (c.P - 1) * c.N)];
+    if (i < c.N) f += c.Fx[(3 + d) * c.N + i];
+  }
+  return f;
+}
+
 // Makes (host side, nothing launched) every buffer the step's pair kernels of
 // this kind need; 1 when anything was allocated, 0 when all existed, -1 on
 // error.
@@ -164,6 +188,8 @@ int pair_prepare(DevPimd *p, int kind);
 int launch_pair_kspace_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
 int launch_pair_real_partial(DevPimd *p, int kind, hipStream_t st);
 int launch_combine_forces(DevPimd *p, const double *Fx, hipStream_t st);
+// The parts launch_combine_forces adds, for a fused consumer of F.
+int force_sums(DevPimd *p, const double *Fx, ForceSums *out);
 int launch_pair_real(DevPimd *p, int kind, hipStream_t st);
 int launch_pair_recip(DevPimd *p, int kind, hipStream_t st);
 bool pair_has_recip(int kind);
@@ -210,7 +236,9 @@ int batch_table_make(BatchTable &t, const void *reps, size_t rep_bytes, const st
 void batch_table_free(BatchTable &t);
 
 // Fused bead-local stages of one system's step (massive thermostats only).
-int launch_step_nhc(const BatchItem &it, int half, hipStream_t st);
+// half 2 with cs (cs->Fpart set): F += the parts first (combined_force),
+// stored, and used by the half-kick.
+int launch_step_nhc(const BatchItem &it, int half, hipStream_t st, const ForceSums *cs = nullptr);
 int launch_step_bead_forces(const BatchItem &it, hipStream_t st);
 
 struct BeadBatch;  // integrator stages
