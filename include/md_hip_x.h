@@ -86,12 +86,12 @@ typedef struct {
 /* Forces at the current positions (step stages B3-B6). */
 int md_pimd_compute_forces_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg);
 /* nsteps full steps (B1-B7).  Eager (use_graph == 0 or nsteps == 1): returns
- * when the work is queued.  Graph (use_graph != 0): up to four consecutive
- * steps are captured once per call into one graph and replayed (steps left
- * over run eagerly); the executable graph is kept with the object and
- * updated in place by its next call (instantiated again only when the
- * graph's shape changed); returns when the steps are done, -1 if the
- * exchange recursion reported an error. */
+ * when the work is queued.  Graph (use_graph != 0): the step (four
+ * consecutive steps in calls of 32 steps or more) is captured once per call
+ * into one graph and replayed (steps left over run eagerly); the executable
+ * graph is kept with the object and updated in place by its next call
+ * (instantiated again only when the graph's shape changed); returns when the
+ * steps are done, -1 if the exchange recursion reported an error. */
 int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps);
 
 /* nsteps steps of n independent systems (replicas of uneven N, P, xi and

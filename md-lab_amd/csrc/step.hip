@@ -282,21 +282,23 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   // first, on the host; when that allocated anything, or on the first call
   // on an object, the first step runs eagerly (so allocation and the
   // capture's kernel-only graph cannot interfere) and the capture runs on the
-  // host while the device works on that step.  One graph holds up to
-  // kStepsPerGraph consecutive steps, which saves most of the boundaries
-  // between graph launches (0.2486 -> 0.2479 ms per step for UEG N=1024
-  // P=32); steps left over run eagerly after the launches.  An
+  // host while the device works on that step.  A call of at least
+  // kLongCall graph steps captures kStepsPerGraph consecutive steps into one
+  // graph, which saves most of the boundaries between graph launches
+  // (0.2486 -> 0.2479 ms per step for UEG N=1024 P=32); shorter calls
+  // capture one (the capture's host time would not be amortised); steps left
+  // over run eagerly after the launches.  An
   // executable graph kept from the previous call on the same object is
   // updated in place when the new graph has its topology, which saves the
   // instantiation.
-  constexpr int kStepsPerGraph = 4;
+  constexpr int kStepsPerGraph = 4, kLongCall = 32;
   Runtime &R = rt();
   DevPimd *p = sp.p;
   const int fresh = pair_prepare(p, sp.pair_kind);
   if (fresh < 0) return -1;
   const int eager = (fresh || !p->step_exec) ? 1 : 0;
   if (eager && enqueue_step(sp, nullptr, false, L)) return -1;
-  const int per = std::min(kStepsPerGraph, nsteps - eager), launches = (nsteps - eager) / per;
+  const int per = nsteps - eager >= kLongCall ? kStepsPerGraph : 1, launches = (nsteps - eager) / per;
   hipGraph_t graph = nullptr;
   XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
   int rc = 0;

@@ -28,8 +28,10 @@ def make(lib, kind, N, P, seed):
     return pyxmd.System(lib, kind, N, P, 0.4 if kind == "ueg" else 1.0, 1.0, seed=seed, L=L, nsum=8)
 
 
-# (time step, Ewald split alpha L, steps) of consecutive calls
-CALLS = [(0.01, -1.0, 4), (0.02, -1.0, 3), (0.02, 6.0, 5), (0.005, -1.0, 2), (0.01, 0.0, 3)]
+# (time step, Ewald split alpha L, steps) of consecutive calls; calls of 32
+# or more graph steps replay four steps per graph and run the rest eagerly
+CALLS = [(0.01, -1.0, 4), (0.02, -1.0, 3), (0.02, 6.0, 5), (0.005, -1.0, 39), (0.005, -1.0, 2), (0.01, 0.0, 3),
+         (0.01, 0.0, 35)]
 
 
 @pytest.mark.parametrize("kind,N,P", [("ueg", 70, 8), ("trap", 40, 16)])
