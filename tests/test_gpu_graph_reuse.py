@@ -56,3 +56,23 @@ def test_graph_reuse_matches_eager(kind, N, P):
         assert np.all(np.isfinite(state(other)[0]))
     finally:
         lib.md_set_ewald_split_x(-1.0)
+
+
+@pytest.mark.parametrize("kind,N,P", [("ueg", 70, 8)])
+def test_concurrent_step_matches_serial_order(kind, N, P):
+    """The three-stream step (real-space partial sums, reciprocal and exchange
+    forces in side buffers, added inside NHC half 2) follows, bit for bit, the
+    serial single-stream order that adds each part into F as it is made."""
+    lib = pyxmd.Lib()
+    lib.md_set_nhc_length_x(4)
+    lib.md_set_coulomb_truc_x(5.0 / ueg_box(70))
+    lib.md_set_coulomb_n_sum_x(8)
+    conc, ser = make(lib, kind, N, P, 5), make(lib, kind, N, P, 5)
+    cc = conc.step_config(0.01, use_graph=1)
+    cs = ser.step_config(0.01, use_graph=0)
+    stages = np.zeros(9)
+    assert lib.md_pimd_run_steps_x(conc.pimd, C.byref(cc), 37) == 0
+    assert lib.md_pimd_profile_steps_x(ser.pimd, C.byref(cs), 37, 1, pyxmd.dptr(stages)) == 0
+    assert lib.md_hip_synchronize_x() == 0
+    for a, b in zip(state(conc), state(ser)):
+        np.testing.assert_array_equal(a, b)
