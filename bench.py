@@ -196,8 +196,9 @@ def main():
         print(json.dumps({"copy_probe_bytes": 8 * n, "dispatches": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build "
-                                   "(replicas or bead slices would shard across GPUs, see DESIGN.md)"}))
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not supported "
+                                   "(independent replicas, or the bead slices of one system, would shard "
+                                   "across GPUs)"}))
         sys.exit(2)
 
     N, P, rs, theta, xi, nsum, dt = args.N, args.P, 2.0, 0.5, 1.0, 8, 0.05
