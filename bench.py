@@ -10,10 +10,14 @@ before timing.  One step = the full reference driver step (NHC half-kick,
 drift, wrap, forces: trap/pair/Ewald, cycle energies, V_B forward/backward,
 exchange forces, NHC half-kick), replayed as a captured hipGraph.
 
-Prints ONE JSON line.  `roofline` is for the dominant kernel (Ewald real
-space): algorithmic FP64 work per launch / its average duration measured with
-HIP events on its own stream.  `cpu_baseline` times the reference's own
-single-thread nocl build (oracle/_ref) on a bounded sample of the same step.
+Prints ONE JSON line.  `roofline` is for the kernel on the step's critical
+path that takes the most device time, the persistent exchange-recursion solve
+k_rec_persist (FP64 vector ALU, limited by its serial dependency chain):
+algorithmic FP64 work per launch / its average duration measured with HIP
+events around it on its own stream.  `kernels` gives the same figures for the
+other large kernels, and `step_rate` the algorithmic work of the whole step
+over ms_per_step.  `cpu_baseline` times the reference's own single-thread
+nocl build (oracle/_ref) on a bounded sample of the same step.
 """
 import argparse
 import ctypes as C
@@ -33,9 +37,40 @@ import pyxmd  # noqa: E402
 
 METRIC = "particle-bead-steps/s, N=1024 P=32 UEG; surrogate vs Ewald at ΔE≤1e-4 Ha"
 FP64_PEAK_TFLOPS = 78.6
-# Algorithmic FP64 flop-equivalents per real-space image term (exp, erfc,
-# sqrt-free count: r^2 5, sqrt 1, erfc 20, exp 20, prefactors 9, force 9).
+# Algorithmic FP64 flop-equivalents (add/mul 1, FMA 2, exp/erfc/log/sincos 20):
+# per real-space image term (r^2 5, sqrt 1, erfc 20, exp 20, prefactors 9, force 9)
 FLOP_PER_IMAGE = 65.0
+# per log-sum-exp term of the V_B / backward recursions (coefficient + value,
+# max, exp, accumulate)
+FLOP_PER_LSE_TERM = 23.0
+# per (bead, half-space k vector) of the reciprocal sum: phase product and
+# structure-factor accumulation (S), weight and force projection (F)
+FLOP_PER_BEAD_K = 37.0
+# per degree of freedom of the massive NHC update, both halves (M = 4)
+FLOP_PER_DOF_NHC = 470.0
+
+
+def recursion_work(N, xi):
+    """Algorithmic flop-eq of both recursion directions (N(N+1)/2 log-sum-exp
+    terms each; xi = 0 keeps only the N k = 1 terms)."""
+    terms = N * (N + 1) / 2 if xi != 0 else N
+    return 2 * terms * FLOP_PER_LSE_TERM
+
+
+def step_work(N, P, xi, alpha, L, rcut, nk):
+    """Algorithmic flop-eq of one full step at this workload, by stage."""
+    pairs = N * (N - 1) / 2 * P
+    w = {
+        "ewald_real": pairs * expected_images(alpha, L, alpha * rcut) * FLOP_PER_IMAGE,
+        "ewald_recip": N * P * nk * FLOP_PER_BEAD_K,
+        "recursions": recursion_work(N, xi),
+        # cycle energies (21 NP + 13.5 N^2) and exchange forces (one exp and
+        # 2 x 7 flops per G entry, N^2 entries per end)
+        "cycle_energies_exchange_forces": 21 * N * P + 13.5 * N * N + 2 * N * N * 34,
+        "nhc_bead_forces": 3 * N * P * (FLOP_PER_DOF_NHC + 10),
+    }
+    w["total"] = sum(w.values())
+    return w
 
 
 def expected_images(alpha, L, xcut):
@@ -82,11 +117,12 @@ def counter_flops(kernel, workload):
 def cpu_baseline(N, P, rs, theta, xi, nsum, pairs):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
     if not os.path.exists(exe):
-        return None
+        return {"unavailable": "oracle/_ref/ref_bench is not built (it is compiled by __graft_entry__.build() "
+                               "from the reference sources, which this machine does not have)"}
     out = subprocess.run(["taskset", "-c", "0", exe, str(N), str(P), str(rs), str(theta), str(xi), str(nsum),
                           str(pairs), "1"], capture_output=True, text=True, timeout=600)
     if out.returncode != 0:
-        return None
+        return {"unavailable": f"ref_bench exited with status {out.returncode}: {out.stderr.strip()[-200:]}"}
     kv = dict(t.split("=") for t in out.stdout.split())
     cpu = "unknown"
     try:
@@ -212,9 +248,9 @@ def main():
     s = pyxmd.System(lib, "ueg", N, P, T, xi, seed=1, L=L, nsum=nsum)
     cfg = s.step_config(dt, use_graph=1)
     names = ["nhc1_wrap", "cycle_energies", "vb_recursions", "bead_forces", "ewald_real", "ewald_recip",
-             "exchange_forces", "nhc2", "step"]
+             "exchange_forces", "nhc2", "step", "rec_persist"]
     if args.serial_only:
-        stages = np.zeros(9)
+        stages = np.zeros(10)
         assert lib.md_pimd_profile_steps_x(s.pimd, C.byref(cfg), args.steps, 1, pyxmd.dptr(stages)) == 0
         print(json.dumps({"serial_only": True, "steps": args.steps,
                           "stages_ms": {k: round(float(v), 5) for k, v in zip(names, stages)}}))
@@ -231,15 +267,21 @@ def main():
     value = N * P * args.steps / sec
 
     # per-stage device times, every stage alone on one stream (HIP events)
-    stages = np.zeros(9)
+    stages = np.zeros(10)
     assert lib.md_pimd_profile_steps_x(s.pimd, C.byref(cfg), 5, 1, pyxmd.dptr(stages)) == 0
-    real_ms = stages[4]
     a_, rc_, nk_ = C.c_double(), C.c_double(), C.c_int()
     assert lib.md_pimd_get_ewald_plan_x(s.pimd, C.byref(a_), C.byref(rc_), C.byref(nk_)) == 0
     alpha, rcut, nk = a_.value, rc_.value, nk_.value
-    pairs = N * (N - 1) / 2 * P
-    work = pairs * expected_images(alpha, L, alpha * rcut) * FLOP_PER_IMAGE
-    achieved = work / (real_ms * 1e-3) / 1e12
+    work = step_work(N, P, xi, alpha, L, rcut, nk)
+    rec_ms = stages[9]
+    achieved = work["recursions"] / (rec_ms * 1e-3) / 1e12
+    nblk = (N + 63) // 64
+    kernels = {}
+    for name, key, ms in (("k_ewald_real", "ewald_real", stages[4]), ("reciprocal (k_phase+k_sk_mfma+"
+                          "k_recip_force_mfma)", "ewald_recip", stages[5])):
+        tf = work[key] / (ms * 1e-3) / 1e12
+        kernels[name] = {"avg_ms": float(ms), "work_flop_eq": work[key], "achieved_tflops": tf,
+                         "frac": tf / FP64_PEAK_TFLOPS}
 
     b = s.host_beads()
     finite = bool(np.all(np.isfinite(b["x"])) and np.all(np.isfinite(b["f"])))
@@ -251,8 +293,9 @@ def main():
         cpu = cpu_baseline(N, P, rs, theta, xi, nsum, args.cpu_pairs)
 
     traffic_key = f"UEG N={N} P={P} alpha'L={alpha * L:.2f}"
-    traffic = measured_traffic("k_ewald_real", traffic_key)
-    hw_flop = counter_flops("k_ewald_real", traffic_key)
+    traffic = measured_traffic("k_rec_persist", traffic_key)
+    hw_flop = counter_flops("k_rec_persist", traffic_key)
+    step_tf = work["total"] / (sec / args.steps) / 1e12
     line = {
         "metric": METRIC, "value": value, "unit": "particle-bead-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -260,19 +303,25 @@ def main():
         "config": {"workload": f"UEG N={N} P={P} r_s={rs} theta={theta} xi={xi}, FP64 Ewald with the reference "
                                f"parameters alpha=5/L, |m|<=8 (converged; evaluated with the engine's split "
                                f"alpha'L={alpha * L:.2f}, r_c={rcut:.2f}, {nk} half-space k), NHC M=4, dt={dt}",
-                   "path": "plain FP64 (no surrogate)", "energy_error_vs_fp64_ewald": 0.0,
+                   "path": "plain FP64 Ewald (no surrogate)", "energy_error_vs_fp64_ewald": None,
                    "hipgraph": True, "finite_state": finite},
-        "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "limiter": "2N-long serial dependency chain (latency), not throughput",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_ewald_real (Ewald real space)", "avg_ms": real_ms,
-                     "work_flop_eq": work,
+                     "kernel": "k_rec_persist (V_B forward + backward recursion, both directions in one launch)",
+                     "avg_ms": float(rec_ms), "work_flop_eq": work["recursions"],
+                     "work_unit": f"2 directions x N(N+1)/2 log-sum-exp terms x {FLOP_PER_LSE_TERM:.0f} flop-eq",
+                     "blocks_per_direction": nblk, "us_per_block": 1e3 * float(rec_ms) / nblk,
                      "counted_fp64_flop": hw_flop,
-                     "counted_tflops": hw_flop / (real_ms * 1e-3) / 1e12 if hw_flop else None},
+                     "counted_tflops": hw_flop / (rec_ms * 1e-3) / 1e12 if hw_flop else None},
+        "kernels": kernels,
+        "step_rate": {"work_flop_eq_per_step": work, "achieved_tflops": step_tf,
+                      "frac": step_tf / FP64_PEAK_TFLOPS},
         "stages_ms": {k: round(float(v), 5) for k, v in zip(names, stages)},
         "cpu_baseline": cpu,
         "batched_replicas": batched,
     }
-    if cpu:
+    if cpu and "value" in cpu:
         line["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(line))
 

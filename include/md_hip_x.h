@@ -20,8 +20,18 @@ extern "C" {
 int md_set_nhc_length_x(int M);
 int md_get_nhc_length_x(void);
 
-/* Waits for all queued device work of the engine. */
+/* Waits for all queued device work of the engine; -1 if the exchange
+ * recursion of any object reported a timed-out hand-off since its last
+ * check (every object with a recursion launched since then is checked; a
+ * reported error is cleared). */
 int md_hip_synchronize_x(void);
+
+/* Test hook for the error path above: the next `count` forward recursion
+ * solves of this object (N > 128, where the solve hands blocks between
+ * workgroups) give up their first hand-off wait at once, as a timed-out
+ * wait does.  Every kernel still completes; the affected results are
+ * invalid and the next status check reports -1. */
+int md_pimd_inject_recursion_timeouts_x(md_pimd_t_x *pimd, int count);
 
 /* Cutoff x_cut of both Ewald series in units of the split alpha: real space
  * within x_cut / alpha, k space within pi |G| / alpha <= x_cut (default 5.25:
@@ -110,8 +120,9 @@ int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *
  *   [0] NHC first half + wrap   [1] cycle energies   [2] V_B recursions
  *   [3] reset/trap/ring springs [4] pair real space  [5] pair reciprocal
  *   [6] exchange end-bead forces [7] NHC second half [8] whole step (wall)
+ *   [9] the persistent recursion solve alone (part of [2]; serial only)
  * Each stage is bracketed by events on the stream it runs on. */
-#define MD_PIMD_NSTAGES_X 9
+#define MD_PIMD_NSTAGES_X 10
 int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, int serial,
                             double *stage_ms);
 

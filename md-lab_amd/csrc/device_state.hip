@@ -289,8 +289,8 @@ static int alloc_pimd(DevPimd *p) {
   // 64-row block the in-block (64*63/2) and previous-block (64*64) coefficients
   const size_t nblk = (N + 63) / 64;
   XMD_CHECK(hipMalloc(&p->rec, (6 * (size_t)(N + 1) + 2 * nblk * (64 * 63 / 2 + 64 * 64)) * sizeof(double)));
-  XMD_CHECK(hipMalloc(&p->recsync, (4 * nblk + 1) * sizeof(int)));
-  XMD_CHECK(hipMemset(p->recsync, 0, (4 * nblk + 1) * sizeof(int)));
+  XMD_CHECK(hipMalloc(&p->recsync, (4 * nblk + 2) * sizeof(int)));
+  XMD_CHECK(hipMemset(p->recsync, 0, (4 * nblk + 2) * sizeof(int)));
   XMD_CHECK(hipMalloc(&p->done, sizeof(int)));
   XMD_CHECK(hipMemset(p->done, 0, sizeof(int)));
   // ln n, n = 0..N, in the first N+1 doubles of the scratch (constant)

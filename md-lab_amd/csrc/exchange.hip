@@ -385,8 +385,11 @@ __device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t, c
   }
 }
 
+// The per-solve reset clears the hand-off flags only: the error word at
+// sync[4 nblk] keeps a timeout of any earlier solve until the host has read
+// it (recursion_status clears it there).
 __device__ __forceinline__ void rec_prep_body(const RecArgs &a, int dir, int first, int t) {
-  if (first && t < 4 * a.nblk + 1) a.sync[t] = 0;
+  if (first && t < 4 * a.nblk) a.sync[t] = 0;
   if (dir == 0)
     rec_prep_one<true>(a, 0, t);
   else
@@ -409,7 +412,7 @@ __global__ __launch_bounds__(256) void k_enk_prep(EnkArgs e, double *__restrict_
   }
   const int w = bx - e.N, dir = w / nprepb;
   const int t = (w - dir * nprepb) * blockDim.x + threadIdx.x;
-  if (dir == 0 && t < 4 * a.nblk + 1) a.sync[t] = 0;
+  if (dir == 0 && t < 4 * a.nblk) a.sync[t] = 0;  // hand-off flags only, as in rec_prep_body
   if (dir == 0)
     rec_prep_one<true, true>(a, 0, t, e);
   else
@@ -559,13 +562,14 @@ struct RecLds {
 };
 
 // Wave-level hand-off wait (lane 0 polls, the whole wave then acquires),
-// bounded like rec_wait.
-__device__ __forceinline__ void rec_wait_wave(int *flag, int target, int *err) {
+// bounded like rec_wait by `spin` polls (lane 0's value); spin < 0 gives up
+// before the first poll (the injected timeout of the test hook below).
+__device__ __forceinline__ void rec_wait_wave(int *flag, int target, int *err, int spin = REC_SPIN_MAX) {
   if ((threadIdx.x & 63) == 0) {
     int it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (spin < 0 || __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (++it > REC_SPIN_MAX) {
+      if (++it > spin) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -600,6 +604,16 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
   const int N = a.N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int *solved = a.sync + dir * a.nblk, *folded = a.sync + (2 + dir) * a.nblk, *err = a.sync + 4 * a.nblk;
+  // Test hook: while sync[4 nblk + 1] (set by the host, never by a normal
+  // run) is positive, a forward solve takes one from it and its first
+  // hand-off wait gives up at once, as a timed-out wait would.
+  int spin = REC_SPIN_MAX;
+  if (!SOLO && FWD && tid == 0) {
+    int *inject = a.sync + 4 * a.nblk + 1;
+    if (__hip_atomic_load(inject, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0 &&
+        __hip_atomic_fetch_add(inject, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0)
+      spin = -1;
+  }
   double *val = a.val2 + (size_t)dir * (N + 1);
   const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
   const double *DGd = a.dg + (size_t)dir * a.nblk * RTRI;
@@ -729,7 +743,10 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __builtin_amdgcn_wave_barrier();
         if (lane == 0) __hip_atomic_store(solved + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (more && b + 1 >= 2) rec_wait_wave(folded + b + 1, b, err);
+        if (more && b + 1 >= 2) {
+          rec_wait_wave(folded + b + 1, b, err, spin);
+          spin = REC_SPIN_MAX;
+        }
       }
       if (!SOLO && lane < nrows1) {
         M = AM[p1 + lane];
@@ -1269,19 +1286,50 @@ int launch_enk(DevPimd *p, int image, hipStream_t st) {
 }
 
 
-DevPimd *g_rec_last = nullptr;  // object of the last recursion launch (error checks)
+// Objects with a recursion launched since their error word was last read
+// (each listed once; objects live until the process ends).
+static std::vector<DevPimd *> g_rec_pending;
 
-// Error flag of the persistent solve (timed-out hand-off); call after a
-// synchronisation.
+static void rec_launched(DevPimd *p) {
+  for (DevPimd *q : g_rec_pending)
+    if (q == p) return;
+  g_rec_pending.push_back(p);
+}
+
+// Error word of the persistent solve (a timed-out hand-off in any solve since
+// the last read): read after a synchronisation, then cleared here on the
+// host, so a timeout in any step of a multi-step call is reported once.
 int recursion_status(DevPimd *p) {
   if (!p || !p->recsync) return 0;
+  for (size_t i = 0; i < g_rec_pending.size(); ++i)
+    if (g_rec_pending[i] == p) {
+      g_rec_pending.erase(g_rec_pending.begin() + i);
+      break;
+    }
   const int nblk = (p->N + RB - 1) / RB;
   int err = 0;
   XMD_CHECK(hipMemcpy(&err, p->recsync + 4 * nblk, sizeof(int), hipMemcpyDeviceToHost));
   if (err) {
     fprintf(stderr, "xmd: exchange recursion hand-off timed out\n");
+    XMD_CHECK(hipMemset(p->recsync + 4 * nblk, 0, sizeof(int)));
     return -1;
   }
+  return 0;
+}
+
+// recursion_status of every object with a recursion launched since its last
+// read (after a device-wide synchronisation).
+int recursion_status_all() {
+  int rc = 0;
+  while (!g_rec_pending.empty())
+    if (recursion_status(g_rec_pending.back())) rc = -1;
+  return rc;
+}
+
+int inject_recursion_timeouts(DevPimd *p, int count) {
+  const int nblk = (p->N + RB - 1) / RB;
+  XMD_CHECK(hipStreamSynchronize(rt().s0));
+  XMD_CHECK(hipMemcpy(p->recsync + 4 * nblk + 1, &count, sizeof(int), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1316,14 +1364,14 @@ int launch_recursion_solve(DevPimd *p, hipStream_t st) {
   const int per = 1 + std::max(0, a.nblk - 2);
   hipLaunchKernelGGL(k_rec_persist, dim3(2 * per), dim3(256), 0, st, a, 0);
   XMD_CHECK(hipGetLastError());
-  g_rec_last = p;
+  rec_launched(p);
   p->vbn_dev = true;
   p->back_dev = true;
   p->back_version = p->enk_version;
   return 0;
 }
 
-int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
+int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st, hipEvent_t mid) {
   const int ndir = (fwd ? 1 : 0) + (bwd ? 1 : 0);
   if (ndir == 0) return 0;
   const int dir0 = fwd ? 0 : 1;
@@ -1332,10 +1380,11 @@ int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st) {
   const int nprep = std::max(N + 1, a.nblk * RB * RB);
   hipLaunchKernelGGL(k_rec_prep, dim3((nprep + 255) / 256, ndir), dim3(256), 0, st, a, dir0);
   XMD_CHECK(hipGetLastError());
+  if (mid) XMD_CHECK(hipEventRecord(mid, st));
   const int per = 1 + std::max(0, a.nblk - 2);
   hipLaunchKernelGGL(k_rec_persist, dim3(ndir * per), dim3(256), 0, st, a, dir0);
   XMD_CHECK(hipGetLastError());
-  g_rec_last = p;
+  rec_launched(p);
   if (fwd) p->vbn_dev = true;
   if (bwd) {
     p->back_dev = true;

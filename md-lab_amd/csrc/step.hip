@@ -121,7 +121,8 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     XMD_CHECK(hipEventRecord(R.fork, R.s0));
     XMD_CHECK(hipStreamWaitEvent(R.s1, R.fork, 0));
   }
-  if (rec(4, sx) || (serial ? launch_recursion(p, true, true, sx) : launch_recursion_solve(p, sx)) || rec(5, sx))
+  if (rec(4, sx) || (serial ? launch_recursion(p, true, true, sx, ev ? ev[16] : nullptr) : launch_recursion_solve(p, sx)) ||
+      rec(5, sx))
     return -1;
   // overlapped: the end-bead exchange forces follow the recursions on s1
   // into a side buffer, added into F after the join
@@ -245,7 +246,12 @@ int md_hip_synchronize_x(void) {
   if (ensure_runtime()) return -1;
   XMD_CHECK(hipStreamSynchronize(rt().s1));
   XMD_CHECK(hipStreamSynchronize(rt().s0));
-  return recursion_status(g_rec_last);
+  return recursion_status_all();
+}
+
+int md_pimd_inject_recursion_timeouts_x(md_pimd_t_x *pimd, int count) {
+  if (count < 0 || ensure_pimd_device(pimd)) return -1;
+  return inject_recursion_timeouts(pimd_state(pimd, false), count);
 }
 
 int md_pimd_get_backward_potentials_x(md_pimd_t_x *pimd, md_num_t_x *Vt) {
@@ -334,7 +340,7 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
                             double *stage_ms) {
   StepPlan sp;
   if (plan_of(pimd, cfg, sp)) return -1;
-  const int NE = 16;
+  const int NE = 17;
   hipEvent_t ev[NE];
   for (int k = 0; k < NE; ++k) XMD_CHECK(hipEventCreate(&ev[k]));
   for (int k = 0; k < MD_PIMD_NSTAGES_X; ++k) stage_ms[k] = 0;
@@ -352,6 +358,10 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
     float ms = 0;
     XMD_CHECK(hipEventElapsedTime(&ms, ev[0], ev[15]));
     stage_ms[8] += ms;
+    if (serial) {
+      XMD_CHECK(hipEventElapsedTime(&ms, ev[16], ev[5]));
+      stage_ms[9] += ms;
+    }
   }
   for (int k = 0; k < MD_PIMD_NSTAGES_X; ++k) stage_ms[k] /= nsteps;
   for (int k = 0; k < NE; ++k) XMD_CHECK(hipEventDestroy(ev[k]));

@@ -103,7 +103,7 @@ struct DevPimd {
   double *work = nullptr;   // estimator scratch
   double *Fx = nullptr;     // exchange forces of the end beads, [slice P-1, slice 0][dim][particle] (6N)
   double *rec = nullptr;    // recursion state: values, accumulators, block coefficients
-  int *recsync = nullptr;   // recursion hand-off flags: 4*nblk + 1
+  int *recsync = nullptr;   // recursion hand-off flags 4*nblk, error word, injected-timeout count
   int *done = nullptr;      // finished-workgroup count of k_chain_scan (zero between launches)
   void *pair_bufs = nullptr;  // pair-kernel tile schedule, partial sums and k-space tables (pair_forces.hip)
   void *step_exec = nullptr;  // executable step graph of the last md_pimd_run_steps_x (hipGraphExec_t)
@@ -202,7 +202,8 @@ int launch_enk(DevPimd *p, int image, hipStream_t st);
 int launch_enk_prep(DevPimd *p, int image, hipStream_t st);
 int launch_recursion_solve(DevPimd *p, hipStream_t st);
 int init_lnn(DevPimd *p);  // ln n table at p->work[0..N], once per object
-int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st);
+// mid (optional): recorded between the set-up and the persistent solve.
+int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st, hipEvent_t mid = nullptr);
 // stored: write the end-bead forces to p->Fx instead of adding them to F;
 // launch_add_exchange_force adds them later (both on the caller's ordering).
 int launch_exchange_force(DevPimd *p, int image, hipStream_t st, bool stored);
@@ -212,8 +213,14 @@ int backward_potentials(DevPimd *p, double *host_Vt);
 int bead_sum(DevSim *d, int mode, const double *trap_c, double w, const double *L, double *out);
 extern double g_ewald_xcut;
 extern double g_ewald_split_aL;
-extern DevPimd *g_rec_last;
+// Error word of an object's persistent recursion solve: recursion_status
+// reads and clears it (-1 when a hand-off timed out since the last read);
+// recursion_status_all does so for every object with a recursion launched
+// since its last read.  inject_recursion_timeouts: the next `count` forward
+// solves of the object give up their first hand-off wait (test hook).
 int recursion_status(DevPimd *p);
+int recursion_status_all();
+int inject_recursion_timeouts(DevPimd *p, int count);
 
 // ------------------------------------------------------ batched replicas
 // One launch per step stage covers every replica of a batch: each stage has

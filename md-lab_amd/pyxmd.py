@@ -135,6 +135,7 @@ _SIGS = {
     "md_set_nhc_length_x": (C.c_int, [C.c_int]),
     "md_get_nhc_length_x": (C.c_int, []),
     "md_hip_synchronize_x": (C.c_int, []),
+    "md_pimd_inject_recursion_timeouts_x": (C.c_int, [PIMD_P, C.c_int]),
     "md_set_ewald_real_cutoff_x": (C.c_int, [C.c_double]),
     "md_set_ewald_split_x": (C.c_int, [C.c_double]),
     "md_pimd_get_ewald_plan_x": (C.c_int, [PIMD_P, DP, DP, C.POINTER(C.c_int)]),
@@ -258,12 +259,15 @@ def ueg_temperature(N, L, theta):
 
 class System:
     """A PIMD system set up through the C ABI in the driver order of the
-    reference (init, allocation, positions, Maxwell velocities, chains)."""
+    reference (init, allocation, positions, Maxwell velocities, chains).
+    kind: "trap" (harmonic trap + Gaussian pair, open boundaries), "ueg"
+    (periodic box, Ewald Coulomb) or "pgauss" (periodic box, periodic
+    Gaussian pair, no trap)."""
 
     def __init__(self, lib, kind, N, P, T, xi, seed=1, L=10.0, nsum=2, fgroup=1, positions=None,
                  ueg_fluc=0.4, alpha_L=5.0):
         self.lib, self.kind, self.N, self.P = lib, kind, N, P
-        self.image = 1 if kind == "ueg" else 0
+        self.image = 0 if kind == "trap" else 1
         self.L = L
         cfg = InitConfig(1, 1, 0, seed, b"")
         if lib.md_init_setup_x(None, C.byref(cfg)) != 0:
@@ -314,11 +318,13 @@ class System:
     def omegaP(self):
         return self.pimd.contents.omegaP
 
+    _PAIR = {"trap": "md_gaussian", "ueg": "md_coulomb_3d_ewald", "pgauss": "md_periodic_gaussian"}
+
     def pair_cb(self):
-        return self.lib.cb["md_gaussian_force_x" if self.kind == "trap" else "md_coulomb_3d_ewald_force_x"]
+        return self.lib.cb[self._PAIR[self.kind] + "_force_x"]
 
     def pair_energy_cb(self):
-        return self.lib.cb["md_gaussian_energy_x" if self.kind == "trap" else "md_coulomb_3d_ewald_energy_x"]
+        return self.lib.cb[self._PAIR[self.kind] + "_energy_x"]
 
     def full_force(self):
         lib = self.lib
@@ -336,7 +342,7 @@ class System:
         lib = self.lib
         if lib.md_update_nhc_VV3_1_x(self.sim, dt) != 0:
             raise RuntimeError("VV3_1 failed")
-        if self.kind == "ueg":
+        if self.image:
             lib.md_periodic_boundary_x(self.sim)
         self.full_force()
         if lib.md_update_nhc_VV3_2_x(self.sim, dt) != 0:
@@ -345,7 +351,7 @@ class System:
     def step_config(self, dt, use_graph=1):
         lib = self.lib
         trap = lib.cb["md_harmonic_trap_force_x"] if self.kind == "trap" else None
-        return StepConfig(trap, self.pair_cb(), dt, 1 if self.kind == "ueg" else 0, self.image, use_graph)
+        return StepConfig(trap, self.pair_cb(), dt, self.image, self.image, use_graph)
 
     def host_beads(self):
         if not self.lib.md_simulation_sync_host_x(self.sim, 1):
