@@ -674,6 +674,133 @@ void orc_fast_force_VB(int N, int P, double beta, double vi, double omegaP, int 
   free(G);
 }
 
+/* Extended-precision twin of orc_fast_fill_ENk + orc_fill_VB +
+ * orc_fast_force_VB (the same formulas, md_pimd_x.c:814-953, 1969-2210,
+ * 2212-2721), every intermediate in x86-64 80-bit long double (64-bit
+ * significand): springs, cycle energies, the forward and backward
+ * log-sum-exp recursions, the connection matrix G and the G-weighted spring
+ * gradients.  On inputs where beta V_B ~ 1e4..1e5 the G entries 1 - exp(-x)
+ * and exp(...) carry the absolute round-off of x in double precision
+ * (~1e-11), which both the reference and any double-precision code inherit;
+ * here that round-off is ~2^11 times smaller, so the result is the truth the
+ * double-precision codes are measured against.  Output: the exchange and
+ * ring-spring forces (accelerations) of every bead, rounded to double, and
+ * V_B^(0..N). */
+typedef long double ld;
+
+static ld ld_comp(ld a, ld Lx, int image) { return image ? a - Lx * roundl(a / Lx) : a; }
+
+static ld ld_spring(const orc_particle_t *a, const orc_particle_t *c, ld om, int image, const double *L) {
+  ld d2 = 0;
+  for (int i = 0; i < 3; ++i) {
+    ld d = ld_comp((ld)a->x[i] - (ld)c->x[i], (ld)L[i], image);
+    d2 += d * d;
+  }
+  return 0.5L * (ld)a->m * om * om * d2;
+}
+
+void orc_exchange_forces_ld(int N, int P, double beta_d, double vi_d, double omegaP, int image, const double *L,
+                            const orc_particle_t *b, double *f_out, double *VBN_out) {
+  const ld beta = beta_d, vi = vi_d, om = omegaP;
+  const ld lnvi = vi > 0 ? logl(vi) : 0;
+  ld *E = (ld *)malloc(sizeof(ld) * (size_t)N * (N + 1) / 2);
+  ld *Eint = (ld *)malloc(sizeof(ld) * N);
+  ld *V = (ld *)malloc(sizeof(ld) * (N + 1));
+  ld *Vt = (ld *)malloc(sizeof(ld) * (N + 1));
+  ld *G = (ld *)malloc(sizeof(ld) * (size_t)N * N);
+  for (int l = 1; l <= N; ++l) {
+    ld e = 0;
+    for (int j = 1; j < P; ++j) e += ld_spring(&b[BEAD(l, j, P)], &b[BEAD(l, j + 1, P)], om, image, L);
+    Eint[l - 1] = e;
+    E[TRI(l, 1)] = e + ld_spring(&b[BEAD(l, P, P)], &b[BEAD(l, 1, P)], om, image, L);
+  }
+  for (int l = 1; l <= N; ++l)
+    for (int k = 2; k <= l; ++k)
+      E[TRI(l, k)] = E[TRI(l, k - 1)] + Eint[l - k] -
+                     ld_spring(&b[BEAD(l, P, P)], &b[BEAD(l - k + 2, 1, P)], om, image, L) +
+                     ld_spring(&b[BEAD(l - k + 1, P, P)], &b[BEAD(l - k + 2, 1, P)], om, image, L) +
+                     ld_spring(&b[BEAD(l, P, P)], &b[BEAD(l - k + 1, 1, P)], om, image, L);
+  /* forward: V^(n) = -(1/beta) ln (1/n) sum_k xi^(k-1) exp(-beta (E_n^(k) + V^(n-k))) */
+  V[0] = 0;
+  for (int n = 1; n <= N; ++n) {
+    ld t = 1e4000L;
+    for (int k = 1; k <= n; ++k) {
+      if (vi == 0 && k != 1) continue;
+      ld x = -(k - 1) * lnvi + beta * (E[TRI(n, k)] + V[n - k]);
+      if (x < t) t = x;
+    }
+    ld s = 0;
+    for (int k = 1; k <= n; ++k) {
+      if (vi == 0 && k != 1) continue;
+      s += expl((k - 1) * lnvi - beta * (E[TRI(n, k)] + V[n - k]) + t);
+    }
+    V[n] = (t - logl(s) + logl((ld)n)) / beta;
+  }
+  /* backward: Vt^(u-1) = -(1/beta) ln sum_{l>=u} xi^(l-u) exp(-beta (E_l^(l-u+1) + Vt^(l))) / l */
+  Vt[N] = 0;
+  for (int u = N; u >= 1; --u) {
+    ld t = 1e4000L;
+    for (int l = u; l <= N; ++l) {
+      int k = l - u + 1;
+      if (vi == 0 && k != 1) continue;
+      ld x = -(k - 1) * lnvi + logl((ld)l) + beta * (E[TRI(l, k)] + Vt[l]);
+      if (x < t) t = x;
+    }
+    ld s = 0;
+    for (int l = u; l <= N; ++l) {
+      int k = l - u + 1;
+      if (vi == 0 && k != 1) continue;
+      s += expl((k - 1) * lnvi - beta * (E[TRI(l, k)] + Vt[l]) + t - logl((ld)l));
+    }
+    Vt[u - 1] = (t - logl(s)) / beta;
+  }
+  for (int l = 1; l <= N; ++l)
+    for (int j = 1; j <= N; ++j) {
+      ld g = 0;
+      if (j == l + 1)
+        g = -expm1l(-beta * (V[l] + Vt[l] - V[N]));
+      else if (j <= l && !(vi == 0 && l != j))
+        g = expl(-beta * (V[j - 1] + E[TRI(l, l - j + 1)] + Vt[l] - V[N]) + (l - j) * lnvi) / l;
+      G[(size_t)(l - 1) * N + (j - 1)] = g;
+    }
+  const ld k2 = om * om;
+  for (int l = 1; l <= N; ++l)
+    for (int j = 1; j <= P; ++j) {
+      const int idx = BEAD(l, j, P);
+      const ld m = b[idx].m;
+      ld f[3] = {0, 0, 0};
+      if (j > 1 && j < P) {
+        for (int i = 0; i < 3; ++i) {
+          ld a = ld_comp((ld)b[idx].x[i] - (ld)b[BEAD(l, j + 1, P)].x[i], (ld)L[i], image);
+          ld c = ld_comp((ld)b[idx].x[i] - (ld)b[BEAD(l, j - 1, P)].x[i], (ld)L[i], image);
+          f[i] = -(m * k2 * (a + c)) / m;
+        }
+      } else {
+        /* bead 1: partners bead P of every j' weighted by G(j', l); bead P:
+         * bead 1 of every j' weighted by G(l, j') */
+        const int inner = (j == 1) ? BEAD(l, 2, P) : BEAD(l, P - 1, P);
+        for (int jp = 1; jp <= N; ++jp) {
+          const ld g = (j == 1) ? G[(size_t)(jp - 1) * N + (l - 1)] : G[(size_t)(l - 1) * N + (jp - 1)];
+          if (g == 0) continue;
+          const int other = (j == 1) ? BEAD(jp, P, P) : BEAD(jp, 1, P);
+          for (int i = 0; i < 3; ++i) {
+            ld a = ld_comp((ld)b[idx].x[i] - (ld)b[inner].x[i], (ld)L[i], image);
+            ld c = ld_comp((ld)b[idx].x[i] - (ld)b[other].x[i], (ld)L[i], image);
+            f[i] += -g * (m * k2 * (a + c)) / m;
+          }
+        }
+      }
+      for (int i = 0; i < 3; ++i) f_out[3 * idx + i] = (double)f[i];
+    }
+  if (VBN_out)
+    for (int n = 0; n <= N; ++n) VBN_out[n] = (double)V[n];
+  free(E);
+  free(Eint);
+  free(V);
+  free(Vt);
+  free(G);
+}
+
 /* md_pimd_x.c:967-995: gradient of the k-cycle ending at n w.r.t. bead (l, j). */
 static void slow_dENk(const orc_particle_t *b, int P, int n, int k, int l, int j, double omegaP,
                       int image, const double *L, double *g) {

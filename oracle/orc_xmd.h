@@ -125,6 +125,10 @@ void orc_slow_force_VB(int N, int P, double beta, double vi, double omegaP, int 
 
 /* Estimators (return the value the reference adds into stats->es). */
 double orc_VBN_energy(int N, int P, double beta, double vi, const double *ENk, const double *VBN);
+/* The exchange path (cycle energies, V_B, backward recursion, G, exchange and
+ * ring-spring forces) in 80-bit long double: the extended-precision truth. */
+void orc_exchange_forces_ld(int N, int P, double beta, double vi, double omegaP, int image, const double *L,
+                            const orc_particle_t *b, double *f_out, double *VBN_out);
 void orc_fast_fill_ENk2(int N, int P, double omegaP, int image, const double *L,
                         const orc_particle_t *b, double *ENk2);
 void orc_fill_ENk2_slow(int N, int P, double omegaP, int image, const double *L,

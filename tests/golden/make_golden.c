@@ -8,6 +8,8 @@
  *
  * Run in the build container only (the reference tree is not shipped):
  *   make -C oracle ref && oracle/_ref/make_golden tests/golden/golden.bin
+ *   oracle/_ref/make_golden tests/golden/configs.bin configs
+ *   oracle/_ref/make_golden tests/golden/metric.bin metric
  */
 #include "md_pimd_x.h"
 #include "md_simulation_x.h"
@@ -106,12 +108,14 @@ typedef struct {
   unsigned seed;
 } gcase;
 
+/* kind 0: trap + Gaussian; 1: UEG, Ewald; 2: periodic box, periodic Gaussian */
 static md_pair_force_t_x pair_force_of(const gcase *c) {
-  return c->kind == 0 ? md_gaussian_force_x : md_coulomb_3d_ewald_force_x;
+  return c->kind == 0 ? md_gaussian_force_x : (c->kind == 1 ? md_coulomb_3d_ewald_force_x : md_periodic_gaussian_force_x);
 }
 
 static md_pair_energy_t_x pair_energy_of(const gcase *c) {
-  return c->kind == 0 ? md_gaussian_energy_x : md_coulomb_3d_ewald_energy_x;
+  return c->kind == 0 ? md_gaussian_energy_x
+                      : (c->kind == 1 ? md_coulomb_3d_ewald_energy_x : md_periodic_gaussian_energy_x);
 }
 
 static void set_params(const gcase *c) {
@@ -262,6 +266,21 @@ static void run_case(const gcase *c) {
   st->es[0] = 0;
   md_pimd_calc_pair_energy_x(pimd, pair_energy_of(c), st);
   dumpf(pf, "e_pair", (double[]){stat_of(st)}, 1);
+  if (c->kind == 1) {
+    /* the Ewald energy splits R, R2 and NI */
+    md_pair_energy_t_x sp[3] = {md_coulomb_3d_ewald_energy_R_x, md_coulomb_3d_ewald_energy_R2_x,
+                                md_coulomb_3d_ewald_energy_NI_x};
+    double es[3];
+    for (int q = 0; q < 3; ++q) {
+      st->es[0] = 0;
+      md_pimd_calc_pair_energy_x(pimd, sp[q], st);
+      es[q] = stat_of(st);
+    }
+    dumpf(pf, "e_pair_R_R2_NI", es, 3);
+  }
+  st->Ts[0] = 0;
+  md_calc_temperature_x(sim, st);
+  dumpf(pf, "e_temperature", (double[]){st->Ts[0]}, 1);
   st->es[0] = 0;
   md_calc_kinetic_x(sim, st);
   dumpf(pf, "e_kin", (double[]){stat_of(st)}, 1);
@@ -291,7 +310,7 @@ static void run_case(const gcase *c) {
   double *ev = malloc(sizeof(double) * (c->steps > 0 ? c->steps : 1));
   for (int s = 0; s < c->steps; ++s) {
     md_update_nhc_VV3_1_x(sim, c->dt);
-    if (c->kind == 1) md_periodic_boundary_x(sim);
+    if (c->kind != 0) md_periodic_boundary_x(sim);
     full_force(c, pimd);
     md_update_nhc_VV3_2_x(sim, c->dt);
     md_pimd_fast_fill_ENk2_x(pimd, c->image);
@@ -640,7 +659,7 @@ static void config_case(const gcase *c, const double *xis, int nxi, int stride) 
   double *ev = malloc(sizeof(double) * (c->steps > 0 ? c->steps : 1));
   for (int s = 0; s < c->steps; ++s) {
     md_update_nhc_VV3_1_x(sim, c->dt);
-    if (c->kind == 1) md_periodic_boundary_x(sim);
+    if (c->kind != 0) md_periodic_boundary_x(sim);
     full_force(c, pimd);
     md_update_nhc_VV3_2_x(sim, c->dt);
     md_pimd_fast_fill_ENk2_x(pimd, c->image);
@@ -657,6 +676,184 @@ static void config_case(const gcase *c, const double *xis, int nxi, int stride) 
   free(f0);
 }
 
+/* Headline configuration (the benchmark workload): spin-polarised UEG
+ * N=1024, P=32, r_s=2, theta=0.5, xi=1, alpha=5/L, n=8, set up from seed 1
+ * exactly as the benchmark sets it up.  Records:
+ *   x0          every bead's position (the seeded state, for a bit-exact check)
+ *   enk_rows    rows n of the cycle-energy table E_n^(1..n) for n in ENK_ROWS
+ *   VBN         V_B^(0..N)
+ *   fx_end      exchange + ring-spring force (md_reset_force_x, then the fast
+ *               path md_pimd_fast_fill_ENk_x / _fill_VB_x /
+ *               _fast_fill_force_VB_x) of beads 1 and P of every particle
+ *   fx_mid      the same for every 31st bead (interior springs)
+ *   e_est       thermodynamic (md_pimd_calc_VBN_energy_x) and virial
+ *               (md_pimd_fast_fill_ENk2_x + md_pimd_calc_vi_virial_energy_x)
+ *               estimators at that force state
+ *   f_slice_j   full Ewald pair force of bead slice j (1-based) for j in
+ *               SLICES: the reference's md_pimd_calc_pair_force_x on a P=1
+ *               object holding that slice's positions (so F itself, not F/P)
+ *   e_slice_j   md_pimd_calc_pair_energy_x of the same P=1 object
+ * The four slice evaluations (N(N-1)/2 pairs of two 17^3 image cubes each,
+ * about 150 s per evaluation on one core) run on four threads; the
+ * reference's interaction callbacks only read the process globals. */
+#include <pthread.h>
+
+enum { M_N = 1024, M_P = 32, M_NSLICE = 2 };
+static const int SLICES[M_NSLICE] = {1, 17};
+static const int ENK_ROWS[] = {1, 2, 63, 64, 65, 129, 500, 1023, 1024};
+
+typedef struct {
+  md_simulation_t_x *src;
+  int slice, energy;
+  double L, T;
+  double *out; /* 3N forces or 1 energy */
+} slice_job;
+
+static void *slice_eval(void *arg) {
+  slice_job *J = arg;
+  md_simulation_t_x *sim = md_alloc_simulation_x(M_N, 1, 0, MD_ND_RECT_BOX_X);
+  md_num_t_x L3[3] = {J->L, J->L, J->L};
+  md_init_nd_rect_box_x(sim->box, L3);
+  md_pimd_t_x *pimd = md_alloc_pimd_x(sim, M_N, 1, 1.0, J->T);
+  md_init_particle_mass_x(sim, 1.0, 0, M_N);
+  for (int l = 1; l <= M_N; ++l)
+    for (int d = 0; d < 3; ++d)
+      sim->particles[l - 1].x[d] = J->src->particles[MD_PIMD_INDEX_X(l, J->slice, M_P)].x[d];
+  if (J->energy) {
+    md_stats_t_x *st = md_alloc_stats_x(1);
+    st->es[0] = 0;
+    md_pimd_calc_pair_energy_x(pimd, md_coulomb_3d_ewald_energy_x, st);
+    J->out[0] = st->es[0];
+  } else {
+    md_reset_force_x(sim);
+    md_pimd_calc_pair_force_x(pimd, md_coulomb_3d_ewald_force_x);
+    for (int l = 0; l < M_N; ++l)
+      for (int d = 0; d < 3; ++d) J->out[3 * l + d] = sim->particles[l].f[d];
+  }
+  return NULL;
+}
+
+static void metric_case(void) {
+  const char *pf = "cfg_ueg1024";
+  const int N = M_N, P = M_P, nb = N * P;
+  const double rs = 2.0, L = pow(4 * M_PI * N / 3.0, 1.0 / 3.0) * rs;
+  const double kF = pow(6 * M_PI * M_PI * N / (L * L * L), 1.0 / 3.0);
+  /*           name N  P  T             vi   kind img L  nsum fgrp steps dt  seed */
+  gcase c = {pf, N, P, 0.25 * kF * kF, 1.0, 1, 1, L, 8, 1, 0, 0.05, 1};
+  set_params(&c);
+  md_set_seed_x(c.seed);
+  md_simulation_t_x *sim = md_alloc_simulation_x(nb, 1, 0, MD_ND_RECT_BOX_X);
+  md_num_t_x L3[3] = {L, L, L};
+  md_init_nd_rect_box_x(sim->box, L3);
+  md_pimd_t_x *pimd = md_alloc_pimd_x(sim, N, P, c.vi, c.T);
+  md_init_particle_mass_x(sim, 1.0, 0, nb);
+  md_num_t_x ctr[3] = {L / 2, L / 2, L / 2};
+  md_pimd_init_particle_uniform_pos_x(pimd, ctr, L3, 0.4);
+  md_periodic_boundary_x(sim);
+  md_init_maxwell_vel_x(sim, c.T);
+  double meta[10] = {N, P, c.T, c.vi, c.kind, c.image, L, c.nsum, 1, c.dt};
+  dumpf(pf, "meta", meta, 10);
+  dump_sub(pf, "x0", sim, 0, 1);
+  /* slice evaluations in the background while the exchange path runs */
+  slice_job jobs[2 * M_NSLICE];
+  pthread_t th[2 * M_NSLICE];
+  for (int q = 0; q < 2 * M_NSLICE; ++q) {
+    jobs[q].src = sim;
+    jobs[q].slice = SLICES[q % M_NSLICE];
+    jobs[q].energy = q >= M_NSLICE;
+    jobs[q].L = L;
+    jobs[q].T = c.T;
+    jobs[q].out = malloc(sizeof(double) * 3 * N);
+  }
+  /* the slice objects copy positions before the exchange path changes f
+   * only; positions stay untouched, so the threads may read them */
+  for (int q = 0; q < 2 * M_NSLICE; ++q) pthread_create(&th[q], NULL, slice_eval, &jobs[q]);
+  md_reset_force_x(sim);
+  md_pimd_fast_fill_ENk_x(pimd, 1);
+  md_pimd_fill_VB_x(pimd);
+  md_pimd_fast_fill_force_VB_x(pimd, 1);
+  int nrows = (int)(sizeof ENK_ROWS / sizeof ENK_ROWS[0]), tot = 0;
+  for (int r = 0; r < nrows; ++r) tot += ENK_ROWS[r];
+  double *rows = malloc(sizeof(double) * tot);
+  int w = 0;
+  for (int r = 0; r < nrows; ++r)
+    for (int k = 1; k <= ENK_ROWS[r]; ++k) rows[w++] = pimd->ENk[ENK_ROWS[r] - 1][k - 1];
+  double rowid[sizeof ENK_ROWS / sizeof ENK_ROWS[0]];
+  for (int r = 0; r < nrows; ++r) rowid[r] = ENK_ROWS[r];
+  dumpf(pf, "enk_row_ids", rowid, nrows);
+  dumpf(pf, "enk_rows", rows, tot);
+  dumpf(pf, "VBN", pimd->VBN, N + 1);
+  double *fe = malloc(sizeof(double) * 6 * N);
+  for (int e = 0; e < 2; ++e)
+    for (int l = 1; l <= N; ++l)
+      for (int d = 0; d < 3; ++d) fe[(e * N + l - 1) * 3 + d] = sim->particles[MD_PIMD_INDEX_X(l, e ? P : 1, P)].f[d];
+  dumpf(pf, "fx_end", fe, 6 * N);
+  dump_sub(pf, "fx_mid", sim, 2, 31);
+  md_stats_t_x *st = md_alloc_stats_x(1);
+  double e_est[2];
+  st->es[0] = 0;
+  md_pimd_calc_VBN_energy_x(pimd, st);
+  e_est[0] = stat_of(st);
+  md_pimd_fast_fill_ENk2_x(pimd, 1);
+  st->es[0] = 0;
+  md_pimd_calc_vi_virial_energy_x(pimd, st, 1);
+  e_est[1] = stat_of(st);
+  dumpf(pf, "e_est", e_est, 2);
+  for (int q = 0; q < 2 * M_NSLICE; ++q) {
+    pthread_join(th[q], NULL);
+    char wname[32];
+    snprintf(wname, sizeof wname, "%s_%d", jobs[q].energy ? "e_slice" : "f_slice", jobs[q].slice);
+    dumpf(pf, wname, jobs[q].out, jobs[q].energy ? 1 : 3 * N);
+    free(jobs[q].out);
+  }
+  free(rows);
+  free(fe);
+}
+
+/* One replica of the batched-replica table, set up exactly as the engine's
+ * test helper sets it up (seed, draw order, N(0,1) trap positions or UEG
+ * uniform particle centres with beads scattered by +-0.4, Maxwell
+ * velocities, chains), and its forces at that state: trap + pair
+ * (f_ext) and trap + pair + fast-path exchange (f_total), every `stride`-th
+ * bead, with the positions of those beads.  UEG pair forces use the
+ * reference's Ewald at alpha = 5/L, n = 8 (converged, the same sum as any
+ * other converged split such as the table's alpha = 2, |m| <= 200). */
+static void replica_case(const char *name, int kind, int N, int P, double xi, double T, double L, unsigned seed,
+                         int stride) {
+  gcase c = {name, N, P, T, xi, kind, kind, L, 8, 1, 0, 0.01, seed};
+  set_params(&c);
+  md_init_config_x cfg = {1, 0, 0, seed, ""};
+  md_init_setup_x(NULL, &cfg);
+  md_set_seed_x(seed);
+  const int nb = N * P;
+  md_simulation_t_x *sim = md_alloc_simulation_x(nb, 1, 0, MD_ND_RECT_BOX_X);
+  md_num_t_x L3[3] = {L, L, L};
+  md_init_nd_rect_box_x(sim->box, L3);
+  md_pimd_t_x *pimd = md_alloc_pimd_x(sim, N, P, xi, T);
+  md_init_particle_mass_x(sim, 1.0, 0, nb);
+  if (kind == 0) {
+    for (int l = 1; l <= N; ++l)
+      for (int j = 1; j <= P; ++j)
+        for (int d = 0; d < 3; ++d) sim->particles[MD_PIMD_INDEX_X(l, j, P)].x[d] = md_random_gaussian_x();
+  } else {
+    md_num_t_x ctr[3] = {L / 2, L / 2, L / 2};
+    md_pimd_init_particle_uniform_pos_x(pimd, ctr, L3, 0.4);
+    md_periodic_boundary_x(sim);
+  }
+  md_init_maxwell_vel_x(sim, T);
+  double meta[10] = {N, P, T, xi, kind, kind, L, 8, stride, seed};
+  dumpf(name, "meta", meta, 10);
+  dump_sub(name, "x0", sim, 0, stride);
+  md_reset_force_x(sim);
+  if (kind == 0) md_pimd_calc_trap_force_x(pimd, md_harmonic_trap_force_x);
+  md_pimd_calc_pair_force_x(pimd, pair_force_of(&c));
+  dump_sub(name, "f_ext", sim, 2, stride);
+  md_pimd_fast_fill_ENk_x(pimd, kind);
+  md_pimd_fill_VB_x(pimd);
+  md_pimd_fast_fill_force_VB_x(pimd, kind);
+  dump_sub(name, "f_total", sim, 2, stride);
+}
+
 int main(int argc, char **argv) {
   const char *path = argc > 1 ? argv[1] : "golden.bin";
   out = fopen(path, "wb");
@@ -664,6 +861,18 @@ int main(int argc, char **argv) {
   md_init_config_x cfg = {1, 0, 0, 1, ""};
   md_init_setup_x(NULL, &cfg);
   md_set_pimd_mode_x(1);
+  if (argc > 2 && strcmp(argv[2], "replica") == 0 && argc == 12) {
+    /* make_golden out.bin replica name kind N P xi T L seed stride */
+    replica_case(argv[3], atoi(argv[4]), atoi(argv[5]), atoi(argv[6]), atof(argv[7]), atof(argv[8]), atof(argv[9]),
+                 (unsigned)strtoul(argv[10], NULL, 10), atoi(argv[11]));
+    fclose(out);
+    return 0;
+  }
+  if (argc > 2 && strcmp(argv[2], "metric") == 0) {
+    metric_case();
+    fclose(out);
+    return 0;
+  }
   if (argc > 2 && strcmp(argv[2], "configs") == 0) {
     double rs = 2.0, L128 = pow(4 * M_PI * 128 / 3.0, 1.0 / 3.0) * rs;
     double kF = pow(6 * M_PI * M_PI * 128 / (L128 * L128 * L128), 1.0 / 3.0);
@@ -699,6 +908,12 @@ int main(int argc, char **argv) {
   observables_case();
   lj_he_case();
   observables2_case();
+  {
+    /* single-species periodic Gaussian (g=3, s=0.5) in a box of edge 3
+     * with minimum-image springs, no trap */
+    gcase pg = {"pgauss12", 12, 8, 1.0, 0.5, 2, 1, 3.0, 2, 1, 5, 0.01, 9};
+    run_case(&pg);
+  }
   fclose(out);
   return 0;
 }

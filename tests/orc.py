@@ -119,6 +119,7 @@ class Oracle:
             "orc_fast_force_VB": (None, [i, i, d, d, d, i, DP, DP, DP, vp, DP, DP]),
             "orc_slow_force_VB": (None, [i, i, d, d, d, i, DP, DP, DP, vp]),
             "orc_VBN_energy": (d, [i, i, d, d, DP, DP]),
+            "orc_exchange_forces_ld": (None, [i, i, d, d, d, i, DP, vp, DP, DP]),
             "orc_fast_fill_ENk2": (None, [i, i, d, i, DP, vp, DP]),
             "orc_fill_ENk2_slow": (None, [i, i, d, i, DP, vp, DP]),
             "orc_vi_virial_energy": (d, [i, i, d, d, i, DP, DP, DP, DP, vp]),
@@ -173,3 +174,16 @@ def rel_err(a, b):
     if rms == 0:
         return float(np.max(np.abs(a - b)))
     return float(np.max(np.abs(a - b)) / rms)
+
+
+def exchange_truth(oracle, x, m, N, P, beta, xi, omegaP, image, L):
+    """Exchange + ring-spring forces (accelerations, reference bead order,
+    3 N P) and V_B^(0..N) from the oracle's 80-bit exchange path: the truth
+    double-precision exchange forces are measured against where beta V_B is
+    large and the connection matrix's exponents lose digits in double."""
+    b = beads_from(np.asarray(x).reshape(-1), m=m)
+    f = np.zeros(3 * N * P)
+    V = np.zeros(N + 1)
+    oracle.exchange_forces_ld(N, P, beta, xi, omegaP, image, dptr(np.array([L, L, L], dtype=float)), ptr(b), dptr(f),
+                              dptr(V))
+    return f, V

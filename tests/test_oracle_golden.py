@@ -12,7 +12,7 @@ import pytest
 from orc import (PAIR_KIND, beads_from, dptr, make_params, nhc_from, ptr, rel_err, tri_size)
 
 CASES = ["trap16", "trap16_g3", "trap8_xi0", "trap8_xi05", "trap24_xi025", "ueg8_n2",
-         "ueg8_n4_xi0", "ueg16_n8"]
+         "ueg8_n4_xi0", "ueg16_n8", "pgauss12"]
 
 
 def case(golden, name):
@@ -23,7 +23,8 @@ def case(golden, name):
              omegaP=g["beta_omegaP"][1])
     c["Lv"] = np.array([L, L, L])
     c["params"] = make_params(alpha=5.0 / L, n=int(nsum))
-    c["pair"] = "gaussian" if c["kind"] == 0 else "ewald"
+    # fixture kinds: 0 trap + Gaussian, 1 UEG Ewald, 2 periodic box + periodic Gaussian
+    c["pair"] = ("gaussian", "ewald", "periodic_gaussian")[c["kind"]]
     return c, g
 
 
@@ -45,7 +46,7 @@ def test_pair_and_trap_forces(golden, oracle, name):
         b = beads0(g)
         oracle.trap_force(ptr_p(c), c["N"], c["P"], ptr(b))
         assert rel_err(forces(b), g["f_trap"]) < 1e-15
-    else:
+    elif c["kind"] == 1:
         for split in ("R", "NI"):
             b = beads0(g)
             oracle.pair_force(PAIR_KIND["ewald_" + split], ptr_p(c), c["N"], c["P"], dptr(c["Lv"]), ptr(b))
@@ -155,7 +156,7 @@ def test_trajectory(golden, oracle, name):
     ev = []
     for s in range(steps):
         oracle.nhc_vv3_1(N * P, len(ch), ptr(ch), ptr(b), c["T"], c["dt"], 8)
-        if c["kind"] == 1:
+        if c["kind"] != 0:
             oracle.periodic_boundary(N * P, dptr(Lv), ptr(b))
         ENk, VBN = full_force()
         oracle.nhc_vv3_2(N * P, len(ch), ptr(ch), ptr(b), c["T"], c["dt"], 8)
@@ -324,3 +325,24 @@ def test_observables_two_systems(golden, oracle):
     assert z[0] == pytest.approx(NA + NB, rel=1e-15)
     # every ITCF entry is counted once (N^2 entries of weight 1/P)
     assert hist(36, o.itcf, NA, NA, P, ptr(a), ptr(a), 0.5, 4.5, 3, 36, 0).sum() == pytest.approx(NA * NA / P)
+
+
+@pytest.mark.parametrize("name", ["ueg8_n2", "ueg8_n4_xi0", "ueg16_n8"])
+def test_ewald_energy_splits_identities(golden, name):
+    """The reference's Ewald energy splits satisfy Ewald = R + NI and
+    R2 = Ewald - sum over pairs of 1/r_min (per slice, divided by P), with
+    r_min the minimum-image distance (md_statistics_x.c:398-495)."""
+    c, g = case(golden, name)
+    N, P, L = c["N"], c["P"], c["L"]
+    eR, eR2, eNI = g["e_pair_R_R2_NI"]
+    e = g["e_pair"][0]
+    assert eR + eNI == pytest.approx(e, rel=1e-12, abs=1e-12)
+    x = g["x0"].reshape(N, P, 3)
+    coul = 0.0
+    for j in range(P):
+        d = x[:, None, j, :] - x[None, :, j, :]
+        d -= L * np.rint(d / L)
+        r = np.sqrt((d ** 2).sum(-1))
+        iu = np.triu_indices(N, 1)
+        coul += (1.0 / r[iu]).sum()
+    assert eR2 == pytest.approx(e - coul / P, rel=1e-11, abs=1e-11)

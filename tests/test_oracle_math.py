@@ -179,3 +179,66 @@ def test_ideal_trapped_bosons_thermodynamic_estimator(oracle):
         V = vb(oracle, E, N, beta, xi)
         e = oracle.VBN_energy(N, P, beta, xi, dptr(E), dptr(V))
         assert e == pytest.approx(3 * N * P / (2 * beta), rel=1e-14)
+
+
+def ld_exchange(oracle, b, N, P, beta, xi, omegaP, image=0, L=1.0):
+    f = np.zeros(3 * N * P)
+    V = np.zeros(N + 1)
+    oracle.exchange_forces_ld(N, P, beta, xi, omegaP, image, dptr(np.array([L] * 3)), ptr(b), dptr(f), dptr(V))
+    return f, V
+
+
+@pytest.mark.parametrize("xi", [0.0, 0.5, 1.0])
+def test_extended_precision_exchange_pinned(oracle, xi):
+    """The 80-bit exchange path (the truth the double-precision codes are
+    measured against on ill-conditioned inputs) against what pins the double
+    oracle: V_B closed forms at xi = 0 and for coincident beads, the
+    double-precision fast path on a well-conditioned input (agreement to
+    double round-off), and -grad V_B by central differences of its own V_B."""
+    N, P, beta = 7, 5, 1.3
+    omegaP = math.sqrt(P) / beta
+    b = random_beads(N, P, 5, 0.3)
+    f, V = ld_exchange(oracle, b, N, P, beta, xi, omegaP)
+    E = enk(oracle, b, N, P, omegaP)
+    Vd = vb(oracle, E, N, beta, xi)
+    np.testing.assert_allclose(V, Vd, rtol=1e-14, atol=1e-14)
+    if xi == 0.0:
+        ring = sum(E[tri_size(l - 1)] for l in range(1, N + 1))
+        assert V[N] == pytest.approx(ring + math.lgamma(N + 1) / beta, rel=1e-14)
+    c = random_beads(N, P, 5, 0.0)
+    _, V0 = ld_exchange(oracle, c, N, P, beta, xi, omegaP)
+    expect = -(sum(math.log1p(m * xi) for m in range(1, N)) - math.lgamma(N + 1)) / beta
+    assert V0[N] == pytest.approx(expect, rel=1e-14, abs=1e-14)
+    fd = b.copy()
+    fd["f"] = 0
+    oracle.fast_force_VB(N, P, beta, xi, omegaP, 0, dptr(np.ones(3)), dptr(E), dptr(Vd), ptr(fd), None, None)
+    np.testing.assert_allclose(f, fd["f"].reshape(-1), rtol=1e-12, atol=1e-13)
+    h = 1e-5
+    for idx in (0, N * P // 2, N * P - 1):
+        for d in range(3):
+            vals = []
+            for s in (+1, -1):
+                cc = b.copy()
+                cc["x"][idx, d] += s * h
+                vals.append(ld_exchange(oracle, cc, N, P, beta, xi, omegaP)[1][N])
+            assert f[3 * idx + d] == pytest.approx(-(vals[0] - vals[1]) / (2 * h), rel=1e-6, abs=1e-7)
+
+
+@pytest.mark.parametrize("name", ["trap16", "trap16_g3", "trap8_xi0", "trap8_xi05", "ueg8_n2", "ueg8_n4_xi0",
+                                  "ueg16_n8", "pgauss12"])
+def test_extended_precision_exchange_vs_reference(oracle, golden, name):
+    """The 80-bit exchange forces against the compiled reference's cycle-walk
+    algorithm (md_pimd_fill_force_VB_x, a formulation independent of the fast
+    backward pass and of its connection matrix): they agree to 2e-12 of the
+    RMS force (measured 5e-16 .. 1.3e-12), while the reference's fast path is
+    off by up to 1.7e-11 on the cases with beta V_B ~ 5e4 -- the double
+    round-off of the connection matrix's exponents.  V_B to 1e-13."""
+    g = {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(name + "/")}
+    N, P, T, vi, kind, image, L, nsum, fgroup, dt = g["meta"]
+    N, P = int(N), int(P)
+    beta, omegaP = g["beta_omegaP"]
+    b = beads_from(g["x0"], m=g["m0"])
+    f, V = ld_exchange(oracle, b, N, P, beta, vi, omegaP, int(image), L)
+    np.testing.assert_allclose(V, g["VBN"], rtol=1e-13, atol=1e-13)
+    rms = np.sqrt(np.mean(g["f_exch_slow"] ** 2))
+    assert np.max(np.abs(f - g["f_exch_slow"])) / rms < 2e-12
