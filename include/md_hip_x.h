@@ -113,6 +113,86 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
  * when the work is done (-1 if any replica's recursion reported an error). */
 int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *cfgs, int n, int nsteps);
 
+/* Learned stand-in for the reciprocal-space Ewald term (no counterpart in
+ * the reference).  A per-particle network on symmetry-function descriptors
+ * of each particle's neighbourhood in its bead slice (64 radial + 32
+ * angular, minimum image, cutoff r_cut <= half the box) predicts that
+ * particle's share e_i of its slice's reciprocal energy (the exact split
+ * E_rec = sum_i e_i, e_i = sum_k A_k (Re(e^{ik.x_i} S_k*) - 1), of the Ewald
+ * split md_coulomb_3d_ewald_force_x uses); forces are -dE/dx through the
+ * network and the descriptors.  96 -> 128 -> 128 -> 1 tanh network, BF16
+ * matrix-core inference with FP32 accumulation, FP32 training (Adam) on
+ * labels from the exact FP64 reciprocal kernels. */
+typedef struct md_ewald_surrogate_x md_ewald_surrogate_x;
+md_ewald_surrogate_x *md_alloc_ewald_surrogate_x(unsigned seed, md_num_t_x r_cut);
+/* iters Adam steps (learning rate lr) on the current configuration's exact
+ * labels (the first call also fixes the feature and label normalisation);
+ * *loss: the normalised training loss before the last step. */
+int md_ewald_surrogate_train_x(md_ewald_surrogate_x *s, md_pimd_t_x *pimd, md_num_t_x lr, int iters,
+                               md_num_t_x *loss);
+/* Surrogate against the exact FP64 reciprocal term on the current
+ * configuration: report[0] mean over slices of (E_pred - E_exact)/N, [1]
+ * largest |E_pred - E_exact|/N of a slice (Hartree per particle), [2..4]
+ * RMS force error per component, [5..7] RMS exact force per component
+ * (Hartree/bohr, slice-energy forces), [8] overall relative RMS force error,
+ * [9] mean exact reciprocal energy per particle. */
+int md_ewald_surrogate_check_x(md_ewald_surrogate_x *s, md_pimd_t_x *pimd, md_num_t_x *report);
+/* Parameters (FP32, 29057 values: W1[96][128], b1[128], W2[128][128], b2,
+ * w3[128], b3) and normalisation (mean[96], 1/std[96], label mean, label
+ * std): get and/or set (either may be NULL); return the count. */
+int md_ewald_surrogate_params_x(md_ewald_surrogate_x *s, float *get, const float *set);
+int md_ewald_surrogate_normalization_x(md_ewald_surrogate_x *s, md_num_t_x *get, const md_num_t_x *set);
+/* Prediction (raw descriptors [b][96], energies [b], slice-energy forces
+ * [b][3]) and exact labels ([b], [b][3]) on the current configuration, and
+ * the training-loss gradient for given per-bead targets, in reference bead
+ * order b = (l-1) P + (j-1).  Outputs may be NULL. */
+int md_ewald_surrogate_eval_x(md_ewald_surrogate_x *s, md_pimd_t_x *pimd, float *desc, md_num_t_x *e_bead,
+                              md_num_t_x *force);
+int md_ewald_surrogate_labels_x(md_ewald_surrogate_x *s, md_pimd_t_x *pimd, md_num_t_x *e_bead, md_num_t_x *force);
+int md_ewald_surrogate_gradient_x(md_ewald_surrogate_x *s, md_pimd_t_x *pimd, const md_num_t_x *e_bead, float *grad,
+                                  md_num_t_x *loss);
+/* Online training from the run itself: nsteps exact MD steps (cfg, the
+ * UEG/Ewald step of md_pimd_run_steps_x), each followed by `iters` Adam
+ * steps on the exact labels of the configuration reached (each bead slice is
+ * one configuration).  report (may be NULL): loss of the first and last
+ * update. */
+int md_pimd_train_surrogate_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, md_ewald_surrogate_x *s,
+                                    int nsteps, md_num_t_x lr, int iters, md_num_t_x *report);
+/* nsteps MD steps with the surrogate standing in for the reciprocal-space
+ * Ewald term (the real-space part, exchange and integrator unchanged;
+ * surrogate steps replayed as graphs).  Every check_every-th step is a judge
+ * step: the exact FP64 terms drive it and the surrogate is compared with
+ * them; when its relative RMS force error exceeds f_threshold or a slice's
+ * energy error per particle exceeds e_threshold (Hartree), the next
+ * check_every steps run exact, each followed by one online training update
+ * (learning rate lr; 0: none), and the surrogate then resumes.  report (8,
+ * may be NULL): surrogate steps, judge steps, exact fallback steps, fallback
+ * events, mean and largest judge relative force error, mean judge energy
+ * error per particle (signed), largest judge |energy error| per particle. */
+int md_pimd_run_steps_surrogate_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, md_ewald_surrogate_x *s,
+                                  int nsteps, int check_every, md_num_t_x f_threshold, md_num_t_x e_threshold,
+                                  md_num_t_x lr, md_num_t_x *report);
+
+/* Autotuner of the Ewald real-space kernel's launch choices, none of which
+ * changes a result bit: the image enumeration (face / near / corner; every
+ * variant as general as the geometry needs takes the same images in the
+ * same order) and the waves per workgroup (1, 2, 4, 8; each wave owns whole
+ * tile pairs).  md_pimd_autotune_x times every valid choice on the object's
+ * current positions (reps launches each; ms[4 * enumeration + log2 waves],
+ * -1 where invalid; choice: the fastest) and keeps it for the object's
+ * bucket (N rounded up to a power of two, P, the most specific valid
+ * enumeration).  md_set_autotune_x(1): use the kept choices, tuning a bucket
+ * on its first use by md_pimd_run_steps_x / md_pimd_run_batch_x (outside any
+ * capture); 0 (default): the rule (most specific enumeration, 4 waves).
+ * md_autotune_save_x / _load_x write / read the kept choices as text (load
+ * returns the count).  md_set_ewald_real_variant_x fixes a choice for
+ * diagnostics (-1, -1: automatic). */
+int md_set_autotune_x(int on);
+int md_pimd_autotune_x(md_pimd_t_x *pimd, int reps, md_num_t_x *ms, int *choice);
+int md_autotune_save_x(const char *path);
+int md_autotune_load_x(const char *path);
+int md_set_ewald_real_variant_x(int img, int wlog);
+
 /* Per-stage device time of nsteps eager steps, averaged per step (ms).  With
  * serial != 0 every stage runs alone on one stream (true kernel times);
  * otherwise the branches overlap as in md_pimd_run_steps_x (stage times are

@@ -49,9 +49,19 @@ md_num_t_x md_harmonic_trap_energy_x(md_num_t_x *p1, md_num_t_x *L, md_num_t_x m
 md_num_t_x md_gaussian_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
 md_num_t_x md_periodic_gaussian_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
 md_num_t_x md_coulomb_3d_ewald_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+/* Ewald energy splits (md_statistics_x.h:57-59, md_statistics_x.c:398-495):
+ * _R_x every term but the minimum image's real-space one, _R2_x that plus
+ * -erf(alpha r)/r of the minimum image, _NI_x the minimum image's
+ * erfc(alpha r)/r alone (Ewald = R + NI). */
+md_num_t_x md_coulomb_3d_ewald_energy_R_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+md_num_t_x md_coulomb_3d_ewald_energy_R2_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
+md_num_t_x md_coulomb_3d_ewald_energy_NI_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
 md_num_t_x md_LJ_periodic_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
 md_num_t_x md_periodic_helium_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x m);
 
+/* md_statistics_x.h:83 (md_statistics_x.c:855-958): Ts[N-1] += sum m v^2 over
+ * the sum of the thermostat groups' degree-of-freedom counts. */
+int md_calc_temperature_x(md_simulation_t_x *sim, md_stats_t_x *stats);
 int md_calc_kinetic_x(md_simulation_t_x *sim, md_stats_t_x *stats);
 int md_calc_nhc_kinetic_x(md_simulation_t_x *sim, md_stats_t_x *stats);
 

@@ -223,6 +223,58 @@ md_num_t_x md_periodic_helium_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_
   return md_he_eps_x * (md_he_A_x * exp(-md_he_alpha_x * x) - disp * he_damp(x));
 }
 
+// One pair's Ewald energy and its splits (md_statistics_x.c:364-495):
+// mode 0 the whole sum, 1 (R) without the minimum image's real-space term,
+// 2 (R2) R plus -erf(alpha r)/r of the minimum image, 3 (NI) the minimum
+// image's erfc(alpha r)/r alone.  The minimum image is recognised as the
+// reference does, per component within 1e-4 of the minimum-image distance.
+static double ewald_energy_pair(const double *p1, const double *p2, const double *L, int mode) {
+  const int n = md_coulomb_n_sum_x;
+  const double al = md_coulomb_truc_x, V = L[0] * L[1] * L[2];
+  double pa[3] = {p1[0], p1[1], p1[2]}, pb[3] = {p2[0], p2[1], p2[2]};
+  if (mode == 3) return md_coulomb_strength_x * erfc(al * md_minimum_image_distance_x(pa, pb, (double *)L)) /
+                        md_minimum_image_distance_x(pa, pb, (double *)L);
+  double res = 0;
+  for (int i = -n; i <= n; ++i)
+    for (int j = -n; j <= n; ++j)
+      for (int k = -n; k <= n; ++k) {
+        if (i == 0 && j == 0 && k == 0) continue;
+        double g[3] = {i / L[0], j / L[1], k / L[2]};
+        double G2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+        double dot = 2 * M_PI * (g[0] * (p1[0] - p2[0]) + g[1] * (p1[1] - p2[1]) + g[2] * (p1[2] - p2[2]));
+        res += cos(dot) / G2 * exp(-M_PI * M_PI * G2 / (al * al)) / (V * M_PI);
+      }
+  res -= M_PI / (al * al * V);
+  for (int i = -n; i <= n; ++i)
+    for (int j = -n; j <= n; ++j)
+      for (int k = -n; k <= n; ++k) {
+        double r[3] = {p1[0] - (p2[0] + i * L[0]), p1[1] - (p2[1] + j * L[1]), p1[2] - (p2[2] + k * L[2])};
+        if (mode != 0) {
+          bool same = true;
+          for (int d = 0; d < 3 && same; ++d)
+            same = fabs(fabs(r[d]) - fabs(md_minimum_image_x(-r[d], L[d]))) < 1e-4;
+          if (same) continue;
+        }
+        double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        res += erfc(al * R) / R;
+      }
+  if (mode == 2) {
+    const double R = md_minimum_image_distance_x(pa, pb, (double *)L);
+    res += -erf(al * R) / R;
+  }
+  return md_coulomb_strength_x * res;
+}
+
+md_num_t_x md_coulomb_3d_ewald_energy_R_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x MD_UNUSED_X(m)) {
+  return ewald_energy_pair(p1, p2, L, 1);
+}
+md_num_t_x md_coulomb_3d_ewald_energy_R2_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x MD_UNUSED_X(m)) {
+  return ewald_energy_pair(p1, p2, L, 2);
+}
+md_num_t_x md_coulomb_3d_ewald_energy_NI_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x MD_UNUSED_X(m)) {
+  return ewald_energy_pair(p1, p2, L, 3);
+}
+
 md_num_t_x md_coulomb_3d_ewald_energy_x(md_num_t_x *p1, md_num_t_x *p2, md_num_t_x *L, md_num_t_x MD_UNUSED_X(m)) {
   const int n = md_coulomb_n_sum_x;
   const double al = md_coulomb_truc_x, V = L[0] * L[1] * L[2];

@@ -669,6 +669,23 @@ int md_calc_kinetic_x(md_simulation_t_x *sim, md_stats_t_x *stats) {
   return 0;
 }
 
+// md_statistics_x.c:855-958 (nocl 946-955): sum m v^2 over the total
+// degree-of-freedom count of the thermostat groups, into Ts[N-1].
+int md_calc_temperature_x(md_simulation_t_x *sim, md_stats_t_x *stats) {
+  long f = 0;
+  for (int i = 0; i < sim->Nf; ++i) f += sim->fs[i];
+  if (f <= 0 || !stats->Ts) return -1;
+  if (ensure_device(sim)) return -1;
+  DevSim *d = sim_state(sim, false);
+  int nblk = (int)std::min<long>(blocks_for(d->nb, 256), 1024);
+  hipLaunchKernelGGL(k_kinetic, dim3(nblk), dim3(256), 0, rt().s0, d->V, d->M, d->nb, d->red);
+  XMD_CHECK(hipGetLastError());
+  double e = 0;  // sum m v^2 / 2
+  if (partial_sum(d, nblk, &e)) return -1;
+  stats->Ts[stats->N - 1] += 2.0 * e / (double)f;
+  return 0;
+}
+
 int md_calc_nhc_kinetic_x(md_simulation_t_x *sim, md_stats_t_x *stats) {
   if (ensure_device(sim)) return -1;
   DevSim *d = sim_state(sim, false);

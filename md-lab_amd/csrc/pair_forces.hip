@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include "ewald_table.h"
@@ -293,6 +294,12 @@ __device__ __forceinline__ void pair_term(double dx, double dy, double dz, const
         gz += dz * w;
       }
     }
+  } else if (MODE == PK_EWALD_R2) {
+    if (ENERGY) {
+      const double rx = pf_min_image(dx, pp.L[0]), ry = pf_min_image(dy, pp.L[1]), rz = pf_min_image(dz, pp.L[2]);
+      const double R = sqrt(rx * rx + ry * ry + rz * rz);
+      e += -pp.q * erf(pp.alpha * R) / R;
+    }
   } else if (NEAR) {
     ewald_real_near<!ENERGY, ENERGY>(dx, dy, dz, pp, MODE == PK_EWALD_R, MODE == PK_EWALD_NI, gx, gy, gz, e);
   } else {
@@ -451,6 +458,7 @@ struct KSpace {
   double L[3] = {0, 0, 0}, alpha = 0, q = 0, gmax2 = 0;
   int n = -2;  // cube half-width, or -1 for the ball of radius^2 gmax2
   int MZ = 0, ZP = 0, ncol = 0, ncolp = 0, nk = 0;  // nk: k vectors with nonzero weight
+  double e0 = 0;  // sum over the k set of coef / (4 pi): each bead's self term of the slice energy
   int2 *cols = nullptr;    // [ncolp] (mx, my), then [ncolp / 16] K-step range of each column tile
   double *coef = nullptr;  // [ncolp][ZP]
   double2 *tx = nullptr, *ty = nullptr, *tz = nullptr, *S = nullptr, *W = nullptr;
@@ -483,6 +491,7 @@ static PairBufs *pair_bufs(DevPimd *p) {
 
 // Device allocations made by the pair buffers (pair_prepare reports them).
 static unsigned g_pair_allocs = 0;
+unsigned pair_alloc_epoch() { return g_pair_allocs; }
 
 static int grow(double2 *&p, size_t &cap, size_t need) {
   if (need <= cap) return 0;
@@ -598,6 +607,9 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
     K->ncol = ncol;
     K->ncolp = ncolp;
     K->nk = nk;
+    double csum = 0;
+    for (double c : coef) csum += c;
+    K->e0 = csum / (4 * M_PI);
   }
   K->stamp = ++pb->ks_clock;
   if ((long long)nb * (3 * K->MZ + 2 + K->ZP) > INT_MAX) {
@@ -795,13 +807,19 @@ __global__ __launch_bounds__(256) void k_sk_mfma(const double2 *__restrict__ tx,
 // reference's per-pair sum (md_simulation_x.c:266-281) since
 // sum_{l != i} sin(k.(x_i - x_l)) = Im(e^{i k.x_i} S_k*).  The Z operands of
 // the 16 beads are the same for every column tile and stay in registers.
-template <int NZT>
+// EN: also the bead's share of its slice's reciprocal energy,
+//   e_i = sum_k A_k (Re(e^{i k.x_i} S_k*) - 1),  A_k = coef_k / (4 pi),
+// (the slice energy sum_k A_k (|S_k|^2 - N) split over its beads), from the
+// same U: Re(A U) = sum_{k of the column} coef_k Re(e^{i k.x_i} S_k*);
+// e0 = sum_k A_k.
+template <int NZT, bool EN = false>
 __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0, const double2 *__restrict__ tx,
                                                  const double2 *__restrict__ ty, const double2 *__restrict__ tz,
                                                  const int2 *__restrict__ cols, int ncolp, int MZ,
                                                  const double2 *__restrict__ W, const double *__restrict__ Mass,
                                                  double *__restrict__ F, int N, int nb, double L0, double L1,
-                                                 double L2, double Pd, double *__restrict__ Frec) {
+                                                 double L2, double Pd, double *__restrict__ Frec,
+                                                 double *__restrict__ Erec = nullptr, double e0 = 0.0) {
   constexpr int ZP = 16 * NZT, KS = 4 * NZT;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int abead = b0 + (lane & 15);  // A operand row of this lane
@@ -824,7 +842,7 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
     const int bead = b0 + 4 * r + beadq;
     sb[r] = slice * N + (bead < N ? bead : 0);
   }
-  double fx[4] = {0, 0, 0, 0}, fy[4] = {0, 0, 0, 0}, fz[4] = {0, 0, 0, 0};
+  double fx[4] = {0, 0, 0, 0}, fy[4] = {0, 0, 0, 0}, fz[4] = {0, 0, 0, 0}, en[4] = {0, 0, 0, 0};
   const double2 *Ws = W + (size_t)slice * ncolp * ZP;
   const int nct = ncolp / 16;
   for (int ct = wv; ct < nct; ct += 4) {
@@ -867,6 +885,7 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
       fx[r] = fma((double)cl.x, g, fx[r]);
       fy[r] = fma((double)cl.y, g, fy[r]);
       fz[r] = fma(ar, vi, fma(ai, vr, fz[r]));
+      if (EN) en[r] = fma(ar, ur, fma(-ai, ui, en[r]));
     }
   }
   // sum over the 16 columns held by the lanes of each 16-lane row
@@ -877,6 +896,7 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
       fx[r] += __shfl_xor(fx[r], o, 64);
       fy[r] += __shfl_xor(fy[r], o, 64);
       fz[r] += __shfl_xor(fz[r], o, 64);
+      if (EN) en[r] += __shfl_xor(en[r], o, 64);
     }
   // red[d][wave][bead]
   if ((lane & 15) == 0) {
@@ -885,6 +905,7 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
       red[(0 * 4 + wv) * 16 + 4 * r + beadq] = fx[r];
       red[(1 * 4 + wv) * 16 + 4 * r + beadq] = fy[r];
       red[(2 * 4 + wv) * 16 + 4 * r + beadq] = fz[r];
+      if (EN) red[(3 * 4 + wv) * 16 + 4 * r + beadq] = en[r];
     }
   }
   __syncthreads();
@@ -907,20 +928,26 @@ __device__ __forceinline__ void recip_force_tile(double *red, int slice, int b0,
         F[nb + s] += g[1] / L1 / m / Pd;
         F[2 * nb + s] += g[2] / L2 / m / Pd;
       }
+      if (EN) {
+        const double e = ((red[(3 * 4 + 0) * 16 + t] + red[(3 * 4 + 1) * 16 + t]) + red[(3 * 4 + 2) * 16 + t]) +
+                         red[(3 * 4 + 3) * 16 + t];
+        Erec[s] = e / (4 * M_PI) - e0;
+      }
     }
   }
 }
 
-template <int NZT>
+template <int NZT, bool EN = false>
 __device__ __forceinline__ void k_recip_force_mfma_body(int bx,
     const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
     const int2 *__restrict__ cols, int ncolp, int MZ, const double2 *__restrict__ W,
     const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
-    double Pd, double *__restrict__ Frec = nullptr) {
-  __shared__ double red[3 * 4 * 16];
+    double Pd, double *__restrict__ Frec = nullptr, double *__restrict__ Erec = nullptr, double e0 = 0.0) {
+  __shared__ double red[(EN ? 4 : 3) * 4 * 16];
   int slice, bt;
   xcd_tile(bx, (N + 15) / 16, P, slice, bt);
-  recip_force_tile<NZT>(red, slice, bt * 16, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, L0, L1, L2, Pd, Frec);
+  recip_force_tile<NZT, EN>(red, slice, bt * 16, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, L0, L1, L2, Pd,
+                            Frec, Erec, e0);
 }
 
 template <int NZT>
@@ -930,6 +957,17 @@ __global__ __launch_bounds__(256) void k_recip_force_mfma(
     const double *__restrict__ Mass, double *__restrict__ F, int N, int nb, int P, double L0, double L1, double L2,
     double Pd, double *__restrict__ Frec) {
   k_recip_force_mfma_body<NZT>(blockIdx.x, tx, ty, tz, cols, ncolp, MZ, W, Mass, F, N, nb, P, L0, L1, L2, Pd, Frec);
+}
+
+// Reciprocal forces (into Frec) and per-bead energies (into Erec) of every slice.
+template <int NZT>
+__global__ __launch_bounds__(256) void k_recip_force_energy(
+    const double2 *__restrict__ tx, const double2 *__restrict__ ty, const double2 *__restrict__ tz,
+    const int2 *__restrict__ cols, int ncolp, int MZ, const double2 *__restrict__ W,
+    const double *__restrict__ Mass, int N, int nb, int P, double L0, double L1, double L2, double Pd,
+    double *__restrict__ Frec, double *__restrict__ Erec, double e0) {
+  k_recip_force_mfma_body<NZT, true>(blockIdx.x, tx, ty, tz, cols, ncolp, MZ, W, Mass, nullptr, N, nb, P, L0, L1, L2,
+                                     Pd, Frec, Erec, e0);
 }
 
 // Reciprocal energy of every slice: sum_{k in half} A_k (|S_k|^2 - N), into partials.
@@ -1069,18 +1107,20 @@ __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, 
 
 // Four tile pairs per workgroup (one per wave) sharing the LDS table; the
 // per-wave schedule is that of k_pair_tiles.
-template <int IMG>
+// WPB waves per workgroup, one tile pair per wave (the erfc table is staged
+// once per workgroup); every choice computes the same partial sums.
+template <int IMG, int WPB = 4>
 __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restrict__ X, const double *__restrict__ M,
                                                     double *__restrict__ Fpart, const int2 *__restrict__ tiles, int N,
                                                     int nb, int T, int ntp, int nitems, EwReal ep) {
   __shared__ double2 tab[kEwTabIntervals * kEwTabStride / 2];
-  __shared__ double jx[4][64], jy[4][64], jz[4][64], rx[4][64], ry[4][64], rz[4][64];
+  __shared__ double jx[WPB][64], jy[WPB][64], jz[WPB][64], rx[WPB][64], ry[WPB][64], rz[WPB][64];
   // the wave index and everything derived from it are wave-uniform: read
   // them as such, so the step loop below runs on scalar registers
   const int tid = threadIdx.x, t = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (int k = tid; k < kEwTabIntervals * kEwTabStride / 2; k += 256)
+  for (int k = tid; k < kEwTabIntervals * kEwTabStride / 2; k += 64 * WPB)
     tab[k] = make_double2(kEwTab[2 * k], kEwTab[2 * k + 1]);
-  const int item = bx * 4 + w;
+  const int item = bx * WPB + w;
   const bool valid = item < nitems;
   int slice = 0, I = 0, J = 0;
   if (valid) {
@@ -1151,11 +1191,11 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
   }
 }
 
-template <int IMG>
-__global__ __launch_bounds__(256) void k_ewald_real(const double *__restrict__ X, const double *__restrict__ M,
-                                                    double *__restrict__ Fpart, const int2 *__restrict__ tiles, int N,
-                                                    int nb, int T, int ntp, int nitems, EwReal ep) {
-  k_ewald_real_body<IMG>(blockIdx.x, X, M, Fpart, tiles, N, nb, T, ntp, nitems, ep);
+template <int IMG, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_ewald_real(const double *__restrict__ X, const double *__restrict__ M,
+                                                         double *__restrict__ Fpart, const int2 *__restrict__ tiles,
+                                                         int N, int nb, int T, int ntp, int nitems, EwReal ep) {
+  k_ewald_real_body<IMG, WPB>(blockIdx.x, X, M, Fpart, tiles, N, nb, T, ntp, nitems, ep);
 }
 
 // ------------------------------------------------------------ launchers
@@ -1175,6 +1215,9 @@ int pair_energy_kind_of(md_pair_energy_t_x pe) {
   if (pe == md_gaussian_energy_x) return PK_GAUSS;
   if (pe == md_periodic_gaussian_energy_x) return PK_PGAUSS;
   if (pe == md_coulomb_3d_ewald_energy_x) return PK_EWALD;
+  if (pe == md_coulomb_3d_ewald_energy_R_x) return PK_EWALD_R;
+  if (pe == md_coulomb_3d_ewald_energy_R2_x) return PK_EWALD_R2;
+  if (pe == md_coulomb_3d_ewald_energy_NI_x) return PK_EWALD_NI;
   if (pe == md_LJ_periodic_energy_x) return PK_LJ;
   if (pe == md_periodic_helium_energy_x) return PK_HE;
   return -1;
@@ -1264,7 +1307,7 @@ static int launch_pair_mode(DevPimd *p, const PairParams &pp, hipStream_t st, do
   DevSim *d = p->ds;
   if (ensure_tiles(p)) return -1;
   TileBufs &B = pair_bufs(p)->tb;
-  const bool near = (MODE == PK_EWALD_NI) || near_ok(pp);
+  const bool near = (MODE == PK_EWALD_NI) || (MODE == PK_EWALD_R2) || near_ok(pp);
   const int items = p->P * B.ntp;
   if (near)
     hipLaunchKernelGGL((k_pair_tiles<MODE, ENERGY, true>), dim3(items), dim3(64), 0, st, d->X, d->M, B.Fpart,
@@ -1395,10 +1438,54 @@ static PairParams ewald_params(DevPimd *p, EwaldPlan &pl, double xcut = g_ewald_
 
 // Image enumeration of the real-space kernel for the cutoff of pp (the face
 // bound keeps a relative margin of 1e-6 against rounding of the distances).
+struct RealChoice {
+  int img;   // IMG_FACE / IMG_NEAR / IMG_CORNER
+  int wlog;  // log2 of the waves per workgroup (0..3)
+};
+
 static int image_mode(const PairParams &pp) {
   const double L2 = lmin3(pp.L) * lmin3(pp.L);
   if (pp.rc2 < 0.5 * L2 * (1.0 - 1e-6)) return IMG_FACE;
   return pp.rc2 >= L2 ? IMG_CORNER : IMG_NEAR;
+}
+
+// ---- launch choices of the real-space kernel and their autotuner ----
+//
+// Result-preserving choices: the image enumeration (every variant at least as
+// general as the geometry needs takes the same images in the same order with
+// the same arithmetic; an out-of-range candidate adds an exact zero) and the
+// waves per workgroup (each wave owns whole tile pairs and their partial-sum
+// slots).  Without tuning: the most specific enumeration, 4 waves.  The
+// autotuner times every valid choice on an object's current positions and
+// keeps the fastest per bucket (N rounded up to a power of two, P, and the
+// most specific valid enumeration).
+using RealKern = void (*)(const double *, const double *, double *, const int2 *, int, int, int, int, int, EwReal);
+static const RealKern k_ewald_real_of[3][4] = {
+    {k_ewald_real<IMG_FACE, 1>, k_ewald_real<IMG_FACE, 2>, k_ewald_real<IMG_FACE, 4>, k_ewald_real<IMG_FACE, 8>},
+    {k_ewald_real<IMG_NEAR, 1>, k_ewald_real<IMG_NEAR, 2>, k_ewald_real<IMG_NEAR, 4>, k_ewald_real<IMG_NEAR, 8>},
+    {k_ewald_real<IMG_CORNER, 1>, k_ewald_real<IMG_CORNER, 2>, k_ewald_real<IMG_CORNER, 4>,
+     k_ewald_real<IMG_CORNER, 8>}};
+
+int g_autotune = 0;                      // use tuned choices
+static int g_real_force_img = -1, g_real_force_wlog = -1;  // fixed choice (diagnostics), -1: none
+static std::map<long long, RealChoice> g_real_tuned;
+
+static long long real_bucket(int N, int P, int img_min) {
+  long long nb2 = 1;
+  while (nb2 < N) nb2 <<= 1;
+  return (nb2 << 24) | ((long long)P << 4) | img_min;
+}
+
+RealChoice real_choice(int N, int P, const PairParams &pp) {
+  const int img_min = image_mode(pp);
+  RealChoice c{img_min, 2};
+  if (g_real_force_img >= img_min) c.img = g_real_force_img;
+  if (g_real_force_wlog >= 0) c.wlog = g_real_force_wlog;
+  if (g_real_force_img < 0 && g_real_force_wlog < 0 && g_autotune) {
+    auto it = g_real_tuned.find(real_bucket(N, P, img_min));
+    if (it != g_real_tuned.end()) c = it->second;
+  }
+  return c;
 }
 
 // Table-driven real space: per-tile partial forces (k_ewald_real), then
@@ -1416,10 +1503,10 @@ static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t s
   ep.al8 = kEwTabInvWidth * pp.alpha;
   ep.rc2 = pp.rc2;
   const int items = p->P * B.ntp;
-  static void (*const kern[3])(const double *, const double *, double *, const int2 *, int, int, int, int, int,
-                               EwReal) = {k_ewald_real<IMG_FACE>, k_ewald_real<IMG_NEAR>, k_ewald_real<IMG_CORNER>};
-  hipLaunchKernelGGL(kern[image_mode(pp)], dim3((items + 3) / 4), dim3(256), 0, st, d->X, d->M, B.Fpart, B.tiles,
-                     p->N, d->nb, B.T, B.ntp, items, ep);
+  const RealChoice ch = real_choice(p->N, p->P, pp);
+  const int wpb = 1 << ch.wlog;
+  hipLaunchKernelGGL(k_ewald_real_of[ch.img][ch.wlog], dim3((items + wpb - 1) / wpb), dim3(64 * wpb), 0, st, d->X, d->M,
+                     B.Fpart, B.tiles, p->N, d->nb, B.T, B.ntp, items, ep);
   XMD_CHECK(hipGetLastError());
   if (!reduce) return 0;
   const int n3 = 3 * d->nb;
@@ -1505,6 +1592,75 @@ int launch_pair_real(DevPimd *p, int kind, hipStream_t st) {
 
 bool pair_has_recip(int kind) { return kind == PK_EWALD || kind == PK_EWALD_R; }
 
+// Times every valid real-space launch choice on the object's current
+// positions (one warm-up, then `reps` launches each, HIP events on the main
+// stream) and keeps the fastest for the object's bucket.  ms[4 img + wlog]:
+// average milliseconds per launch, -1 for invalid choices.
+int autotune_real(DevPimd *p, int reps, double *ms, int *choice) {
+  DevSim *d = p->ds;
+  if (!d->uniform_mass) return -1;
+  EwaldPlan pl;
+  PairParams pp = ewald_params(p, pl);
+  if (!near_ok(pp) || ensure_tiles(p)) return -1;
+  TileBufs &B = pair_bufs(p)->tb;
+  EwReal ep;
+  for (int k = 0; k < 3; ++k) {
+    ep.L[k] = pp.L[k];
+    ep.iL[k] = 1.0 / pp.L[k];
+  }
+  ep.q = pp.q;
+  ep.al8 = kEwTabInvWidth * pp.alpha;
+  ep.rc2 = pp.rc2;
+  const int items = p->P * B.ntp, img_min = image_mode(pp);
+  hipStream_t st = rt().s0;
+  hipEvent_t e0, e1;
+  XMD_CHECK(hipEventCreate(&e0));
+  XMD_CHECK(hipEventCreate(&e1));
+  RealChoice best{img_min, 2};
+  double tbest = 1e300;
+  for (int img = 0; img < 3; ++img)
+    for (int wl = 0; wl < 4; ++wl) {
+      if (ms) ms[4 * img + wl] = -1;
+      if (img < img_min) continue;
+      const int wpb = 1 << wl;
+      auto launch = [&]() {
+        hipLaunchKernelGGL(k_ewald_real_of[img][wl], dim3((items + wpb - 1) / wpb), dim3(64 * wpb), 0, st, d->X,
+                           d->M, B.Fpart, B.tiles, p->N, d->nb, B.T, B.ntp, items, ep);
+      };
+      launch();
+      XMD_CHECK(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; ++r) launch();
+      XMD_CHECK(hipEventRecord(e1, st));
+      XMD_CHECK(hipEventSynchronize(e1));
+      XMD_CHECK(hipGetLastError());
+      float t = 0;
+      XMD_CHECK(hipEventElapsedTime(&t, e0, e1));
+      const double avg = (double)t / reps;
+      if (ms) ms[4 * img + wl] = avg;
+      if (avg < tbest) {
+        tbest = avg;
+        best = RealChoice{img, wl};
+      }
+    }
+  XMD_CHECK(hipEventDestroy(e0));
+  XMD_CHECK(hipEventDestroy(e1));
+  g_real_tuned[real_bucket(p->N, p->P, img_min)] = best;
+  if (choice) {
+    choice[0] = best.img;
+    choice[1] = best.wlog;
+  }
+  return 0;
+}
+
+// With autotuning on: tune the object's bucket unless it is tuned already.
+int autotune_if_needed(DevPimd *p, int kind) {
+  if (!g_autotune || kind != PK_EWALD || !p->ds->uniform_mass) return 0;
+  EwaldPlan pl;
+  PairParams pp = ewald_params(p, pl);
+  if (!near_ok(pp) || g_real_tuned.count(real_bucket(p->N, p->P, image_mode(pp)))) return 0;
+  return autotune_real(p, 5, nullptr, nullptr);
+}
+
 int launch_pair_recip(DevPimd *p, int kind, hipStream_t st) {
   if (kind == PK_EWALD) {
     EwaldPlan pl;
@@ -1515,6 +1671,34 @@ int launch_pair_recip(DevPimd *p, int kind, hipStream_t st) {
   return launch_recip(p, pp, pp.nsum, 0.0, st, true, nullptr);
 }
 
+static void (*const k_frce_of[4])(const double2 *, const double2 *, const double2 *, const int2 *, int, int,
+                                  const double2 *, const double *, int, int, int, double, double, double, double,
+                                  double *, double *, double) = {k_recip_force_energy<1>, k_recip_force_energy<2>,
+                                                                 k_recip_force_energy<3>, k_recip_force_energy<4>};
+
+int launch_recip_labels(DevPimd *p, double *Frec, double *Erec, hipStream_t st, double *alpha_out) {
+  DevSim *d = p->ds;
+  if (!d->uniform_mass) {
+    fprintf(stderr, "xmd: the Ewald kernels need equal bead masses\n");
+    return -1;
+  }
+  EwaldPlan pl;
+  PairParams pp = ewald_params(p, pl);
+  if (alpha_out) *alpha_out = pp.alpha;
+  KSpace *K = nullptr;
+  if (launch_recip_sk(p, pp, pl.cube_n, pl.gmax2, st, &K)) return -1;
+  hipLaunchKernelGGL(k_frce_of[K->ZP / 16 - 1], dim3(((p->N + 15) / 16) * p->P), dim3(256), 0, st, K->tx, K->ty,
+                     K->tz, K->cols, K->ncolp, K->MZ, K->W, d->M, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P,
+                     Frec, Erec, K->e0);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+double ewald_split_alpha(DevPimd *p) {
+  EwaldPlan pl;
+  return ewald_params(p, pl).alpha;
+}
+
 int launch_pair_kspace_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join) {
   if (kind != PK_EWALD || !p->ds->uniform_mass) return 0;
   EwaldPlan pl;
@@ -1523,9 +1707,15 @@ int launch_pair_kspace_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2
   XMD_CHECK(hipEventRecord(fork, st));
   XMD_CHECK(hipStreamWaitEvent(s2, fork, 0));
   if (ensure_tiles(p)) return -1;
-  KSpace *K = nullptr;
-  if (launch_recip_sk(p, pp, pl.cube_n, pl.gmax2, s2, &K)) return -1;
-  if (launch_recip_force(p, pp, K, s2, pair_bufs(p)->tb.Frec)) return -1;
+  if (p->sur_mode) {
+    // the learned stand-in (or the judge step: exact terms plus the
+    // surrogate's for comparison) fills the same reciprocal-force buffer
+    if (surrogate_kspace(p, pair_bufs(p)->tb.Frec, s2)) return -1;
+  } else {
+    KSpace *K = nullptr;
+    if (launch_recip_sk(p, pp, pl.cube_n, pl.gmax2, s2, &K)) return -1;
+    if (launch_recip_force(p, pp, K, s2, pair_bufs(p)->tb.Frec)) return -1;
+  }
   XMD_CHECK(hipEventRecord(join, s2));
   return 1;
 }
@@ -1606,6 +1796,33 @@ int pair_energy(DevPimd *p, int kind, double *out) {
     if (launch_pair_mode<PK_EWALD, true>(p, pp, st, &e)) return -1;
     KSpace *K = nullptr;
     if (launch_recip(p, pp, pl.cube_n, pl.gmax2, st, false, &K)) return -1;
+    const int nb2 = 256;
+    hipLaunchKernelGGL(k_recip_energy, dim3(nb2), dim3(256), 0, st, K->S, K->coef, K->ncolp * K->ZP, p->P, p->N,
+                       d->red);
+    XMD_CHECK(hipGetLastError());
+    double er = 0;
+    if (device_sum(d->red, nb2, &er)) return -1;
+    const double V = pp.L[0] * pp.L[1] * pp.L[2];
+    const double bg = -pp.q * M_PI / (pp.alpha * pp.alpha * V) * 0.5 * p->N * (p->N - 1) * p->P;
+    e += er + bg;
+  } else if (kind == PK_EWALD_NI) {
+    if (launch_pair_mode<PK_EWALD_NI, true>(p, pp, st, &e)) return -1;
+  } else if (kind == PK_EWALD_R || kind == PK_EWALD_R2) {
+    // the reference's own alpha and k cube (as the R force split): real
+    // space without the minimum image, reciprocal space, background; R2
+    // adds -erf(alpha r)/r of the minimum image
+    if (!d->uniform_mass) {
+      fprintf(stderr, "xmd: the Ewald kernels need equal bead masses\n");
+      return -1;
+    }
+    if (launch_pair_mode<PK_EWALD_R, true>(p, pp, st, &e)) return -1;
+    if (kind == PK_EWALD_R2) {
+      double e2 = 0;
+      if (launch_pair_mode<PK_EWALD_R2, true>(p, pp, st, &e2)) return -1;
+      e += e2;
+    }
+    KSpace *K = nullptr;
+    if (launch_recip(p, pp, pp.nsum, 0.0, st, false, &K)) return -1;
     const int nb2 = 256;
     hipLaunchKernelGGL(k_recip_energy, dim3(nb2), dim3(256), 0, st, K->S, K->coef, K->ncolp * K->ZP, p->P, p->N,
                        d->red);
@@ -1748,10 +1965,10 @@ __global__ __launch_bounds__(64) void kb_pair_tiles(PAIR_REP) {
   k_pair_tiles_body<MODE, false, true>(bx, R.X, R.M, R.Fpart, nullptr, R.tiles, R.N, R.nb, R.T, R.ntp, R.pp);
 }
 
-template <int IMG>
-__global__ __launch_bounds__(256) void kb_ewald_real(PAIR_REP) {
+template <int IMG, int WPB>
+__global__ __launch_bounds__(64 * WPB) void kb_ewald_real(PAIR_REP) {
   PAIR_R
-  k_ewald_real_body<IMG>(bx, R.X, R.M, R.Fpart, R.tiles, R.N, R.nb, R.T, R.ntp, R.items, R.ep);
+  k_ewald_real_body<IMG, WPB>(bx, R.X, R.M, R.Fpart, R.tiles, R.N, R.nb, R.T, R.ntp, R.items, R.ep);
 }
 
 __global__ __launch_bounds__(256) void kb_reduce_tiles(PAIR_REP) {
@@ -1784,7 +2001,7 @@ __global__ __launch_bounds__(256) void kb_recip_force_mfma(PAIR_REP) {
 // real space: face, near, corner enumeration), G_RED (tile reduction of all
 // of them, in that order), G_PHASE / G_SK / G_FRC (reciprocal space; one
 // group per slot-tile count ZP / 16 = 1..4).
-enum { G_GAUSS, G_PGAUSS, G_EW, G_RED = G_EW + 3, G_PHASE, G_SK, G_FRC = G_SK + 4, G_COUNT = G_FRC + 4 };
+enum { G_GAUSS, G_PGAUSS, G_EW, G_RED = G_EW + 12, G_PHASE, G_SK, G_FRC = G_SK + 4, G_COUNT = G_FRC + 4 };
 
 struct PairBatch {
   BatchTable g[G_COUNT];
@@ -1853,8 +2070,10 @@ PairBatch *pair_batch_create(const std::vector<BatchItem> &items) {
     R.ep.rc2 = pp.rc2;
     int grp;
     if (ew) {
-      grp = G_EW + image_mode(pp);
-      blocks[grp].push_back((R.items + 3) / 4);
+      // Ewald real space grouped by launch choice (the autotuner's per bucket)
+      const RealChoice ch = real_choice(p->N, p->P, pp);
+      grp = G_EW + 4 * ch.img + ch.wlog;
+      blocks[grp].push_back((R.items + (1 << ch.wlog) - 1) >> ch.wlog);
     } else {
       grp = kind == PK_GAUSS ? G_GAUSS : G_PGAUSS;
       blocks[grp].push_back(R.items);
@@ -1891,9 +2110,18 @@ int pair_batch_forces(PairBatch *b, hipStream_t st) {
   };
   go(G_GAUSS, kb_pair_tiles<PK_GAUSS>, 64);
   go(G_PGAUSS, kb_pair_tiles<PK_PGAUSS>, 64);
-  go(G_EW + IMG_FACE, kb_ewald_real<IMG_FACE>, 256);
-  go(G_EW + IMG_NEAR, kb_ewald_real<IMG_NEAR>, 256);
-  go(G_EW + IMG_CORNER, kb_ewald_real<IMG_CORNER>, 256);
+  go(G_EW + 0, kb_ewald_real<IMG_FACE, 1>, 64);
+  go(G_EW + 1, kb_ewald_real<IMG_FACE, 2>, 128);
+  go(G_EW + 2, kb_ewald_real<IMG_FACE, 4>, 256);
+  go(G_EW + 3, kb_ewald_real<IMG_FACE, 8>, 512);
+  go(G_EW + 4, kb_ewald_real<IMG_NEAR, 1>, 64);
+  go(G_EW + 5, kb_ewald_real<IMG_NEAR, 2>, 128);
+  go(G_EW + 6, kb_ewald_real<IMG_NEAR, 4>, 256);
+  go(G_EW + 7, kb_ewald_real<IMG_NEAR, 8>, 512);
+  go(G_EW + 8, kb_ewald_real<IMG_CORNER, 1>, 64);
+  go(G_EW + 9, kb_ewald_real<IMG_CORNER, 2>, 128);
+  go(G_EW + 10, kb_ewald_real<IMG_CORNER, 4>, 256);
+  go(G_EW + 11, kb_ewald_real<IMG_CORNER, 8>, 512);
   go(G_RED, kb_reduce_tiles, 256);
   go(G_PHASE, kb_phase, 256);
   go(G_SK, kb_sk_mfma<1>, 256);
@@ -1994,6 +2222,54 @@ int md_pimd_calc_pair_energy_x(md_pimd_t_x *pimd, md_pair_energy_t_x pe, md_stat
   if (pair_energy(pimd_state(pimd, false), kind, &e)) return -1;
   stats->es[stats->N - 1] += e;
   return 0;
+}
+
+int md_set_autotune_x(int on) {
+  g_autotune = on ? 1 : 0;
+  return 0;
+}
+
+int md_set_ewald_real_variant_x(int img, int wlog) {
+  if (img < -1 || img > 2 || wlog < -1 || wlog > 3) return -1;
+  g_real_force_img = img;
+  g_real_force_wlog = wlog;
+  return 0;
+}
+
+int md_pimd_autotune_x(md_pimd_t_x *pimd, int reps, md_num_t_x *ms, int *choice) {
+  if (reps < 1 || ensure_pimd_device(pimd)) return -1;
+  return autotune_real(pimd_state(pimd, false), reps, ms, choice);
+}
+
+int md_autotune_save_x(const char *path) {
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  fprintf(f, "# MI355X PIMD engine: Ewald real-space launch choices per (N bucket, P, enumeration) bucket\n");
+  for (const auto &kv : g_real_tuned)
+    fprintf(f, "%lld %lld %lld %d %d\n", kv.first >> 24, (kv.first >> 4) & 0xfffff, kv.first & 15, kv.second.img,
+            kv.second.wlog);
+  return fclose(f) == 0 ? 0 : -1;
+}
+
+int md_autotune_load_x(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char line[256];
+  int n = 0;
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    long long nb2, P, imin;
+    int img, wl;
+    if (sscanf(line, "%lld %lld %lld %d %d", &nb2, &P, &imin, &img, &wl) != 5 || img < imin || img > 2 || wl < 0 ||
+        wl > 3) {
+      fclose(f);
+      return -1;
+    }
+    g_real_tuned[(nb2 << 24) | (P << 4) | imin] = RealChoice{img, wl};
+    ++n;
+  }
+  fclose(f);
+  return n;
 }
 
 }  // extern "C"

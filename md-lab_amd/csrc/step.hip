@@ -22,6 +22,7 @@
 // into one hipGraph per batch step.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "md_hip_x.h"
@@ -204,6 +205,43 @@ struct Batch {
   }
 };
 
+// A batch's tables and executable step graph are kept after a call and reused
+// by the next call on the same objects with the same configs, interaction
+// globals and object state (temperature, exchange parameter, box), as long
+// as no pair-kernel buffer was reallocated in between; anything else
+// rebuilds them.
+struct BatchCache {
+  std::vector<md_pimd_t_x *> pimds;
+  std::vector<md_pimd_step_config_x> cfgs;
+  std::vector<double> key;
+  unsigned epoch = 0;
+  Batch *B = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+static BatchCache g_bcache;
+
+static std::vector<double> batch_key(md_pimd_t_x *const *pimds, int n) {
+  std::vector<double> k = {md_trap_frequency_x, md_trap_center_x[0], md_trap_center_x[1], md_trap_center_x[2],
+                           md_gaussian_strength_x, md_gaussian_range_x, md_coulomb_strength_x, md_coulomb_truc_x,
+                           (double)md_coulomb_n_sum_x, (double)g_nhc_len, g_ewald_xcut, g_ewald_split_aL,
+                           md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C8_x, md_he_C10_x, md_he_D_x,
+                           md_he_rm_x};
+  for (int i = 0; i < n; ++i) {
+    k.push_back(pimds[i]->beta);
+    k.push_back(pimds[i]->omegaP);
+    k.push_back(pimds[i]->vi);
+    k.push_back(pimds[i]->sim->T);
+    for (int d = 0; d < 3; ++d) k.push_back(box_L(pimds[i]->sim, d));
+  }
+  return k;
+}
+
+static void batch_cache_drop() {
+  if (g_bcache.exec) (void)hipGraphExecDestroy(g_bcache.exec);
+  delete g_bcache.B;
+  g_bcache = BatchCache();
+}
+
 static int enqueue_batch(Batch &B, const Lane &L) {
   if (bead_batch_nhc(B.bead, 1, L.s0) || exch_batch_enk(B.exch, L.s0)) return -1;
   XMD_CHECK(hipEventRecord(L.fork, L.s0));
@@ -300,6 +338,7 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   constexpr int kStepsPerGraph = 4, kLongCall = 32;
   Runtime &R = rt();
   DevPimd *p = sp.p;
+  if (autotune_if_needed(p, sp.pair_kind)) return -1;
   const int fresh = pair_prepare(p, sp.pair_kind);
   if (fresh < 0) return -1;
   const int eager = (fresh || !p->step_exec) ? 1 : 0;
@@ -380,40 +419,70 @@ int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *
       }
     StepPlan sp;
     if (plan_of(pimds[i], &cfgs[i], sp)) return -1;
+    if (autotune_if_needed(sp.p, sp.pair_kind)) return -1;
     items[i] = item_of(sp);
   }
-  Batch B;
-  B.bead = bead_batch_create(items);
-  B.exch = exch_batch_create(items);
-  B.pair = pair_batch_create(items);
-  if (!B.bead || !B.exch || !B.pair) return -1;
   const Lane L = main_lane();
   Runtime &R = rt();
+  const bool graph = cfgs[0].use_graph != 0;
+  if (graph) {
+    // the kept graph of the previous call, when nothing it depends on changed
+    const std::vector<double> key = batch_key(pimds, n);
+    bool same = g_bcache.exec && g_bcache.pimds.size() == (size_t)n && g_bcache.key == key &&
+                g_bcache.epoch == pair_alloc_epoch();
+    for (int i = 0; same && i < n; ++i)
+      same = g_bcache.pimds[i] == pimds[i] && !memcmp(&g_bcache.cfgs[i], &cfgs[i], sizeof(md_pimd_step_config_x));
+    if (same) {
+      for (int s = 0; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(g_bcache.exec, R.s0));
+      XMD_CHECK(hipStreamSynchronize(R.s0));
+      return exch_batch_status(g_bcache.B->exch);
+    }
+    batch_cache_drop();
+  }
+  Batch *B = new Batch();
+  B->bead = bead_batch_create(items);
+  B->exch = exch_batch_create(items);
+  B->pair = pair_batch_create(items);
+  if (!B->bead || !B->exch || !B->pair) {
+    delete B;
+    return -1;
+  }
   // The first step runs eagerly; the rest replay one captured graph of a
   // whole batch step (or run eagerly when the configs ask for no graph).
-  if (enqueue_batch(B, L)) return -1;
-  XMD_CHECK(hipStreamSynchronize(R.s0));
-  if (nsteps > 1 && !cfgs[0].use_graph) {
-    for (int s = 1; s < nsteps; ++s)
-      if (enqueue_batch(B, L)) return -1;
-  } else if (nsteps > 1) {
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
-    int rc = enqueue_batch(B, L);
-    hipError_t ec = hipStreamEndCapture(R.s0, &graph);
-    if (rc || ec != hipSuccess) {
-      fprintf(stderr, "xmd: batch capture failed\n");
-      return -1;
-    }
-    XMD_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    for (int s = 1; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
-    XMD_CHECK(hipStreamSynchronize(R.s0));
-    XMD_CHECK(hipGraphExecDestroy(exec));
-    XMD_CHECK(hipGraphDestroy(graph));
+  if (enqueue_batch(*B, L)) {
+    delete B;
+    return -1;
   }
   XMD_CHECK(hipStreamSynchronize(R.s0));
-  return exch_batch_status(B.exch);
+  if (!graph) {
+    int rc = 0;
+    for (int s = 1; s < nsteps && !rc; ++s) rc = enqueue_batch(*B, L);
+    if (!rc && hipStreamSynchronize(R.s0) != hipSuccess) rc = -1;
+    if (!rc) rc = exch_batch_status(B->exch);
+    delete B;
+    return rc;
+  }
+  hipGraph_t g;
+  hipGraphExec_t exec;
+  XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
+  int rc = enqueue_batch(*B, L);
+  hipError_t ec = hipStreamEndCapture(R.s0, &g);
+  if (rc || ec != hipSuccess) {
+    fprintf(stderr, "xmd: batch capture failed\n");
+    delete B;
+    return -1;
+  }
+  XMD_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+  XMD_CHECK(hipGraphDestroy(g));
+  g_bcache.pimds.assign(pimds, pimds + n);
+  g_bcache.cfgs.assign(cfgs, cfgs + n);
+  g_bcache.key = batch_key(pimds, n);
+  g_bcache.epoch = pair_alloc_epoch();
+  g_bcache.B = B;
+  g_bcache.exec = exec;
+  for (int s = 1; s < nsteps; ++s) XMD_CHECK(hipGraphLaunch(exec, R.s0));
+  XMD_CHECK(hipStreamSynchronize(R.s0));
+  return exch_batch_status(B->exch);
 }
 
 }  // extern "C"

@@ -107,6 +107,8 @@ struct DevPimd {
   int *done = nullptr;      // finished-workgroup count of k_chain_scan (zero between launches)
   void *pair_bufs = nullptr;  // pair-kernel tile schedule, partial sums and k-space tables (pair_forces.hip)
   void *step_exec = nullptr;  // executable step graph of the last md_pimd_run_steps_x (hipGraphExec_t)
+  void *sur = nullptr;        // Ewald surrogate attached for md_pimd_run_steps_surrogate_x (Surrogate)
+  int sur_mode = 0;           // 0 exact reciprocal space, 1 surrogate, 2 judge (exact used, surrogate compared)
   bool enk_dev = false, enk2_dev = false, vbn_dev = false, back_dev = false;
   bool enk_host_stale = false;
   unsigned enk_version = 0, back_version = 0;
@@ -139,7 +141,10 @@ int launch_pbc(DevSim *d, const double *L, hipStream_t st);
 int launch_nhc(DevSim *d, int half, double h, hipStream_t st);
 int launch_ring_springs(DevPimd *p, int image, const double *L, hipStream_t st);
 
-enum PairKind { PK_GAUSS = 0, PK_PGAUSS = 1, PK_EWALD = 2, PK_EWALD_R = 3, PK_EWALD_NI = 4, PK_LJ = 5, PK_HE = 6 };
+// PK_EWALD_R2 (energies only): -erf(alpha r)/r of the minimum image, the
+// term md_coulomb_3d_ewald_energy_R2_x adds to the R split.
+enum PairKind { PK_GAUSS = 0, PK_PGAUSS = 1, PK_EWALD = 2, PK_EWALD_R = 3, PK_EWALD_NI = 4, PK_LJ = 5, PK_HE = 6,
+                PK_EWALD_R2 = 7 };
 
 // helium potential parameters (md_simulation_x.c:365-383), set by md_set_helium_parameters_x
 extern "C" md_num_t_x md_he_eps_x, md_he_A_x, md_he_alpha_x, md_he_C6_x, md_he_C8_x, md_he_C10_x, md_he_D_x,
@@ -175,6 +180,12 @@ __device__ __forceinline__ double combined_force(const double *__restrict__ F, c
 // this kind need; 1 when anything was allocated, 0 when all existed, -1 on
 // error.
 int pair_prepare(DevPimd *p, int kind);
+// Count of device (re)allocations made by the pair kernels' buffers so far
+// (a captured graph that names them stays valid while it is unchanged).
+unsigned pair_alloc_epoch();
+// Real-space Ewald launch choice autotuning (pair_forces.hip): with it on,
+// tune the object's (N, P) bucket before first use (never during capture).
+int autotune_if_needed(DevPimd *p, int kind);
 // Pair forces in parts, with reciprocal space on a side stream:
 // launch_pair_kspace_fork forks s2 from st (through `fork`) and queues the
 // phase tables, structure factors and reciprocal forces there (they read
@@ -191,6 +202,16 @@ int launch_combine_forces(DevPimd *p, const double *Fx, hipStream_t st);
 // The parts launch_combine_forces adds, for a fused consumer of F.
 int force_sums(DevPimd *p, const double *Fx, ForceSums *out);
 int launch_pair_real(DevPimd *p, int kind, hipStream_t st);
+// Exact reciprocal-space Ewald terms of the step's plan (alpha, k set of
+// md_coulomb_3d_ewald_force_x): forces (accelerations / P, [3][nb]) into Frec
+// and each bead's share of its slice's reciprocal energy into Erec [nb];
+// alpha_out (may be null) receives the split.  ewald_split_alpha: that split.
+int launch_recip_labels(DevPimd *p, double *Frec, double *Erec, hipStream_t st, double *alpha_out);
+double ewald_split_alpha(DevPimd *p);
+// Reciprocal-space forces of the step from the attached surrogate (sur_mode
+// 1) or exact with the surrogate's error against them recorded (sur_mode 2),
+// into Frec ([3][nb], accelerations / P), on st (surrogate.hip).
+int surrogate_kspace(DevPimd *p, double *Frec, hipStream_t st);
 int launch_pair_recip(DevPimd *p, int kind, hipStream_t st);
 bool pair_has_recip(int kind);
 int pair_energy(DevPimd *p, int kind, double *out);

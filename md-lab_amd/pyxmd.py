@@ -128,6 +128,7 @@ _SIGS = {
     "md_pimd_calc_vi_virial_energy_x": (C.c_int, [PIMD_P, STATS_P, C.c_int]),
     "md_pimd_calc_vi_sign_x": (C.c_int, [PIMD_P, STATS_P, STATS_P, C.c_double]),
     "md_calc_kinetic_x": (C.c_int, [SIM_P, STATS_P]),
+    "md_calc_temperature_x": (C.c_int, [SIM_P, STATS_P]),
     "md_calc_nhc_kinetic_x": (C.c_int, [SIM_P, STATS_P]),
     "md_alloc_stats_x": (STATS_P, [C.c_int]),
     "md_update_stats_x": (STATS_P, [STATS_P]),
@@ -183,12 +184,30 @@ _SIGS = {
     "md_pimd_run_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), C.c_int]),
     "md_pimd_run_batch_x": (C.c_int, [C.POINTER(PIMD_P), C.POINTER(StepConfig), C.c_int, C.c_int]),
     "md_pimd_profile_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), C.c_int, C.c_int, DP]),
+    "md_set_autotune_x": (C.c_int, [C.c_int]),
+    "md_pimd_autotune_x": (C.c_int, [PIMD_P, C.c_int, DP, C.POINTER(C.c_int)]),
+    "md_autotune_save_x": (C.c_int, [C.c_char_p]),
+    "md_autotune_load_x": (C.c_int, [C.c_char_p]),
+    "md_set_ewald_real_variant_x": (C.c_int, [C.c_int, C.c_int]),
+    "md_alloc_ewald_surrogate_x": (VP, [C.c_uint, C.c_double]),
+    "md_ewald_surrogate_train_x": (C.c_int, [VP, PIMD_P, C.c_double, C.c_int, DP]),
+    "md_ewald_surrogate_check_x": (C.c_int, [VP, PIMD_P, DP]),
+    "md_ewald_surrogate_params_x": (C.c_int, [VP, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "md_ewald_surrogate_normalization_x": (C.c_int, [VP, DP, DP]),
+    "md_ewald_surrogate_eval_x": (C.c_int, [VP, PIMD_P, C.POINTER(C.c_float), DP, DP]),
+    "md_ewald_surrogate_labels_x": (C.c_int, [VP, PIMD_P, DP, DP]),
+    "md_ewald_surrogate_gradient_x": (C.c_int, [VP, PIMD_P, DP, C.POINTER(C.c_float), DP]),
+    "md_pimd_train_surrogate_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), VP, C.c_int, C.c_double, C.c_int,
+                                                  DP]),
+    "md_pimd_run_steps_surrogate_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), VP, C.c_int, C.c_int, C.c_double,
+                                                C.c_double, C.c_double, DP]),
 }
 
 CALLBACKS = ["md_harmonic_trap_force_x", "md_gaussian_force_x", "md_periodic_gaussian_force_x",
              "md_coulomb_3d_ewald_force_x", "md_coulomb_3d_ewald_force_R_x", "md_coulomb_3d_ewald_force_NI_x",
              "md_harmonic_trap_energy_x", "md_gaussian_energy_x", "md_periodic_gaussian_energy_x",
-             "md_coulomb_3d_ewald_energy_x", "md_LJ_periodic_force_x", "md_LJ_periodic_energy_x",
+             "md_coulomb_3d_ewald_energy_x", "md_coulomb_3d_ewald_energy_R_x", "md_coulomb_3d_ewald_energy_R2_x",
+             "md_coulomb_3d_ewald_energy_NI_x", "md_LJ_periodic_force_x", "md_LJ_periodic_energy_x",
              "md_periodic_helium_force_x", "md_periodic_helium_energy_x"]
 
 
@@ -212,6 +231,10 @@ class Lib:
 
 def dptr(a):
     return a.ctypes.data_as(DP)
+
+
+def fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def beads(sim):

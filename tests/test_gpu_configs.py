@@ -11,19 +11,21 @@ Same initial state on both sides (positions and velocities stored whole,
 chains from md_init_nhc_x).  Per xi: the total
 force (trap + pair + exchange; every stride-th bead), V_B^(N) and the
 thermodynamic, virial and pair estimators; at the last xi a short fused-step
-trajectory.  Bars: forces max|dF|/RMS <= 1e-11, or twice the reference's own
-disagreement between its fast and cycle-walk exchange algorithms on the same
-input when that is larger (at beta=2 with N(0,1) beads, beta V_B ~ 1.6e6, so
-the 1 - exp(-x) entries of the connection matrix that should vanish carry the
-round-off of x, ~1e-10, in both codes); energies 1e-12 relative; trajectories
-1e-10 (x) / 1e-9 (v)."""
+trajectory.  Force bar: the total force (trap + pair from the reference,
+plus the exchange and ring-spring part from the oracle's 80-bit exchange
+path, which the reference's cycle-walk algorithm pins to ~1e-15 on the golden
+cases) is the truth; the engine is held to max(1e-11, four times the reference
+fast path's own distance from it).  At beta=2 with N(0,1) beads
+beta V_B ~ 1.6e6, so the connection-matrix exponents carry ~1e-10 of absolute
+round-off in any double-precision code.  Energies 1e-12 relative;
+trajectories 1e-10 (x) / 1e-9 (v)."""
 import os
 
 import numpy as np
 import pytest
 
 import pyxmd
-from orc import read_golden, rel_err
+from orc import Oracle, exchange_truth, read_golden, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +53,7 @@ def test_config_parity(name):
     s = pyxmd.System(lib, kind, N, P, T, float(xis[0]), L=L, nsum=nsum, positions=g["x0"])
     pyxmd.load_state(s, g["x0"], g["v0"])  # chains: md_init_nhc_x, as on the reference side
     sub = slice(0, None, stride)
+    o = Oracle()
     for q, xi in enumerate(xis):
         s.pimd.contents.vi = float(xi)
         s.full_force()
@@ -63,12 +66,14 @@ def test_config_parity(name):
         assert lib.md_pimd_fast_fill_ENk2_x(s.pimd, image) == 0
         e_vir = stat(lib, lib.md_pimd_calc_vi_virial_energy_x, s.pimd, image)
         e_pair = stat(lib, lib.md_pimd_calc_pair_energy_x, s.pimd, s.pair_energy_cb(), pos=2)
-        # the reference's own floor: its fast and cycle-walk exchange algorithms on this input
-        floor = rel_err(g[f"f_total_{q}"], g[f"f_total_slow_{q}"])
-        tol = max(1e-11, 2 * floor)
-        print(f"{name} xi={xi}: force rel err {err:.2e} (reference fast vs slow {floor:.2e}); "
-              f"V_B {vbn:.12g} vs {e[0]:.12g}")
+        fx, _ = exchange_truth(o, g["x0"], np.ones(N * P), N, P, 1.0 / T, float(xi), np.sqrt(P) * T, image, L)
+        truth = g["f_ext"] + fx.reshape(-1, 3)[sub].reshape(-1)
+        err, ref = rel_err(f, truth), rel_err(g[f"f_total_{q}"], truth)
+        tol = max(1e-11, 4 * ref)
+        print(f"{name} xi={xi}: total force vs 80-bit truth: engine {err:.2e}, reference fast path {ref:.2e} "
+              f"(reference cycle walk {rel_err(g[f'f_total_slow_{q}'], truth):.2e}); V_B {vbn:.12g} vs {e[0]:.12g}")
         assert err < tol
+        tol = tol + ref  # engine vs reference, for the virial bar below
         for got, ref in zip((vbn, e_vbn, e_pair), (e[0], e[1], e[3])):
             assert got == pytest.approx(ref, rel=1e-12, abs=1e-12)
         # the virial estimator adds sum_j>1 (x_j - x_1) . f_j m / 2, so the force bar

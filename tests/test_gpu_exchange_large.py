@@ -4,19 +4,20 @@ against the CPU oracle on the same beads (the golden cases stop at N = 24,
 one block).  Also the size-independent identities Vt(0) = V_B(N) and G
 doubly stochastic at full size.
 
-Force tolerance: max(1e-11, twice the disagreement between the reference's
-own two exchange algorithms (fast backward pass vs slow cycle walk, both in
-the oracle) on the same input).  The G entries 1 - exp(x) carry the absolute
-round-off of x ~ beta V_B(N) ~ 1e3-1e4, so at N = 700 the reference's two
-algorithms already differ by ~2e-10.  The slow walk costs O(N^3 P); at
-N = 1024 the forces follow from the checked G and springs, and the test
-checks V_B, Vt and G only.
+Forces are measured against the oracle's 80-bit exchange path (the truth;
+the reference's cycle-walk algorithm pins it to ~1e-15 on the golden cases).
+The connection-matrix entries carry the absolute round-off of their
+exponents ~ beta V_B(N) ~ 1e3-7e4 in any double-precision code, so the
+reference's own fast path (which the double oracle reproduces bit for bit)
+is 1e-12 .. 4e-9 of the RMS force away from the truth on these inputs; the
+engine is held to max(1e-11, four times that distance; engine / reference
+ranges 0.15 .. 3.3 over the inputs measured).
 """
 import numpy as np
 import pytest
 
 import pyxmd
-from orc import Oracle, beads_from, dptr, ptr, rel_err, tri_size
+from orc import Oracle, beads_from, dptr, exchange_truth, ptr, rel_err, tri_size
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +28,7 @@ def lib():
 
 
 CASES = [("trap", 130, 4, 0.0), ("trap", 200, 3, 0.5), ("trap", 300, 4, 1.0), ("ueg", 700, 2, 0.25),
-         ("ueg", 1024, 2, 1.0)]
+         ("ueg", 1024, 32, 1.0)]
 
 
 @pytest.mark.parametrize("kind,N,P,xi", CASES)
@@ -70,18 +71,15 @@ def test_exchange_multiblock(lib, kind, N, P, xi):
     np.testing.assert_allclose(vbn, V, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(Vt_g, Vt, rtol=1e-12, atol=1e-12)
     scale = max(1.0, abs(s.beta * V[N]))
-    np.testing.assert_allclose(G_g, G, rtol=1e-10, atol=max(1e-12, 64 * 2.2e-16 * scale))
-    err = rel_err(f, fo)
-    if N <= 700:
-        b2 = beads_from(hb["x"].reshape(-1), m=hb["m"])
-        o.slow_force_VB(N, P, s.beta, xi, s.omegaP, img, dptr(L), dptr(E), dptr(V), ptr(b2))
-        floor = rel_err(fo, b2["f"].reshape(-1))
-        tol = max(1e-11, 2 * floor)
-        print(f"{kind} N={N} P={P} xi={xi}: beta V_B(N)={s.beta * V[N]:.1f}, exchange force rel err {err:.2e} "
-              f"(reference fast vs slow {floor:.2e})")
-        assert err < tol
-    else:
-        print(f"{kind} N={N} P={P} xi={xi}: beta V_B(N)={s.beta * V[N]:.1f}, exchange force rel err {err:.2e}")
+    tol_g = max(1e-12, 64 * 2.2e-16 * scale)
+    np.testing.assert_allclose(G_g, G, rtol=1e-10, atol=tol_g)
+    ft, Vtruth = exchange_truth(o, hb["x"].reshape(-1), hb["m"], N, P, s.beta, xi, s.omegaP, img, L[0])
+    err, ref = rel_err(f, ft), rel_err(fo, ft)
+    tol = max(1e-11, 4 * ref)
+    print(f"{kind} N={N} P={P} xi={xi}: beta V_B(N)={s.beta * V[N]:.1f}, exchange force vs 80-bit truth: "
+          f"engine {err:.2e}, reference fast path {ref:.2e}")
+    assert err < tol
+    np.testing.assert_allclose(vbn, Vtruth, rtol=1e-12, atol=1e-12)
     # identities at full size
     assert Vt_g[0] == pytest.approx(vbn[N], rel=1e-12, abs=1e-12)
     Gm = G_g.reshape(N, N)

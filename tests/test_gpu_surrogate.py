@@ -1,0 +1,162 @@
+"""The learned Ewald stand-in (md_alloc_ewald_surrogate_x and friends) on the
+GPU against its FP64 CPU restatement (tests/sur_ref.py, itself pinned by
+brute-force triplet sums and finite differences):
+
+* descriptors (FP32 kernels) per slice, to FP32 round-off;
+* per-particle energies and forces of the BF16 matrix-core network with
+  FP32 accumulation, to BF16 round-off (energies 2 % of their spread,
+  forces 3 % RMS), and Newton's third law of the deterministic two-pass
+  force gather;
+* the FP32 training gradient of the loss, against the CPU restatement's
+  analytic gradient;
+* the exact per-particle labels: their per-slice sums are the reciprocal
+  Ewald energy sum_k A_k (|S_k|^2 - N) (numpy, the reference's alpha and k
+  cube), and their forces minus the finite differences of that energy;
+* online training on exact labels lowers the loss and the energy and force
+  errors against the FP64 reciprocal term.
+UEG N=64, P=4, r_s=2 (L = 13.0 bohr), cutoff 5 bohr."""
+import numpy as np
+import pytest
+
+import pyxmd
+import sur_ref as R
+
+pytestmark = pytest.mark.gpu
+
+N, P, RC = 64, 4, 5.0
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return pyxmd.Lib()
+
+
+def system(lib, seed=7, nsum=8, alpha_L=5.0):
+    L = pyxmd.ueg_box(N, 2.0)
+    return pyxmd.System(lib, "ueg", N, P, pyxmd.ueg_temperature(N, L, 0.5), 1.0, seed=seed, L=L, nsum=nsum,
+                        alpha_L=alpha_L, ueg_fluc=1.5)
+
+
+def random_model(lib, sur, seed=5):
+    rng = np.random.default_rng(seed)
+    par = (rng.normal(size=R.NPAR) * 0.15).astype(np.float32)
+    assert lib.md_ewald_surrogate_params_x(sur, None, pyxmd.fptr(par)) == R.NPAR
+    norm = np.concatenate([rng.uniform(0.0, 2.0, R.ND), np.exp(rng.normal(size=R.ND) * 0.3) * 0.4, [0.05, 0.02]])
+    assert lib.md_ewald_surrogate_normalization_x(sur, None, pyxmd.dptr(norm)) == 2 * R.ND + 2
+    got = np.zeros(2 * R.ND + 2)
+    lib.md_ewald_surrogate_normalization_x(sur, pyxmd.dptr(got), None)
+    # the device keeps mean and 1/std in FP32
+    return par.astype(np.float64), (got[:R.ND], got[R.ND:2 * R.ND], got[-2], got[-1])
+
+
+def slices(s):
+    x = s.host_beads()["x"].reshape(N, P, 3)
+    return [x[:, j] for j in range(P)]
+
+
+def test_descriptors_energies_forces(lib):
+    s = system(lib)
+    sur = lib.md_alloc_ewald_surrogate_x(1, RC)
+    assert sur
+    par, norm = random_model(lib, sur)
+    desc = np.zeros((N * P, R.ND), dtype=np.float32)
+    e = np.zeros(N * P)
+    F = np.zeros((N * P, 3))
+    assert lib.md_ewald_surrogate_eval_x(sur, s.pimd, pyxmd.fptr(desc), pyxmd.dptr(e), pyxmd.dptr(F)) == 0
+    desc = desc.reshape(N, P, R.ND)
+    e = e.reshape(N, P)
+    F = F.reshape(N, P, 3)
+    for j, x in enumerate(slices(s)):
+        D, _ = R.descriptors(x, s.L, RC)
+        scale = np.abs(D).max(0) + 1e-30
+        derr = (np.abs(desc[:, j] - D) / scale).max()
+        er = R.energies(x, s.L, RC, par, norm)
+        fr = R.forces(x, s.L, RC, par, norm)
+        eerr = np.abs(e[:, j] - er).max() / er.std()
+        ferr = np.sqrt(((F[:, j] - fr) ** 2).mean() / (fr ** 2).mean())
+        print(f"slice {j}: descriptors {derr:.2e}, energies {eerr:.2e} of spread, forces {ferr:.2e} RMS")
+        assert derr < 2e-5
+        assert eerr < 2e-2
+        assert ferr < 3e-2
+        assert np.abs(F[:, j].sum(0)).max() < 1e-5 * np.abs(F[:, j]).max() * N
+
+
+def test_training_gradient(lib):
+    s = system(lib, seed=8)
+    sur = lib.md_alloc_ewald_surrogate_x(2, RC)
+    par, norm = random_model(lib, sur, seed=6)
+    rng = np.random.default_rng(4)
+    target = rng.normal(size=N * P) * 0.02 + 0.05
+    g = np.zeros(R.NPAR, dtype=np.float32)
+    loss = np.zeros(1)
+    assert lib.md_ewald_surrogate_gradient_x(sur, s.pimd, pyxmd.dptr(target), pyxmd.fptr(g), pyxmd.dptr(loss)) == 0
+    mean, istd, ymu, ysd = norm
+    xs = s.host_beads()["x"].reshape(N, P, 3)
+    Dn = np.concatenate([(R.descriptors(xs[:, j], s.L, RC)[0] - mean) * istd for j in range(P)])
+    t = np.concatenate([(target.reshape(N, P)[:, j] - ymu) / ysd for j in range(P)])
+    lref, gref = R.loss_and_gradient(Dn, t, par)
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print(f"loss {loss[0]:.6g} vs {lref:.6g}; gradient rel err {rel:.2e}")
+    assert loss[0] == pytest.approx(lref, rel=1e-4)
+    assert rel < 1e-4
+
+
+def recip_energy(x, L, alpha, n):
+    """sum over the half space of the cube |m| <= n of A_k (|S_k|^2 - N),
+    A_k = exp(-pi^2 G^2 / alpha^2) / (pi V G^2), G = m / L (unit charges)."""
+    V = L ** 3
+    e = 0.0
+    m = np.arange(-n, n + 1)
+    mx, my, mz = np.meshgrid(m, m, m, indexing="ij")
+    mx, my, mz = mx.ravel(), my.ravel(), mz.ravel()
+    half = (mx > 0) | ((mx == 0) & (my > 0)) | ((mx == 0) & (my == 0) & (mz > 0))
+    k = np.stack([mx[half], my[half], mz[half]], 1) / L
+    G2 = (k ** 2).sum(1)
+    A = np.exp(-np.pi ** 2 * G2 / alpha ** 2) / (np.pi * V * G2)
+    ph = 2 * np.pi * x @ k.T
+    S2 = np.cos(ph).sum(0) ** 2 + np.sin(ph).sum(0) ** 2
+    return (A * (S2 - len(x))).sum()
+
+
+def test_exact_labels(lib):
+    lib.md_set_ewald_split_x(0.0)  # the reference's alpha and k cube
+    try:
+        s = system(lib, seed=9, nsum=4, alpha_L=5.0)
+        sur = lib.md_alloc_ewald_surrogate_x(3, RC)
+        e = np.zeros(N * P)
+        F = np.zeros((N * P, 3))
+        assert lib.md_ewald_surrogate_labels_x(sur, s.pimd, pyxmd.dptr(e), pyxmd.dptr(F)) == 0
+        e = e.reshape(N, P)
+        F = F.reshape(N, P, 3)
+        alpha = 5.0 / s.L
+        for j, x in enumerate(slices(s)):
+            ref = recip_energy(x, s.L, alpha, 4)
+            assert e[:, j].sum() == pytest.approx(ref, rel=1e-12, abs=1e-12)
+            h = 1e-5
+            for b in (0, 33):
+                for d in range(3):
+                    xp, xm = x.copy(), x.copy()
+                    xp[b, d] += h
+                    xm[b, d] -= h
+                    fd = -(recip_energy(xp, s.L, alpha, 4) - recip_energy(xm, s.L, alpha, 4)) / (2 * h)
+                    assert F[b, j, d] == pytest.approx(fd, rel=1e-6, abs=1e-8)
+    finally:
+        lib.md_set_ewald_split_x(-1.0)
+
+
+def test_online_training_reduces_error(lib):
+    s = system(lib, seed=10)
+    sur = lib.md_alloc_ewald_surrogate_x(4, RC)
+    rep0 = np.zeros(10)
+    loss = np.zeros(1)
+    assert lib.md_ewald_surrogate_train_x(sur, s.pimd, 1e-3, 1, pyxmd.dptr(loss)) == 0
+    l0 = loss[0]
+    assert lib.md_ewald_surrogate_check_x(sur, s.pimd, pyxmd.dptr(rep0)) == 0
+    assert lib.md_ewald_surrogate_train_x(sur, s.pimd, 1e-3, 300, pyxmd.dptr(loss)) == 0
+    rep = np.zeros(10)
+    assert lib.md_ewald_surrogate_check_x(sur, s.pimd, pyxmd.dptr(rep)) == 0
+    print(f"loss {l0:.3e} -> {loss[0]:.3e}; |dE|/N {abs(rep0[0]):.2e} -> {abs(rep[0]):.2e} Ha; "
+          f"force RMS rel {rep0[8]:.3f} -> {rep[8]:.3f}; exact recip force RMS {rep[5:8]}")
+    assert np.all(np.isfinite(rep))
+    assert loss[0] < 0.1 * l0
+    assert rep[8] < rep0[8]
