@@ -74,6 +74,26 @@ int md_checkpoint_read_x(FILE *in, md_simulation_t_x *sim);
  * (one workgroup per xi'). */
 int md_pimd_calc_vi_sign_grid_x(md_pimd_t_x *pimd, int n, const md_num_t_x *vi2, md_num_t_x *factor);
 
+/* Allocation-free variant of md_pimd_fill_VB_x.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Input d_ENk: the cycle-energy triangle in the reference's row
+ * layout (E_n^(k) at (n-1)n/2 + k-1, N(N+1)/2 doubles) in device memory;
+ * output d_VBN: V_B^(0..N) (N+1 doubles, device memory) and, when d_Vt is not
+ * NULL, the backward potentials Vt^(0..N) (device memory).  1 <= N <= 1024,
+ * beta > 0, vi >= 0; larger N is rejected with an error.  The working memory
+ * (5.8 MiB: the column copy of the cycle energies, the solve state and the
+ * hand-off flags) is a static array in the library's device code that every
+ * call on the same GPU shares: calls that overlap on one GPU race.  The call
+ * allocates nothing (no hipMalloc, hipFree or allocating library routine);
+ * it needs the library set up (md_init_setup_x or any earlier engine call)
+ * and is queued on the engine's main stream, ordered after earlier engine
+ * calls; the caller makes d_ENk ready before the call and reads the outputs
+ * after md_hip_synchronize_x, which also reports a timed-out hand-off of the
+ * solve (-1). */
+int md_pimd_fill_VB_noalloc_x(int N, md_num_t_x beta, md_num_t_x vi, const md_num_t_x *d_ENk, md_num_t_x *d_VBN,
+                              md_num_t_x *d_Vt);
+
 /* Internal exchange quantities after md_pimd_fill_VB_x (tests, diagnostics):
  * backward potentials Vt^(0..N) and the connection-probability matrix
  * G[(l-1)*N + (j-1)] = P(bead P of l links to bead 1 of j). */

@@ -36,6 +36,7 @@ int ensure_runtime() {
   hipDeviceProp_t prop;
   XMD_CHECK(hipGetDeviceProperties(&prop, g_rt.device));
   g_rt.num_cus = prop.multiProcessorCount;
+  if (noalloc_init(g_rt.device)) return -1;
   g_rt.ready = true;
   return 0;
 }

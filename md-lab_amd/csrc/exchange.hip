@@ -1319,8 +1319,10 @@ int recursion_status(DevPimd *p) {
 
 // recursion_status of every object with a recursion launched since its last
 // read (after a device-wide synchronisation).
+static int noalloc_status();
+
 int recursion_status_all() {
-  int rc = 0;
+  int rc = noalloc_status();
   while (!g_rec_pending.empty())
     if (recursion_status(g_rec_pending.back())) rc = -1;
   return rc;
@@ -1406,6 +1408,120 @@ int launch_exchange_force(DevPimd *p, int image, hipStream_t st, bool stored) {
 int launch_add_exchange_force(DevPimd *p, hipStream_t st) {
   hipLaunchKernelGGL(k_add_exch, dim3((3 * p->N + 255) / 256), dim3(256), 0, st, p->ds->F, p->Fx, p->N, p->P);
   XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------ allocation-free V_B
+//
+// md_pimd_fill_VB_noalloc_x runs the recursion of md_pimd_fill_VB_x (the same
+// set-up and persistent-solve kernels) on cycle energies the caller already
+// holds in device memory, with all of its working memory in the file-scope
+// array `scratch` below instead of buffers the library allocates.  Layout
+// (doubles): the column copy of the cycle energies, ln n, the forward and
+// backward values, the solve state (values, accumulators, block
+// coefficients of both directions), then the hand-off flags as ints.  The
+// flags are placed so that the error word and the injected-timeout count sit
+// at fixed positions for every N (the flags of a smaller N end where those of
+// N_max end), so a call never mistakes an earlier call's hand-off flags for
+// an error; the per-solve set-up clears the flags, the host clears the error
+// word when it reports it.
+constexpr int NA_NMAX = 1024;
+constexpr int NA_NBLK = (NA_NMAX + RB - 1) / RB;
+constexpr size_t NA_ET = 0;
+constexpr size_t NA_LNN = NA_ET + (size_t)NA_NMAX * (NA_NMAX + 1) / 2;
+constexpr size_t NA_LF = NA_LNN + (NA_NMAX + 1);
+constexpr size_t NA_RB = NA_LF + (NA_NMAX + 1);
+constexpr size_t NA_REC = NA_RB + (NA_NMAX + 1);
+constexpr size_t NA_SYNC = NA_REC + 6 * (size_t)(NA_NMAX + 1) + 2 * (size_t)NA_NBLK * (RTRI + RB * RB);
+constexpr int NA_NSYNC = 4 * NA_NBLK + 2;  // flags of N_max, error word, injected-timeout count (unused here)
+constexpr size_t NA_DOUBLES = NA_SYNC + (NA_NSYNC + 1) / 2;
+static_assert(NA_DOUBLES * sizeof(double) < 6u * 1024 * 1024, "scratch stays below 6 MiB");
+
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+// Device address of `scratch` per device (the host-side name is not a device
+// address), taken once per device by noalloc_init at runtime set-up.
+constexpr int NA_MAXDEV = 64;
+static double *g_scratch[NA_MAXDEV];
+static bool g_noalloc_pending = false;  // a solve was queued since the last status read
+
+int noalloc_init(int device) {
+  if (device < 0 || device >= NA_MAXDEV) return -1;
+  if (g_scratch[device]) return 0;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    fprintf(stderr, "xmd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+    return -1;
+  }
+  g_scratch[device] = (double *)ptr;
+  return 0;
+}
+
+static int *noalloc_err_word(double *base) { return (int *)(base + NA_SYNC) + 4 * NA_NBLK; }
+
+// ln n (n = 0..N) into the scratch's table.
+__global__ void k_noalloc_lnn(double *__restrict__ lnn, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= N) lnn[i] = i > 0 ? log((double)i) : 0.0;
+}
+
+// Vt^(m) = -R(m) / beta.
+__global__ void k_noalloc_vt(const double *__restrict__ Rb, double *__restrict__ Vt, int N, double beta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= N) Vt[i] = -Rb[i] / beta;
+}
+
+static int noalloc_fill_vb(int N, double beta, double vi, const double *ENk, double *VBN, double *Vt) {
+  Runtime &R = rt();
+  double *base = g_scratch[R.device];
+  const int nblk = (N + RB - 1) / RB;
+  RecArgs a;
+  a.ENk = ENk;
+  a.ET = base + NA_ET;
+  a.lnn = base + NA_LNN;
+  a.Lf = base + NA_LF;
+  a.Rb = base + NA_RB;
+  a.VBN = VBN;
+  a.val2 = base + NA_REC;
+  a.accM = a.val2 + 2 * (N + 1);
+  a.accS = a.val2 + 4 * (N + 1);
+  a.dg = a.val2 + 6 * (N + 1);
+  a.la = a.dg + 2 * (size_t)nblk * RTRI;
+  a.sync = (int *)(base + NA_SYNC) + 4 * (NA_NBLK - nblk);  // error word at the fixed slot 4 NA_NBLK
+  a.N = N;
+  a.nblk = nblk;
+  a.beta = beta;
+  a.xi0 = vi == 0.0;
+  a.lnxi = a.xi0 ? 0.0 : log(vi);
+  const hipStream_t st = R.s0;
+  hipLaunchKernelGGL(k_noalloc_lnn, dim3((N + 256) / 256), dim3(256), 0, st, base + NA_LNN, N);
+  hipLaunchKernelGGL(k_enk_transpose, dim3(N), dim3(256), 0, st, ENk, base + NA_ET, N);
+  const int ndir = Vt ? 2 : 1;
+  const int nprep = std::max(N + 1, nblk * RB * RB);
+  hipLaunchKernelGGL(k_rec_prep, dim3((nprep + 255) / 256, ndir), dim3(256), 0, st, a, 0);
+  const int per = 1 + std::max(0, nblk - 2);
+  hipLaunchKernelGGL(k_rec_persist, dim3(ndir * per), dim3(256), 0, st, a, 0);
+  if (Vt) hipLaunchKernelGGL(k_noalloc_vt, dim3((N + 256) / 256), dim3(256), 0, st, base + NA_RB, Vt, N, beta);
+  XMD_CHECK(hipGetLastError());
+  g_noalloc_pending = true;
+  return 0;
+}
+
+// Error word of the allocation-free solves queued since the last read
+// (call after a synchronisation); cleared once reported.
+static int noalloc_status() {
+  if (!g_noalloc_pending) return 0;
+  g_noalloc_pending = false;
+  int *err = noalloc_err_word(g_scratch[rt().device]);
+  int v = 0;
+  XMD_CHECK(hipMemcpy(&v, err, sizeof(int), hipMemcpyDeviceToHost));
+  if (v) {
+    fprintf(stderr, "xmd: exchange recursion hand-off timed out (allocation-free solve)\n");
+    const int zero = 0;
+    XMD_CHECK(hipMemcpy(err, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return -1;
+  }
   return 0;
 }
 
@@ -1766,6 +1882,23 @@ int md_pimd_fast_fill_ENk_x(md_pimd_t_x *pimd, int image) {
 // The cycle-walk algorithm (md_pimd_x.c:593-665) defines the same E_n^(k);
 // the engine fills it with the closed form above.
 int md_pimd_fill_ENk_x(md_pimd_t_x *pimd, int image) { return md_pimd_fast_fill_ENk_x(pimd, image); }
+
+int md_pimd_fill_VB_noalloc_x(int N, md_num_t_x beta, md_num_t_x vi, const md_num_t_x *d_ENk, md_num_t_x *d_VBN,
+                              md_num_t_x *d_Vt) {
+  if (!rt().ready || !g_scratch[rt().device]) {
+    fprintf(stderr, "xmd: md_pimd_fill_VB_noalloc_x needs the library set up first (md_init_setup_x)\n");
+    return -1;
+  }
+  if (N < 1 || N > NA_NMAX) {
+    fprintf(stderr, "xmd: md_pimd_fill_VB_noalloc_x supports 1 <= N <= %d (got %d)\n", NA_NMAX, N);
+    return -1;
+  }
+  if (!(beta > 0) || !(vi >= 0) || !d_ENk || !d_VBN) {
+    fprintf(stderr, "xmd: md_pimd_fill_VB_noalloc_x needs beta > 0, vi >= 0 and device buffers for ENk and VBN\n");
+    return -1;
+  }
+  return noalloc_fill_vb(N, beta, vi, d_ENk, d_VBN, d_Vt);
+}
 
 int md_pimd_fill_VB_x(md_pimd_t_x *pimd) {
   if (ensure_pimd_device(pimd)) return -1;

@@ -241,6 +241,9 @@ extern double g_ewald_split_aL;
 // solves of the object give up their first hand-off wait (test hook).
 int recursion_status(DevPimd *p);
 int recursion_status_all();
+// Device address of the allocation-free recursion's scratch array for this
+// device (once per device, at runtime set-up).
+int noalloc_init(int device);
 int inject_recursion_timeouts(DevPimd *p, int count);
 
 // ------------------------------------------------------ batched replicas
