@@ -145,6 +145,14 @@ int md_pimd_run_batch_x(md_pimd_t_x *const *pimds, const md_pimd_step_config_x *
  * labels from the exact FP64 reciprocal kernels. */
 typedef struct md_ewald_surrogate_x md_ewald_surrogate_x;
 md_ewald_surrogate_x *md_alloc_ewald_surrogate_x(unsigned seed, md_num_t_x r_cut);
+/* Keep the low reciprocal shells |m| <= nlow (G = m / L, ball along the
+ * longest side) exact in FP64 and let the network stand in for the rest of
+ * the sum only (0, the default: the network stands in for all of it).  The
+ * reciprocal force of a dense periodic system is dominated by these few
+ * long-wavelength terms, which no cutoff descriptor can see.  Set before
+ * training (a change discards the normalisation); the labels, checks, judge
+ * and surrogate steps then all use the split. */
+int md_ewald_surrogate_set_exact_shells_x(md_ewald_surrogate_x *s, int nlow);
 /* iters Adam steps (learning rate lr) on the current configuration's exact
  * labels (the first call also fixes the feature and label normalisation);
  * *loss: the normalised training loss before the last step. */
@@ -162,8 +170,9 @@ int md_ewald_surrogate_check_x(md_ewald_surrogate_x *s, md_pimd_t_x *pimd, md_nu
  * std): get and/or set (either may be NULL); return the count. */
 int md_ewald_surrogate_params_x(md_ewald_surrogate_x *s, float *get, const float *set);
 int md_ewald_surrogate_normalization_x(md_ewald_surrogate_x *s, md_num_t_x *get, const md_num_t_x *set);
-/* Prediction (raw descriptors [b][96], energies [b], slice-energy forces
- * [b][3]) and exact labels ([b], [b][3]) on the current configuration, and
+/* Network prediction (raw descriptors [b][96], energies [b], slice-energy
+ * forces [b][3]) and its exact targets ([b], [b][3]: the labels less the
+ * exact low shells) on the current configuration, and
  * the training-loss gradient for given per-bead targets, in reference bead
  * order b = (l-1) P + (j-1).  Outputs may be NULL. */
 int md_ewald_surrogate_eval_x(md_ewald_surrogate_x *s, md_pimd_t_x *pimd, float *desc, md_num_t_x *e_bead,

@@ -476,11 +476,13 @@ struct TileBufs {
 };
 
 // Per-object buffers of the pair kernels, so that independent systems can
-// run concurrently: the tile schedule and partial sums, and two k-space table
-// sets (most recently used kept).
+// run concurrently: the tile schedule and partial sums, and three k-space
+// table sets (most recently used kept: the step's, the estimators', and the
+// low shells of a surrogate).
+constexpr int NKS = 3;
 struct PairBufs {
   TileBufs tb;
-  KSpace ks[2];
+  KSpace ks[NKS];
   int ks_clock = 0;
 };
 
@@ -515,7 +517,7 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
   PairBufs *pb = pair_bufs(p);
   KSpace *ks = pb->ks;
   KSpace *K = nullptr;
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < NKS; ++i) {
     KSpace &k = ks[i];
     if (k.n == n && k.alpha == alpha && k.q == q && k.gmax2 == gmax2 && k.L[0] == L[0] && k.L[1] == L[1] &&
         k.L[2] == L[2])
@@ -523,7 +525,9 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
   }
   if (!K) {
     ++g_pair_allocs;
-    K = ks[0].stamp <= ks[1].stamp ? &ks[0] : &ks[1];
+    K = &ks[0];
+    for (int i = 1; i < NKS; ++i)
+      if (ks[i].stamp < K->stamp) K = &ks[i];
     int M = n;
     if (n < 0) {
       M = 0;
@@ -1687,6 +1691,22 @@ int launch_recip_labels(DevPimd *p, double *Frec, double *Erec, hipStream_t st, 
   if (alpha_out) *alpha_out = pp.alpha;
   KSpace *K = nullptr;
   if (launch_recip_sk(p, pp, pl.cube_n, pl.gmax2, st, &K)) return -1;
+  hipLaunchKernelGGL(k_frce_of[K->ZP / 16 - 1], dim3(((p->N + 15) / 16) * p->P), dim3(256), 0, st, K->tx, K->ty,
+                     K->tz, K->cols, K->ncolp, K->MZ, K->W, d->M, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P,
+                     Frec, Erec, K->e0);
+  XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_recip_shells(DevPimd *p, int nlow, double *Frec, double *Erec, hipStream_t st) {
+  DevSim *d = p->ds;
+  if (!d->uniform_mass || nlow < 1) return -1;
+  EwaldPlan pl;
+  PairParams pp = ewald_params(p, pl);
+  const double Lmax = std::max(pp.L[0], std::max(pp.L[1], pp.L[2]));
+  const double g = (nlow + 1e-6) / Lmax;  // |m| <= nlow along the longest side
+  KSpace *K = nullptr;
+  if (launch_recip_sk(p, pp, -1, g * g, st, &K)) return -1;
   hipLaunchKernelGGL(k_frce_of[K->ZP / 16 - 1], dim3(((p->N + 15) / 16) * p->P), dim3(256), 0, st, K->tx, K->ty,
                      K->tz, K->cols, K->ncolp, K->MZ, K->W, d->M, p->N, d->nb, p->P, pp.L[0], pp.L[1], pp.L[2], pp.P,
                      Frec, Erec, K->e0);

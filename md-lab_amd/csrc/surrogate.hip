@@ -463,10 +463,11 @@ __global__ __launch_bounds__(128) void k_sur_force1(const double *__restrict__ X
 
 // Pass 2, one thread per bead l: F_l = own part - sum over its neighbours i
 // of the reaction stored in i's list at l's slot (found by bisection), as an
-// acceleration of the slice energy's 1/P share: out[d][s] = F / (m P).
+// acceleration of the slice energy's 1/P share: out[d][s] = F / (m P), or
+// added to out when add is set.
 __global__ __launch_bounds__(128) void k_sur_force2(Geo g, const int *__restrict__ nbr, const int *__restrict__ cnt,
                                                     const double *__restrict__ Fown, const float *__restrict__ V,
-                                                    const double *__restrict__ M, double Pd,
+                                                    const double *__restrict__ M, double Pd, int add,
                                                     double *__restrict__ out) {
   const int s = blockIdx.x * 128 + threadIdx.x;
   if (s >= g.nb) return;
@@ -492,9 +493,15 @@ __global__ __launch_bounds__(128) void k_sur_force2(Geo g, const int *__restrict
     fz -= v[2];
   }
   const double w = 1.0 / (M[s] * Pd);
-  out[s] = fx * w;
-  out[g.nb + s] = fy * w;
-  out[2 * g.nb + s] = fz * w;
+  if (add) {
+    out[s] += fx * w;
+    out[g.nb + s] += fy * w;
+    out[2 * g.nb + s] += fz * w;
+  } else {
+    out[s] = fx * w;
+    out[g.nb + s] = fy * w;
+    out[2 * g.nb + s] = fz * w;
+  }
 }
 
 // ------------------------------------------------------------ training
@@ -708,18 +715,23 @@ __global__ void k_sur_convert(const float *__restrict__ par, unsigned short *__r
 
 __global__ void k_sur_zero(int *__restrict__ v) { *v = 0; }
 
-// Targets t_i = (e_i - ymu) / ysd from the FP64 labels.
-__global__ void k_sur_targets(const double *__restrict__ E, int n, double ymu, double ysd, float *__restrict__ T) {
+// Targets t_i = (e_i - el_i - ymu) / ysd from the FP64 labels e and the
+// exact low-shell part el (null: none).
+__global__ void k_sur_targets(const double *__restrict__ E, const double *__restrict__ El, int n, double ymu,
+                              double ysd, float *__restrict__ T) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) T[t] = (float)((E[t] - ymu) / ysd);
+  if (t < n) T[t] = (float)((E[t] - (El ? El[t] : 0.0) - ymu) / ysd);
 }
 
 // Error sums of a prediction against the exact terms: [0..2] sum (dF_d)^2,
 // [3..5] sum F_d^2 (exact), per workgroup; per-slice energy differences
-// sum_i (e_pred - e_exact) into dE[slice] (one workgroup per slice).
+// sum_i (e_pred - e_exact) into dE[slice] (one workgroup per slice), the
+// prediction being the network's eps plus the exact low-shell el (null:
+// none).
 __global__ __launch_bounds__(256) void k_sur_err(const double *__restrict__ Fs, const double *__restrict__ Fx,
-                                                 const float *__restrict__ eps, const double *__restrict__ Ex,
-                                                 int N, int nb, double *__restrict__ fpart, double *__restrict__ dE,
+                                                 const float *__restrict__ eps, const double *__restrict__ El,
+                                                 const double *__restrict__ Ex, int N, int nb,
+                                                 double *__restrict__ fpart, double *__restrict__ dE,
                                                  double *__restrict__ Eex) {
   __shared__ double red[8][256];
   const int slice = blockIdx.x, t = threadIdx.x;
@@ -732,7 +744,7 @@ __global__ __launch_bounds__(256) void k_sur_err(const double *__restrict__ Fs, 
       a[d] += e * e;
       a[3 + d] += x * x;
     }
-    a[6] += (double)eps[s] - Ex[s];
+    a[6] += (double)eps[s] + (El ? El[s] : 0.0) - Ex[s];
     a[7] += Ex[s];
   }
 #pragma unroll
@@ -762,6 +774,7 @@ using namespace sur;
 
 struct Surrogate {
   double rc = 8.0;
+  int nlow = 0;  // low reciprocal shells |m| <= nlow kept exact (0: none)
   float *par = nullptr, *am = nullptr, *av = nullptr, *grad = nullptr, *W2T = nullptr;
   unsigned short *w1t = nullptr, *w1 = nullptr, *w2t = nullptr, *w2 = nullptr;
   float *mean = nullptr, *istd = nullptr;
@@ -773,7 +786,7 @@ struct Surrogate {
   int *nbr = nullptr, *cnt = nullptr, *ovf = nullptr;
   float *D = nullptr, *Mom = nullptr, *Dn = nullptr, *dD = nullptr, *eps = nullptr, *V = nullptr;
   unsigned short *Db = nullptr;
-  double *Fown = nullptr, *Fs = nullptr, *Fx = nullptr, *Ex = nullptr;
+  double *Fown = nullptr, *Fs = nullptr, *Fx = nullptr, *Ex = nullptr, *El = nullptr;
   float *H1 = nullptr, *H2 = nullptr, *Y = nullptr, *DY = nullptr, *D1 = nullptr, *D2 = nullptr, *T = nullptr;
   float *lpart = nullptr, *gpart = nullptr;
   double *fpart = nullptr, *dE = nullptr, *Eex = nullptr;
@@ -817,7 +830,7 @@ static int sur_ensure(Surrogate *S, DevPimd *p) {
         dalloc(S->Mom, (size_t)nb * NMOM) || dalloc(S->Dn, (size_t)nb * ND) || dalloc(S->Db, (size_t)nb * ND) ||
         dalloc(S->dD, (size_t)nb * ND) || dalloc(S->eps, nb) || dalloc(S->V, (size_t)nb * MAXNB * 3) ||
         dalloc(S->Fown, 3 * (size_t)nb) || dalloc(S->Fs, 3 * (size_t)nb) || dalloc(S->Fx, 3 * (size_t)nb) ||
-        dalloc(S->Ex, nb) || dalloc(S->H1, (size_t)nb * H) || dalloc(S->H2, (size_t)nb * H) || dalloc(S->Y, nb) ||
+        dalloc(S->Ex, nb) || dalloc(S->El, nb) || dalloc(S->H1, (size_t)nb * H) || dalloc(S->H2, (size_t)nb * H) || dalloc(S->Y, nb) ||
         dalloc(S->DY, nb) || dalloc(S->D1, (size_t)nb * H) || dalloc(S->D2, (size_t)nb * H) || dalloc(S->T, nb) ||
         dalloc(S->lpart, (nb + TB - 1) / TB))
       return -1;
@@ -889,9 +902,9 @@ static int sur_features(Surrogate *S, DevPimd *p, hipStream_t st) {
   return 0;
 }
 
-// Predicted per-bead energies and forces (accelerations of the 1/P share,
-// [3][nb]) into out.
-static int sur_infer(Surrogate *S, DevPimd *p, double *out, hipStream_t st) {
+// Predicted per-bead energies (eps) and forces (accelerations of the 1/P
+// share, [3][nb]) stored into out, or added to it when add is set.
+static int sur_infer(Surrogate *S, DevPimd *p, double *out, hipStream_t st, int add = 0) {
   const Geo g = sur_geo(S, p);
   MlpArgs a;
   a.Db = S->Db;
@@ -915,7 +928,7 @@ static int sur_infer(Surrogate *S, DevPimd *p, double *out, hipStream_t st) {
   hipLaunchKernelGGL(k_sur_force1, dim3((g.nb + 127) / 128), dim3(128), 0, st, p->ds->X, g, S->nbr, S->cnt, S->dD,
                      S->Mom, S->Fown, S->V);
   hipLaunchKernelGGL(k_sur_force2, dim3((g.nb + 127) / 128), dim3(128), 0, st, g, S->nbr, S->cnt, S->Fown, S->V,
-                     p->ds->M, (double)p->P, out);
+                     p->ds->M, (double)p->P, add, out);
   XMD_CHECK(hipGetLastError());
   return 0;
 }
@@ -935,9 +948,11 @@ static int sur_normalize(Surrogate *S, DevPimd *p) {
   const int nb = p->N * p->P;
   XMD_CHECK(hipStreamSynchronize(rt().s0));
   std::vector<float> D((size_t)nb * ND);
-  std::vector<double> E(nb);
+  std::vector<double> E(nb), El(nb, 0.0);
   XMD_CHECK(hipMemcpy(D.data(), S->D, D.size() * sizeof(float), hipMemcpyDeviceToHost));
   XMD_CHECK(hipMemcpy(E.data(), S->Ex, nb * sizeof(double), hipMemcpyDeviceToHost));
+  if (S->nlow) XMD_CHECK(hipMemcpy(El.data(), S->El, nb * sizeof(double), hipMemcpyDeviceToHost));
+  for (int b = 0; b < nb; ++b) E[b] -= El[b];  // the network's targets
   std::vector<float> mean(ND), istd(ND);
   for (int f = 0; f < ND; ++f) {
     double s = 0, s2 = 0;
@@ -989,8 +1004,11 @@ static int sur_gradient(Surrogate *S, int nb, double *loss, hipStream_t st) {
   return 0;
 }
 
+// Exact labels Fx, Ex; with low shells kept exact, also their part: forces
+// stored in Fs (the base of a prediction), energies in El.
 static int sur_labels(Surrogate *S, DevPimd *p, hipStream_t st) {
-  return launch_recip_labels(p, S->Fx, S->Ex, st, &S->alpha);
+  if (launch_recip_labels(p, S->Fx, S->Ex, st, &S->alpha)) return -1;
+  return S->nlow ? launch_recip_shells(p, S->nlow, S->Fs, S->El, st) : 0;
 }
 
 int sur_prepare(Surrogate *S, DevPimd *p) {
@@ -1004,7 +1022,8 @@ int sur_train(Surrogate *S, DevPimd *p, double lr, int iters, double *loss) {
   const int nb = p->N * p->P;
   if (sur_labels(S, p, st) || sur_features(S, p, st)) return -1;
   if (!S->normed && sur_normalize(S, p)) return -1;
-  hipLaunchKernelGGL(k_sur_targets, dim3((nb + 255) / 256), dim3(256), 0, st, S->Ex, nb, S->ymu, S->ysd, S->T);
+  hipLaunchKernelGGL(k_sur_targets, dim3((nb + 255) / 256), dim3(256), 0, st, S->Ex, S->nlow ? S->El : nullptr, nb,
+                     S->ymu, S->ysd, S->T);
   XMD_CHECK(hipGetLastError());
   const float b1 = 0.9f, b2 = 0.999f;
   for (int it = 0; it < iters; ++it) {
@@ -1028,9 +1047,9 @@ int sur_train(Surrogate *S, DevPimd *p, double lr, int iters, double *loss) {
 int sur_check(Surrogate *S, DevPimd *p, double *out) {
   if (sur_prepare(S, p)) return -1;
   hipStream_t st = rt().s0;
-  if (sur_labels(S, p, st) || sur_features(S, p, st) || sur_infer(S, p, S->Fs, st)) return -1;
-  hipLaunchKernelGGL(k_sur_err, dim3(p->P), dim3(256), 0, st, S->Fs, S->Fx, S->eps, S->Ex, p->N, p->N * p->P,
-                     S->fpart, S->dE, S->Eex);
+  if (sur_labels(S, p, st) || sur_features(S, p, st) || sur_infer(S, p, S->Fs, st, S->nlow > 0)) return -1;
+  hipLaunchKernelGGL(k_sur_err, dim3(p->P), dim3(256), 0, st, S->Fs, S->Fx, S->eps, S->nlow ? S->El : nullptr, S->Ex,
+                     p->N, p->N * p->P, S->fpart, S->dE, S->Eex);
   XMD_CHECK(hipGetLastError());
   XMD_CHECK(hipStreamSynchronize(st));
   std::vector<double> fp(6 * p->P), de(p->P), ee(p->P);
@@ -1057,17 +1076,24 @@ int sur_check(Surrogate *S, DevPimd *p, double *out) {
 }
 
 // The step's reciprocal-space stage with a surrogate attached (see
-// md_pimd_run_steps_surrogate_x): mode 1 the surrogate's forces; mode 2 the
-// exact FP64 terms (forces used by the step, per-bead energies kept) and the
-// surrogate's prediction beside them, with the error sums left in fpart / dE.
+// md_pimd_run_steps_surrogate_x): mode 1 the surrogate's forces (the exact
+// low shells plus the network); mode 2 the exact FP64 terms (forces used by
+// the step, per-bead energies kept) and the surrogate's prediction beside
+// them, with the error sums left in fpart / dE.
 int surrogate_kspace(DevPimd *p, double *Frec, hipStream_t st) {
   Surrogate *S = (Surrogate *)p->sur;
   if (!S) return -1;
-  if (p->sur_mode == 1) return (sur_features(S, p, st) || sur_infer(S, p, Frec, st)) ? -1 : 0;
-  if (launch_recip_labels(p, Frec, S->Ex, st, &S->alpha) || sur_features(S, p, st) || sur_infer(S, p, S->Fs, st))
+  const int add = S->nlow > 0;
+  if (p->sur_mode == 1) {
+    if (add && launch_recip_shells(p, S->nlow, Frec, S->El, st)) return -1;
+    return (sur_features(S, p, st) || sur_infer(S, p, Frec, st, add)) ? -1 : 0;
+  }
+  if (launch_recip_labels(p, Frec, S->Ex, st, &S->alpha) ||
+      (add && launch_recip_shells(p, S->nlow, S->Fs, S->El, st)) || sur_features(S, p, st) ||
+      sur_infer(S, p, S->Fs, st, add))
     return -1;
-  hipLaunchKernelGGL(k_sur_err, dim3(p->P), dim3(256), 0, st, S->Fs, Frec, S->eps, S->Ex, p->N, p->N * p->P, S->fpart,
-                     S->dE, S->Eex);
+  hipLaunchKernelGGL(k_sur_err, dim3(p->P), dim3(256), 0, st, S->Fs, Frec, S->eps, add ? S->El : nullptr, S->Ex, p->N,
+                     p->N * p->P, S->fpart, S->dE, S->Eex);
   XMD_CHECK(hipGetLastError());
   return 0;
 }
@@ -1119,6 +1145,13 @@ md_ewald_surrogate_x *md_alloc_ewald_surrogate_x(unsigned seed, md_num_t_x r_cut
   md_ewald_surrogate_x *h = new md_ewald_surrogate_x;
   h->s = S;
   return h;
+}
+
+int md_ewald_surrogate_set_exact_shells_x(md_ewald_surrogate_x *h, int nlow) {
+  if (!h || nlow < 0) return -1;
+  if (nlow != h->s->nlow) h->s->normed = false;  // the targets change
+  h->s->nlow = nlow;
+  return 0;
 }
 
 int md_ewald_surrogate_train_x(md_ewald_surrogate_x *h, md_pimd_t_x *pimd, md_num_t_x lr, int iters,
@@ -1214,7 +1247,9 @@ int md_ewald_surrogate_eval_x(md_ewald_surrogate_x *h, md_pimd_t_x *pimd, float 
 }
 
 // Exact labels of the current configuration (reference bead order): each
-// bead's share of its slice's reciprocal energy and the slice-energy forces.
+// bead's share of its slice's reciprocal energy and the slice-energy forces,
+// less the exact low-shell part when low shells are kept (the network's
+// targets).
 int md_ewald_surrogate_labels_x(md_ewald_surrogate_x *h, md_pimd_t_x *pimd, md_num_t_x *e_bead, md_num_t_x *force) {
   DevPimd *p;
   if (!h || sur_pimd(pimd, p)) return -1;
@@ -1222,15 +1257,19 @@ int md_ewald_surrogate_labels_x(md_ewald_surrogate_x *h, md_pimd_t_x *pimd, md_n
   if (sur_prepare(S, p) || sur_labels(S, p, rt().s0)) return -1;
   XMD_CHECK(hipStreamSynchronize(rt().s0));
   const int N = p->N, P = p->P, nb = N * P;
-  std::vector<double> E(nb), F(3 * (size_t)nb), M(nb);
+  std::vector<double> E(nb), F(3 * (size_t)nb), M(nb), El(nb, 0.0), Fl(3 * (size_t)nb, 0.0);
   XMD_CHECK(hipMemcpy(E.data(), S->Ex, nb * sizeof(double), hipMemcpyDeviceToHost));
   XMD_CHECK(hipMemcpy(F.data(), S->Fx, F.size() * sizeof(double), hipMemcpyDeviceToHost));
   XMD_CHECK(hipMemcpy(M.data(), p->ds->M, nb * sizeof(double), hipMemcpyDeviceToHost));
+  if (S->nlow) {
+    XMD_CHECK(hipMemcpy(El.data(), S->El, nb * sizeof(double), hipMemcpyDeviceToHost));
+    XMD_CHECK(hipMemcpy(Fl.data(), S->Fs, Fl.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
   for (int s = 0; s < nb; ++s) {
     const int b = slot_to_bead(s, N, P);
-    if (e_bead) e_bead[b] = E[s];
+    if (e_bead) e_bead[b] = E[s] - El[s];
     if (force)
-      for (int d = 0; d < 3; ++d) force[3 * b + d] = F[d * (size_t)nb + s] * M[s] * P;
+      for (int d = 0; d < 3; ++d) force[3 * b + d] = (F[d * (size_t)nb + s] - Fl[d * (size_t)nb + s]) * M[s] * P;
   }
   return 0;
 }

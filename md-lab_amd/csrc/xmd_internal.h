@@ -208,6 +208,9 @@ int launch_pair_real(DevPimd *p, int kind, hipStream_t st);
 // alpha_out (may be null) receives the split.  ewald_split_alpha: that split.
 int launch_recip_labels(DevPimd *p, double *Frec, double *Erec, hipStream_t st, double *alpha_out);
 double ewald_split_alpha(DevPimd *p);
+// The same reciprocal terms restricted to the low shells |G| <= nlow / L_max
+// (G = m / L): forces stored in Frec, per-bead energies in Erec.
+int launch_recip_shells(DevPimd *p, int nlow, double *Frec, double *Erec, hipStream_t st);
 // Reciprocal-space forces of the step from the attached surrogate (sur_mode
 // 1) or exact with the surrogate's error against them recorded (sur_mode 2),
 // into Frec ([3][nb], accelerations / P), on st (surrogate.hip).

@@ -197,6 +197,7 @@ _SIGS = {
     "md_ewald_surrogate_normalization_x": (C.c_int, [VP, DP, DP]),
     "md_ewald_surrogate_eval_x": (C.c_int, [VP, PIMD_P, C.POINTER(C.c_float), DP, DP]),
     "md_ewald_surrogate_labels_x": (C.c_int, [VP, PIMD_P, DP, DP]),
+    "md_ewald_surrogate_set_exact_shells_x": (C.c_int, [VP, C.c_int]),
     "md_ewald_surrogate_gradient_x": (C.c_int, [VP, PIMD_P, DP, C.POINTER(C.c_float), DP]),
     "md_pimd_train_surrogate_steps_x": (C.c_int, [PIMD_P, C.POINTER(StepConfig), VP, C.c_int, C.c_double, C.c_int,
                                                   DP]),

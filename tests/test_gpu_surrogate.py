@@ -12,9 +12,13 @@ brute-force triplet sums and finite differences):
 * the exact per-particle labels: their per-slice sums are the reciprocal
   Ewald energy sum_k A_k (|S_k|^2 - N) (numpy, the reference's alpha and k
   cube), and their forces minus the finite differences of that energy;
+  with the low shells |m| <= 2 kept exact, the labels drop by exactly the
+  ball's share of that sum (energies and finite-difference forces);
 * online training on exact labels lowers the loss and the energy and force
   errors against the FP64 reciprocal term.
 UEG N=64, P=4, r_s=2 (L = 13.0 bohr), cutoff 5 bohr."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -101,15 +105,17 @@ def test_training_gradient(lib):
     assert rel < 1e-4
 
 
-def recip_energy(x, L, alpha, n):
-    """sum over the half space of the cube |m| <= n of A_k (|S_k|^2 - N),
-    A_k = exp(-pi^2 G^2 / alpha^2) / (pi V G^2), G = m / L (unit charges)."""
+def recip_energy(x, L, alpha, n, ball=False):
+    """sum over the half space of the cube |m| <= n (or the ball |m|^2 <= n^2)
+    of A_k (|S_k|^2 - N), A_k = exp(-pi^2 G^2 / alpha^2) / (pi V G^2),
+    G = m / L (unit charges)."""
     V = L ** 3
-    e = 0.0
     m = np.arange(-n, n + 1)
     mx, my, mz = np.meshgrid(m, m, m, indexing="ij")
     mx, my, mz = mx.ravel(), my.ravel(), mz.ravel()
     half = (mx > 0) | ((mx == 0) & (my > 0)) | ((mx == 0) & (my == 0) & (mz > 0))
+    if ball:
+        half &= mx * mx + my * my + mz * mz <= n * n
     k = np.stack([mx[half], my[half], mz[half]], 1) / L
     G2 = (k ** 2).sum(1)
     A = np.exp(-np.pi ** 2 * G2 / alpha ** 2) / (np.pi * V * G2)
@@ -140,6 +146,24 @@ def test_exact_labels(lib):
                     xm[b, d] -= h
                     fd = -(recip_energy(xp, s.L, alpha, 4) - recip_energy(xm, s.L, alpha, 4)) / (2 * h)
                     assert F[b, j, d] == pytest.approx(fd, rel=1e-6, abs=1e-8)
+        # low shells exact: the network's targets are the labels less the ball |m| <= 2
+        assert lib.md_ewald_surrogate_set_exact_shells_x(sur, 2) == 0
+        e2 = np.zeros(N * P)
+        F2 = np.zeros((N * P, 3))
+        assert lib.md_ewald_surrogate_labels_x(sur, s.pimd, pyxmd.dptr(e2), pyxmd.dptr(F2)) == 0
+        de = (e.reshape(-1) - e2).reshape(N, P)
+        dF = (F.reshape(-1, 3) - F2).reshape(N, P, 3)
+        for j, x in enumerate(slices(s)):
+            ref = recip_energy(x, s.L, alpha, 2, ball=True)
+            assert de[:, j].sum() == pytest.approx(ref, rel=1e-12, abs=1e-12)
+            h = 1e-5
+            for b in (5, 40):
+                for d in range(3):
+                    xp, xm = x.copy(), x.copy()
+                    xp[b, d] += h
+                    xm[b, d] -= h
+                    fd = -(recip_energy(xp, s.L, alpha, 2, True) - recip_energy(xm, s.L, alpha, 2, True)) / (2 * h)
+                    assert dF[b, j, d] == pytest.approx(fd, rel=1e-6, abs=1e-8)
     finally:
         lib.md_set_ewald_split_x(-1.0)
 
@@ -160,3 +184,37 @@ def test_online_training_reduces_error(lib):
     assert np.all(np.isfinite(rep))
     assert loss[0] < 0.1 * l0
     assert rep[8] < rep0[8]
+
+
+def test_network_gain_over_exact_low_shells(lib):
+    """With the shells |m| <= 2 exact, the network trained online on MD steps
+    of this system lowers the energy error per particle against the FP64
+    reciprocal term on later configurations (three, 10 steps apart),
+    relative to the low shells alone (whose error is minus the sum of the
+    network's targets): the mean over the slices (the systematic part) by a
+    large factor, the worst slice too.  (Energy-only training at this size
+    does not improve the forces: they are printed, not asserted.)"""
+    s = system(lib, seed=12)
+    cfg = s.step_config(0.05, use_graph=0)
+    sur = lib.md_alloc_ewald_surrogate_x(6, RC)
+    assert lib.md_ewald_surrogate_set_exact_shells_x(sur, 2) == 0
+    rep = np.zeros(2)
+    assert lib.md_pimd_train_surrogate_steps_x(s.pimd, C.byref(cfg), sur, 300, 3e-3, 8, pyxmd.dptr(rep)) == 0
+    res = []
+    for _ in range(3):
+        assert lib.md_pimd_run_steps_x(s.pimd, C.byref(cfg), 10) == 0
+        chk = np.zeros(10)
+        assert lib.md_ewald_surrogate_check_x(sur, s.pimd, pyxmd.dptr(chk)) == 0
+        assert np.all(np.isfinite(chk))
+        et = np.zeros(N * P)
+        ft = np.zeros((N * P, 3))
+        assert lib.md_ewald_surrogate_labels_x(sur, s.pimd, pyxmd.dptr(et), pyxmd.dptr(ft)) == 0
+        per_slice = -et.reshape(N, P).sum(0) / N
+        f_without = np.sqrt((ft ** 2).sum() / (np.square(chk[5:8]).sum() * N * P))
+        res.append((per_slice.mean(), chk[0], np.abs(per_slice).max(), chk[1], f_without, chk[8]))
+    r = np.array(res)
+    print(f"loss {rep[0]:.3e} -> {rep[1]:.3e}; dE/N mean over slices: low shells alone {r[:, 0].mean():.2e}, with the "
+          f"network {r[:, 1].mean():.2e}; worst slice {r[:, 2].max():.2e} -> {r[:, 3].max():.2e}; force rel RMS: "
+          f"{r[:, 4].mean():.3f} -> {r[:, 5].mean():.3f}")
+    assert abs(r[:, 1].mean()) < 0.25 * abs(r[:, 0].mean())
+    assert r[:, 3].max() < r[:, 2].max()
