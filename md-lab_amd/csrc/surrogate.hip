@@ -73,114 +73,204 @@ __device__ __forceinline__ void separation(const double *__restrict__ X, const G
 // ------------------------------------------------------------ neighbours
 
 // Partners of every bead within r_c in its own slice, in increasing index
-// order (the second force pass finds a bead's slot in its partner's list by
-// bisection).  cnt > MAXNB raises *overflow (the host reports it).
-__global__ __launch_bounds__(64) void k_sur_nlist(const double *__restrict__ X, Geo g, int *__restrict__ nbr,
-                                                  int *__restrict__ cnt, int *__restrict__ overflow) {
-  __shared__ double px[64], py[64], pz[64];
-  const int slice = blockIdx.y, lane = threadIdx.x;
-  const int i = blockIdx.x * 64 + lane;
+// order (the force pass finds a bead's slot in its partner's list by
+// bisection).  One workgroup per (slice, NLB beads): the slice's fractional
+// coordinates in LDS (FP32, wrapped to [0, 1)), one wave per bead at a time,
+// its lanes testing 64 partners per round; a ballot and the lane's rank in it
+// place the hits in order.  The test is the same FP32 arithmetic for (i, l)
+// and (l, i) (negated differences), so the lists are symmetric.  cnt >
+// MAXNB raises *overflow (the host reports it).  The ballots are kept too:
+// bm[s][w] has bit k set when bead 64 w + k is a partner of s, and pc[s][w]
+// counts the partners below 64 w, so the slot of i in l's list is
+// pc[l][i/64] + popcount(bm[l][i/64] below bit i%64) without a search.
+constexpr int NLB = 16;  // beads per neighbour-list workgroup
+__global__ __launch_bounds__(256) void k_sur_nlist(const double *__restrict__ X, Geo g, int nw, int *__restrict__ nbr,
+                                                   int *__restrict__ cnt, unsigned long long *__restrict__ bm,
+                                                   int *__restrict__ pc, int *__restrict__ overflow) {
+  extern __shared__ float sp[];  // [3][N]
+  const int slice = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int base = slice * g.N;
-  const bool act = i < g.N;
-  double xi = 0, yi = 0, zi = 0;
-  if (act) {
-    xi = X[base + i];
-    yi = X[g.nb + base + i];
-    zi = X[2 * g.nb + base + i];
+  const double iL[3] = {1.0 / g.L0, 1.0 / g.L1, 1.0 / g.L2};
+  for (int t = tid; t < 3 * g.N; t += 256) {
+    const int d = t / g.N, l = t - d * g.N;
+    const double u = X[(size_t)d * g.nb + base + l] * iL[d];
+    sp[t] = (float)(u - floor(u));
   }
-  int c = 0;
-  int *out = nbr + (size_t)(base + (act ? i : 0)) * MAXNB;
-  for (int t0 = 0; t0 < g.N; t0 += 64) {
-    __syncthreads();
-    const int l = t0 + lane;
-    if (l < g.N) {
-      px[lane] = X[base + l];
-      py[lane] = X[g.nb + base + l];
-      pz[lane] = X[2 * g.nb + base + l];
-    }
-    __syncthreads();
-    const int m = min(64, g.N - t0);
-    if (act)
-      for (int q = 0; q < m; ++q) {
-        const double dx = mimg(px[q] - xi, g.L0), dy = mimg(py[q] - yi, g.L1), dz = mimg(pz[q] - zi, g.L2);
-        if (t0 + q != i && dx * dx + dy * dy + dz * dz < g.rc2) {
-          if (c < MAXNB) out[c] = t0 + q;
-          ++c;
-        }
+  __syncthreads();
+  const float L0 = (float)g.L0, L1 = (float)g.L1, L2 = (float)g.L2, rc2 = (float)g.rc2;
+  for (int k = wv; k < NLB; k += 4) {
+    const int i = blockIdx.x * NLB + k;
+    if (i >= g.N) break;
+    const float xi = sp[i], yi = sp[g.N + i], zi = sp[2 * g.N + i];
+    int *out = nbr + (size_t)(base + i) * MAXNB;
+    int c = 0;
+    for (int t0 = 0; t0 < g.N; t0 += 64) {
+      const int l = t0 + lane;
+      bool hit = false;
+      if (l < g.N && l != i) {
+        float dx = sp[l] - xi, dy = sp[g.N + l] - yi, dz = sp[2 * g.N + l] - zi;
+        dx = (dx - rintf(dx)) * L0;
+        dy = (dy - rintf(dy)) * L1;
+        dz = (dz - rintf(dz)) * L2;
+        hit = dx * dx + dy * dy + dz * dz < rc2;
       }
-  }
-  if (act) {
-    cnt[base + i] = min(c, MAXNB);
-    if (c > MAXNB) atomicMax(overflow, c);
+      const unsigned long long m = __ballot(hit);
+      const int pos = c + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      if (hit && pos < MAXNB) out[pos] = l;
+      if (lane == 0) {
+        bm[(size_t)(base + i) * nw + (t0 >> 6)] = m;
+        pc[(size_t)(base + i) * nw + (t0 >> 6)] = c;
+      }
+      c += __popcll(m);
+    }
+    if (lane == 0) {
+      cnt[base + i] = min(c, MAXNB);
+      if (c > MAXNB) atomicMax(overflow, c);
+    }
   }
 }
 
 // ------------------------------------------------------------ descriptors
 
-__device__ __forceinline__ float cutf(const Geo &g, float r) { return 0.5f * (cosf(g.pirc * r) + 1.0f); }
-__device__ __forceinline__ float dcutf(const Geo &g, float r) { return -0.5f * g.pirc * sinf(g.pirc * r); }
+// The descriptors are FP32 quantities: the cutoff function and the basis
+// exps use the hardware approximations (__cosf, __sinf, __expf: relative
+// error ~1e-6 over the arguments met here, |x| < 30), not the correctly
+// rounded library calls, which cost several times the instructions.
+__device__ __forceinline__ float cutf(const Geo &g, float r) { return 0.5f * (__cosf(g.pirc * r) + 1.0f); }
+__device__ __forceinline__ float dcutf(const Geo &g, float r) { return -0.5f * g.pirc * __sinf(g.pirc * r); }
 
-// One thread per bead: raw descriptors D [nb][ND] and moments Mom [nb][NMOM]
-// (per channel M0, M1 x y z, M2 xx yy zz xy xz yz).
-__global__ __launch_bounds__(128) void k_sur_desc(const double *__restrict__ X, Geo g, const int *__restrict__ nbr,
+// Eight consecutive radial Gaussians e_k = exp(-eta_r (r - (f0 + k) dmu_r)^2),
+// k = 0..7.  With eta_r dmu_r^2 = 1/8 and rho = r / dmu_r, e_k = exp(-(f0 + k
+// - rho)^2 / 8) and e_{k+1} = e_k R_k, R_k = exp(-(2 (f0 + k - rho) + 1) / 8),
+// R_{k+1} = R_k e^{-1/4}: two exps and products (relative error ~1e-6; a
+// start far from r may underflow to 0, where every member is below e^-28 of
+// the peak).
+__device__ __forceinline__ void radial_gauss8(const Geo &g, float r, int f0, float *e) {
+  const float d0 = f0 - r / g.dmu_r, q = 0.77880078307f;  // e^{-1/4}
+  float v = __expf(-0.125f * d0 * d0), R = __expf(-0.125f * (2.f * d0 + 1.f));
+  e[0] = v;
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    v *= R;
+    R *= q;
+    e[k] = v;
+  }
+}
+
+// Geometry of one pair (bead s, partner slot sl): distance r, unit vector u
+// from s to the minimum image of sl, cutoff value fc (and its derivative)
+// and whether it counts (r < r_c).  In the descriptor and force kernels each
+// of a bead's SUB lanes computes the geometry of one of SUB consecutive
+// partners and the lanes then share it (pair_from), instead of every lane
+// computing every partner's.
+struct PairGeo {
+  float r, ux, uy, uz, fc, dfc;
+  int in;
+};
+__device__ __forceinline__ PairGeo pair_geo(const double *__restrict__ X, const Geo &g, int s, int sl, bool valid) {
+  PairGeo p;
+  float rx, ry, rz;
+  separation(X, g, s, sl, rx, ry, rz);
+  p.r = sqrtf(rx * rx + ry * ry + rz * rz);
+  p.in = valid && p.r < g.rc && p.r > 0.f;
+  const float ir = p.in ? 1.0f / p.r : 0.f;
+  p.ux = rx * ir;
+  p.uy = ry * ir;
+  p.uz = rz * ir;
+  p.fc = cutf(g, p.r);
+  p.dfc = dcutf(g, p.r);
+  return p;
+}
+__device__ __forceinline__ PairGeo pair_from(const PairGeo &p, int src) {
+  PairGeo q;
+  q.r = __shfl(p.r, src, 64);
+  q.ux = __shfl(p.ux, src, 64);
+  q.uy = __shfl(p.uy, src, 64);
+  q.uz = __shfl(p.uz, src, 64);
+  q.fc = __shfl(p.fc, src, 64);
+  q.dfc = __shfl(p.dfc, src, 64);
+  q.in = __shfl(p.in, src, 64);
+  return q;
+}
+
+// Sum over the SUB lanes of a bead (consecutive lanes), every lane left
+// holding the total (fixed butterfly order: deterministic).
+constexpr int SUB = 8;  // lanes per bead in the descriptor and force kernels
+// DPP steps: swap neighbours (quad_perm 1,0,3,2), swap pairs (quad_perm
+// 2,3,0,1), then lane i of each group of eight takes lane 7 - i
+// (row_half_mirror); every lane ends with the same bits (the partial sums
+// meet in the same pairings on every lane, and addition commutes exactly).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+static_assert(SUB == 8, "the DPP sum covers groups of eight lanes");
+__device__ __forceinline__ float sub_sum(float v) {
+  v += dpp_f<0xB1>(v);
+  v += dpp_f<0x4E>(v);
+  v += dpp_f<0x141>(v);
+  return v;
+}
+
+// SUB = 8 lanes per bead, each owning eight radial functions (f = 8 sub ..
+// 8 sub + 7) and one angular channel (c = sub), over all of the bead's
+// neighbours: raw descriptors D [nb][ND] and moments Mom [nb][NMOM] (per
+// channel M0, M1 x y z, M2 xx yy zz xy xz yz).
+static_assert(NR == 8 * SUB && NCH == SUB, "one lane per channel, eight radial functions per lane");
+__global__ __launch_bounds__(256) void k_sur_desc(const double *__restrict__ X, Geo g, const int *__restrict__ nbr,
                                                   const int *__restrict__ cnt, float *__restrict__ D,
                                                   float *__restrict__ Mom) {
-  const int s = blockIdx.x * 128 + threadIdx.x;
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int s = gt / SUB, sub = gt % SUB;
   if (s >= g.nb) return;
-  const int base = (s / g.N) * g.N;
-  float G[NR], m[NMOM];
+  const int base = (s / g.N) * g.N, f0 = 8 * sub;
+  const float mu_a = (sub + 0.5f) * g.dmu_a;
+  float G[8], m[10];
 #pragma unroll
-  for (int f = 0; f < NR; ++f) G[f] = 0.f;
+  for (int k = 0; k < 8; ++k) G[k] = 0.f;
 #pragma unroll
-  for (int k = 0; k < NMOM; ++k) m[k] = 0.f;
+  for (int k = 0; k < 10; ++k) m[k] = 0.f;
   const int n = cnt[s];
   const int *lst = nbr + (size_t)s * MAXNB;
-  for (int q = 0; q < n; ++q) {
-    const int sl = base + lst[q];
-    float rx, ry, rz;
-    separation(X, g, s, sl, rx, ry, rz);
-    const float r = sqrtf(rx * rx + ry * ry + rz * rz);
-    if (!(r < g.rc)) continue;
-    const float fc = cutf(g, r);
+  const int grp = (threadIdx.x & 63) & ~(SUB - 1);
+  for (int q0 = 0; q0 < n; q0 += SUB) {
+    const int qm = min(q0 + sub, n - 1);
+    const PairGeo mine = pair_geo(X, g, s, base + lst[qm], q0 + sub < n);
 #pragma unroll
-    for (int f = 0; f < NR; ++f) {
-      const float d = r - f * g.dmu_r;
-      G[f] += expf(-g.eta_r * d * d) * fc;
-    }
-    const float ir = 1.0f / r, ux = rx * ir, uy = ry * ir, uz = rz * ir;
+    for (int j = 0; j < SUB; ++j) {
+      const PairGeo p = pair_from(mine, grp + j);
+      if (!p.in) continue;
+      float e[8];
+      radial_gauss8(g, p.r, f0, e);
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      const float d = r - (c + 0.5f) * g.dmu_a;
-      const float w = expf(-g.eta_a * d * d) * fc;
-      float *mc = m + 10 * c;
-      mc[0] += w;
-      mc[1] += w * ux;
-      mc[2] += w * uy;
-      mc[3] += w * uz;
-      mc[4] += w * ux * ux;
-      mc[5] += w * uy * uy;
-      mc[6] += w * uz * uz;
-      mc[7] += w * ux * uy;
-      mc[8] += w * ux * uz;
-      mc[9] += w * uy * uz;
+      for (int k = 0; k < 8; ++k) G[k] += e[k] * p.fc;
+      const float d = p.r - mu_a;
+      const float w = __expf(-g.eta_a * d * d) * p.fc;
+      const float ux = p.ux, uy = p.uy, uz = p.uz;
+      m[0] += w;
+      m[1] += w * ux;
+      m[2] += w * uy;
+      m[3] += w * uz;
+      m[4] += w * ux * ux;
+      m[5] += w * uy * uy;
+      m[6] += w * uz * uz;
+      m[7] += w * ux * uy;
+      m[8] += w * ux * uz;
+      m[9] += w * uy * uz;
     }
   }
   float *Ds = D + (size_t)s * ND;
 #pragma unroll
-  for (int f = 0; f < NR; ++f) Ds[f] = G[f];
+  for (int k = 0; k < 8; ++k) Ds[f0 + k] = G[k];
+  const float a0 = m[0] * m[0], a1 = m[1] * m[1] + m[2] * m[2] + m[3] * m[3];
+  const float a2 = m[4] * m[4] + m[5] * m[5] + m[6] * m[6] + 2.f * (m[7] * m[7] + m[8] * m[8] + m[9] * m[9]);
+  Ds[NR + 4 * sub + 0] = a0 + a1;
+  Ds[NR + 4 * sub + 1] = a0 - a1;
+  Ds[NR + 4 * sub + 2] = a0 + 2.f * a1 + a2;
+  Ds[NR + 4 * sub + 3] = a0 - 2.f * a1 + a2;
+  float *Ms = Mom + (size_t)s * NMOM + 10 * sub;
 #pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    const float *mc = m + 10 * c;
-    const float a0 = mc[0] * mc[0], a1 = mc[1] * mc[1] + mc[2] * mc[2] + mc[3] * mc[3];
-    const float a2 = mc[4] * mc[4] + mc[5] * mc[5] + mc[6] * mc[6] + 2.f * (mc[7] * mc[7] + mc[8] * mc[8] + mc[9] * mc[9]);
-    Ds[NR + 4 * c + 0] = a0 + a1;
-    Ds[NR + 4 * c + 1] = a0 - a1;
-    Ds[NR + 4 * c + 2] = a0 + 2.f * a1 + a2;
-    Ds[NR + 4 * c + 3] = a0 - 2.f * a1 + a2;
-  }
-  float *Ms = Mom + (size_t)s * NMOM;
-#pragma unroll
-  for (int k = 0; k < NMOM; ++k) Ms[k] = m[k];
+  for (int k = 0; k < 10; ++k) Ms[k] = m[k];
 }
 
 // Round to nearest even BF16 (finite inputs).
@@ -247,12 +337,15 @@ struct MlpArgs {
 // layout A[i][k]: lane i + 16 (k / 4), B[k][j]: lane j + 16 (k / 4) (four
 // consecutive k per lane), D[i][j]: lane j + 16 (i / 4), register i % 4.
 // Layer outputs go through a per-wave LDS tile (BF16, [16][128]) to become
-// the next layer's A operand.
-constexpr int MW = 4;  // waves per workgroup
+// the next layer's A operand.  One tile per wave serves every layer: a layer
+// loads its whole A operand into registers before it writes its output over
+// the tile, and a wave's LDS accesses take effect in program order, so the
+// waves need no barriers after the weights are staged.
+constexpr int MW = 8;  // waves per workgroup (two per SIMD, sharing the staged weights)
 __global__ __launch_bounds__(64 * MW) void k_sur_mlp(MlpArgs a) {
   extern __shared__ unsigned short lds[];
   unsigned short *sw1t = lds, *sw1 = sw1t + H * ND, *sw2t = sw1 + ND * H, *sw2 = sw2t + H * H;
-  unsigned short *scr = sw2 + H * H;  // [MW][2][16][H]
+  unsigned short *scr = sw2 + H * H;  // [MW][16][H]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int t = tid; t < H * ND / 4; t += 64 * MW) {
     ((uint2 *)sw1t)[t] = ((const uint2 *)a.w1t)[t];
@@ -263,7 +356,7 @@ __global__ __launch_bounds__(64 * MW) void k_sur_mlp(MlpArgs a) {
     ((uint2 *)sw2)[t] = ((const uint2 *)a.w2)[t];
   }
   __syncthreads();
-  unsigned short *s0 = scr + wv * 2 * 16 * H, *s1 = s0 + 16 * H;
+  unsigned short *s0 = scr + wv * 16 * H;
   const int ri = lane & 15, kq = 4 * (lane >> 4);  // A/B operand row/column and k offset
   const int ntile = (a.nb + 15) / 16;
   const int rounds = (ntile + gridDim.x * MW - 1) / (gridDim.x * MW);
@@ -294,7 +387,6 @@ __global__ __launch_bounds__(64 * MW) void k_sur_mlp(MlpArgs a) {
         s0[(kq + v) * H + j] = to_bf16(h);
       }
     }
-    __syncthreads();
     // layer 2 and the output
     float y[4] = {0.f, 0.f, 0.f, 0.f};
     {
@@ -314,7 +406,7 @@ __global__ __launch_bounds__(64 * MW) void k_sur_mlp(MlpArgs a) {
           dh2[ct][v] = 1.f - h * h;
           y[v] += w3 * h;
           // delta2 = w3 (1 - h2^2), the next A operand
-          s1[(kq + v) * H + j] = to_bf16(w3 * dh2[ct][v]);
+          s0[(kq + v) * H + j] = to_bf16(w3 * dh2[ct][v]);
         }
       }
     }
@@ -328,12 +420,11 @@ __global__ __launch_bounds__(64 * MW) void k_sur_mlp(MlpArgs a) {
         const int bead = b0 + kq + v;
         if (bead < a.nb) a.eps[bead] = (y[v] + a.b3[0]) * a.ysd + a.ymu;
       }
-    __syncthreads();
     // delta1 = (delta2 W2^T) (1 - h1^2)
     {
       v4s da[H / 16];
 #pragma unroll
-      for (int ks = 0; ks < H / 16; ++ks) da[ks] = lds4(s1 + ri * H + 16 * ks + kq);
+      for (int ks = 0; ks < H / 16; ++ks) da[ks] = lds4(s0 + ri * H + 16 * ks + kq);
 #pragma unroll
       for (int ct = 0; ct < 8; ++ct) {
         v4f c = {0.f, 0.f, 0.f, 0.f};
@@ -344,7 +435,6 @@ __global__ __launch_bounds__(64 * MW) void k_sur_mlp(MlpArgs a) {
         for (int v = 0; v < 4; ++v) s0[(kq + v) * H + j] = to_bf16(c[v] * dh1[ct][v]);
       }
     }
-    __syncthreads();
     // dy/dDn = delta1 W1^T, then back to raw descriptors and Hartree
     {
       v4s da[H / 16];
@@ -365,132 +455,131 @@ __global__ __launch_bounds__(64 * MW) void k_sur_mlp(MlpArgs a) {
           }
       }
     }
-    __syncthreads();
   }
 }
 
-constexpr size_t MLP_LDS = (size_t)(2 * H * ND + 2 * H * H + MW * 2 * 16 * H) * sizeof(unsigned short);
+constexpr size_t MLP_LDS = (size_t)(2 * H * ND + 2 * H * H + MW * 16 * H) * sizeof(unsigned short);
+static_assert(MLP_LDS <= 160 * 1024, "staged weights and per-wave tiles fit the LDS");
 
 // ------------------------------------------------------------ forces
 
-// Pass 1, one thread per bead i: zeta = de_i / d(moments) from de_i / dD_i,
-// then for every neighbour l the gradient of e_i with respect to the
-// separation r = x_l - x_i,
+// Pass 1, SUB = 8 lanes per bead i (lane sub: radial functions 8 sub ..
+// 8 sub + 7 and angular channel sub, as in the descriptors): zeta = de_i /
+// d(moments) from de_i / dD_i, then for every neighbour l the gradient of e_i
+// with respect to the separation r = x_l - x_i,
 //   grad = (sum_f gam_f g_f'(r) + sum_c w_c'(r) (z0 + z1.u + u.Z2.u)) u
 //          + sum_c w_c(r) [(z1 - (z1.u) u) + 2 (Z2 u - (u.Z2.u) u)] / r,
-// added to bead i's force (F_i = -de_i/dx_i = +grad) and stored as the
-// reaction on l (F_l -= grad) in V[i][slot].
-__global__ __launch_bounds__(128) void k_sur_force1(const double *__restrict__ X, Geo g, const int *__restrict__ nbr,
-                                                    const int *__restrict__ cnt, const float *__restrict__ dD,
+// each lane's share added across the bead's lanes; added to bead i's force
+// (F_i = -de_i/dx_i = +grad) and stored as the reaction on l at i's slot in
+// l's list (from the list bitmaps of k_sur_nlist), V[l][slot], so that each
+// bead's reactions lie contiguous for the second pass.
+__global__ __launch_bounds__(256) void k_sur_force1(const double *__restrict__ X, Geo g, int nw,
+                                                    const int *__restrict__ nbr, const int *__restrict__ cnt,
+                                                    const unsigned long long *__restrict__ bm,
+                                                    const int *__restrict__ pc, const float *__restrict__ dD,
                                                     const float *__restrict__ Mom, double *__restrict__ Fown,
                                                     float *__restrict__ V) {
-  const int s = blockIdx.x * 128 + threadIdx.x;
-  if (s >= g.nb) return;
-  const int base = (s / g.N) * g.N;
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int s0 = gt / SUB, sub = gt % SUB;
+  // lanes past the last bead follow bead nb-1 (their loads stay in bounds and
+  // the shuffles see full groups) but store nothing
+  const bool act = s0 < g.nb;
+  const int s = act ? s0 : g.nb - 1;
+  const int base = (s / g.N) * g.N, me = s - base, f0 = 8 * sub;
+  const float mu_a = (sub + 0.5f) * g.dmu_a;
   const float *dd = dD + (size_t)s * ND;
-  const float *mm = Mom + (size_t)s * NMOM;
-  float gam[NR];
+  float gam[8];
 #pragma unroll
-  for (int f = 0; f < NR; ++f) gam[f] = dd[f];
-  float z[NMOM];
-#pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    const float d0 = dd[NR + 4 * c], d1 = dd[NR + 4 * c + 1], d2 = dd[NR + 4 * c + 2], d3 = dd[NR + 4 * c + 3];
-    const float *mc = mm + 10 * c;
-    float *zc = z + 10 * c;
-    zc[0] = 2.f * mc[0] * (d0 + d1 + d2 + d3);
+  for (int k = 0; k < 8; ++k) gam[k] = dd[f0 + k];
+  float z[10];
+  {
+    const float d0 = dd[NR + 4 * sub], d1 = dd[NR + 4 * sub + 1], d2 = dd[NR + 4 * sub + 2], d3 = dd[NR + 4 * sub + 3];
+    const float *mc = Mom + (size_t)s * NMOM + 10 * sub;
+    z[0] = 2.f * mc[0] * (d0 + d1 + d2 + d3);
     const float k1 = 2.f * (d0 - d1 + 2.f * d2 - 2.f * d3);
 #pragma unroll
-    for (int k = 1; k < 4; ++k) zc[k] = k1 * mc[k];
+    for (int k = 1; k < 4; ++k) z[k] = k1 * mc[k];
     const float k2 = 2.f * (d2 + d3);
 #pragma unroll
-    for (int k = 4; k < 10; ++k) zc[k] = k2 * mc[k];
+    for (int k = 4; k < 10; ++k) z[k] = k2 * mc[k];
   }
   double fx = 0, fy = 0, fz = 0;
   const int n = cnt[s];
   const int *lst = nbr + (size_t)s * MAXNB;
-  float *Vs = V + (size_t)s * MAXNB * 3;
-  for (int q = 0; q < n; ++q) {
-    const int sl = base + lst[q];
-    float rx, ry, rz;
-    separation(X, g, s, sl, rx, ry, rz);
-    const float r = sqrtf(rx * rx + ry * ry + rz * rz);
-    float gx = 0.f, gy = 0.f, gz = 0.f;
-    if (r < g.rc) {
-      const float fc = cutf(g, r), dfc = dcutf(g, r);
-      float rad = 0.f;
+  const int grp = (threadIdx.x & 63) & ~(SUB - 1);
+  for (int q0 = 0; q0 < n; q0 += SUB) {
+    const int qm = min(q0 + sub, n - 1);
+    const int slm = base + lst[qm];
+    const PairGeo mine = pair_geo(X, g, s, slm, q0 + sub < n);
+    const size_t wi = (size_t)slm * nw + (me >> 6);
+    const int slot = pc[wi] + __popcll(bm[wi] & ((1ull << (me & 63)) - 1ull));
 #pragma unroll
-      for (int f = 0; f < NR; ++f) {
-        const float d = r - f * g.dmu_r;
-        const float e = expf(-g.eta_r * d * d);
-        rad += gam[f] * e * (dfc - 2.f * g.eta_r * d * fc);
-      }
-      const float ir = 1.0f / r, ux = rx * ir, uy = ry * ir, uz = rz * ir;
-      float px = 0.f, py = 0.f, pz = 0.f;
+    for (int j = 0; j < SUB; ++j) {
+      if (q0 + j >= n) break;
+      const PairGeo p = pair_from(mine, grp + j);
+      float gx = 0.f, gy = 0.f, gz = 0.f;
+      if (p.in) {
+        const float r = p.r, fc = p.fc, dfc = p.dfc, ux = p.ux, uy = p.uy, uz = p.uz, ir = 1.0f / r;
+        float e[8];
+        radial_gauss8(g, r, f0, e);
+        float rad = 0.f;
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const float d = r - (c + 0.5f) * g.dmu_a;
-        const float e = expf(-g.eta_a * d * d);
-        const float w = e * fc, dw = e * (dfc - 2.f * g.eta_a * d * fc);
-        const float *zc = z + 10 * c;
-        const float t1 = zc[1] * ux + zc[2] * uy + zc[3] * uz;
+        for (int k = 0; k < 8; ++k) {
+          const float d = r - (f0 + k) * g.dmu_r;
+          rad += gam[k] * e[k] * (dfc - 2.f * g.eta_r * d * fc);
+        }
+        const float d = r - mu_a;
+        const float ea = __expf(-g.eta_a * d * d);
+        const float w = ea * fc, dw = ea * (dfc - 2.f * g.eta_a * d * fc);
+        const float t1 = z[1] * ux + z[2] * uy + z[3] * uz;
         // Z2 u with the full symmetric tensor (xx yy zz xy xz yz)
-        const float qx = zc[4] * ux + zc[7] * uy + zc[8] * uz;
-        const float qy = zc[7] * ux + zc[5] * uy + zc[9] * uz;
-        const float qz = zc[8] * ux + zc[9] * uy + zc[6] * uz;
+        const float qx = z[4] * ux + z[7] * uy + z[8] * uz;
+        const float qy = z[7] * ux + z[5] * uy + z[9] * uz;
+        const float qz = z[8] * ux + z[9] * uy + z[6] * uz;
         const float t2 = qx * ux + qy * uy + qz * uz;
-        rad += dw * (zc[0] + t1 + t2);
+        rad += dw * (z[0] + t1 + t2);
         const float wr = w * ir;
-        px += wr * ((zc[1] - t1 * ux) + 2.f * (qx - t2 * ux));
-        py += wr * ((zc[2] - t1 * uy) + 2.f * (qy - t2 * uy));
-        pz += wr * ((zc[3] - t1 * uz) + 2.f * (qz - t2 * uz));
+        gx = rad * ux + wr * ((z[1] - t1 * ux) + 2.f * (qx - t2 * ux));
+        gy = rad * uy + wr * ((z[2] - t1 * uy) + 2.f * (qy - t2 * uy));
+        gz = rad * uz + wr * ((z[3] - t1 * uz) + 2.f * (qz - t2 * uz));
       }
-      gx = rad * ux + px;
-      gy = rad * uy + py;
-      gz = rad * uz + pz;
+      gx = sub_sum(gx);
+      gy = sub_sum(gy);
+      gz = sub_sum(gz);
+      fx += gx;
+      fy += gy;
+      fz += gz;
+      // lane j owns pair q0 + j: its reaction goes to the partner's slot
+      if (act && sub == j && slot < MAXNB) {
+        float *v = V + ((size_t)slm * MAXNB + slot) * 3;
+        v[0] = gx;
+        v[1] = gy;
+        v[2] = gz;
+      }
     }
-    fx += gx;
-    fy += gy;
-    fz += gz;
-    Vs[3 * q] = gx;
-    Vs[3 * q + 1] = gy;
-    Vs[3 * q + 2] = gz;
   }
-  Fown[s] = fx;
-  Fown[g.nb + s] = fy;
-  Fown[2 * g.nb + s] = fz;
+  if (act && sub == 0) {
+    Fown[s] = fx;
+    Fown[g.nb + s] = fy;
+    Fown[2 * g.nb + s] = fz;
+  }
 }
 
-// Pass 2, one thread per bead l: F_l = own part - sum over its neighbours i
-// of the reaction stored in i's list at l's slot (found by bisection), as an
-// acceleration of the slice energy's 1/P share: out[d][s] = F / (m P), or
-// added to out when add is set.
-__global__ __launch_bounds__(128) void k_sur_force2(Geo g, const int *__restrict__ nbr, const int *__restrict__ cnt,
-                                                    const double *__restrict__ Fown, const float *__restrict__ V,
-                                                    const double *__restrict__ M, double Pd, int add,
-                                                    double *__restrict__ out) {
+// Pass 2, one thread per bead l: F_l = own part - the reactions left in its
+// slots (in slot order), as an acceleration of the slice energy's 1/P share:
+// out[d][s] = F / (m P), or added to out when add is set.
+__global__ __launch_bounds__(128) void k_sur_force2(Geo g, const int *__restrict__ cnt, const double *__restrict__ Fown,
+                                                    const float *__restrict__ V, const double *__restrict__ M,
+                                                    double Pd, int add, double *__restrict__ out) {
   const int s = blockIdx.x * 128 + threadIdx.x;
   if (s >= g.nb) return;
-  const int base = (s / g.N) * g.N, me = s - base;
   double fx = Fown[s], fy = Fown[g.nb + s], fz = Fown[2 * g.nb + s];
   const int n = cnt[s];
-  const int *lst = nbr + (size_t)s * MAXNB;
+  const float *v = V + (size_t)s * MAXNB * 3;
   for (int q = 0; q < n; ++q) {
-    const int si = base + lst[q];
-    const int *li = nbr + (size_t)si * MAXNB;
-    int lo = 0, hi = cnt[si] - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (li[mid] < me)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    if (li[lo] != me) continue;  // not in its list (cannot happen for a symmetric cutoff)
-    const float *v = V + ((size_t)si * MAXNB + lo) * 3;
-    fx -= v[0];
-    fy -= v[1];
-    fz -= v[2];
+    fx -= v[3 * q];
+    fy -= v[3 * q + 1];
+    fz -= v[3 * q + 2];
   }
   const double w = 1.0 / (M[s] * Pd);
   if (add) {
@@ -783,7 +872,9 @@ struct Surrogate {
   int adam_t = 0;
   // per-size buffers (beads, slices)
   int cap = 0, cap_sl = 0, cap_gb = 0;
-  int *nbr = nullptr, *cnt = nullptr, *ovf = nullptr;
+  size_t cap_bm = 0;  // list bitmap words (nb * ceil(N / 64))
+  int *nbr = nullptr, *cnt = nullptr, *ovf = nullptr, *pc = nullptr;
+  unsigned long long *bm = nullptr;
   float *D = nullptr, *Mom = nullptr, *Dn = nullptr, *dD = nullptr, *eps = nullptr, *V = nullptr;
   unsigned short *Db = nullptr;
   double *Fown = nullptr, *Fs = nullptr, *Fx = nullptr, *Ex = nullptr, *El = nullptr;
@@ -803,7 +894,8 @@ static Geo sur_geo(const Surrogate *S, DevPimd *p) {
   g.rc2 = S->rc * S->rc;
   g.pirc = (float)(M_PI / S->rc);
   g.dmu_r = (float)(S->rc / NR);
-  g.eta_r = 1.0f / (2.0f * (2.0f * g.dmu_r) * (2.0f * g.dmu_r));  // Gaussian width two spacings
+  // Gaussian width two spacings: eta_r dmu_r^2 = 1/8 (radial_gauss relies on it)
+  g.eta_r = 1.0f / (2.0f * (2.0f * g.dmu_r) * (2.0f * g.dmu_r));
   g.dmu_a = (float)(S->rc / NCH);
   g.eta_a = 1.0f / (2.0f * g.dmu_a * g.dmu_a);
   g.N = p->N;
@@ -825,12 +917,22 @@ static int sur_ensure(Surrogate *S, DevPimd *p) {
     fprintf(stderr, "xmd: surrogate cutoff %.3g exceeds half the box\n", S->rc);
     return -1;
   }
+  if (p->N > 4096) {  // the neighbour-list kernel holds a slice's coordinates in LDS
+    fprintf(stderr, "xmd: the surrogate supports N <= 4096 particles per slice (got %d)\n", p->N);
+    return -1;
+  }
+  const size_t nwb = (size_t)nb * ((p->N + 63) / 64);
+  if (nwb > S->cap_bm) {
+    if (dalloc(S->bm, nwb) || dalloc(S->pc, nwb)) return -1;
+    S->cap_bm = nwb;
+  }
   if (nb > S->cap) {
     if (dalloc(S->nbr, (size_t)nb * MAXNB) || dalloc(S->cnt, nb) || dalloc(S->D, (size_t)nb * ND) ||
         dalloc(S->Mom, (size_t)nb * NMOM) || dalloc(S->Dn, (size_t)nb * ND) || dalloc(S->Db, (size_t)nb * ND) ||
         dalloc(S->dD, (size_t)nb * ND) || dalloc(S->eps, nb) || dalloc(S->V, (size_t)nb * MAXNB * 3) ||
         dalloc(S->Fown, 3 * (size_t)nb) || dalloc(S->Fs, 3 * (size_t)nb) || dalloc(S->Fx, 3 * (size_t)nb) ||
-        dalloc(S->Ex, nb) || dalloc(S->El, nb) || dalloc(S->H1, (size_t)nb * H) || dalloc(S->H2, (size_t)nb * H) || dalloc(S->Y, nb) ||
+        dalloc(S->Ex, nb) || dalloc(S->El, nb) || dalloc(S->H1, (size_t)nb * H) || dalloc(S->H2, (size_t)nb * H) ||
+        dalloc(S->Y, nb) ||
         dalloc(S->DY, nb) || dalloc(S->D1, (size_t)nb * H) || dalloc(S->D2, (size_t)nb * H) || dalloc(S->T, nb) ||
         dalloc(S->lpart, (nb + TB - 1) / TB))
       return -1;
@@ -893,8 +995,9 @@ Surrogate *sur_create(unsigned seed, double rc) {
 static int sur_features(Surrogate *S, DevPimd *p, hipStream_t st) {
   const Geo g = sur_geo(S, p);
   hipLaunchKernelGGL(k_sur_zero, dim3(1), dim3(1), 0, st, S->ovf);  // (a kernel: this may be captured)
-  hipLaunchKernelGGL(k_sur_nlist, dim3((p->N + 63) / 64, p->P), dim3(64), 0, st, p->ds->X, g, S->nbr, S->cnt, S->ovf);
-  hipLaunchKernelGGL(k_sur_desc, dim3((g.nb + 127) / 128), dim3(128), 0, st, p->ds->X, g, S->nbr, S->cnt, S->D,
+  hipLaunchKernelGGL(k_sur_nlist, dim3((p->N + NLB - 1) / NLB, p->P), dim3(256), 3 * p->N * sizeof(float), st,
+                     p->ds->X, g, (p->N + 63) / 64, S->nbr, S->cnt, S->bm, S->pc, S->ovf);
+  hipLaunchKernelGGL(k_sur_desc, dim3((g.nb * SUB + 255) / 256), dim3(256), 0, st, p->ds->X, g, S->nbr, S->cnt, S->D,
                      S->Mom);
   const int n = g.nb * ND;
   hipLaunchKernelGGL(k_sur_norm, dim3((n + 255) / 256), dim3(256), 0, st, S->D, S->mean, S->istd, n, S->Dn, S->Db);
@@ -925,10 +1028,10 @@ static int sur_infer(Surrogate *S, DevPimd *p, double *out, hipStream_t st, int 
   const int ntile = (g.nb + 15) / 16;
   const int grid = std::max(1, std::min(rt().num_cus, (ntile + MW - 1) / MW));
   hipLaunchKernelGGL(k_sur_mlp, dim3(grid), dim3(64 * MW), MLP_LDS, st, a);
-  hipLaunchKernelGGL(k_sur_force1, dim3((g.nb + 127) / 128), dim3(128), 0, st, p->ds->X, g, S->nbr, S->cnt, S->dD,
-                     S->Mom, S->Fown, S->V);
-  hipLaunchKernelGGL(k_sur_force2, dim3((g.nb + 127) / 128), dim3(128), 0, st, g, S->nbr, S->cnt, S->Fown, S->V,
-                     p->ds->M, (double)p->P, add, out);
+  hipLaunchKernelGGL(k_sur_force1, dim3((g.nb * SUB + 255) / 256), dim3(256), 0, st, p->ds->X, g, (p->N + 63) / 64,
+                     S->nbr, S->cnt, S->bm, S->pc, S->dD, S->Mom, S->Fown, S->V);
+  hipLaunchKernelGGL(k_sur_force2, dim3((g.nb + 127) / 128), dim3(128), 0, st, g, S->cnt, S->Fown, S->V, p->ds->M,
+                     (double)p->P, add, out);
   XMD_CHECK(hipGetLastError());
   return 0;
 }
