@@ -33,6 +33,17 @@ int md_hip_synchronize_x(void);
  * invalid and the next status check reports -1. */
 int md_pimd_inject_recursion_timeouts_x(md_pimd_t_x *pimd, int count);
 
+/* Test hook for the exact path of the exchange recursion: each 64-position
+ * block is solved in the frame of its single-particle chain, where every
+ * scaled value is >= 1, and accepted only while those values (and the
+ * look-ahead sums into the next block) stay at or below `limit` (default
+ * 1e250, the frame's safe range); a block or row beyond it is solved or
+ * folded again exactly.  A limit below 1 sends every block through the exact
+ * path; moderate limits mix the two.  Applies to recursions launched or
+ * graphs captured afterwards, for every object; limit <= 0 restores the
+ * default. */
+int md_set_recursion_frame_limit_x(double limit);
+
 /* Cutoff x_cut of both Ewald series in units of the split alpha: real space
  * within x_cut / alpha, k space within pi |G| / alpha <= x_cut (default 5.25:
  * omitted terms below erfc(5.25) = 1.4e-13 and exp(-5.25^2) = 1.1e-12 of the

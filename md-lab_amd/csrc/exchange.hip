@@ -257,14 +257,17 @@ struct RecArgs {
   double *Lf;         // forward output, N+1
   double *Rb;         // backward output, N+1
   double *VBN;        // V_B^(n) = -L(n)/beta
-  double *val2;       // [2][N+1] solved values per direction, position order
-  double *accM, *accS;  // [2][N+1] off-block log-sum-exp accumulators
-  double *dg;         // [2][nblk][RB(RB-1)/2] in-block coefficients
-  double *la;         // [2][nblk][RB][RB] coefficients of the block on the previous block
-  int *sync;          // solved[2][nblk], folded[2][nblk], error flag
+  double *val2;       // [2][N+1] solved values per direction, position order (hand-off: NaN until solved)
+  double *accM, *accS;  // [2][N+1] accumulators of column 0 (log-sum-exp pairs)
+  double *finM, *finS;  // [2][N+1] row blocks' accumulators over blocks 0..G-2 (hand-off: NaN until folded)
+  double *dg;         // [2][nblk][RB(RB-1)/2] in-block coefficients, chain frame, exponentiated
+  double *la;         // [2][nblk][RB][RB] the block's rows on the previous block, likewise
+  double *ech;        // [2][nblk][RB+1] chain offsets E(0..RB-1) and the step Delta into the next block
+  int *sync;          // [4 nblk] unused, error word, injected-timeout count
   int N, nblk;
   double beta, lnxi;
   int xi0;            // xi == 0: only k = 1 terms
+  double big;         // upper end of the chain frame's accepted range
 };
 
 // Position p = 0..N along the solve order; p' < p are solved earlier.
@@ -300,13 +303,14 @@ __device__ __forceinline__ double rec_coef_direct(const RecArgs &a, const EnkArg
 // ---- blocked right-looking solve (positions in blocks of RB rows) ----
 //
 // Direction dir (0 forward, 1 backward) keeps, per position p, the solved
-// value val[p] and a running log-sum-exp (accM[p], accS[p]) of the terms
-// coef(p, p') + val[p'] over solved positions p' outside p's own block.
+// value val[p] and a running log-sum-exp (M, S) of the terms coef(p, p') +
+// val[p'] over solved positions p' outside p's own block.
 //   k_rec_prep   (one launch; in the fused step its work is part of
-//                k_enk_prep) initial accumulators from position 0, and the
-//                coefficients of every diagonal block and of every block on
-//                its predecessor, so the serial kernels read them with
-//                independent, coalesced loads;
+//                k_enk_prep) accumulators from position 0, the hand-off
+//                buffers' NaN marks, and the chain-frame coefficients of
+//                every diagonal block and of every block on its predecessor,
+//                so the serial kernels read them with independent, coalesced
+//                loads;
 //   k_rec_persist (one launch) per direction a solver workgroup that solves
 //                block b and folds it into block b+1's rows (look-ahead),
 //                and one updater workgroup per later row block that folds
@@ -343,67 +347,141 @@ __device__ __forceinline__ void tri_decode(int w, int &q, int &r) {
 }
 
 // DIRECT: coefficients from positions and prefix sums (rec_coef_direct, e
-// used) instead of the cycle-energy tables, so this may run beside the
+// used) instead of the cycle-energy tables, so the set-up may run beside the
 // kernel that fills them.
-template <bool FWD, bool DIRECT = false>
-__device__ __forceinline__ void rec_prep_one(const RecArgs &a, int dir, int t, const EnkArgs &e = EnkArgs{}) {
-  auto coef = [&](int p, int pp) { return DIRECT ? rec_coef_direct<FWD>(a, e, p, pp) : rec_coef<FWD>(a, p, pp); };
-  auto coef_col = [&](int p, int pp) {
-    return DIRECT ? rec_coef_direct<FWD>(a, e, p, pp) : rec_coef_col<FWD>(a, p, pp);
-  };
-  const int N = a.N;
-  if (t <= N) {
-    double *val = a.val2 + (size_t)dir * (N + 1);
-    double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
-    if (t == 0) {
-      val[0] = 0.0;
-      if (FWD) {
-        a.Lf[0] = 0.0;
-        a.VBN[0] = 0.0;
-      } else {
-        a.Rb[N] = 0.0;
-      }
-    } else {
-      const double c = coef(t, 0);
-      AM[t] = c > NEG ? c : NEG;
-      AS[t] = c > NEG ? 1.0 : 0.0;
-    }
+template <bool FWD, bool DIRECT>
+struct RecCoef {
+  const RecArgs &a;
+  const EnkArgs &e;
+  __device__ double operator()(int p, int pp) const {
+    return DIRECT ? rec_coef_direct<FWD>(a, e, p, pp) : rec_coef<FWD>(a, p, pp);
   }
-  const int ndg = a.nblk * RTRI;
-  if (t < ndg) {
-    const int b = t / RTRI, w = t - b * RTRI;
+  __device__ double col(int p, int pp) const {
+    return DIRECT ? rec_coef_direct<FWD>(a, e, p, pp) : rec_coef_col<FWD>(a, p, pp);
+  }
+};
+
+// Value of position 0 and the accumulators of column 0 (t = position).
+template <bool FWD, bool DIRECT>
+__device__ __forceinline__ void rec_prep_acc(const RecArgs &a, int dir, int t, const EnkArgs &e) {
+  const RecCoef<FWD, DIRECT> coef{a, e};
+  const int N = a.N;
+  if (t > N) return;
+  double *val = a.val2 + (size_t)dir * (N + 1);
+  double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  a.finM[(size_t)dir * (N + 1) + t] = __builtin_nan("");
+  a.finS[(size_t)dir * (N + 1) + t] = __builtin_nan("");
+  if (t == 0) {
+    val[0] = 0.0;
+    if (FWD) {
+      a.Lf[0] = 0.0;
+      a.VBN[0] = 0.0;
+    } else {
+      a.Rb[N] = 0.0;
+    }
+  } else {
+    val[t] = __builtin_nan("");
+    const double c = coef(t, 0);
+    AM[t] = c > NEG ? c : NEG;
+    AS[t] = c > NEG ? 1.0 : 0.0;
+  }
+}
+
+// Chain offsets of row block b into E[0 .. RB], lane i of the calling wave
+// computing E[i]: E(0) = 0, E(i) = E(i-1) + coef(p0+i, p0+i-1) over the
+// block's rows (constant past them), and E[RB] = Delta = E(63) +
+// coef(p0+64, p0+63), the k = 1 step into the next block (0 for the last
+// block).  Every workgroup that needs block b's offsets computes them with
+// this code, in the same order, so they agree bit for bit.
+template <bool FWD, bool DIRECT>
+__device__ __forceinline__ void rec_chain_offsets(const RecArgs &a, int b, const EnkArgs &e, double *E) {
+  const RecCoef<FWD, DIRECT> coef{a, e};
+  const int lane = threadIdx.x & 63;
+  const int p0 = 1 + b * RB, nrows = min(RB, a.N - p0 + 1);
+  double d = (lane >= 1 && lane < nrows) ? coef(p0 + lane, p0 + lane - 1) : 0.0;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_up(d, o, 64);
+    if (lane >= o) d += t;
+  }
+  E[lane] = d;
+  if (lane == 63) E[RB] = b + 1 < a.nblk ? d + coef(p0 + RB, p0 + RB - 1) : 0.0;
+}
+
+// Coefficients of row block b in the frame of its k = 1 chain, one part of
+// REC_PREP_PARTS (every workgroup of the block computes the offsets, part 0
+// stores them):
+//   in-block    A(q, r)  = exp(c(p0+q, p0+r) + E(r) - E(q)),             r < q
+//   look-ahead  F(q, r)  = exp(c(p0+q, p0-RB+r) + E'(r) - Delta' - E(q))
+// with E' and Delta' those of block b-1 (F: block b's rows on block b-1's
+// columns, stored [r][q]).  Terms without a coefficient are 0; an exponent
+// past the FP64 range gives inf, which the solver's range vote catches.
+constexpr int REC_PREP_PARTS = 4;
+
+template <bool FWD, bool DIRECT>
+__device__ __forceinline__ void rec_prep_block(const RecArgs &a, int dir, int b, int part, const EnkArgs &e,
+                                               double (*E)[RB + 1]) {
+  const RecCoef<FWD, DIRECT> coef{a, e};
+  const int tid = threadIdx.x, wv = tid >> 6;
+  if (wv == 0) rec_chain_offsets<FWD, DIRECT>(a, b, e, E[0]);
+  if (wv == 1 && b > 0) rec_chain_offsets<FWD, DIRECT>(a, b - 1, e, E[1]);
+  __syncthreads();
+  const int p0 = 1 + b * RB, nrows = min(RB, a.N - p0 + 1);
+  if (part == 0 && tid <= RB) a.ech[((size_t)dir * a.nblk + b) * (RB + 1) + tid] = E[0][tid];
+  double *dg = a.dg + ((size_t)dir * a.nblk + b) * RTRI;
+  for (int w = part * 256 + tid; w < RTRI; w += REC_PREP_PARTS * 256) {
     int q, r;
     tri_decode(w, q, r);
-    const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
-    a.dg[(size_t)dir * ndg + t] = q < nrows ? coef(p0 + q, p0 + r) : NEG;
+    const double c = q < nrows ? coef(p0 + q, p0 + r) : NEG;
+    dg[w] = c > NEG ? exp(c + E[0][r] - E[0][q]) : 0.0;
   }
-  const int nla = a.nblk * RB * RB;
-  if (t < nla) {  // transposed: [b][r][q]
-    const int b = t / (RB * RB), w = t - b * (RB * RB), r = w / RB, q = w - r * RB;
-    const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
-    a.la[(size_t)dir * nla + t] = (b > 0 && q < nrows) ? coef_col(p0 + q, p0 - RB + r) : NEG;
+  double *la = a.la + ((size_t)dir * a.nblk + b) * (RB * RB);
+  for (int w = part * 256 + tid; w < RB * RB; w += REC_PREP_PARTS * 256) {
+    const int r = w / RB, q = w - r * RB;
+    const double c = (b > 0 && q < nrows) ? coef.col(p0 + q, p0 - RB + r) : NEG;
+    la[w] = c > NEG ? exp(c + E[1][r] - E[1][RB] - E[0][q]) : 0.0;
   }
 }
 
-// The per-solve reset clears the hand-off flags only: the error word at
-// sync[4 nblk] keeps a timeout of any earlier solve until the host has read
-// it (recursion_status clears it there).
-__device__ __forceinline__ void rec_prep_body(const RecArgs &a, int dir, int first, int t) {
-  if (first && t < 4 * a.nblk) a.sync[t] = 0;
-  if (dir == 0)
-    rec_prep_one<true>(a, 0, t);
-  else
-    rec_prep_one<false>(a, 1, t);
+// Workgroups of one direction's set-up: [0, rec_prep_acc_wgs) the values
+// and accumulators (thread = position), then REC_PREP_PARTS per row block
+// for the coefficients.  The error word at sync[4 nblk] keeps a timeout of
+// any earlier solve until the host has read it (recursion_status clears it
+// there).
+__host__ __device__ inline int rec_prep_acc_wgs(int N) { return (N + 1 + 255) / 256; }
+__host__ __device__ inline int rec_prep_wgs(int N, int nblk) { return rec_prep_acc_wgs(N) + nblk * REC_PREP_PARTS; }
+
+template <bool DIRECT>
+__device__ __forceinline__ void rec_prep_wg(const RecArgs &a, int dir, int wg, const EnkArgs &e,
+                                            double (*E)[RB + 1]) {
+  const int nacc = rec_prep_acc_wgs(a.N);
+  if (wg < nacc) {
+    const int t = wg * 256 + threadIdx.x;
+    if (dir == 0)
+      rec_prep_acc<true, DIRECT>(a, 0, t, e);
+    else
+      rec_prep_acc<false, DIRECT>(a, 1, t, e);
+  } else {
+    const int w = wg - nacc;
+    if (dir == 0)
+      rec_prep_block<true, DIRECT>(a, 0, w / REC_PREP_PARTS, w % REC_PREP_PARTS, e, E);
+    else
+      rec_prep_block<false, DIRECT>(a, 1, w / REC_PREP_PARTS, w % REC_PREP_PARTS, e, E);
+  }
 }
 
-__global__ void k_rec_prep(RecArgs a, int dir0) {
-  rec_prep_body(a, dir0 + blockIdx.y, blockIdx.y == 0, blockIdx.x * blockDim.x + threadIdx.x);
+// grid (rec_prep_wgs, directions)
+__global__ __launch_bounds__(256) void k_rec_prep(RecArgs a, int dir0) {
+  __shared__ double E[2][RB + 1];
+  rec_prep_wg<false>(a, dir0 + blockIdx.y, blockIdx.x, EnkArgs{}, E);
 }
 
-// k_enk_fill (workgroups 0 .. N-1) and k_rec_prep of both directions (the
-// next 2 * nprepb workgroups, coefficients computed directly) in one launch.
+// k_enk_fill (workgroups 0 .. N-1) and the recursion set-up of both
+// directions (the next 2 nprepb workgroups, coefficients computed directly)
+// in one launch.
 __global__ __launch_bounds__(256) void k_enk_prep(EnkArgs e, double *__restrict__ ENk, double *__restrict__ ET,
                                                   RecArgs a, int nprepb) {
+  __shared__ double E[2][RB + 1];
   chain_priority();
   const int bx = blockIdx.x;
   if (bx < e.N) {
@@ -411,12 +489,7 @@ __global__ __launch_bounds__(256) void k_enk_prep(EnkArgs e, double *__restrict_
     return;
   }
   const int w = bx - e.N, dir = w / nprepb;
-  const int t = (w - dir * nprepb) * blockDim.x + threadIdx.x;
-  if (dir == 0 && t < 4 * a.nblk) a.sync[t] = 0;  // hand-off flags only, as in rec_prep_body
-  if (dir == 0)
-    rec_prep_one<true, true>(a, 0, t, e);
-  else
-    rec_prep_one<false, true>(a, 1, t, e);
+  rec_prep_wg<true>(a, dir, w - dir * nprepb, e, E);
 }
 
 // (M, S) <- log-sum-exp merge with (m, s).
@@ -448,96 +521,111 @@ __device__ __forceinline__ void lse16(const double *t, double &M, double &S) {
 // the look-ahead fold) and one updater workgroup per row block G >= 2 (folds
 // blocks 0 .. G-2 into its 64 rows as soon as each is solved).  All of the
 // launch's workgroups are resident from the start, so no block waits for a
-// new dispatch while other kernels fill the device.  Hand-off through
-// device-scope release/acquire flags:
-//   solved[dir][b] = 1       block b's values are in val[]
-//   folded[dir][G] = G - 1   row block G's accumulators hold blocks 0..G-2
-// Every wait is bounded; a timeout raises the error flag (checked by the
-// host at synchronisation points) and lets all waits fall through, so the
-// grid always drains.
+// new dispatch while other kernels fill the device.  The hand-offs carry
+// their own readiness: the set-up marks val[] and fin[MS][] with NaN, the
+// solver writes block b's values into val[], an updater its rows' finished
+// accumulators into fin[MS][], both with device-scope relaxed stores (to the
+// coherence point, past the writer's L2), and the reader polls the values
+// themselves with device-scope loads until none is NaN.  No flag, so no
+// cache-wide write-back or invalidate per hand-off.  Every poll is bounded;
+// a timeout raises the error word (checked by the host at synchronisation
+// points) and lets all polls fall through, so the grid always drains.
 constexpr int REC_SPIN_MAX = 1 << 22;
 
-// The poll itself is relaxed; the acquire fence follows it once per wait
-// (in every wave, after the barrier), so the caches are invalidated once per
-// hand-off instead of once per poll.
-__device__ __forceinline__ void rec_wait(int *flag, int target, int *err) {
-  if (threadIdx.x == 0) {
-    int it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (++it > REC_SPIN_MAX) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
+__device__ __forceinline__ void st_handoff(double *p, double v) {
+  __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double ld_handoff(const double *p) {
+  return __longlong_as_double(
+      (long long)__hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Lanes [0, n) of the calling wave read p[lane] (and q[lane] if q) until no
+// value is NaN; at most `spin` rounds (wave-uniform; spin < 0 gives up at
+// once: the injected timeout of the test hook), then the error word is
+// raised and the values are whatever was read last.
+__device__ __forceinline__ void rec_poll(const double *p, const double *q, int n, int *err, int spin, double &pv,
+                                         double &qv) {
+  const int lane = threadIdx.x & 63;
+  for (int it = 0;; ++it) {
+    pv = lane < n ? ld_handoff(p + lane) : 0.0;
+    qv = (q && lane < n) ? ld_handoff(q + lane) : 0.0;
+    if (spin >= 0 && !__any(pv != pv || qv != qv)) return;
+    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+      return;
+    if (it >= spin) {
+      if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
     }
+    __builtin_amdgcn_s_sleep(2);
   }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// Every wave publishes its own stores (release fence: write-back, no
-// invalidate), then thread 0 raises the flag.
-__device__ __forceinline__ void rec_post(int *flag, int value) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Folds solved blocks 0 .. G-2 into the accumulators of row block G (lane =
-// row, the four waves split each block's 64 columns), in block order.  WAIT:
-// another workgroup solves the blocks (wait for each); otherwise this
-// workgroup solved them itself.
-template <bool FWD, bool WAIT>
-__device__ __forceinline__ void rec_fold_rows(const RecArgs &a, int dir, int G, double (*pm)[RB], double (*ps)[RB]) {
+// Folds blocks 0 .. G-2 into row block G's accumulators (lane = row, the
+// four waves split each block's 64 columns, 16 each), in block order,
+// starting from column 0's (M, S); wave 0 returns the result.  Each wave
+// polls its 16 columns' values itself; the next block's coefficients are in
+// flight while a block is folded; the four partials of a block merge in
+// wave 0 through double-buffered LDS (one barrier per block).
+template <bool FWD>
+__device__ __forceinline__ void rec_fold_rows(const RecArgs &a, int dir, int G, double (*pm)[4][RB],
+                                              double (*ps)[4][RB], double &Ma, double &Sa) {
   const int N = a.N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  int *solved = a.sync + dir * a.nblk, *err = a.sync + 4 * a.nblk;
+  int *err = a.sync + 4 * a.nblk;
   const double *val = a.val2 + (size_t)dir * (N + 1);
-  double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
   const int p = 1 + G * RB + lane;
   const bool act = p <= N;
-  double Ma = NEG, Sa = 0.0;
+  Ma = NEG;
+  Sa = 0.0;
   if (wv == 0 && act) {
     Ma = AM[p];
     Sa = AS[p];
   }
+  double c[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) c[j] = act ? rec_coef_col<FWD>(a, p, 1 + 16 * wv + j) : NEG;
   for (int b = 0; b <= G - 2; ++b) {
     const int pprev = 1 + b * RB;
-    // the coefficients do not depend on block b's values: their loads are
-    // in flight while this workgroup waits for the block
-    double c[16];
+    double v, unused;
+    rec_poll(val + pprev + 16 * wv, nullptr, 16, err, REC_SPIN_MAX, v, unused);
+    double cn[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) c[j] = act ? rec_coef_col<FWD>(a, p, pprev + 16 * wv + j) : NEG;
-    if (WAIT) rec_wait(solved + b, 1, err);
+    for (int j = 0; j < 16; ++j) cn[j] = (act && b + 1 <= G - 2) ? rec_coef_col<FWD>(a, p, pprev + RB + 16 * wv + j) : NEG;
     double t[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const int pp = pprev + 16 * wv + j;
-      t[j] = c[j] > NEG ? c[j] + val[pp] : NEG;
+      const double vj = readlane_d(v, j);
+      t[j] = c[j] > NEG ? c[j] + vj : NEG;
     }
     double M, S;
     lse16(t, M, S);
-    pm[wv][lane] = M;
-    ps[wv][lane] = S;
+    const int buf = b & 1;
+    pm[buf][wv][lane] = M;
+    ps[buf][wv][lane] = S;
     __syncthreads();
     if (wv == 0) {
 #pragma unroll
-      for (int w = 0; w < 4; ++w) lse_pair(Ma, Sa, pm[w][lane], ps[w][lane]);
+      for (int w = 0; w < 4; ++w) lse_pair(Ma, Sa, pm[buf][w][lane], ps[buf][w][lane]);
     }
-    __syncthreads();
-  }
-  if (wv == 0 && act) {
-    AM[p] = Ma;
-    AS[p] = Sa;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) c[j] = cn[j];
   }
 }
 
 template <bool FWD>
-__device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[RB], double (*ps)[RB]) {
-  int *folded = a.sync + (2 + dir) * a.nblk;
-  rec_fold_rows<FWD, true>(a, dir, G, pm, ps);
-  rec_post(folded + G, G - 1);
+__device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[4][RB],
+                                            double (*ps)[4][RB]) {
+  double Ma, Sa;
+  rec_fold_rows<FWD>(a, dir, G, pm, ps, Ma, Sa);
+  const int p = 1 + G * RB + (threadIdx.x & 63);
+  if (threadIdx.x < 64 && p <= a.N) {
+    st_handoff(a.finM + (size_t)dir * (a.N + 1) + p, Ma);
+    st_handoff(a.finS + (size_t)dir * (a.N + 1) + p, Sa);
+  }
 }
 
 // (q, r) of the in-block coefficient w = q (q - 1) / 2 + r, packed q << 8 | r.
@@ -552,50 +640,93 @@ struct TriTable {
 __constant__ TriTable kTri = TriTable();
 
 struct RecLds {
-  double dg[2][RTRI];           // in-block coefficients, then scaled exps (ping-pong)
-  double mo[RB], so[RB];        // row accumulators (max, scaled sum)
-  double sg[RB];                // per-row shifts
-  double wl[RB];                // scaled values W of the block solved last
-  double tmax[3][RB], tacc[3][RB];  // look-ahead partials of waves 1..3 (shift, scaled sum)
-  double pm[4][RB], ps[4][RB];  // updater partials
-  double fm[4][RB], fs[4][RB];  // partials of a self-fold (SOLO)
+  double dg[2][RTRI];           // in-block coefficients (ping-pong)
+  double sg[RB];                // per-row shifts of an exact block solve
+  double wl[RB];                // W of the block solved last
+  double lv[RB];                // its values L
+  double tacc[4][RB];           // look-ahead partial sums of the four waves
+  double nm[RB], ns[RB];        // next block's accumulators, loaded by wave 1
+  double pm[2][4][RB], ps[2][4][RB];  // fold partials (updater, or a self-fold in SOLO)
 };
 
-// Wave-level hand-off wait (lane 0 polls, the whole wave then acquires),
-// bounded like rec_wait by `spin` polls (lane 0's value); spin < 0 gives up
-// before the first poll (the injected timeout of the test hook below).
-__device__ __forceinline__ void rec_wait_wave(int *flag, int target, int *err, int spin = REC_SPIN_MAX) {
-  if ((threadIdx.x & 63) == 0) {
-    int it = 0;
-    while (spin < 0 || __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (++it > spin) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-  }
+// LDS written by some lanes of a wave and read by others of the same wave:
+// the fence keeps the compiler from forwarding or reordering across it.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// The solver walks the blocks in order; wave 0 carries the serial chains.
-// Per block b (its rows' accumulators over all earlier blocks already merged
-// into mo, so):
-//   issue the loads of block b+1's coefficients (in-block: all waves; on
-//   block b: waves 1..3, columns r = wave - 1 mod 3);
-//   wave 0: the (max,+) scan for the per-row shifts sigma;
-//   all: the scaled in-block exps;
-//   wave 0: the linear substitution, values of block b; waves 1..3
-//   meanwhile: tau_q = max_r c(q, r) + sigma_r over their columns and
-//   f(q, r) = exp(c(q, r) + sigma_r - tau_q) in registers for block b+1's
-//   rows q;
-//   publish block b; waves 1..3: the look-ahead sums sum_r f(q, r) W_r;
-//   wave 0 meanwhile: wait for block b+1's updater, load its accumulators;
-//   wave 0: merge the three (tau + ln sum) into them.
-// W_r = exp(L_r - sigma_r) is the substitution's own unknown, so the
-// look-ahead is the log-sum-exp of c(q, r) + L_r without a log/exp round trip.
+// Upper end of the chain frame's range (RecArgs::big, lowered only by a
+// test hook): W up to e^575, so no sum of non-negative terms overflows, and
+// a term that underflows is below e^-133 of its row's W (>= 1).
+constexpr double REC_BIG = 1e250;
+
+// Exact solve of row block b by wave 0 alone, without the chain frame:
+// coefficients from the tables, per-row shifts sigma_q = max(M_q, max_{r<q}
+// c(q,r) + sigma_r) by a (max,+) scan (no transcendental on its chain), the
+// scaled exps a(q,r) = exp(c(q,r) + sigma_r - sigma_q) and the substitution
+// W_q = b_q + sum_{r<q} a(q,r) W_r, W_q = exp(L(q) - sigma_q) (every term
+// obeys c(q,r) + L(r) <= L(q) <= sigma_q + (rows) ln N).  Returns L of the
+// lane's row (lane = row, accumulators (M, Sm)).
+template <bool FWD>
+__device__ __forceinline__ double rec_block_exact(const RecArgs &a, int p0, int nrows, double M, double Sm,
+                                                  double *dg, double *sg) {
+  const int lane = threadIdx.x & 63, rowq = lane * (lane - 1) / 2;
+  for (int w = lane; w < RTRI; w += 64) {
+    const int q = kTri.v[w] >> 8, r = kTri.v[w] & 255;
+    dg[w] = q < nrows ? rec_coef<FWD>(a, p0 + q, p0 + r) : NEG;
+  }
+  wave_lds_sync();
+  // row q's coefficients start at rowq; the reads past them stay inside dg
+  // (rowq + RB - 2 < RTRI) and are masked
+  double c[RB - 1];
+#pragma unroll
+  for (int r = 0; r < RB - 1; ++r) {
+    const double x = dg[rowq + r];
+    c[r] = r < lane ? x : NEG;
+  }
+  double sig = lane < nrows ? M : NEG;
+#pragma unroll
+  for (int r = 0; r < RB - 1; ++r) sig = fmax(sig, c[r] + readlane_d(sig, r));
+  sg[lane] = lane < nrows ? sig : NEG;
+  wave_lds_sync();
+  for (int w = lane; w < RTRI; w += 64) {
+    const int q = kTri.v[w] >> 8, r = kTri.v[w] & 255;
+    const double x = dg[w];
+    dg[w] = (q < nrows && x > NEG) ? exp(x + sg[r] - sg[q]) : 0.0;
+  }
+  wave_lds_sync();
+  double W = 0.0;
+  if (lane < nrows && M > NEG && M > sig - 745.0) W = Sm * exp(M - sig);
+#pragma unroll
+  for (int r = 0; r < RB - 1; ++r) {
+    const double x = dg[rowq + r];
+    W = fma(r < lane ? x : 0.0, readlane_d(W, r), W);
+  }
+  return sig + log(W);
+}
+
+// The solver walks the blocks in order.  Block b is solved in the frame of
+// its k = 1 chain: row q's unknown is W_q = exp(L(q) - sigma - E(q)), with
+// sigma = L(p0) (the block's first row has no in-block predecessor, so its
+// accumulator is its value) and E the chain offsets of the set-up.  Since
+// L(q) >= L(q-1) + c(q, q-1), L(q) >= sigma + E(q): every W_q >= 1.  The
+// frame's coefficients A and look-ahead weights F come exponentiated from the
+// set-up, so per block:
+//   all waves: issue the loads of block b+1's coefficients;
+//   wave 0: right-hand sides W_q = S_q exp(M_q - sigma - E(q)) (one exp),
+//           forward substitution W_q += sum_{r<q} A(q, r) W_r (one FMA per
+//           step), L(q) = sigma + E(q) + ln W_q, publish the block;
+//   wave 1 meanwhile: poll block b+1's accumulators from its updater;
+//   all waves: look-ahead sums Y(q') = sum_r F(q', r) W_r over a quarter of
+//           the columns each (block b+1's row q' = lane);
+//   wave 0: merge sigma + Delta + E'(q') + ln Y(q') into block b+1's rows
+//           (E', Delta: block b+1's offsets and the step into it, so this is
+//           the log-sum-exp of c(q', r) + L(r) over block b).
+// A block whose W leaves [0, REC_BIG] (a row more than ~575 above its
+// chain, or an overflowed coefficient) fails a wave vote; wave 0 then solves
+// it with rec_block_exact and folds it into block b+1 from its values, as it
+// also does for a row whose look-ahead sum left the range.
 // SOLO: no updater workgroups; the solver folds blocks 0 .. b-1 into block
 // b+1 itself (same order and arithmetic as an updater), so it waits on no
 // other workgroup.
@@ -603,113 +734,81 @@ template <bool FWD, bool SOLO = false>
 __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S) {
   const int N = a.N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  int *solved = a.sync + dir * a.nblk, *folded = a.sync + (2 + dir) * a.nblk, *err = a.sync + 4 * a.nblk;
+  int *err = a.sync + 4 * a.nblk;
   // Test hook: while sync[4 nblk + 1] (set by the host, never by a normal
-  // run) is positive, a forward solve takes one from it and its first
-  // hand-off wait gives up at once, as a timed-out wait would.
+  // run) is positive, a forward solve takes one from it and its first poll
+  // of an updater (wave 1's) gives up at once, as a timed-out poll would.
   int spin = REC_SPIN_MAX;
-  if (!SOLO && FWD && tid == 0) {
+  if (!SOLO && FWD && tid == 64) {
     int *inject = a.sync + 4 * a.nblk + 1;
     if (__hip_atomic_load(inject, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0 &&
         __hip_atomic_fetch_add(inject, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0)
       spin = -1;
   }
+  if (wv == 1) spin = __builtin_amdgcn_readfirstlane(spin);
   double *val = a.val2 + (size_t)dir * (N + 1);
   const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
+  const double *FM = a.finM + (size_t)dir * (N + 1), *FS = a.finS + (size_t)dir * (N + 1);
   const double *DGd = a.dg + (size_t)dir * a.nblk * RTRI;
   const double *LAd = a.la + (size_t)dir * a.nblk * (RB * RB);
+  const double *ECd = a.ech + (size_t)dir * a.nblk * (RB + 1);
   constexpr int DPT = (RTRI + 255) / 256;  // in-block coefficients per thread
-  constexpr int LPW = (RB + 2) / 3;        // look-ahead coefficients per lane of waves 1..3
-  // block 0: coefficients and accumulators (column 0 only)
+  constexpr int LPW = RB / 4;              // look-ahead columns per lane
+  // block 0: coefficients, accumulators (column 0 only), chain offsets
   for (int w = tid; w < RTRI; w += 256) S.dg[0][w] = DGd[w];
+  double M = NEG, Sm = 0.0, E = 0.0, Dl = 0.0;
   if (wv == 0) {
     const int nr0 = min(RB, N);
-    S.mo[lane] = lane < nr0 ? AM[1 + lane] : NEG;
-    S.so[lane] = lane < nr0 ? AS[1 + lane] : 0.0;
+    M = lane < nr0 ? AM[1 + lane] : NEG;
+    Sm = lane < nr0 ? AS[1 + lane] : 0.0;
+    E = ECd[lane];
+    Dl = ECd[RB];
   }
   const int rowq = lane * (lane - 1) / 2;
-  // (q, r) of this thread's in-block coefficients, the same for every block
-  int tq[DPT], tr[DPT];
-#pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    const int w = tid + 256 * k;
-    const int qr = w < RTRI ? kTri.v[w] : 0;
-    tq[k] = qr >> 8;
-    tr[k] = qr & 255;
-  }
   for (int b = 0; b < a.nblk; ++b) {
     const int cur = b & 1;
     const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
     const bool more = b + 1 < a.nblk;
     const int p1 = p0 + RB, nrows1 = more ? min(RB, N - p1 + 1) : 0;
     __syncthreads();
-    // loads of block b+1's coefficients, in flight during the scan
-    double dgr[DPT], lar[LPW];
+    // block b+1's coefficients, in flight during this block's solve
+    double dgr[DPT], lar[LPW], En = 0.0, Dn = 0.0;
     if (more) {
 #pragma unroll
       for (int k = 0; k < DPT; ++k) {
         const int w = tid + 256 * k;
-        dgr[k] = w < RTRI ? DGd[(size_t)(b + 1) * RTRI + w] : NEG;
+        dgr[k] = w < RTRI ? DGd[(size_t)(b + 1) * RTRI + w] : 0.0;
       }
-      if (wv > 0) {
 #pragma unroll
-        for (int k = 0; k < LPW; ++k) {
-          const int r = wv - 1 + 3 * k;  // [r][q] layout: row q = lane of block b+1 on column r of block b
-          lar[k] = r < RB ? LAd[(size_t)(b + 1) * (RB * RB) + r * RB + lane] : NEG;
-        }
+      for (int k = 0; k < LPW; ++k) lar[k] = LAd[(size_t)(b + 1) * (RB * RB) + (wv + 4 * k) * RB + lane];
+      if (wv == 0) {
+        En = ECd[(size_t)(b + 1) * (RB + 1) + lane];
+        Dn = ECd[(size_t)(b + 1) * (RB + 1) + RB];
       }
     }
     double *dg = S.dg[cur];
-    // per-row shift sigma_q = max(M_q, max_{r<q} c(q,r) + sigma_r): a (max,+)
-    // scan with no transcendental on its serial chain; each lane holds its
-    // row of c in registers.  Every term obeys c(q,r) + L(r) <= L(q) <=
-    // sigma_q + (rows) ln N, so the scaled values below stay in FP64 range.
+    double sigma = 0.0;
+    bool fast = true;
     if (wv == 0) {
-      // row q's coefficients c(q, 0 .. q-1) start at rowq; the reads past
-      // them stay inside dg (rowq + RB - 2 < RTRI) and are masked
-      double c[RB - 1];
-#pragma unroll
-      for (int r = 0; r < RB - 1; ++r) {
-        const double x = dg[rowq + r];
-        c[r] = r < lane ? x : NEG;
-      }
-      double sig = lane < nrows ? S.mo[lane] : NEG;
-#pragma unroll
-      for (int r = 0; r < RB - 1; ++r) sig = fmax(sig, c[r] + readlane_d(sig, r));
-      S.sg[lane] = lane < nrows ? sig : NEG;
-    }
-    __syncthreads();
-    // scaled coefficients a(q,r) = exp(c(q,r) + sigma_r - sigma_q), evenly
-    // spread over the workgroup
-#pragma unroll
-    for (int k = 0; k < DPT; ++k) {
-      const int w = tid + 256 * k;
-      if (w < RTRI) {
-        const int q = tq[k], r = tr[k];
-        const double c = dg[w];
-        dg[w] = (q < nrows && c > NEG) ? exp(c + S.sg[r] - S.sg[q]) : 0.0;
-      }
-    }
-    __syncthreads();
-    if (wv == 0) {
-      // linear forward substitution W_q = b_q + sum_{r<q} a(q,r) W_r, with
-      // W_q = exp(L(q) - sigma_q): one FMA per step (a(q,r) = 0 for r >= q)
+      sigma = readlane_d(M, 0) + log(readlane_d(Sm, 0));
+      double W = lane < nrows ? Sm * exp(M - sigma - E) : 0.0;
+      // row q's coefficients start at rowq; the reads past them stay inside
+      // dg (rowq + RB - 2 < RTRI) and are masked
       double av[RB - 1];
 #pragma unroll
       for (int r = 0; r < RB - 1; ++r) {
         const double x = dg[rowq + r];
         av[r] = r < lane ? x : 0.0;
       }
-      double W = 0.0;
-      if (lane < nrows && S.mo[lane] > NEG && S.mo[lane] > S.sg[lane] - 745.0)
-        W = S.so[lane] * exp(S.mo[lane] - S.sg[lane]);
 #pragma unroll
       for (int r = 0; r < RB - 1; ++r) W = fma(av[r], readlane_d(W, r), W);
+      fast = __all(W <= a.big);  // NaN fails too
+      const double v = fast ? sigma + E + log(W) : rec_block_exact<FWD>(a, p0, nrows, M, Sm, dg, S.sg);
       S.wl[lane] = W;
-      const double v = S.sg[lane] + log(W);
+      S.lv[lane] = v;
       if (lane < nrows) {
         const int p = p0 + lane;
-        val[p] = v;
+        st_handoff(val + p, v);  // the updaters poll these
         if (FWD) {
           a.Lf[p] = v;
           a.VBN[p] = -v / a.beta;
@@ -717,85 +816,75 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
           a.Rb[N - p] = v;
         }
       }
-    } else if (more) {
-      // look-ahead weights of block b+1's row q = lane over this wave's
-      // columns r: f = exp(c(q, r) + sigma_r - tau), tau their maximum
-      double tau = NEG;
-#pragma unroll
-      for (int k = 0; k < LPW; ++k) {
-        const int r = wv - 1 + 3 * k;
-        if (r < RB && lar[k] > NEG) tau = fmax(tau, lar[k] + S.sg[r]);
+    } else if (wv == 1 && !SOLO && more) {
+      // block b+1's accumulators: column 0 (set-up) for block 1, blocks
+      // 0 .. b-1 from its updater otherwise
+      double m = NEG, sv = 0.0;
+      if (b + 1 >= 2) {
+        rec_poll(FM + p1, FS + p1, nrows1, err, spin, m, sv);
+        spin = REC_SPIN_MAX;
+      } else if (lane < nrows1) {
+        m = AM[p1 + lane];
+        sv = AS[p1 + lane];
       }
-#pragma unroll
-      for (int k = 0; k < LPW; ++k) {
-        const int r = wv - 1 + 3 * k;
-        lar[k] = (r < RB && lar[k] > NEG && tau > NEG) ? exp(lar[k] + S.sg[r] - tau) : 0.0;
-      }
-      S.tmax[wv - 1][lane] = tau;
+      S.nm[lane] = lane < nrows1 ? m : NEG;
+      S.ns[lane] = lane < nrows1 ? sv : 0.0;
     }
-    // wave 0 wrote every global result of the block: it publishes them
-    // itself (updaters only need these values), then waits for block b+1's
-    // updater and issues the loads of its accumulators, while waves 1..3
-    // finish their look-ahead weights
-    double M = NEG, Sm = 0.0;
-    if (wv == 0) {
-      if (!SOLO) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) __hip_atomic_store(solved + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (more && b + 1 >= 2) {
-          rec_wait_wave(folded + b + 1, b, err, spin);
-          spin = REC_SPIN_MAX;
-        }
-      }
-      if (!SOLO && lane < nrows1) {
-        M = AM[p1 + lane];
-        Sm = AS[p1 + lane];
-      }
-    }
-    if (more) {
-#pragma unroll
-      for (int k = 0; k < DPT; ++k) {
-        const int w = tid + 256 * k;
-        if (w < RTRI) S.dg[cur ^ 1][w] = dgr[k];
-      }
-    }
-    if (!more) break;
     __syncthreads();
-    // waves 1..3: the look-ahead sums over their columns
-    if (wv > 0) {
+    if (!more) break;
+    {
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < LPW; ++k) {
-        const int r = wv - 1 + 3 * k;
-        if (r < RB) acc = fma(lar[k], S.wl[r], acc);
-      }
-      S.tacc[wv - 1][lane] = acc;
+      for (int k = 0; k < LPW; ++k) acc = fma(lar[k], S.wl[wv + 4 * k], acc);
+      S.tacc[wv][lane] = acc;
     }
-    if (SOLO) {
-      // block b+1's accumulators over blocks 0 .. b-1, folded here
-      if (b + 1 >= 2) rec_fold_rows<FWD, false>(a, dir, b + 1, S.fm, S.fs);
-      if (wv == 0 && lane < nrows1) {
-        M = AM[p1 + lane];
-        Sm = AS[p1 + lane];
-      }
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+      const int w = tid + 256 * k;
+      if (w < RTRI) S.dg[cur ^ 1][w] = dgr[k];
     }
+    // SOLO: block b+1's accumulators over blocks 0 .. b-1, folded here
+    double Mf = NEG, Sf = 0.0;
+    if (SOLO && b + 1 >= 2) rec_fold_rows<FWD>(a, dir, b + 1, S.pm, S.ps, Mf, Sf);
     __syncthreads();
     if (wv == 0) {
+      double Mn, Sn;
+      if (SOLO) {
+        Mn = b + 1 >= 2 ? Mf : (lane < nrows1 ? AM[p1 + lane] : NEG);
+        Sn = b + 1 >= 2 ? Sf : (lane < nrows1 ? AS[p1 + lane] : 0.0);
+      } else {
+        Mn = S.nm[lane];
+        Sn = S.ns[lane];
+      }
+      const double Y = (S.tacc[0][lane] + S.tacc[1][lane]) + (S.tacc[2][lane] + S.tacc[3][lane]);
       if (lane < nrows1) {
-        // wave w's look-ahead sum is exp(tmax_w) tacc_w with tacc_w >= 1
-        // (its largest weight is 1 and W >= 1); their total in log form
-        const double t = fmax(fmax(S.tmax[0][lane], S.tmax[1][lane]), S.tmax[2][lane]);
-        if (t > NEG) {
-          double sum = 0.0;
-#pragma unroll
-          for (int w = 0; w < 3; ++w)
-            if (S.tmax[w][lane] > NEG) sum += S.tacc[w][lane] * exp(S.tmax[w][lane] - t);
-          if (sum > 0.0) lse_pair(M, Sm, t + log(sum), 1.0);
+        if (fast && Y <= a.big) {
+          if (Y > 0.0) lse_pair(Mn, Sn, sigma + Dl + En + log(Y), 1.0);
+        } else {
+          // block b folded into row q' from its values (block b is full)
+          const int p = p1 + lane;
+          double m = NEG;
+#pragma unroll 4
+          for (int r = 0; r < RB; ++r) {
+            const double c = rec_coef_col<FWD>(a, p, p0 + r);
+            if (c > NEG) m = fmax(m, c + S.lv[r]);
+          }
+          if (m > NEG) {
+            double s = 0.0;
+#pragma unroll 4
+            for (int r = 0; r < RB; ++r) {
+              const double c = rec_coef_col<FWD>(a, p, p0 + r);
+              const double t = c > NEG ? c + S.lv[r] : NEG;
+              if (t > m - LSE_SKIP) s += exp(t - m);
+            }
+            lse_pair(Mn, Sn, m, s);
+          }
         }
       }
-      S.mo[lane] = M;
-      S.so[lane] = Sm;
+      M = Mn;
+      Sm = Sn;
+      E = En;
+      Dl = Dn;
     }
   }
 }
@@ -1199,10 +1288,11 @@ __global__ __launch_bounds__(256) void kb_enk_fill(const ExRep *__restrict__ rep
 __global__ __launch_bounds__(256) void kb_rec_prep(const ExRep *__restrict__ reps, const int *__restrict__ start,
                                                    int nrep) {
   const int r = batch_rep(start, nrep, blockIdx.x);
+  __shared__ double E[2][RB + 1];
   const ExRep &R = reps[r];
   const int lb = blockIdx.x - start[r];
   const int dir = lb >= R.prep_blocks ? 1 : 0, bx = lb - dir * R.prep_blocks;
-  rec_prep_body(R.a, dir, dir == 0, bx * 256 + threadIdx.x);
+  rec_prep_wg<false>(R.a, dir, bx, EnkArgs{}, E);
 }
 
 // one workgroup per (replica, direction), each solving its whole recursion
@@ -1238,6 +1328,10 @@ static Geo geo_of(DevPimd *p, int image) {
   return g;
 }
 
+static double g_rec_frame_limit = REC_BIG;
+
+void set_recursion_frame_limit(double limit) { g_rec_frame_limit = limit > 0.0 ? limit : REC_BIG; }
+
 static RecArgs rec_args(DevPimd *p) {
   RecArgs a;
   a.ENk = p->ENk;
@@ -1249,9 +1343,13 @@ static RecArgs rec_args(DevPimd *p) {
   a.val2 = p->rec;
   a.accM = p->rec + 2 * (p->N + 1);
   a.accS = p->rec + 4 * (p->N + 1);
+  a.finM = p->rec + 6 * (p->N + 1);
+  a.finS = p->rec + 8 * (p->N + 1);
   a.nblk = (p->N + RB - 1) / RB;
-  a.dg = p->rec + 6 * (p->N + 1);
+  a.dg = p->rec + 10 * (p->N + 1);
   a.la = a.dg + 2 * (size_t)a.nblk * RTRI;
+  a.ech = a.la + 2 * (size_t)a.nblk * (RB * RB);
+  a.big = g_rec_frame_limit;
   a.sync = p->recsync;
   a.N = p->N;
   a.beta = p->host->beta;
@@ -1352,7 +1450,7 @@ int launch_enk_prep(DevPimd *p, int image, hipStream_t st) {
                      p->ENk, p->ET, p->Cpre, p->done);
   XMD_CHECK(hipGetLastError());
   RecArgs a = rec_args(p);
-  const int nprepb = (std::max(N + 1, a.nblk * RB * RB) + 255) / 256;
+  const int nprepb = rec_prep_wgs(N, a.nblk);
   hipLaunchKernelGGL(k_enk_prep, dim3(N + 2 * nprepb), dim3(256), 0, st, e, p->ENk, p->ET, a, nprepb);
   XMD_CHECK(hipGetLastError());
   p->enk_dev = true;
@@ -1379,8 +1477,7 @@ int launch_recursion(DevPimd *p, bool fwd, bool bwd, hipStream_t st, hipEvent_t 
   const int dir0 = fwd ? 0 : 1;
   const int N = p->N;
   RecArgs a = rec_args(p);
-  const int nprep = std::max(N + 1, a.nblk * RB * RB);
-  hipLaunchKernelGGL(k_rec_prep, dim3((nprep + 255) / 256, ndir), dim3(256), 0, st, a, dir0);
+  hipLaunchKernelGGL(k_rec_prep, dim3(rec_prep_wgs(N, a.nblk), ndir), dim3(256), 0, st, a, dir0);
   XMD_CHECK(hipGetLastError());
   if (mid) XMD_CHECK(hipEventRecord(mid, st));
   const int per = 1 + std::max(0, a.nblk - 2);
@@ -1432,7 +1529,7 @@ constexpr size_t NA_LNN = NA_ET + (size_t)NA_NMAX * (NA_NMAX + 1) / 2;
 constexpr size_t NA_LF = NA_LNN + (NA_NMAX + 1);
 constexpr size_t NA_RB = NA_LF + (NA_NMAX + 1);
 constexpr size_t NA_REC = NA_RB + (NA_NMAX + 1);
-constexpr size_t NA_SYNC = NA_REC + 6 * (size_t)(NA_NMAX + 1) + 2 * (size_t)NA_NBLK * (RTRI + RB * RB);
+constexpr size_t NA_SYNC = NA_REC + 10 * (size_t)(NA_NMAX + 1) + 2 * (size_t)NA_NBLK * (RTRI + RB * RB + RB + 1);
 constexpr int NA_NSYNC = 4 * NA_NBLK + 2;  // flags of N_max, error word, injected-timeout count (unused here)
 constexpr size_t NA_DOUBLES = NA_SYNC + (NA_NSYNC + 1) / 2;
 static_assert(NA_DOUBLES * sizeof(double) < 6u * 1024 * 1024, "scratch stays below 6 MiB");
@@ -1486,8 +1583,12 @@ static int noalloc_fill_vb(int N, double beta, double vi, const double *ENk, dou
   a.val2 = base + NA_REC;
   a.accM = a.val2 + 2 * (N + 1);
   a.accS = a.val2 + 4 * (N + 1);
-  a.dg = a.val2 + 6 * (N + 1);
+  a.finM = a.val2 + 6 * (N + 1);
+  a.finS = a.val2 + 8 * (N + 1);
+  a.dg = a.val2 + 10 * (N + 1);
   a.la = a.dg + 2 * (size_t)nblk * RTRI;
+  a.ech = a.la + 2 * (size_t)nblk * (RB * RB);
+  a.big = g_rec_frame_limit;
   a.sync = (int *)(base + NA_SYNC) + 4 * (NA_NBLK - nblk);  // error word at the fixed slot 4 NA_NBLK
   a.N = N;
   a.nblk = nblk;
@@ -1498,8 +1599,7 @@ static int noalloc_fill_vb(int N, double beta, double vi, const double *ENk, dou
   hipLaunchKernelGGL(k_noalloc_lnn, dim3((N + 256) / 256), dim3(256), 0, st, base + NA_LNN, N);
   hipLaunchKernelGGL(k_enk_transpose, dim3(N), dim3(256), 0, st, ENk, base + NA_ET, N);
   const int ndir = Vt ? 2 : 1;
-  const int nprep = std::max(N + 1, nblk * RB * RB);
-  hipLaunchKernelGGL(k_rec_prep, dim3((nprep + 255) / 256, ndir), dim3(256), 0, st, a, 0);
+  hipLaunchKernelGGL(k_rec_prep, dim3(rec_prep_wgs(N, nblk), ndir), dim3(256), 0, st, a, 0);
   const int per = 1 + std::max(0, nblk - 2);
   hipLaunchKernelGGL(k_rec_persist, dim3(ndir * per), dim3(256), 0, st, a, 0);
   if (Vt) hipLaunchKernelGGL(k_noalloc_vt, dim3((N + 256) / 256), dim3(256), 0, st, base + NA_RB, Vt, N, beta);
@@ -1554,8 +1654,7 @@ ExchBatch *exch_batch_create(const std::vector<BatchItem> &items) {
     R.ENk = p->ENk;
     R.ET = p->ET;
     R.a = rec_args(p);
-    const int nprep = std::max(p->N + 1, R.a.nblk * RB * RB);
-    R.prep_blocks = (nprep + 255) / 256;
+    R.prep_blocks = rec_prep_wgs(p->N, R.a.nblk);
     reps.push_back(R);
     bc.push_back((p->N + 63) / 64);
     be.push_back(p->N);

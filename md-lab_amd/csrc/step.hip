@@ -292,6 +292,11 @@ int md_pimd_inject_recursion_timeouts_x(md_pimd_t_x *pimd, int count) {
   return inject_recursion_timeouts(pimd_state(pimd, false), count);
 }
 
+int md_set_recursion_frame_limit_x(double limit) {
+  set_recursion_frame_limit(limit);
+  return 0;
+}
+
 int md_pimd_get_backward_potentials_x(md_pimd_t_x *pimd, md_num_t_x *Vt) {
   DevPimd *p = pimd_state(pimd, false);
   if (!p || !p->back_dev) return -1;

@@ -248,6 +248,8 @@ int recursion_status_all();
 // device (once per device, at runtime set-up).
 int noalloc_init(int device);
 int inject_recursion_timeouts(DevPimd *p, int count);
+// Range limit of the chain-frame block solve (test hook; <= 0: default).
+void set_recursion_frame_limit(double limit);
 
 // ------------------------------------------------------ batched replicas
 // One launch per step stage covers every replica of a batch: each stage has

@@ -137,6 +137,7 @@ _SIGS = {
     "md_get_nhc_length_x": (C.c_int, []),
     "md_hip_synchronize_x": (C.c_int, []),
     "md_pimd_inject_recursion_timeouts_x": (C.c_int, [PIMD_P, C.c_int]),
+    "md_set_recursion_frame_limit_x": (C.c_int, [C.c_double]),
     "md_pimd_fill_VB_noalloc_x": (C.c_int, [C.c_int, C.c_double, C.c_double, VP, VP, VP]),
     "md_set_ewald_real_cutoff_x": (C.c_int, [C.c_double]),
     "md_set_ewald_split_x": (C.c_int, [C.c_double]),
