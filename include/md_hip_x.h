@@ -93,8 +93,8 @@ int md_pimd_calc_vi_sign_grid_x(md_pimd_t_x *pimd, int n, const md_num_t_x *vi2,
  * output d_VBN: V_B^(0..N) (N+1 doubles, device memory) and, when d_Vt is not
  * NULL, the backward potentials Vt^(0..N) (device memory).  1 <= N <= 1024,
  * beta > 0, vi >= 0; larger N is rejected with an error.  The working memory
- * (5.8 MiB: the column copy of the cycle energies, the solve state and the
- * hand-off flags) is a static array in the library's device code that every
+ * (6.2 MiB: the column copy of the cycle energies, the solve state and its
+ * coefficients) is a static array in the library's device code that every
  * call on the same GPU shares: calls that overlap on one GPU race.  The call
  * allocates nothing (no hipMalloc, hipFree or allocating library routine);
  * it needs the library set up (md_init_setup_x or any earlier engine call)

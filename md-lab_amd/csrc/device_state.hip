@@ -287,10 +287,10 @@ static int alloc_pimd(DevPimd *p) {
   XMD_CHECK(hipMalloc(&p->work, 8 * (N + 1) * sizeof(double) + 64 * 1024 * sizeof(double)));
   XMD_CHECK(hipMalloc(&p->Fx, 6 * N * sizeof(double)));
   // recursion state: values and accumulators 10(N+1), then per direction and
-  // 64-row block the in-block (64*63/2) and previous-block (64*64)
+  // 64-row block the in-block (64*64) and previous-block (64*64)
   // coefficients, then the chain offsets (65) per direction and block
   const size_t nblk = (N + 63) / 64;
-  XMD_CHECK(hipMalloc(&p->rec, (10 * (size_t)(N + 1) + 2 * nblk * (64 * 63 / 2 + 64 * 64 + 65)) * sizeof(double)));
+  XMD_CHECK(hipMalloc(&p->rec, (10 * (size_t)(N + 1) + 2 * nblk * (64 * 64 + 64 * 64 + 65)) * sizeof(double)));
   XMD_CHECK(hipMalloc(&p->recsync, (4 * nblk + 2) * sizeof(int)));
   XMD_CHECK(hipMemset(p->recsync, 0, (4 * nblk + 2) * sizeof(int)));
   XMD_CHECK(hipMalloc(&p->done, sizeof(int)));

@@ -260,7 +260,7 @@ struct RecArgs {
   double *val2;       // [2][N+1] solved values per direction, position order (hand-off: NaN until solved)
   double *accM, *accS;  // [2][N+1] accumulators of column 0 (log-sum-exp pairs)
   double *finM, *finS;  // [2][N+1] row blocks' accumulators over blocks 0..G-2 (hand-off: NaN until folded)
-  double *dg;         // [2][nblk][RB(RB-1)/2] in-block coefficients, chain frame, exponentiated
+  double *dg;         // [2][nblk][RB][RB] in-block coefficients [r][q] (0 unless r < q), chain frame, exponentiated
   double *la;         // [2][nblk][RB][RB] the block's rows on the previous block, likewise
   double *ech;        // [2][nblk][RB+1] chain offsets E(0..RB-1) and the step Delta into the next block
   int *sync;          // [4 nblk] unused, error word, injected-timeout count
@@ -319,7 +319,7 @@ __device__ __forceinline__ double rec_coef_direct(const RecArgs &a, const EnkArg
 // updater) and block B-1 (the solver's look-ahead).
 // Lanes are rows throughout in the updaters: log-sum-exps run along each
 // lane, so no cross-lane reductions are needed.
-constexpr int RTRI = RB * (RB - 1) / 2;
+constexpr int RSQ = RB * RB;
 
 // coef(p, p') read through the other layout (column p' of the forward
 // problem is contiguous in ET, of the backward problem in ENk), so that
@@ -339,11 +339,38 @@ __device__ __forceinline__ double rec_coef_col(const RecArgs &a, int p, int pp) 
   return (k - 1) * a.lnxi - a.beta * E - ln_n;
 }
 
-__device__ __forceinline__ void tri_decode(int w, int &q, int &r) {
-  q = (int)((1.0 + sqrt(1.0 + 8.0 * w)) / 2.0);
-  while (q * (q - 1) / 2 > w) --q;
-  while ((q + 1) * q / 2 <= w) ++q;
-  r = w - q * (q - 1) / 2;
+// rec_coef without branches (ok false: NEG), so that unrolled callers issue
+// their loads together.
+template <bool FWD>
+__device__ __forceinline__ double rec_coef_nb(const RecArgs &a, int p, int pp, bool ok) {
+  const int k = p - pp;
+  ok = ok && !(a.xi0 && k != 1);
+  const size_t idx = ok ? (FWD ? tri_row(p) : col_off(a.N - p, a.N)) + k - 1 : 0;
+  const double E = FWD ? a.ENk[idx] : a.ET[idx];
+  const double ln_n = a.lnn[ok ? (FWD ? p : a.N - pp) : 0];
+  return ok ? (k - 1) * a.lnxi - a.beta * E - ln_n : NEG;
+}
+
+// rec_coef_col for the 16 columns pp0 .. pp0+15 of row p, all loads issued
+// before any is used (clamped indices instead of branches, which would make
+// each load wait for the one before); act false: every coefficient NEG.
+template <bool FWD>
+__device__ __forceinline__ void rec_coef_col16(const RecArgs &a, int p, int pp0, bool act, double *c) {
+  double E[16], ln[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int pp = pp0 + j, k = p - pp;
+    const bool ok = act && !(a.xi0 && k != 1);
+    const size_t idx = ok ? (FWD ? col_off(pp, a.N) : tri_row(a.N - pp)) + k - 1 : 0;
+    E[j] = FWD ? a.ET[idx] : a.ENk[idx];
+    ln[j] = a.lnn[ok ? (FWD ? p : a.N - pp) : 0];
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int pp = pp0 + j, k = p - pp;
+    const bool ok = act && !(a.xi0 && k != 1);
+    c[j] = ok ? (k - 1) * a.lnxi - a.beta * E[j] - ln[j] : NEG;
+  }
 }
 
 // DIRECT: coefficients from positions and prefix sums (rec_coef_direct, e
@@ -411,7 +438,7 @@ __device__ __forceinline__ void rec_chain_offsets(const RecArgs &a, int b, const
 // Coefficients of row block b in the frame of its k = 1 chain, one part of
 // REC_PREP_PARTS (every workgroup of the block computes the offsets, part 0
 // stores them):
-//   in-block    A(q, r)  = exp(c(p0+q, p0+r) + E(r) - E(q)),             r < q
+//   in-block    A(q, r)  = exp(c(p0+q, p0+r) + E(r) - E(q)),   r < q, stored [r][q]
 //   look-ahead  F(q, r)  = exp(c(p0+q, p0-RB+r) + E'(r) - Delta' - E(q))
 // with E' and Delta' those of block b-1 (F: block b's rows on block b-1's
 // columns, stored [r][q]).  Terms without a coefficient are 0; an exponent
@@ -428,11 +455,10 @@ __device__ __forceinline__ void rec_prep_block(const RecArgs &a, int dir, int b,
   __syncthreads();
   const int p0 = 1 + b * RB, nrows = min(RB, a.N - p0 + 1);
   if (part == 0 && tid <= RB) a.ech[((size_t)dir * a.nblk + b) * (RB + 1) + tid] = E[0][tid];
-  double *dg = a.dg + ((size_t)dir * a.nblk + b) * RTRI;
-  for (int w = part * 256 + tid; w < RTRI; w += REC_PREP_PARTS * 256) {
-    int q, r;
-    tri_decode(w, q, r);
-    const double c = q < nrows ? coef(p0 + q, p0 + r) : NEG;
+  double *dg = a.dg + ((size_t)dir * a.nblk + b) * RSQ;
+  for (int w = part * 256 + tid; w < RSQ; w += REC_PREP_PARTS * 256) {
+    const int r = w / RB, q = w - r * RB;
+    const double c = (r < q && q < nrows) ? coef(p0 + q, p0 + r) : NEG;
     dg[w] = c > NEG ? exp(c + E[0][r] - E[0][q]) : 0.0;
   }
   double *la = a.la + ((size_t)dir * a.nblk + b) * (RB * RB);
@@ -545,14 +571,20 @@ __device__ __forceinline__ double ld_handoff(const double *p) {
 // Lanes [0, n) of the calling wave read p[lane] (and q[lane] if q) until no
 // value is NaN; at most `spin` rounds (wave-uniform; spin < 0 gives up at
 // once: the injected timeout of the test hook), then the error word is
-// raised and the values are whatever was read last.
+// raised and the values are whatever was read last.  While the last value
+// is still NaN a round reads only that one (a single request per wave: many
+// workgroups poll the same lines).
 __device__ __forceinline__ void rec_poll(const double *p, const double *q, int n, int *err, int spin, double &pv,
                                          double &qv) {
   const int lane = threadIdx.x & 63;
+  const double *last = q ? q + n - 1 : p + n - 1;
   for (int it = 0;; ++it) {
-    pv = lane < n ? ld_handoff(p + lane) : 0.0;
-    qv = (q && lane < n) ? ld_handoff(q + lane) : 0.0;
-    if (spin >= 0 && !__any(pv != pv || qv != qv)) return;
+    const double probe = ld_handoff(last);
+    if (spin >= 0 && probe == probe) {
+      pv = lane < n ? ld_handoff(p + lane) : 0.0;
+      qv = (q && lane < n) ? ld_handoff(q + lane) : 0.0;
+      if (!__any(pv != pv || qv != qv)) return;
+    }
     if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
       return;
     if (it >= spin) {
@@ -586,15 +618,13 @@ __device__ __forceinline__ void rec_fold_rows(const RecArgs &a, int dir, int G, 
     Sa = AS[p];
   }
   double c[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) c[j] = act ? rec_coef_col<FWD>(a, p, 1 + 16 * wv + j) : NEG;
+  rec_coef_col16<FWD>(a, p, 1 + 16 * wv, act, c);
   for (int b = 0; b <= G - 2; ++b) {
     const int pprev = 1 + b * RB;
     double v, unused;
     rec_poll(val + pprev + 16 * wv, nullptr, 16, err, REC_SPIN_MAX, v, unused);
     double cn[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) cn[j] = (act && b + 1 <= G - 2) ? rec_coef_col<FWD>(a, p, pprev + RB + 16 * wv + j) : NEG;
+    rec_coef_col16<FWD>(a, p, pprev + RB + 16 * wv, act && b + 1 <= G - 2, cn);
     double t[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -628,19 +658,8 @@ __device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, do
   }
 }
 
-// (q, r) of the in-block coefficient w = q (q - 1) / 2 + r, packed q << 8 | r.
-struct TriTable {
-  unsigned short v[RTRI];
-  constexpr TriTable() : v() {
-    int w = 0;
-    for (int q = 1; q < RB; ++q)
-      for (int r = 0; r < q; ++r) v[w++] = (unsigned short)((q << 8) | r);
-  }
-};
-__constant__ TriTable kTri = TriTable();
-
 struct RecLds {
-  double dg[2][RTRI];           // in-block coefficients (ping-pong)
+  double dg[2][RSQ];            // in-block coefficients [r][q] (ping-pong)
   double sg[RB];                // per-row shifts of an exact block solve
   double wl[RB];                // W of the block solved last
   double lv[RB];                // its values L
@@ -671,38 +690,21 @@ constexpr double REC_BIG = 1e250;
 template <bool FWD>
 __device__ __forceinline__ double rec_block_exact(const RecArgs &a, int p0, int nrows, double M, double Sm,
                                                   double *dg, double *sg) {
-  const int lane = threadIdx.x & 63, rowq = lane * (lane - 1) / 2;
-  for (int w = lane; w < RTRI; w += 64) {
-    const int q = kTri.v[w] >> 8, r = kTri.v[w] & 255;
-    dg[w] = q < nrows ? rec_coef<FWD>(a, p0 + q, p0 + r) : NEG;
-  }
-  wave_lds_sync();
-  // row q's coefficients start at rowq; the reads past them stay inside dg
-  // (rowq + RB - 2 < RTRI) and are masked
-  double c[RB - 1];
-#pragma unroll
-  for (int r = 0; r < RB - 1; ++r) {
-    const double x = dg[rowq + r];
-    c[r] = r < lane ? x : NEG;
-  }
+  const int lane = threadIdx.x & 63;
+  // lane = row q, element [r][q] = c(q, r) for r < q (NEG otherwise)
+#pragma unroll 16
+  for (int r = 0; r < RB - 1; ++r) dg[r * RB + lane] = rec_coef_nb<FWD>(a, p0 + lane, p0 + r, r < lane && lane < nrows);
   double sig = lane < nrows ? M : NEG;
-#pragma unroll
-  for (int r = 0; r < RB - 1; ++r) sig = fmax(sig, c[r] + readlane_d(sig, r));
+  for (int r = 0; r < RB - 1; ++r) sig = fmax(sig, dg[r * RB + lane] + readlane_d(sig, r));
   sg[lane] = lane < nrows ? sig : NEG;
   wave_lds_sync();
-  for (int w = lane; w < RTRI; w += 64) {
-    const int q = kTri.v[w] >> 8, r = kTri.v[w] & 255;
-    const double x = dg[w];
-    dg[w] = (q < nrows && x > NEG) ? exp(x + sg[r] - sg[q]) : 0.0;
+  for (int r = 0; r < RB - 1; ++r) {
+    const double x = dg[r * RB + lane];
+    dg[r * RB + lane] = x > NEG ? exp(x + sg[r] - sig) : 0.0;
   }
-  wave_lds_sync();
   double W = 0.0;
   if (lane < nrows && M > NEG && M > sig - 745.0) W = Sm * exp(M - sig);
-#pragma unroll
-  for (int r = 0; r < RB - 1; ++r) {
-    const double x = dg[rowq + r];
-    W = fma(r < lane ? x : 0.0, readlane_d(W, r), W);
-  }
+  for (int r = 0; r < RB - 1; ++r) W = fma(dg[r * RB + lane], readlane_d(W, r), W);
   return sig + log(W);
 }
 
@@ -749,13 +751,13 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
   double *val = a.val2 + (size_t)dir * (N + 1);
   const double *AM = a.accM + (size_t)dir * (N + 1), *AS = a.accS + (size_t)dir * (N + 1);
   const double *FM = a.finM + (size_t)dir * (N + 1), *FS = a.finS + (size_t)dir * (N + 1);
-  const double *DGd = a.dg + (size_t)dir * a.nblk * RTRI;
+  const double *DGd = a.dg + (size_t)dir * a.nblk * RSQ;
   const double *LAd = a.la + (size_t)dir * a.nblk * (RB * RB);
   const double *ECd = a.ech + (size_t)dir * a.nblk * (RB + 1);
-  constexpr int DPT = (RTRI + 255) / 256;  // in-block coefficients per thread
+  constexpr int DPT = RSQ / 256;           // in-block coefficients per thread
   constexpr int LPW = RB / 4;              // look-ahead columns per lane
   // block 0: coefficients, accumulators (column 0 only), chain offsets
-  for (int w = tid; w < RTRI; w += 256) S.dg[0][w] = DGd[w];
+  for (int w = tid; w < RSQ; w += 256) S.dg[0][w] = DGd[w];
   double M = NEG, Sm = 0.0, E = 0.0, Dl = 0.0;
   if (wv == 0) {
     const int nr0 = min(RB, N);
@@ -764,7 +766,6 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
     E = ECd[lane];
     Dl = ECd[RB];
   }
-  const int rowq = lane * (lane - 1) / 2;
   for (int b = 0; b < a.nblk; ++b) {
     const int cur = b & 1;
     const int p0 = 1 + b * RB, nrows = min(RB, N - p0 + 1);
@@ -777,7 +778,7 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
 #pragma unroll
       for (int k = 0; k < DPT; ++k) {
         const int w = tid + 256 * k;
-        dgr[k] = w < RTRI ? DGd[(size_t)(b + 1) * RTRI + w] : 0.0;
+        dgr[k] = DGd[(size_t)(b + 1) * RSQ + w];
       }
 #pragma unroll
       for (int k = 0; k < LPW; ++k) lar[k] = LAd[(size_t)(b + 1) * (RB * RB) + (wv + 4 * k) * RB + lane];
@@ -792,16 +793,9 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
     if (wv == 0) {
       sigma = readlane_d(M, 0) + log(readlane_d(Sm, 0));
       double W = lane < nrows ? Sm * exp(M - sigma - E) : 0.0;
-      // row q's coefficients start at rowq; the reads past them stay inside
-      // dg (rowq + RB - 2 < RTRI) and are masked
-      double av[RB - 1];
+      // column r of dg holds A(q, r) at row q = lane (0 unless r < q)
 #pragma unroll
-      for (int r = 0; r < RB - 1; ++r) {
-        const double x = dg[rowq + r];
-        av[r] = r < lane ? x : 0.0;
-      }
-#pragma unroll
-      for (int r = 0; r < RB - 1; ++r) W = fma(av[r], readlane_d(W, r), W);
+      for (int r = 0; r < RB - 1; ++r) W = fma(dg[r * RB + lane], readlane_d(W, r), W);
       fast = __all(W <= a.big);  // NaN fails too
       const double v = fast ? sigma + E + log(W) : rec_block_exact<FWD>(a, p0, nrows, M, Sm, dg, S.sg);
       S.wl[lane] = W;
@@ -841,7 +835,7 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
       const int w = tid + 256 * k;
-      if (w < RTRI) S.dg[cur ^ 1][w] = dgr[k];
+      S.dg[cur ^ 1][w] = dgr[k];
     }
     // SOLO: block b+1's accumulators over blocks 0 .. b-1, folded here
     double Mf = NEG, Sf = 0.0;
@@ -863,22 +857,16 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
         } else {
           // block b folded into row q' from its values (block b is full)
           const int p = p1 + lane;
-          double m = NEG;
-#pragma unroll 4
-          for (int r = 0; r < RB; ++r) {
-            const double c = rec_coef_col<FWD>(a, p, p0 + r);
-            if (c > NEG) m = fmax(m, c + S.lv[r]);
+          double m = NEG, s = 0.0;
+          for (int r0 = 0; r0 < RB; r0 += 16) {
+            double c[16], t[16], m16, s16;
+            rec_coef_col16<FWD>(a, p, p0 + r0, true, c);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) t[j] = c[j] > NEG ? c[j] + S.lv[r0 + j] : NEG;
+            lse16(t, m16, s16);
+            lse_pair(m, s, m16, s16);
           }
-          if (m > NEG) {
-            double s = 0.0;
-#pragma unroll 4
-            for (int r = 0; r < RB; ++r) {
-              const double c = rec_coef_col<FWD>(a, p, p0 + r);
-              const double t = c > NEG ? c + S.lv[r] : NEG;
-              if (t > m - LSE_SKIP) s += exp(t - m);
-            }
-            lse_pair(Mn, Sn, m, s);
-          }
+          lse_pair(Mn, Sn, m, s);
         }
       }
       M = Mn;
@@ -1347,7 +1335,7 @@ static RecArgs rec_args(DevPimd *p) {
   a.finS = p->rec + 8 * (p->N + 1);
   a.nblk = (p->N + RB - 1) / RB;
   a.dg = p->rec + 10 * (p->N + 1);
-  a.la = a.dg + 2 * (size_t)a.nblk * RTRI;
+  a.la = a.dg + 2 * (size_t)a.nblk * RSQ;
   a.ech = a.la + 2 * (size_t)a.nblk * (RB * RB);
   a.big = g_rec_frame_limit;
   a.sync = p->recsync;
@@ -1529,10 +1517,10 @@ constexpr size_t NA_LNN = NA_ET + (size_t)NA_NMAX * (NA_NMAX + 1) / 2;
 constexpr size_t NA_LF = NA_LNN + (NA_NMAX + 1);
 constexpr size_t NA_RB = NA_LF + (NA_NMAX + 1);
 constexpr size_t NA_REC = NA_RB + (NA_NMAX + 1);
-constexpr size_t NA_SYNC = NA_REC + 10 * (size_t)(NA_NMAX + 1) + 2 * (size_t)NA_NBLK * (RTRI + RB * RB + RB + 1);
+constexpr size_t NA_SYNC = NA_REC + 10 * (size_t)(NA_NMAX + 1) + 2 * (size_t)NA_NBLK * (2 * RSQ + RB + 1);
 constexpr int NA_NSYNC = 4 * NA_NBLK + 2;  // flags of N_max, error word, injected-timeout count (unused here)
 constexpr size_t NA_DOUBLES = NA_SYNC + (NA_NSYNC + 1) / 2;
-static_assert(NA_DOUBLES * sizeof(double) < 6u * 1024 * 1024, "scratch stays below 6 MiB");
+static_assert(NA_DOUBLES * sizeof(double) < 7u * 1024 * 1024, "scratch stays below 7 MiB");
 
 static __device__ __align__(256) double scratch[NA_DOUBLES];
 
@@ -1586,7 +1574,7 @@ static int noalloc_fill_vb(int N, double beta, double vi, const double *ENk, dou
   a.finM = a.val2 + 6 * (N + 1);
   a.finS = a.val2 + 8 * (N + 1);
   a.dg = a.val2 + 10 * (N + 1);
-  a.la = a.dg + 2 * (size_t)nblk * RTRI;
+  a.la = a.dg + 2 * (size_t)nblk * RSQ;
   a.ech = a.la + 2 * (size_t)nblk * (RB * RB);
   a.big = g_rec_frame_limit;
   a.sync = (int *)(base + NA_SYNC) + 4 * (NA_NBLK - nblk);  // error word at the fixed slot 4 NA_NBLK
