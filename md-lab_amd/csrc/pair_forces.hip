@@ -1032,31 +1032,67 @@ __device__ __forceinline__ double rsqrt_pos(double x) {
   return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
-// Force of one image r of the pair, added to g when `in` holds.  r is never
-// zero: distinct beads of a periodic system.
+// Force of one image r of a pair, in three parts so that callers can keep
+// the table loads of several images in flight together: ew_prep (distance,
+// interval, local coordinate), ew_load (the interval's coefficients from the
+// LDS table), ew_finish (Horner in y, force added to g when `in` holds).  r
+// is never zero: distinct beads of a periodic system.
+constexpr int kEwNC2 = kEwTabStride / 2;  // coefficient pairs per interval
+
+struct EwImg {
+  double rx, ry, rz, ir, y;
+  int iv;
+  bool in;
+};
+
+__device__ __forceinline__ EwImg ew_prep(double rx, double ry, double rz, bool in, const EwReal &ep) {
+  EwImg m;
+  m.rx = rx;
+  m.ry = ry;
+  m.rz = rz;
+  m.in = in;
+  const double R2 = fma(rx, rx, fma(ry, ry, rz * rz));
+  m.ir = rsqrt_pos(R2);
+  const double t8 = (R2 * m.ir) * ep.al8;
+  m.iv = min((int)t8, kEwTabIntervals - 1);
+  m.y = fma(2.0, t8, -(double)(2 * m.iv + 1));
+  return m;
+}
+
+__device__ __forceinline__ void ew_load(const EwImg &m, const double2 *__restrict__ tab, double2 *cc) {
+  const double2 *c = tab + m.iv * kEwNC2;
+#pragma unroll
+  for (int k = 0; k < kEwNC2; ++k) cc[k] = c[k];
+}
+
+__device__ __forceinline__ void ew_finish(const EwImg &m, const double2 *cc, const EwReal &ep, double &gx,
+                                          double &gy, double &gz) {
+  // coefficients c_0 .. c_DEG of the interval as (c_2k, c_2k+1) pairs
+  double h = fma(cc[kEwNC2 - 1].y, m.y, cc[kEwNC2 - 1].x);
+#pragma unroll
+  for (int k = kEwNC2 - 2; k >= 0; --k) {
+    h = fma(h, m.y, cc[k].y);
+    h = fma(h, m.y, cc[k].x);
+  }
+  const double w = m.in ? ep.q * h * (m.ir * m.ir * m.ir) : 0.0;
+  gx = fma(w, m.rx, gx);
+  gy = fma(w, m.ry, gy);
+  gz = fma(w, m.rz, gz);
+}
+
+// The loaded coefficients are pinned before use, so that the compiler issues
+// every load of the group first (one wait) instead of waiting for each.
+__device__ __forceinline__ void ew_pin(const double2 *cc, int n) {
+  for (int k = 0; k < n; ++k) __asm__ volatile("" ::"v"(cc[k].x), "v"(cc[k].y));
+}
+
 __device__ __forceinline__ void ewald_image(double rx, double ry, double rz, bool in, const EwReal &ep,
                                             const double2 *__restrict__ tab, double &gx, double &gy, double &gz) {
-  const double R2 = fma(rx, rx, fma(ry, ry, rz * rz));
-  const double ir = rsqrt_pos(R2);
-  const double t8 = (R2 * ir) * ep.al8;
-  const int iv = min((int)t8, kEwTabIntervals - 1);
-  const double y = fma(2.0, t8, -(double)(2 * iv + 1));
-  // coefficients c_0 .. c_DEG of the interval as (c_2k, c_2k+1) pairs; Horner in y
-  constexpr int NC2 = kEwTabStride / 2;
-  const double2 *c = tab + iv * NC2;
-  double2 cc[NC2];
-#pragma unroll
-  for (int k = 0; k < NC2; ++k) cc[k] = c[k];
-  double h = fma(cc[NC2 - 1].y, y, cc[NC2 - 1].x);
-#pragma unroll
-  for (int k = NC2 - 2; k >= 0; --k) {
-    h = fma(h, y, cc[k].y);
-    h = fma(h, y, cc[k].x);
-  }
-  const double w = in ? ep.q * h * (ir * ir * ir) : 0.0;
-  gx = fma(w, rx, gx);
-  gy = fma(w, ry, gy);
-  gz = fma(w, rz, gz);
+  const EwImg m = ew_prep(rx, ry, rz, in, ep);
+  double2 cc[kEwNC2];
+  ew_load(m, tab, cc);
+  ew_pin(cc, kEwNC2);
+  ew_finish(m, cc, ep, gx, gy, gz);
 }
 
 template <int IMG>
@@ -1074,8 +1110,14 @@ __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, 
   const double xy[4] = {ax2 + ay2, bx2 + ay2, ax2 + by2, bx2 + by2};
   if (IMG == IMG_FACE) {
     const bool kx = xy[1] + az2 <= rc2, ky = xy[2] + az2 <= rc2, kz = xy[0] + bz2 <= rc2;
-    ewald_image(ax, ay, az, xy[0] + az2 <= rc2, ep, tab, gx, gy, gz);
-    ewald_image(kx ? bx : ax, ky ? by : ay, kz ? bz : az, kx || ky || kz, ep, tab, gx, gy, gz);
+    const EwImg m0 = ew_prep(ax, ay, az, xy[0] + az2 <= rc2, ep);
+    const EwImg m1 = ew_prep(kx ? bx : ax, ky ? by : ay, kz ? bz : az, kx || ky || kz, ep);
+    double2 cc[2 * kEwNC2];
+    ew_load(m0, tab, cc);
+    ew_load(m1, tab, cc + kEwNC2);
+    ew_pin(cc, 2 * kEwNC2);
+    ew_finish(m0, cc, ep, gx, gy, gz);
+    ew_finish(m1, cc + kEwNC2, ep, gx, gy, gz);
     return;
   }
   unsigned lo = 0, hi = 0;
@@ -1109,33 +1151,40 @@ __device__ __forceinline__ void ewald_pair_tab(double dx, double dy, double dz, 
   }
 }
 
-// Four tile pairs per workgroup (one per wave) sharing the LDS table; the
-// per-wave schedule is that of k_pair_tiles.
+// Rotates a double one lane down the wave (lane i takes lane i+1's value,
+// lane 63 lane 0's); every lane of the wave must be active.
+__device__ __forceinline__ double wave_rol1(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
 // WPB waves per workgroup, one tile pair per wave (the erfc table is staged
-// once per workgroup); every choice computes the same partial sums.
+// once per workgroup); every choice computes the same partial sums.  Lane t
+// holds bead i = I*64 + t of tile I and, at step s, the record of bead
+// j = J*64 + (t + s) mod 64 of tile J (its position and the reaction
+// accumulated so far); the records move one lane down the wave after each
+// step (DPP rotation, no LDS), so after 64 steps every record is home.  Each
+// reaction thus gathers its terms in step order, lane t's own force likewise.
 template <int IMG, int WPB = 4>
 __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restrict__ X, const double *__restrict__ M,
                                                     double *__restrict__ Fpart, const int2 *__restrict__ tiles, int N,
                                                     int nb, int T, int ntp, int nitems, EwReal ep) {
   __shared__ double2 tab[kEwTabIntervals * kEwTabStride / 2];
-  __shared__ double jx[WPB][64], jy[WPB][64], jz[WPB][64], rx[WPB][64], ry[WPB][64], rz[WPB][64];
   // the wave index and everything derived from it are wave-uniform: read
   // them as such, so the step loop below runs on scalar registers
   const int tid = threadIdx.x, t = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int k = tid; k < kEwTabIntervals * kEwTabStride / 2; k += 64 * WPB)
     tab[k] = make_double2(kEwTab[2 * k], kEwTab[2 * k + 1]);
+  __syncthreads();
   const int item = bx * WPB + w;
-  const bool valid = item < nitems;
-  int slice = 0, I = 0, J = 0;
-  if (valid) {
-    slice = item / ntp;
-    const int2 IJ = tiles[item - slice * ntp];
-    I = __builtin_amdgcn_readfirstlane(IJ.x);
-    J = __builtin_amdgcn_readfirstlane(IJ.y);
-  }
+  if (item >= nitems) return;
+  const int slice = item / ntp;
+  const int2 IJ = tiles[item - slice * ntp];
+  const int I = __builtin_amdgcn_readfirstlane(IJ.x), J = __builtin_amdgcn_readfirstlane(IJ.y);
   const int base = slice * N;
   const int i = I * 64 + t, jl = J * 64 + t;
-  const bool act = valid && i < N, jact = valid && jl < N;
+  const bool act = i < N, jact = jl < N;
   double xi = 0, yi = 0, zi = 0, wi = 1;
   if (act) {
     xi = X[base + i];
@@ -1143,44 +1192,36 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
     zi = X[2 * nb + base + i];
     wi = 1.0 / M[base + i];
   }
-  jx[w][t] = jact ? X[base + jl] : 0.0;
-  jy[w][t] = jact ? X[nb + base + jl] : 0.0;
-  jz[w][t] = jact ? X[2 * nb + base + jl] : 0.0;
-  rx[w][t] = 0;
-  ry[w][t] = 0;
-  rz[w][t] = 0;
-  __syncthreads();
-  const bool diag = I == J;
+  double jx = jact ? X[base + jl] : 0.0, jy = jact ? X[nb + base + jl] : 0.0, jz = jact ? X[2 * nb + base + jl] : 0.0;
+  double rx = 0, ry = 0, rz = 0;
+  const bool diag = I == J, full = J * 64 + 64 <= N;
   double fx = 0, fy = 0, fz = 0;
-  // lane t meets bead (t + s) mod 64 of tile J at step s; a diagonal tile
-  // takes s = 1..32 and only lanes t < 32 at s = 32 (each pair once)
-  auto pair = [&](int j) {
-    double gx = 0, gy = 0, gz = 0;
-    ewald_pair_tab<IMG>(xi - jx[w][j], yi - jy[w][j], zi - jz[w][j], ep, tab, gx, gy, gz);
-    // the pair's force over the mass of its higher-index bead; the table
-    // kernels run only for equal masses, so that is this lane's 1 / m_i
-    const double iw = wi;
-    fx = fma(gx, iw, fx);
-    fy = fma(gy, iw, fy);
-    fz = fma(gz, iw, fz);
-    rx[w][j] -= gx * iw;
-    ry[w][j] -= gy * iw;
-    rz[w][j] -= gz * iw;
-  };
-  if (act) {
-    const int s_lo = diag ? 1 : 0, s_end = diag ? 31 : 63;
-    if (J * 64 + 64 <= N) {
-      for (int s = s_lo; s <= s_end; ++s) pair((t + s) & 63);
-    } else {
-      for (int s = s_lo; s <= s_end; ++s) {
-        const int j = (t + s) & 63;
-        if (J * 64 + j < N) pair(j);
-      }
+  // a diagonal tile takes s = 1..31 and only lanes t < 32 at s = 32 (each
+  // pair once); its remaining steps only rotate the records home
+  for (int s = 0; s < 64; ++s) {
+    const int j = (t + s) & 63;
+    bool on = act && (full || J * 64 + j < N);
+    if (diag) on = on && (s >= 1 && s <= 32) && (s < 32 || t < 32);
+    if (on) {
+      double gx = 0, gy = 0, gz = 0;
+      ewald_pair_tab<IMG>(xi - jx, yi - jy, zi - jz, ep, tab, gx, gy, gz);
+      // the pair's force over the mass of its higher-index bead; the table
+      // kernels run only for equal masses, so that is this lane's 1 / m_i
+      const double iw = wi;
+      fx = fma(gx, iw, fx);
+      fy = fma(gy, iw, fy);
+      fz = fma(gz, iw, fz);
+      rx -= gx * iw;
+      ry -= gy * iw;
+      rz -= gz * iw;
     }
-    if (diag && t < 32 && J * 64 + t + 32 < N) pair(t + 32);
+    jx = wave_rol1(jx);
+    jy = wave_rol1(jy);
+    jz = wave_rol1(jz);
+    rx = wave_rol1(rx);
+    ry = wave_rol1(ry);
+    rz = wave_rol1(rz);
   }
-  __syncthreads();
-  if (!valid) return;
   auto put = [&](int bead, int slot, double vx, double vy, double vz) {
     const size_t s0 = (size_t)slot * 3 * nb + base + bead;
     Fpart[s0] = vx;
@@ -1188,15 +1229,18 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
     Fpart[s0 + 2 * (size_t)nb] = vz;
   };
   if (diag) {
-    if (act) put(i, I, fx + rx[w][t], fy + ry[w][t], fz + rz[w][t]);
+    if (act) put(i, I, fx + rx, fy + ry, fz + rz);
   } else {
     if (act) put(i, J, fx, fy, fz);
-    if (jact) put(jl, I, rx[w][t], ry[w][t], rz[w][t]);
+    if (jact) put(jl, I, rx, ry, rz);
   }
 }
 
+// Register budget for five waves per SIMD (every wave of the workload is
+// resident at once; more registers keep the table loads of a step in
+// flight together).
 template <int IMG, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_ewald_real(const double *__restrict__ X, const double *__restrict__ M,
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_ewald_real(const double *__restrict__ X, const double *__restrict__ M,
                                                          double *__restrict__ Fpart, const int2 *__restrict__ tiles,
                                                          int N, int nb, int T, int ntp, int nitems, EwReal ep) {
   k_ewald_real_body<IMG, WPB>(blockIdx.x, X, M, Fpart, tiles, N, nb, T, ntp, nitems, ep);
