@@ -10,14 +10,16 @@ before timing.  One step = the full reference driver step (NHC half-kick,
 drift, wrap, forces: trap/pair/Ewald, cycle energies, V_B forward/backward,
 exchange forces, NHC half-kick), replayed as a captured hipGraph.
 
-Prints ONE JSON line.  `roofline` is for the kernel on the step's critical
-path that takes the most device time, the persistent exchange-recursion solve
-k_rec_persist (FP64 vector ALU, limited by its serial dependency chain):
-algorithmic FP64 work per launch / its average duration measured with HIP
-events around it on its own stream.  `kernels` gives the same figures for the
-other large kernels, and `step_rate` the algorithmic work of the whole step
-over ms_per_step.  `cpu_baseline` times the reference's own single-thread
-nocl build (oracle/_ref) on a bounded sample of the same step.
+Prints ONE JSON line.  `roofline` is for the kernel of the step that takes
+the most device time, chosen live between the real-space Ewald kernel
+k_ewald_real (FP64 vector ALU; timed alone with HIP events on its stream,
+default launch choice) and the persistent exchange-recursion solve
+k_rec_persist (FP64 vector ALU, limited by its serial dependency chain; its
+stage timed alone the same way): algorithmic FP64 work per launch / that
+average duration.  `kernels` gives the same figures for the other large
+kernels, and `step_rate` the algorithmic work of the whole step over
+ms_per_step.  `cpu_baseline` times the reference's own single-thread nocl
+build (oracle/_ref) on a bounded sample of the same step.
 """
 import argparse
 import ctypes as C
@@ -350,11 +352,18 @@ def main():
     alpha, rcut, nk = a_.value, rc_.value, nk_.value
     work = step_work(N, P, xi, alpha, L, rcut, nk)
     rec_ms = stages[9]
-    achieved = work["recursions"] / (rec_ms * 1e-3) / 1e12
     nblk = (N + 63) // 64
+    # the real-space kernel alone (its partial-force reduction is part of
+    # stage 4): every launch choice timed with HIP events, 20 launches each;
+    # the step uses the most specific image enumeration with 4 waves
+    choice_ms = np.zeros(12)
+    chosen = C.c_int()
+    assert lib.md_pimd_autotune_x(s.pimd, 20, pyxmd.dptr(choice_ms), C.byref(chosen)) == 0
+    real_ms = float(next(choice_ms[4 * img + 2] for img in range(3) if choice_ms[4 * img + 2] > 0))
     kernels = {}
-    for name, key, ms in (("k_ewald_real", "ewald_real", stages[4]), ("reciprocal (k_phase+k_sk_mfma+"
-                          "k_recip_force_mfma)", "ewald_recip", stages[5])):
+    for name, key, ms in (("k_ewald_real", "ewald_real", real_ms),
+                          ("k_rec_persist (V_B forward + backward recursion)", "recursions", rec_ms),
+                          ("reciprocal stage (k_phase+k_sk_mfma+k_recip_force_mfma)", "ewald_recip", stages[5])):
         tf = work[key] / (ms * 1e-3) / 1e12
         kernels[name] = {"avg_ms": float(ms), "work_flop_eq": work[key], "achieved_tflops": tf,
                          "frac": tf / FP64_PEAK_TFLOPS}
@@ -371,9 +380,22 @@ def main():
         cpu = cpu_baseline(N, P, rs, theta, xi, nsum, args.cpu_pairs)
 
     traffic_key = f"UEG N={N} P={P} alpha'L={alpha * L:.2f}"
-    traffic = measured_traffic("k_rec_persist", traffic_key)
-    hw_flop = counter_flops("k_rec_persist", traffic_key)
     step_tf = work["total"] / (sec / args.steps) / 1e12
+    if real_ms >= rec_ms:
+        kname, kwork, kms = "k_ewald_real", work["ewald_real"], real_ms
+        roof_extra = {"kernel": "k_ewald_real (real-space Ewald, 64x64 bead tile pairs, both images of each pair)",
+                      "limiter": "FP64 VALU issue and LDS table lookups",
+                      "work_unit": f"pairs x mean images in r_c x {FLOP_PER_IMAGE:.0f} flop-eq",
+                      "launch_choice_ms": [round(float(x), 5) for x in choice_ms]}
+    else:
+        kname, kwork, kms = "k_rec_persist", work["recursions"], rec_ms
+        roof_extra = {"kernel": "k_rec_persist (V_B forward + backward recursion, both directions in one launch)",
+                      "limiter": "2N-long serial dependency chain (latency), not throughput",
+                      "work_unit": f"2 directions x N(N+1)/2 log-sum-exp terms x {FLOP_PER_LSE_TERM:.0f} flop-eq",
+                      "blocks_per_direction": nblk, "us_per_block": 1e3 * float(rec_ms) / nblk}
+    traffic = measured_traffic(kname, traffic_key)
+    hw_flop = counter_flops(kname, traffic_key)
+    achieved = kwork / (kms * 1e-3) / 1e12
     line = {
         "metric": METRIC, "value": value, "unit": "particle-bead-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -383,15 +405,10 @@ def main():
                                f"alpha'L={alpha * L:.2f}, r_c={rcut:.2f}, {nk} half-space k), NHC M=4, dt={dt}",
                    "path": "plain FP64 Ewald (no surrogate)", "energy_error_vs_fp64_ewald": None,
                    "hipgraph": True, "finite_state": finite},
-        "roofline": {"bound": "fp64_valu", "limiter": "2N-long serial dependency chain (latency), not throughput",
-                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_rec_persist (V_B forward + backward recursion, both directions in one launch)",
-                     "avg_ms": float(rec_ms), "work_flop_eq": work["recursions"],
-                     "work_unit": f"2 directions x N(N+1)/2 log-sum-exp terms x {FLOP_PER_LSE_TERM:.0f} flop-eq",
-                     "blocks_per_direction": nblk, "us_per_block": 1e3 * float(rec_ms) / nblk,
-                     "counted_fp64_flop": hw_flop,
-                     "counted_tflops": hw_flop / (rec_ms * 1e-3) / 1e12 if hw_flop else None},
+        "roofline": dict({"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                          "avg_ms": float(kms), "work_flop_eq": kwork, "counted_fp64_flop": hw_flop,
+                          "counted_tflops": hw_flop / (kms * 1e-3) / 1e12 if hw_flop else None}, **roof_extra),
         "kernels": kernels,
         "step_rate": {"work_flop_eq_per_step": work, "achieved_tflops": step_tf,
                       "frac": step_tf / FP64_PEAK_TFLOPS},
