@@ -12,11 +12,10 @@ exchange forces, NHC half-kick), replayed as a captured hipGraph.
 
 Prints ONE JSON line.  `roofline` is for the kernel of the step that takes
 the most device time, chosen live between the real-space Ewald kernel
-k_ewald_real (FP64 vector ALU; timed alone with HIP events on its stream,
-default launch choice) and the persistent exchange-recursion solve
-k_rec_persist (FP64 vector ALU, limited by its serial dependency chain; its
-stage timed alone the same way): algorithmic FP64 work per launch / that
-average duration.  `kernels` gives the same figures for the other large
+k_ewald_real (FP64 vector ALU) and the persistent exchange-recursion solve
+k_rec_persist (FP64 vector ALU, limited by its serial dependency chain),
+each timed alone with HIP events around it on its stream in the serial
+stage profile: algorithmic FP64 work per launch / that average duration.  `kernels` gives the same figures for the other large
 kernels, and `step_rate` the algorithmic work of the whole step over
 ms_per_step.  `cpu_baseline` times the reference's own single-thread nocl
 build (oracle/_ref) on a bounded sample of the same step.
@@ -326,9 +325,9 @@ def main():
     s = pyxmd.System(lib, "ueg", N, P, T, xi, seed=1, L=L, nsum=nsum)
     cfg = s.step_config(dt, use_graph=1)
     names = ["nhc1_wrap", "cycle_energies", "vb_recursions", "bead_forces", "ewald_real", "ewald_recip",
-             "exchange_forces", "nhc2", "step", "rec_persist"]
+             "exchange_forces", "nhc2", "step", "rec_persist", "ewald_real_kernel"]
     if args.serial_only:
-        stages = np.zeros(10)
+        stages = np.zeros(11)
         assert lib.md_pimd_profile_steps_x(s.pimd, C.byref(cfg), args.steps, 1, pyxmd.dptr(stages)) == 0
         print(json.dumps({"serial_only": True, "steps": args.steps,
                           "stages_ms": {k: round(float(v), 5) for k, v in zip(names, stages)}}))
@@ -345,7 +344,7 @@ def main():
     value = N * P * args.steps / sec
 
     # per-stage device times, every stage alone on one stream (HIP events)
-    stages = np.zeros(10)
+    stages = np.zeros(11)
     assert lib.md_pimd_profile_steps_x(s.pimd, C.byref(cfg), 5, 1, pyxmd.dptr(stages)) == 0
     a_, rc_, nk_ = C.c_double(), C.c_double(), C.c_int()
     assert lib.md_pimd_get_ewald_plan_x(s.pimd, C.byref(a_), C.byref(rc_), C.byref(nk_)) == 0
@@ -353,13 +352,9 @@ def main():
     work = step_work(N, P, xi, alpha, L, rcut, nk)
     rec_ms = stages[9]
     nblk = (N + 63) // 64
-    # the real-space kernel alone (its partial-force reduction is part of
-    # stage 4): every launch choice timed with HIP events, 20 launches each;
-    # the step uses the most specific image enumeration with 4 waves
-    choice_ms = np.zeros(12)
-    chosen = C.c_int()
-    assert lib.md_pimd_autotune_x(s.pimd, 20, pyxmd.dptr(choice_ms), C.byref(chosen)) == 0
-    real_ms = float(next(choice_ms[4 * img + 2] for img in range(3) if choice_ms[4 * img + 2] > 0))
+    # the real-space kernel alone (stage 10: HIP events around it on its
+    # stream in the serial profile, without its partial-force reduction)
+    real_ms = float(stages[10]) if stages[10] > 0 else float(stages[4])
     kernels = {}
     for name, key, ms in (("k_ewald_real", "ewald_real", real_ms),
                           ("k_rec_persist (V_B forward + backward recursion)", "recursions", rec_ms),
@@ -385,8 +380,7 @@ def main():
         kname, kwork, kms = "k_ewald_real", work["ewald_real"], real_ms
         roof_extra = {"kernel": "k_ewald_real (real-space Ewald, 64x64 bead tile pairs, both images of each pair)",
                       "limiter": "FP64 VALU issue and LDS table lookups",
-                      "work_unit": f"pairs x mean images in r_c x {FLOP_PER_IMAGE:.0f} flop-eq",
-                      "launch_choice_ms": [round(float(x), 5) for x in choice_ms]}
+                      "work_unit": f"pairs x mean images in r_c x {FLOP_PER_IMAGE:.0f} flop-eq"}
     else:
         kname, kwork, kms = "k_rec_persist", work["recursions"], rec_ms
         roof_extra = {"kernel": "k_rec_persist (V_B forward + backward recursion, both directions in one launch)",

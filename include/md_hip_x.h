@@ -241,8 +241,10 @@ int md_set_ewald_real_variant_x(int img, int wlog);
  *   [3] reset/trap/ring springs [4] pair real space  [5] pair reciprocal
  *   [6] exchange end-bead forces [7] NHC second half [8] whole step (wall)
  *   [9] the persistent recursion solve alone (part of [2]; serial only)
+ *   [10] the tabulated real-space Ewald kernel alone, without its partial-
+ *        force reduction (part of [4]; serial only; 0 for other pair forces)
  * Each stage is bracketed by events on the stream it runs on. */
-#define MD_PIMD_NSTAGES_X 10
+#define MD_PIMD_NSTAGES_X 11
 int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int nsteps, int serial,
                             double *stage_ms);
 

@@ -1538,6 +1538,19 @@ RealChoice real_choice(int N, int P, const PairParams &pp) {
 
 // Table-driven real space: per-tile partial forces (k_ewald_real), then
 // their sum into F (k_reduce_tiles) unless `reduce` is false.
+// Event recorded after the tabulated real-space kernel (before its partial-
+// force reduction) while a serial stage profile runs; g_real_probe_hit tells
+// the profile that it was.
+static hipEvent_t g_real_probe = nullptr;
+static bool g_real_probe_hit = false;
+
+void set_real_space_probe(hipEvent_t e) {
+  g_real_probe = e;
+  g_real_probe_hit = false;
+}
+
+bool real_space_probe_hit() { return g_real_probe_hit; }
+
 static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t st, bool reduce = true) {
   DevSim *d = p->ds;
   if (ensure_tiles(p)) return -1;
@@ -1556,6 +1569,10 @@ static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t s
   hipLaunchKernelGGL(k_ewald_real_of[ch.img][ch.wlog], dim3((items + wpb - 1) / wpb), dim3(64 * wpb), 0, st, d->X, d->M,
                      B.Fpart, B.tiles, p->N, d->nb, B.T, B.ntp, items, ep);
   XMD_CHECK(hipGetLastError());
+  if (g_real_probe) {
+    XMD_CHECK(hipEventRecord(g_real_probe, st));
+    g_real_probe_hit = true;
+  }
   if (!reduce) return 0;
   const int n3 = 3 * d->nb;
   hipLaunchKernelGGL(k_reduce_tiles, dim3((n3 + 255) / 256), dim3(256), 0, st, B.Fpart, d->F, n3, B.T, pp.P);

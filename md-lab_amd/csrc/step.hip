@@ -384,14 +384,18 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
                             double *stage_ms) {
   StepPlan sp;
   if (plan_of(pimd, cfg, sp)) return -1;
-  const int NE = 17;
+  const int NE = 18;
   hipEvent_t ev[NE];
   for (int k = 0; k < NE; ++k) XMD_CHECK(hipEventCreate(&ev[k]));
   for (int k = 0; k < MD_PIMD_NSTAGES_X; ++k) stage_ms[k] = 0;
   // stage k <- events (a, b)
   const int pairs[8][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {10, 11}, {12, 13}, {14, 15}};
   for (int s = 0; s < nsteps; ++s) {
-    if (enqueue_step(sp, ev, serial != 0, main_lane())) return -1;
+    if (serial) set_real_space_probe(ev[17]);
+    const int bad = enqueue_step(sp, ev, serial != 0, main_lane());
+    const bool probe = serial && real_space_probe_hit();
+    set_real_space_probe(nullptr);
+    if (bad) return -1;
     XMD_CHECK(hipStreamSynchronize(rt().s0));
     if (recursion_status(sp.p)) return -1;
     for (int k = 0; k < 8; ++k) {
@@ -405,6 +409,8 @@ int md_pimd_profile_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg,
     if (serial) {
       XMD_CHECK(hipEventElapsedTime(&ms, ev[16], ev[5]));
       stage_ms[9] += ms;
+      if (probe) XMD_CHECK(hipEventElapsedTime(&ms, ev[8], ev[17]));
+      stage_ms[10] += probe ? ms : 0.0f;
     }
   }
   for (int k = 0; k < MD_PIMD_NSTAGES_X; ++k) stage_ms[k] /= nsteps;

@@ -250,6 +250,10 @@ int noalloc_init(int device);
 int inject_recursion_timeouts(DevPimd *p, int count);
 // Range limit of the chain-frame block solve (test hook; <= 0: default).
 void set_recursion_frame_limit(double limit);
+// Serial stage profile: event recorded after the tabulated real-space Ewald
+// kernel (nullptr: none), and whether a launch recorded it since it was set.
+void set_real_space_probe(hipEvent_t e);
+bool real_space_probe_hit();
 
 // ------------------------------------------------------ batched replicas
 // One launch per step stage covers every replica of a batch: each stage has

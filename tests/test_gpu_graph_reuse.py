@@ -70,7 +70,7 @@ def test_concurrent_step_matches_serial_order(kind, N, P):
     conc, ser = make(lib, kind, N, P, 5), make(lib, kind, N, P, 5)
     cc = conc.step_config(0.01, use_graph=1)
     cs = ser.step_config(0.01, use_graph=0)
-    stages = np.zeros(10)
+    stages = np.zeros(11)
     assert lib.md_pimd_run_steps_x(conc.pimd, C.byref(cc), 37) == 0
     assert lib.md_pimd_profile_steps_x(ser.pimd, C.byref(cs), 37, 1, pyxmd.dptr(stages)) == 0
     assert lib.md_hip_synchronize_x() == 0
