@@ -10,12 +10,14 @@ of 5 x 10^4 steps (20 per trajectory): the estimators stay correlated over
 ~10^4 steps (the reference's own block means drift for ~2 x 10^5 steps after
 the random start), so shorter blocks understate the error.
 Trajectories separate chaotically after a few hundred steps, so only
-averages can agree.  Single 10^6-step trajectories of the engine (seeds
-7 .. 10, with this and with an earlier recursion) scatter 1.2-1.5 times
-wider than their block standard errors, and one of them sat 3.6 sigma from
-the reference on all four correlated estimators at once; so the engine's
-mean over both trajectories must lie within 3 sigma of the reference's,
-sigma combining both block standard errors.
+averages can agree: the engine's mean over both trajectories must lie
+within 2 sigma of the reference's, sigma combining both block standard
+errors.  (Single 10^6-step trajectories of the engine, seeds 7 .. 10 with
+this and with an earlier recursion, scatter 1.2-1.5 times wider than their
+block standard errors, and one sat 2.4-3.6 sigma from the reference on all
+four correlated estimators at once; averaging two trajectories halves that
+scatter.  Both trajectories are deterministic, so the outcome does not vary
+between runs of the same build.)
 """
 import ctypes as C
 import json
@@ -81,4 +83,4 @@ def test_estimator_averages_trap16():
         zs[name] = (m - r["mean"]) / sig
         print(f"{name}: engine {m:.5f} +- {sem:.5f}, reference {r['mean']:.5f} +- {r['sem']:.5f}, z = {zs[name]:+.2f}")
     for name, z in zs.items():
-        assert abs(z) < 3.0, (name, z)
+        assert abs(z) < 2.0, (name, z)
