@@ -25,8 +25,8 @@
 // particle contiguous) for the backward pass and the exchange-force columns.
 //
 // The recursions are solved as a blocked right-looking triangular solve in
-// log-sum-exp form (see the block-solve section below): both directions
-// share each launch, one launch per 64 positions.
+// log-sum-exp form (see the block-solve section below): both directions in
+// one persistent launch, 64 positions per block.
 #include <cmath>
 
 #include "xmd_device.h"
@@ -35,7 +35,7 @@
 namespace xmd {
 
 constexpr double NEG = -1.0e300;  // "no term" marker (exp(NEG - x) == 0)
-constexpr int RB = 64;             // recursion block (rows solved per diagonal launch)
+constexpr int RB = 64;             // recursion block (rows per diagonal block)
 constexpr int RT = 1024;           // estimator workgroup
 constexpr double LSE_SKIP = 40.0;  // log-sum-exp terms below max - 40 are < 4.3e-18 relative
 
@@ -1503,13 +1503,12 @@ int launch_add_exchange_force(DevPimd *p, hipStream_t st) {
 // holds in device memory, with all of its working memory in the file-scope
 // array `scratch` below instead of buffers the library allocates.  Layout
 // (doubles): the column copy of the cycle energies, ln n, the forward and
-// backward values, the solve state (values, accumulators, block
-// coefficients of both directions), then the hand-off flags as ints.  The
-// flags are placed so that the error word and the injected-timeout count sit
-// at fixed positions for every N (the flags of a smaller N end where those of
-// N_max end), so a call never mistakes an earlier call's hand-off flags for
-// an error; the per-solve set-up clears the flags, the host clears the error
-// word when it reports it.
+// backward values, the solve state (values, accumulators, hand-off values,
+// block coefficients of both directions), then the sync words as ints.  They
+// are placed so that the error word and the injected-timeout count sit at
+// fixed positions for every N (the 4 nblk unused words of a smaller N end
+// where those of N_max end); the host clears the error word when it reports
+// it.
 constexpr int NA_NMAX = 1024;
 constexpr int NA_NBLK = (NA_NMAX + RB - 1) / RB;
 constexpr size_t NA_ET = 0;

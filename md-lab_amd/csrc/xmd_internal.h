@@ -103,7 +103,7 @@ struct DevPimd {
   double *work = nullptr;   // estimator scratch
   double *Fx = nullptr;     // exchange forces of the end beads, [slice P-1, slice 0][dim][particle] (6N)
   double *rec = nullptr;    // recursion state: values, accumulators, block coefficients
-  int *recsync = nullptr;   // recursion hand-off flags 4*nblk, error word, injected-timeout count
+  int *recsync = nullptr;   // recursion sync words: 4*nblk unused, error word, injected-timeout count
   int *done = nullptr;      // finished-workgroup count of k_chain_scan (zero between launches)
   void *pair_bufs = nullptr;  // pair-kernel tile schedule, partial sums and k-space tables (pair_forces.hip)
   void *step_exec = nullptr;  // executable step graph of the last md_pimd_run_steps_x (hipGraphExec_t)
