@@ -597,11 +597,13 @@ __device__ __forceinline__ void rec_poll(const double *p, const double *q, int n
 
 // Folds blocks 0 .. G-2 into row block G's accumulators (lane = row, the
 // four waves split each block's 64 columns, 16 each), in block order,
-// starting from column 0's (M, S); wave 0 returns the result.  Each wave
-// polls its 16 columns' values itself; the next block's coefficients are in
-// flight while a block is folded; the four partials of a block merge in
-// wave 0 through double-buffered LDS (one barrier per block).
-template <bool FWD>
+// starting from column 0's (M, S); wave 0 returns the result.  POLL (an
+// updater): each wave polls its 16 columns' values itself; otherwise this
+// workgroup solved the blocks (its barriers order those stores before these
+// plain loads).  The next block's coefficients are in flight while a block
+// is folded; the four partials of a block merge in wave 0 through
+// double-buffered LDS (one barrier per block).
+template <bool FWD, bool POLL>
 __device__ __forceinline__ void rec_fold_rows(const RecArgs &a, int dir, int G, double (*pm)[4][RB],
                                               double (*ps)[4][RB], double &Ma, double &Sa) {
   const int N = a.N;
@@ -622,7 +624,10 @@ __device__ __forceinline__ void rec_fold_rows(const RecArgs &a, int dir, int G, 
   for (int b = 0; b <= G - 2; ++b) {
     const int pprev = 1 + b * RB;
     double v, unused;
-    rec_poll(val + pprev + 16 * wv, nullptr, 16, err, REC_SPIN_MAX, v, unused);
+    if (POLL)
+      rec_poll(val + pprev + 16 * wv, nullptr, 16, err, REC_SPIN_MAX, v, unused);
+    else
+      v = lane < 16 ? val[pprev + 16 * wv + lane] : 0.0;
     double cn[16];
     rec_coef_col16<FWD>(a, p, pprev + RB + 16 * wv, act && b + 1 <= G - 2, cn);
     double t[16];
@@ -650,7 +655,7 @@ template <bool FWD>
 __device__ __forceinline__ void rec_updater(const RecArgs &a, int dir, int G, double (*pm)[4][RB],
                                             double (*ps)[4][RB]) {
   double Ma, Sa;
-  rec_fold_rows<FWD>(a, dir, G, pm, ps, Ma, Sa);
+  rec_fold_rows<FWD, true>(a, dir, G, pm, ps, Ma, Sa);
   const int p = 1 + G * RB + (threadIdx.x & 63);
   if (threadIdx.x < 64 && p <= a.N) {
     st_handoff(a.finM + (size_t)dir * (a.N + 1) + p, Ma);
@@ -839,7 +844,7 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
     }
     // SOLO: block b+1's accumulators over blocks 0 .. b-1, folded here
     double Mf = NEG, Sf = 0.0;
-    if (SOLO && b + 1 >= 2) rec_fold_rows<FWD>(a, dir, b + 1, S.pm, S.ps, Mf, Sf);
+    if (SOLO && b + 1 >= 2) rec_fold_rows<FWD, false>(a, dir, b + 1, S.pm, S.ps, Mf, Sf);
     __syncthreads();
     if (wv == 0) {
       double Mn, Sn;
