@@ -165,12 +165,13 @@ __global__ __launch_bounds__(256) void k_scan_dd(const double *__restrict__ in, 
   scan_dd_body(in, n, pre);
 }
 
-// In the concurrent step the cycle energies, the recursion set-up and the
-// end-bead exchange forces lie on the longest chain of dependent kernels,
-// while the pair-force grids fill the compute units beside them: the waves
-// of these short kernels take the highest issue priority (measured 0.267 ->
-// 0.249 ms per step for UEG N=1024 P=32; the same for the persistent
-// recursion itself made no difference).
+// In the concurrent step the cycle energies, the recursion set-up, the
+// recursion and the end-bead exchange forces form a chain of dependent
+// kernels while the pair-force grids fill the compute units beside them: the
+// waves of these kernels take the highest issue priority (measured 0.267 ->
+// 0.249 ms per step for UEG N=1024 P=32 for the short kernels; for the
+// persistent recursion 0.235 -> 0.226 ms once the faster k-space kernels had
+// put it on the longest chain).
 __device__ __forceinline__ void chain_priority() { __builtin_amdgcn_s_setprio(3); }
 
 // k_chain and k_scan_dd in one launch: the last workgroup to finish its
@@ -884,6 +885,7 @@ __device__ __forceinline__ void rec_solver(const RecArgs &a, int dir, RecLds &S)
 
 __global__ __launch_bounds__(256) void k_rec_persist(RecArgs a, int dir0) {
   __shared__ RecLds S;
+  chain_priority();
   const int per = 1 + max(0, a.nblk - 2);
   const int dir = dir0 + blockIdx.x / per, role = blockIdx.x % per;
   if (role == 0) {

@@ -2,20 +2,26 @@
 //
 // Stream layout of one step (the reference driver's order of stages):
 //   s0: NHC half 1 (+ wrap) --fork2--> cycle energies --fork--> reset F,
-//       trap, ring springs, pair real --wait join, join2--> add the pair
-//       and exchange end-bead forces, NHC half 2 (one kernel with massive
-//       thermostats)
+//       trap, ring springs (pair forces here unless they are Ewald) --wait
+//       join, join2--> add the pair and exchange end-bead forces, NHC half 2
+//       (one kernel with massive thermostats)
 // With massive thermostats NHC half 1 and the wrap are one kernel, and so are
 // the force reset, trap and interior springs.
 //   s1: (after fork) forward + backward recursions, end-bead exchange forces
 //       into a side buffer --join--> (added into F on s0)
-//   s2: (Ewald) phase tables, structure factors and reciprocal forces into a
-//       side buffer, forked from s0 right after NHC half 1 (they read
-//       positions only, so they overlap the cycle energies) --join2--> s0
-//       then adds the real-space, reciprocal and exchange forces
+//   s2: (Ewald) real-space tile sums, then phase tables, structure factors
+//       and reciprocal forces into a side buffer, forked from s0 right after
+//       NHC half 1 (they read positions only) --join2--> s0 then adds the
+//       real-space, reciprocal and exchange forces
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces are added into F after both branches are done
 // (the serial profiling order computes and adds them there in one kernel).
+// The real-space grid goes first on s2: it fills the device while the short
+// cycle-energy kernels (highest wave priority) run beside it, and the
+// recursions then overlap the matrix-core k-space kernels, which leave them
+// more of the compute units than the real-space waves do (UEG N=1024 P=32:
+// 0.219 -> 0.199 ms per step against the real-space grid on s0 after the
+// recursion fork).
 //
 // Batches of independent replicas (md_pimd_run_batch_x) run the same stages
 // with one launch per stage over a packed table of all replicas, captured
@@ -110,7 +116,7 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   // Reciprocal space (positions only) forks onto s2 first: the phase tables
   // and structure factors overlap the cycle energies.
   const int kfork =
-      (!serial && sp.pair_kind >= 0) ? launch_pair_kspace_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
+      (!serial && sp.pair_kind >= 0) ? launch_pair_ewald_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
   if (kfork < 0) return -1;
   // Cycle energies (short, position-only); then the two recursions are
   // forked onto s1 before the real-space grid is queued, so their
@@ -140,7 +146,6 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   }
   if (rec(7, R.s0)) return -1;
   if (kfork) {
-    if (launch_pair_real_partial(p, sp.pair_kind, R.s0)) return -1;
     if (rec(8, R.s0) || rec(9, R.s0) || rec(10, R.s0) || rec(11, R.s0)) return -1;
   } else if (sp.pair_kind >= 0) {
     if (rec(8, R.s0) || launch_pair_real(p, sp.pair_kind, R.s0) || rec(9, R.s0)) return -1;

@@ -186,18 +186,16 @@ unsigned pair_alloc_epoch();
 // Real-space Ewald launch choice autotuning (pair_forces.hip): with it on,
 // tune the object's (N, P) bucket before first use (never during capture).
 int autotune_if_needed(DevPimd *p, int kind);
-// Pair forces in parts, with reciprocal space on a side stream:
-// launch_pair_kspace_fork forks s2 from st (through `fork`) and queues the
-// phase tables, structure factors and reciprocal forces there (they read
-// positions only; the forces go to a side buffer), recording `join`;
-// launch_pair_real_partial queues the real-space tile kernel on st; after
-// both (and the exchange branch) launch_combine_forces adds the real-space,
-// reciprocal and, when Fx is given, exchange end-bead forces into F in the
-// order of the unsplit kernels.  The fork returns 1 when it applies (Ewald
-// with the table kernels), 0 when nothing was launched (use
+// Ewald forces on a side stream: launch_pair_ewald_fork forks s2 from st
+// (through `fork`) and queues there the real-space tile kernel (per-tile
+// partial forces), then the phase tables, structure factors and reciprocal
+// forces (into a side buffer); all read positions only.  It records `join`;
+// after it (and the exchange branch) launch_combine_forces adds the
+// real-space, reciprocal and, when Fx is given, exchange end-bead forces into
+// F in the order of the unsplit kernels.  The fork returns 1 when it applies
+// (Ewald with the table kernels), 0 when nothing was launched (use
 // launch_pair_force), -1 on error.
-int launch_pair_kspace_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
-int launch_pair_real_partial(DevPimd *p, int kind, hipStream_t st);
+int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
 int launch_combine_forces(DevPimd *p, const double *Fx, hipStream_t st);
 // The parts launch_combine_forces adds, for a fused consumer of F.
 int force_sums(DevPimd *p, const double *Fx, ForceSums *out);
