@@ -451,7 +451,7 @@ __global__ void k_reduce_tiles(const double *__restrict__ Fpart, double *__restr
 // columns of a tile (consecutive m_y, mostly one m_x) read one bead's
 // contiguous run:
 //   tx[s * (MZ + 1) + m]        e^{i 2 pi m x / L_x},  m = 0..MZ
-//   ty[s * (2 MZ + 1) + m + MZ] e^{i 2 pi m y / L_y},  m = -MZ..MZ
+//   ty[s * (MZ + 1) + m]        e^{i 2 pi m y / L_y},  m = 0..MZ (m_y < 0: the conjugate)
 //   tz[s * (MZ + 1) + m]        e^{i 2 pi m z / L_z},  m = 0..MZ
 constexpr int MAX_KM = 31;     // largest table half-width of the k kernels
 
@@ -626,47 +626,39 @@ static KSpace *ensure_kspace(DevPimd *p, const double *L, double alpha, double q
     K->e0 = csum / (4 * M_PI);
   }
   K->stamp = ++pb->ks_clock;
-  if ((long long)nb * (4 * K->MZ + 3) > INT_MAX) {
+  if ((long long)nb * 3 * (K->MZ + 1) > INT_MAX) {
     fprintf(stderr, "xmd: %d beads exceed the Ewald phase-table index range\n", nb);
     return nullptr;
   }
   const size_t slots = (size_t)K->ncolp * K->ZP;
-  if (grow(K->tx, K->tx_cap, (size_t)(K->MZ + 1) * nb) || grow(K->ty, K->ty_cap, (size_t)(2 * K->MZ + 1) * nb) ||
+  if (grow(K->tx, K->tx_cap, (size_t)(K->MZ + 1) * nb) || grow(K->ty, K->ty_cap, (size_t)(K->MZ + 1) * nb) ||
       grow(K->tz, K->tz_cap, (size_t)(K->MZ + 1) * nb) || grow(K->S, K->s_cap, (size_t)P * slots) ||
       grow(K->W, K->w_cap, (size_t)P * slots))
     return nullptr;
   return K;
 }
 
-// e^{i pi m u}, u = 2 x / L, one thread per table entry: threads run over
-// tx, then ty, then tz, each in storage order, so the stores of a wave are
-// consecutive.  Negative m take the conjugate of the |m| entry.
-__host__ __device__ inline int phase_entries(int nb, int MZ) { return nb * (4 * MZ + 3); }
+// e^{i pi m u}, u = 2 x / L, m = 0..MZ, one thread per table entry: threads
+// run over tx, then ty, then tz, each in storage order, so the stores of a
+// wave are consecutive.
+__host__ __device__ inline int phase_entries(int nb, int MZ) { return 3 * nb * (MZ + 1); }
 
 __device__ __forceinline__ void k_phase_body(int bx, const double *__restrict__ X, int nb, int MZ, double L0, double L1,
                                              double L2, double2 *__restrict__ tx, double2 *__restrict__ ty,
                                              double2 *__restrict__ tz) {
-  int t = bx * blockDim.x + threadIdx.x;
-  const int nx = nb * (MZ + 1), ny = nb * (2 * MZ + 1), nz = nb * (MZ + 1);
-  int d, s, m;
-  double2 *out;
-  if (t < nx) {
-    d = 0, s = t / (MZ + 1), m = t - s * (MZ + 1), out = tx + t;
-  } else if ((t -= nx) < ny) {
-    d = 1, s = t / (2 * MZ + 1), m = t - s * (2 * MZ + 1) - MZ, out = ty + t;
-  } else if ((t -= ny) < nz) {
-    d = 2, s = t / (MZ + 1), m = t - s * (MZ + 1), out = tz + t;
-  } else {
-    return;
-  }
+  const int t = bx * blockDim.x + threadIdx.x;
+  const int n1 = nb * (MZ + 1);
+  if (t >= 3 * n1) return;
+  const int d = t / n1, e = t - d * n1, s = e / (MZ + 1), m = e - s * (MZ + 1);
+  double2 *out = (d == 0 ? tx : (d == 1 ? ty : tz)) + e;
   if (m == 0) {
     *out = make_double2(1.0, 0.0);
     return;
   }
   const double u = 2.0 * X[d * nb + s] / (d == 0 ? L0 : (d == 1 ? L1 : L2));
   double sn, cs;
-  sincospi((m < 0 ? -m : m) * u, &sn, &cs);
-  *out = make_double2(cs, m < 0 ? -sn : sn);
+  sincospi(m * u, &sn, &cs);
+  *out = make_double2(cs, sn);
 }
 
 __global__ void k_phase(const double *__restrict__ X, int nb, int MZ, double L0, double L1, double L2,
@@ -707,10 +699,11 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
   const int mb = lane & 15;  // B operand column: m within an m tile
   const int q0 = (N * wv) / 4, q1 = (N * (wv + 1)) / 4;
   const int nfull = (q1 - q0) >> 2;
-  const int sx = 4 * (MZ + 1), sy = 4 * (2 * MZ + 1);
+  const int sx = 4 * (MZ + 1);
   const size_t s0 = (size_t)slice * N + q0 + kb;
   const double2 *px = tx + s0 * (MZ + 1) + cl.x;
-  const double2 *py = ty + s0 * (2 * MZ + 1) + MZ + cl.y;
+  const bool yneg = cl.y < 0;  // e^{-i m y} = conj(e^{i m y})
+  const double2 *py = ty + s0 * (MZ + 1) + (yneg ? -cl.y : cl.y);
   const double2 *pz = tz + s0 * MH + mb;
   // m tiles beyond the column tile's largest |m_z| (kr.y: its last 4-wide
   // step) hold zero weights: their sums are not formed (S and PQ stay zero)
@@ -720,6 +713,7 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
 #pragma unroll
   for (int t = 0; t < NMT; ++t) xcr[t] = xci[t] = xsr[t] = xsi[t] = v4d{0, 0, 0, 0};
   auto step = [&](double2 ex, double2 ey, const double2 *z, bool keep) {
+    if (yneg) ey.y = -ey.y;
     double ar = ex.x * ey.x - ex.y * ey.y, ai = ex.x * ey.y + ex.y * ey.x;
     if (!keep) ar = ai = 0.0;
 #pragma unroll
@@ -737,12 +731,12 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       ex[u] = px[u * sx];
-      ey[u] = py[u * sy];
+      ey[u] = py[u * sx];
 #pragma unroll
       for (int t = 0; t < NMT; ++t) z[u][t] = zok(t) ? pz[u * 4 * MH + 16 * t] : make_double2(0.0, 0.0);
     }
     px += 4 * sx;
-    py += 4 * sy;
+    py += 4 * sx;
     pz += 16 * MH;
 #pragma unroll
     for (int u = 0; u < 4; ++u) step(ex[u], ey[u], z[u], true);
@@ -753,7 +747,7 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
     for (int t = 0; t < NMT; ++t) z[t] = zok(t) ? pz[16 * t] : make_double2(0.0, 0.0);
     step(px[0], py[0], z, true);
     px += sx;
-    py += sy;
+    py += sx;
     pz += 4 * MH;
   }
   if ((q1 - q0) & 3) {
@@ -764,7 +758,7 @@ __device__ __forceinline__ void sk_tile(double *red, int slice, int c0, const do
     double2 z[NMT];
 #pragma unroll
     for (int t = 0; t < NMT; ++t) z[t] = zok(t) ? pz[16 * t - back * MH] : make_double2(0.0, 0.0);
-    step(px[-back * (MZ + 1)], py[-back * (2 * MZ + 1)], z, ok);
+    step(px[-back * (MZ + 1)], py[-back * (MZ + 1)], z, ok);
   }
   // D[i = column][j = m]: lane j + 16 (i%4), register i/4
   const size_t base = (size_t)slice * ncolp * ZP;
@@ -872,17 +866,17 @@ __device__ __forceinline__ void recip_force_tile(FrcLds<NZT> &sh, int slice, int
                                                  double *__restrict__ Erec = nullptr, double e0 = 0.0) {
   using S = FrcLds<NZT>;  // MZ = 8 NZT - 1 (table_mz)
   (void)MZ;
-  constexpr int ZP = S::ZP, MH = S::RX, KS = MH / 4, RX = S::RX, RY = 2 * S::MZ + 1, WS = S::WS, TE = 16 * ZP;
+  constexpr int ZP = S::ZP, MH = S::RX, KS = MH / 4, RX = S::RX, WS = S::WS, TE = 16 * ZP;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nloc = min(64, N - b0);  // beads of this workgroup
   // phase rows of the workgroup's beads (zero past the last bead)
   {
     const double2 *gx = tx + (size_t)(slice * N + b0) * RX;
-    const double2 *gy = ty + (size_t)(slice * N + b0) * RY;
-    for (int e = tid; e < 64 * RX; e += 256) sh.rx[e] = e < nloc * RX ? gx[e] : make_double2(0.0, 0.0);
+    const double2 *gy = ty + (size_t)(slice * N + b0) * RX;
     for (int e = tid; e < 64 * RX; e += 256) {
-      const int bl = e / RX;
-      sh.ry[e] = bl < nloc ? gy[bl * RY + S::MZ + (e - bl * RX)] : make_double2(0.0, 0.0);
+      const bool ok = e < nloc * RX;
+      sh.rx[e] = ok ? gx[e] : make_double2(0.0, 0.0);
+      sh.ry[e] = ok ? gy[e] : make_double2(0.0, 0.0);
     }
   }
   const double2 *Ws = PQ + (size_t)slice * ncolp * ZP;
@@ -1649,9 +1643,9 @@ static void (*const k_frc_of[4])(const double2 *, const double2 *, const double2
                                  double, double *) = {k_recip_force_mfma<1>, k_recip_force_mfma<2>, k_recip_force_mfma<3>,
                                             k_recip_force_mfma<4>};
 
-// Phase tables and structure factors S, W of every slice (reads X only).
-static int launch_recip_sk(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st,
-                           KSpace **out) {
+// Phase tables of every slice (reads X only) for the k set of pp.
+static int launch_recip_phase(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st,
+                              KSpace **out) {
   DevSim *d = p->ds;
   KSpace *K = ensure_kspace(p, pp.L, pp.alpha, pp.q, cube_n, gmax2, d->nb, p->P);
   if (!K) return -1;
@@ -1659,10 +1653,22 @@ static int launch_recip_sk(DevPimd *p, const PairParams &pp, int cube_n, double 
   hipLaunchKernelGGL(k_phase, dim3((phase_entries(d->nb, K->MZ) + 255) / 256), dim3(256), 0, st, d->X, d->nb, K->MZ,
                      pp.L[0], pp.L[1], pp.L[2], K->tx, K->ty, K->tz);
   XMD_CHECK(hipGetLastError());
+  return 0;
+}
+
+// Structure factors S and the force operands of every slice from the phase tables.
+static int launch_sk(DevPimd *p, const KSpace *K, hipStream_t st) {
   hipLaunchKernelGGL(k_sk_of[K->ZP / 16 - 1], dim3((K->ncolp / 16) * p->P), dim3(256), 0, st, K->tx, K->ty, K->tz,
                      K->cols, K->coef, K->ncolp, K->MZ, p->N, p->P, K->S, K->W);
   XMD_CHECK(hipGetLastError());
   return 0;
+}
+
+// Phase tables, then structure factors S and the force operands (reads X only).
+static int launch_recip_sk(DevPimd *p, const PairParams &pp, int cube_n, double gmax2, hipStream_t st,
+                           KSpace **out) {
+  if (launch_recip_phase(p, pp, cube_n, gmax2, st, out)) return -1;
+  return launch_sk(p, *out, st);
 }
 
 // Reciprocal forces from the tables and W of launch_recip_sk, added into F
@@ -1841,7 +1847,7 @@ double ewald_split_alpha(DevPimd *p) {
   return ewald_params(p, pl).alpha;
 }
 
-int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join) {
+int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork) {
   if (kind != PK_EWALD || !p->ds->uniform_mass) return 0;
   EwaldPlan pl;
   PairParams pp = ewald_params(p, pl);
@@ -1853,17 +1859,27 @@ int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2,
   // kernels run beside it, so the recursions start next to the matrix-core
   // k-space kernels rather than the real-space waves
   if (launch_ewald_real_tab(p, pp, s2, false)) return -1;
+  return 1;
+}
+
+int launch_pair_kspace(DevPimd *p, hipStream_t st, hipStream_t s2, hipEvent_t phase, hipEvent_t join) {
   if (p->sur_mode) {
     // the learned stand-in (or the judge step: exact terms plus the
     // surrogate's for comparison) fills the same reciprocal-force buffer
     if (surrogate_kspace(p, pair_bufs(p)->tb.Frec, s2)) return -1;
   } else {
+    // the phase tables run on st beside the end of the real-space grid (a
+    // short grid that fills its tail), the structure factors wait for both
+    EwaldPlan pl;
+    PairParams pp = ewald_params(p, pl);
     KSpace *K = nullptr;
-    if (launch_recip_sk(p, pp, pl.cube_n, pl.gmax2, s2, &K)) return -1;
-    if (launch_recip_force(p, pp, K, s2, pair_bufs(p)->tb.Frec)) return -1;
+    if (launch_recip_phase(p, pp, pl.cube_n, pl.gmax2, st, &K)) return -1;
+    XMD_CHECK(hipEventRecord(phase, st));
+    XMD_CHECK(hipStreamWaitEvent(s2, phase, 0));
+    if (launch_sk(p, K, s2) || launch_recip_force(p, pp, K, s2, pair_bufs(p)->tb.Frec)) return -1;
   }
   XMD_CHECK(hipEventRecord(join, s2));
-  return 1;
+  return 0;
 }
 
 // F += real-space tile sums / P, then the reciprocal forces, then (when Fx
