@@ -326,18 +326,21 @@ struct BeadRep {
 };
 
 // NHC half-kick (+ drift for HALF 1), then the box wrap after HALF 1, for
-// degree of freedom e < 3 nb.
+// degree of freedom e < 3 nb.  HALF 3: half 2 of one step and half 1 of the
+// next in one pass (the force does not change between them; every chain is
+// its own degree of freedom's, so the arithmetic is that of the two kernels).
 template <int HALF>
 __device__ __forceinline__ void nhc_wrap_body(const BeadRep &R, int e) {
   double f;
-  if (HALF == 2 && R.cs.Fpart) {
+  if (HALF != 1 && R.cs.Fpart) {
     f = combined_force(R.F, R.cs, 3 * R.nb, e);
     R.F[e] = f;
   } else {
     f = R.F[e];
   }
-  nhc_massive_body<HALF>(R.X, R.V, f, R.M, R.th, R.vth, R.Q, R.nb, R.Mlen, R.T, R.h, e);
-  if (HALF == 1 && R.pbc) {
+  if (HALF != 1) nhc_massive_body<2>(R.X, R.V, f, R.M, R.th, R.vth, R.Q, R.nb, R.Mlen, R.T, R.h, e);
+  if (HALF != 2) nhc_massive_body<1>(R.X, R.V, f, R.M, R.th, R.vth, R.Q, R.nb, R.Mlen, R.T, R.h, e);
+  if (HALF != 2 && R.pbc) {
     const int d = e / R.nb;
     const double L = R.L[d];
     double x = R.X[e];
@@ -443,12 +446,14 @@ static BeadRep bead_rep(const BatchItem &it) {
 // NHC half (+ wrap after half 1), or reset + trap + interior springs.
 int launch_step_nhc(const BatchItem &it, int half, hipStream_t st, const ForceSums *cs) {
   BeadRep R = bead_rep(it);
-  if (half == 2 && cs) R.cs = *cs;
+  if (half != 1 && cs) R.cs = *cs;
   const dim3 g((3 * R.nb + 255) / 256);
   if (half == 1)
     hipLaunchKernelGGL(k_nhc_wrap<1>, g, dim3(256), 0, st, R);
-  else
+  else if (half == 2)
     hipLaunchKernelGGL(k_nhc_wrap<2>, g, dim3(256), 0, st, R);
+  else
+    hipLaunchKernelGGL(k_nhc_wrap<3>, g, dim3(256), 0, st, R);
   XMD_CHECK(hipGetLastError());
   return 0;
 }

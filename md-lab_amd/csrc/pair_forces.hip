@@ -1251,12 +1251,7 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
   double rx = 0, ry = 0, rz = 0;
   const bool diag = I == J, full = J * 64 + 64 <= N;
   double fx = 0, fy = 0, fz = 0;
-  // a diagonal tile takes s = 1..31 and only lanes t < 32 at s = 32 (each
-  // pair once); its remaining steps only rotate the records home
-  for (int s = 0; s < 64; ++s) {
-    const int j = (t + s) & 63;
-    bool on = act && (full || J * 64 + j < N);
-    if (diag) on = on && (s >= 1 && s <= 32) && (s < 32 || t < 32);
+  auto step = [&](bool on) {
     if (on) {
       double gx = 0, gy = 0, gz = 0;
       ewald_pair_tab<IMG>(xi - jx, yi - jy, zi - jz, ep, tab, gx, gy, gz);
@@ -1276,6 +1271,19 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
     rx = wave_rol1(rx);
     ry = wave_rol1(ry);
     rz = wave_rol1(rz);
+  };
+  // an off-diagonal pair of full tiles meets a partner on every lane at
+  // every step; a diagonal tile takes s = 1..31 and only lanes t < 32 at
+  // s = 32 (each pair once), its remaining steps only rotate the records home
+  const bool every = !diag && full && I * 64 + 64 <= N;  // wave-uniform
+  for (int s = 0; s < 64; ++s) {
+    bool on = true;
+    if (!every) {
+      const int j = (t + s) & 63;
+      on = act && (full || J * 64 + j < N);
+      if (diag) on = on && (s >= 1 && s <= 32) && (s < 32 || t < 32);
+    }
+    step(on);
   }
   auto put = [&](int bead, int slot, double vx, double vy, double vz) {
     const size_t s0 = (size_t)slot * 3 * nb + base + bead;

@@ -180,10 +180,15 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   return 0;
 }
 
-static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R) {
+// head: NHC half 1 starts the step (false: the previous step's chained tail
+// did it).  chain (massive thermostats): NHC half 2 and the next step's half
+// 1 run as one kernel at the end of this step.
+static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R, bool head = true,
+                        bool chain = false) {
   DevSim *d = sp.p->ds;
   if (ev) XMD_CHECK(hipEventRecord(ev[0], R.s0));
-  if (d->massive) {
+  if (!head) {
+  } else if (d->massive) {
     if (launch_step_nhc(item_of(sp), 1, R.s0)) return -1;
   } else {
     if (launch_nhc(d, 1, sp.dt, R.s0)) return -1;
@@ -193,7 +198,7 @@ static int enqueue_step(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane &R
   ForceSums fused;
   if (enqueue_forces(sp, ev, serial, R, &fused)) return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[14], R.s0));
-  if (d->massive ? launch_step_nhc(item_of(sp), 2, R.s0, fused.Fpart ? &fused : nullptr)
+  if (d->massive ? launch_step_nhc(item_of(sp), chain ? 3 : 2, R.s0, fused.Fpart ? &fused : nullptr)
                  : launch_nhc(d, 2, sp.dt, R.s0))
     return -1;
   if (ev) XMD_CHECK(hipEventRecord(ev[15], R.s0));
@@ -345,7 +350,10 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   // graph, which saves most of the boundaries between graph launches
   // (0.2486 -> 0.2479 ms per step for UEG N=1024 P=32); shorter calls
   // capture one (the capture's host time would not be amortised); steps left
-  // over run eagerly after the launches.  An
+  // over run eagerly after the launches.  With massive thermostats the
+  // captured steps chain NHC half 2 of one step and half 1 of the next into
+  // one kernel (one launch and one pass over the thermostat state fewer per
+  // step boundary inside the graph).  An
   // executable graph kept from the previous call on the same object is
   // updated in place when the new graph has its topology, which saves the
   // instantiation.
@@ -361,7 +369,8 @@ int md_pimd_run_steps_x(md_pimd_t_x *pimd, const md_pimd_step_config_x *cfg, int
   hipGraph_t graph = nullptr;
   XMD_CHECK(hipStreamBeginCapture(R.s0, hipStreamCaptureModeGlobal));
   int rc = 0;
-  for (int k = 0; k < per && !rc; ++k) rc = enqueue_step(sp, nullptr, false, L);
+  const bool chain = p->ds->massive;
+  for (int k = 0; k < per && !rc; ++k) rc = enqueue_step(sp, nullptr, false, L, k == 0 || !chain, chain && k + 1 < per);
   const hipError_t ec = hipStreamEndCapture(R.s0, &graph);
   if (rc || ec != hipSuccess || !graph) {
     fprintf(stderr, "xmd: step capture failed\n");

@@ -279,7 +279,8 @@ void batch_table_free(BatchTable &t);
 
 // Fused bead-local stages of one system's step (massive thermostats only).
 // half 2 with cs (cs->Fpart set): F += the parts first (combined_force),
-// stored, and used by the half-kick.
+// stored, and used by the half-kick.  half 3: half 2 (with cs as for half 2)
+// and then the next step's half 1 (+ wrap) in one kernel.
 int launch_step_nhc(const BatchItem &it, int half, hipStream_t st, const ForceSums *cs = nullptr);
 int launch_step_bead_forces(const BatchItem &it, hipStream_t st);
 
