@@ -61,7 +61,9 @@ def test_split_independence(lib, N, P, xcut):
     a0, _, nk0 = plan(lib, s)
     assert a0 == pytest.approx(5.0 / s.L, rel=1e-15) and nk0 == (17 * 17 * 17 - 1) // 2
     f0, e0 = force_energy(lib, s)
-    for aL in (4.5, 6.0, 8.0, 12.0, -1.0):
+    # alpha' L = 4.5 .. 16 take k tables of one to four 16-slot tiles per
+    # column (half-widths MZ = 7, 15, 23, 31) through the k kernels
+    for aL in (4.5, 6.0, 8.0, 12.0, 16.0, -1.0):
         lib.md_set_ewald_split_x(aL)
         a, rc, nk = plan(lib, s)
         if aL > 0:
