@@ -480,6 +480,7 @@ struct KSpace {
 struct TileBufs {
   int N = -1, T = 0, ntp = 0;
   int2 *tiles = nullptr;
+  int2 *tiles_r = nullptr;  // the same pairs, off-diagonal first (real-space Ewald order)
   double *Fpart = nullptr, *Epart = nullptr;
   double *Frec = nullptr;  // reciprocal forces kept apart for k_combine_forces (3 N P)
   size_t fcap = 0, ecap = 0, rcap = 0;
@@ -1215,7 +1216,12 @@ __device__ __forceinline__ double wave_rol1(double v) {
 }
 
 // WPB waves per workgroup, one tile pair per wave (the erfc table is staged
-// once per workgroup); every choice computes the same partial sums.  Lane t
+// once per workgroup); every choice computes the same partial sums.  Items
+// run over the off-diagonal tile pairs of every slice first and the diagonal
+// ones (half the rotation steps' work) last: tiles holds a slice's noff =
+// ntp - T off-diagonal pairs followed by its T diagonal ones.  The grid's
+// last workgroups, the fifth on some compute units when the grid holds 4.25
+// waves per SIMD, then carry half a tile pair's work.  Lane t
 // holds bead i = I*64 + t of tile I and, at step s, the record of bead
 // j = J*64 + (t + s) mod 64 of tile J (its position and the reaction
 // accumulated so far); the records move one lane down the wave after each
@@ -1234,8 +1240,16 @@ __device__ __forceinline__ void k_ewald_real_body(int bx, const double *__restri
   __syncthreads();
   const int item = bx * WPB + w;
   if (item >= nitems) return;
-  const int slice = item / ntp;
-  const int2 IJ = tiles[item - slice * ntp];
+  const int noff = ntp - T, P = nitems / ntp;
+  int slice, idx;
+  if (item < P * noff) {
+    slice = item / noff;
+    idx = item - slice * noff;
+  } else {
+    slice = (item - P * noff) / T;
+    idx = noff + (item - P * noff - slice * T);
+  }
+  const int2 IJ = tiles[idx];
   const int I = __builtin_amdgcn_readfirstlane(IJ.x), J = __builtin_amdgcn_readfirstlane(IJ.y);
   const int base = slice * N;
   const int i = I * 64 + t, jl = J * 64 + t;
@@ -1374,9 +1388,17 @@ static int ensure_tiles(DevPimd *p) {
     std::vector<int2> tl;
     for (int I = 0; I < T; ++I)
       for (int J = I; J < T; ++J) tl.push_back(make_int2(I, J));
+    std::vector<int2> tr;
+    for (const int2 &ij : tl)
+      if (ij.x != ij.y) tr.push_back(ij);
+    for (const int2 &ij : tl)
+      if (ij.x == ij.y) tr.push_back(ij);
     if (B.tiles) (void)hipFree(B.tiles);
+    if (B.tiles_r) (void)hipFree(B.tiles_r);
     XMD_CHECK(hipMalloc(&B.tiles, tl.size() * sizeof(int2)));
     XMD_CHECK(hipMemcpy(B.tiles, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice));
+    XMD_CHECK(hipMalloc(&B.tiles_r, tr.size() * sizeof(int2)));
+    XMD_CHECK(hipMemcpy(B.tiles_r, tr.data(), tr.size() * sizeof(int2), hipMemcpyHostToDevice));
     B.N = N;
     B.T = T;
     B.ntp = (int)tl.size();
@@ -1630,7 +1652,7 @@ static int launch_ewald_real_tab(DevPimd *p, const PairParams &pp, hipStream_t s
   const RealChoice ch = real_choice(p->N, p->P, pp);
   const int wpb = 1 << ch.wlog;
   hipLaunchKernelGGL(k_ewald_real_of[ch.img][ch.wlog], dim3((items + wpb - 1) / wpb), dim3(64 * wpb), 0, st, d->X, d->M,
-                     B.Fpart, B.tiles, p->N, d->nb, B.T, B.ntp, items, ep);
+                     B.Fpart, B.tiles_r, p->N, d->nb, B.T, B.ntp, items, ep);
   XMD_CHECK(hipGetLastError());
   if (g_real_probe) {
     XMD_CHECK(hipEventRecord(g_real_probe, st));
@@ -1765,7 +1787,7 @@ int autotune_real(DevPimd *p, int reps, double *ms, int *choice) {
       const int wpb = 1 << wl;
       auto launch = [&]() {
         hipLaunchKernelGGL(k_ewald_real_of[img][wl], dim3((items + wpb - 1) / wpb), dim3(64 * wpb), 0, st, d->X,
-                           d->M, B.Fpart, B.tiles, p->N, d->nb, B.T, B.ntp, items, ep);
+                           d->M, B.Fpart, B.tiles_r, p->N, d->nb, B.T, B.ntp, items, ep);
       };
       launch();
       XMD_CHECK(hipEventRecord(e0, st));
@@ -2107,7 +2129,7 @@ static void launch_pair2_energy(DevPimd *a, DevPimd *b, const PairParams &pp, hi
 struct PairRep {
   const double *X, *M;
   double *F, *Fpart;
-  const int2 *tiles;
+  const int2 *tiles, *tiles_r;
   int N, nb, T, ntp, P, items;
   PairParams pp;
   EwReal ep;
@@ -2132,7 +2154,7 @@ __global__ __launch_bounds__(64) void kb_pair_tiles(PAIR_REP) {
 template <int IMG, int WPB>
 __global__ __launch_bounds__(64 * WPB) void kb_ewald_real(PAIR_REP) {
   PAIR_R
-  k_ewald_real_body<IMG, WPB>(bx, R.X, R.M, R.Fpart, R.tiles, R.N, R.nb, R.T, R.ntp, R.items, R.ep);
+  k_ewald_real_body<IMG, WPB>(bx, R.X, R.M, R.Fpart, R.tiles_r, R.N, R.nb, R.T, R.ntp, R.items, R.ep);
 }
 
 __global__ __launch_bounds__(256) void kb_reduce_tiles(PAIR_REP) {
@@ -2221,6 +2243,7 @@ PairBatch *pair_batch_create(const std::vector<BatchItem> &items) {
     TileBufs &B = pair_bufs(p)->tb;
     R.Fpart = B.Fpart;
     R.tiles = B.tiles;
+    R.tiles_r = B.tiles_r;
     R.T = B.T;
     R.ntp = B.ntp;
     R.items = p->P * B.ntp;
