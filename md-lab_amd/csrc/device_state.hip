@@ -33,7 +33,6 @@ int ensure_runtime() {
   XMD_CHECK(hipEventCreateWithFlags(&g_rt.join, hipEventDisableTiming));
   XMD_CHECK(hipEventCreateWithFlags(&g_rt.fork2, hipEventDisableTiming));
   XMD_CHECK(hipEventCreateWithFlags(&g_rt.join2, hipEventDisableTiming));
-  XMD_CHECK(hipEventCreateWithFlags(&g_rt.phase, hipEventDisableTiming));
   hipDeviceProp_t prop;
   XMD_CHECK(hipGetDeviceProperties(&prop, g_rt.device));
   g_rt.num_cus = prop.multiProcessorCount;

@@ -1877,7 +1877,7 @@ double ewald_split_alpha(DevPimd *p) {
   return ewald_params(p, pl).alpha;
 }
 
-int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork) {
+int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join) {
   if (kind != PK_EWALD || !p->ds->uniform_mass) return 0;
   EwaldPlan pl;
   PairParams pp = ewald_params(p, pl);
@@ -1889,26 +1889,24 @@ int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2,
   // kernels run beside it, so the recursions start next to the matrix-core
   // k-space kernels rather than the real-space waves
   if (launch_ewald_real_tab(p, pp, s2, false)) return -1;
+  // the learned stand-in (or the judge step: exact terms plus the
+  // surrogate's for comparison) fills the reciprocal-force buffer here
+  if (p->sur_mode && surrogate_kspace(p, pair_bufs(p)->tb.Frec, s2)) return -1;
+  XMD_CHECK(hipEventRecord(join, s2));
   return 1;
 }
 
-int launch_pair_kspace(DevPimd *p, hipStream_t st, hipStream_t s2, hipEvent_t phase, hipEvent_t join) {
-  if (p->sur_mode) {
-    // the learned stand-in (or the judge step: exact terms plus the
-    // surrogate's for comparison) fills the same reciprocal-force buffer
-    if (surrogate_kspace(p, pair_bufs(p)->tb.Frec, s2)) return -1;
-  } else {
-    // the phase tables run on st beside the end of the real-space grid (a
-    // short grid that fills its tail), the structure factors wait for both
-    EwaldPlan pl;
-    PairParams pp = ewald_params(p, pl);
-    KSpace *K = nullptr;
-    if (launch_recip_phase(p, pp, pl.cube_n, pl.gmax2, st, &K)) return -1;
-    XMD_CHECK(hipEventRecord(phase, st));
-    XMD_CHECK(hipStreamWaitEvent(s2, phase, 0));
-    if (launch_sk(p, K, s2) || launch_recip_force(p, pp, K, s2, pair_bufs(p)->tb.Frec)) return -1;
-  }
-  XMD_CHECK(hipEventRecord(join, s2));
+int launch_pair_kspace(DevPimd *p, hipStream_t st) {
+  if (p->sur_mode) return 0;
+  // phase tables, structure factors and reciprocal forces one after another
+  // on st: they start beside the end of the real-space grid (which fills its
+  // tail) without waiting for it
+  EwaldPlan pl;
+  PairParams pp = ewald_params(p, pl);
+  KSpace *K = nullptr;
+  if (launch_recip_phase(p, pp, pl.cube_n, pl.gmax2, st, &K) || launch_sk(p, K, st) ||
+      launch_recip_force(p, pp, K, st, pair_bufs(p)->tb.Frec))
+    return -1;
   return 0;
 }
 

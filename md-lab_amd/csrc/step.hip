@@ -3,17 +3,16 @@
 // Stream layout of one step (the reference driver's order of stages):
 //   s0: NHC half 1 (+ wrap) --fork2--> cycle energies --fork--> reset F,
 //       trap, ring springs (pair forces here unless they are Ewald), Ewald
-//       phase tables --phase--> ... --wait join, join2--> add the pair and
-//       exchange end-bead forces, NHC half 2 (one kernel with massive
-//       thermostats)
+//       phase tables, structure factors and reciprocal forces (into a side
+//       buffer) --wait join, join2--> add the pair and exchange end-bead
+//       forces, NHC half 2 (one kernel with massive thermostats)
 // With massive thermostats NHC half 1 and the wrap are one kernel, and so are
 // the force reset, trap and interior springs.
 //   s1: (after fork) forward + backward recursions, end-bead exchange forces
 //       into a side buffer --join--> (added into F on s0)
-//   s2: (Ewald) real-space tile sums, forked from s0 right after NHC half 1,
-//       then (after `phase`) structure factors and reciprocal forces into a
-//       side buffer (all read positions only) --join2--> s0 then adds the
-//       real-space, reciprocal and exchange forces
+//   s2: (Ewald) real-space tile sums, forked from s0 right after NHC half 1
+//       (positions only) --join2--> s0 then adds the real-space,
+//       reciprocal and exchange forces
 // The recursions only read cycle energies, so they overlap the pair forces;
 // the end-bead exchange forces are added into F after both branches are done
 // (the serial profiling order computes and adds them there in one kernel).
@@ -22,8 +21,10 @@
 // recursions then overlap the matrix-core k-space kernels, which leave them
 // more of the compute units than the real-space waves do (UEG N=1024 P=32:
 // 0.219 -> 0.199 ms per step against the real-space grid on s0 after the
-// recursion fork).  The phase tables run on s0 beside the end of the
-// real-space grid rather than after it on s2 (0.1975 -> 0.192 ms).
+// recursion fork).  The k-space kernels run on s0 after the bead-local
+// forces, beside the end of the real-space grid, rather than after it on s2
+// (0.1975 -> 0.192 ms with the phase tables moved, 0.185 -> 0.1815 ms with
+// the structure factors and forces following them).
 //
 // Batches of independent replicas (md_pimd_run_batch_x) run the same stages
 // with one launch per stage over a packed table of all replicas, captured
@@ -86,7 +87,7 @@ static BatchItem item_of(const StepPlan &sp) {
 // recursion branch (fork / join events between them).
 struct Lane {
   hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, fork2 = nullptr, join2 = nullptr, phase = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, fork2 = nullptr, join2 = nullptr;
 };
 
 static Lane main_lane() {
@@ -99,7 +100,6 @@ static Lane main_lane() {
   L.join = R.join;
   L.fork2 = R.fork2;
   L.join2 = R.join2;
-  L.phase = R.phase;
   return L;
 }
 
@@ -119,7 +119,7 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
   // Reciprocal space (positions only) forks onto s2 first: the phase tables
   // and structure factors overlap the cycle energies.
   const int kfork =
-      (!serial && sp.pair_kind >= 0) ? launch_pair_ewald_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2) : 0;
+      (!serial && sp.pair_kind >= 0) ? launch_pair_ewald_fork(p, sp.pair_kind, R.s0, R.s2, R.fork2, R.join2) : 0;
   if (kfork < 0) return -1;
   // Cycle energies (short, position-only); then the two recursions are
   // forked onto s1 before the real-space grid is queued, so their
@@ -148,7 +148,7 @@ static int enqueue_forces(StepPlan &sp, hipEvent_t *ev, bool serial, const Lane 
     if (launch_ring_springs(p, sp.image, sp.L, R.s0)) return -1;
   }
   if (rec(7, R.s0)) return -1;
-  if (kfork && launch_pair_kspace(p, R.s0, R.s2, R.phase, R.join2)) return -1;
+  if (kfork && launch_pair_kspace(p, R.s0)) return -1;
   if (kfork) {
     if (rec(8, R.s0) || rec(9, R.s0) || rec(10, R.s0) || rec(11, R.s0)) return -1;
   } else if (sp.pair_kind >= 0) {

@@ -55,7 +55,7 @@ struct Runtime {
   hipStream_t s0 = nullptr;   // main stream (all API work is ordered on it)
   hipStream_t s1 = nullptr;   // exchange branch of the fused step
   hipStream_t s2 = nullptr;   // reciprocal-space branch of the fused step
-  hipEvent_t fork = nullptr, join = nullptr, fork2 = nullptr, join2 = nullptr, phase = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, fork2 = nullptr, join2 = nullptr;
   int num_cus = 0;
 };
 Runtime &rt();
@@ -186,20 +186,18 @@ unsigned pair_alloc_epoch();
 // Real-space Ewald launch choice autotuning (pair_forces.hip): with it on,
 // tune the object's (N, P) bucket before first use (never during capture).
 int autotune_if_needed(DevPimd *p, int kind);
-// Ewald forces on a side stream: launch_pair_ewald_fork forks s2 from st
-// (through `fork`) and queues there the real-space tile kernel (per-tile
-// partial forces).  launch_pair_kspace, called later in st's order, queues
-// the phase tables on st (recording `phase`), then on s2 after `phase` the
-// structure factors and reciprocal forces (into a side buffer), and records
-// `join` on s2; with the learned stand-in in use its kernels follow the
-// real-space kernel on s2 instead.  All of these read positions only.  After
-// `join` (and the exchange branch) launch_combine_forces adds the real-space,
-// reciprocal and, when Fx is given, exchange end-bead forces into F in the
-// order of the unsplit kernels.  The fork returns 1 when it applies (Ewald
-// with the table kernels), 0 when nothing was launched (use
-// launch_pair_force), -1 on error.
-int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork);
-int launch_pair_kspace(DevPimd *p, hipStream_t st, hipStream_t s2, hipEvent_t phase, hipEvent_t join);
+// Ewald forces in parts: launch_pair_ewald_fork forks s2 from st (through
+// `fork`), queues there the real-space tile kernel (per-tile partial forces;
+// with the learned stand-in in use, then its kernels) and records `join`;
+// launch_pair_kspace, called later in st's order, queues the phase tables,
+// structure factors and reciprocal forces (into a side buffer) on st.  All of
+// these read positions only.  After `join` (and the exchange branch)
+// launch_combine_forces adds the real-space, reciprocal and, when Fx is
+// given, exchange end-bead forces into F in the order of the unsplit
+// kernels.  The fork returns 1 when it applies (Ewald with the table
+// kernels), 0 when nothing was launched (use launch_pair_force), -1 on error.
+int launch_pair_ewald_fork(DevPimd *p, int kind, hipStream_t st, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
+int launch_pair_kspace(DevPimd *p, hipStream_t st);
 int launch_combine_forces(DevPimd *p, const double *Fx, hipStream_t st);
 // The parts launch_combine_forces adds, for a fused consumer of F.
 int force_sums(DevPimd *p, const double *Fx, ForceSums *out);
