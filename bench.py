@@ -289,6 +289,8 @@ def main():
                     help="steps of the batched-replica measurement (0 skips it)")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched-replica measurement (for profiling it)")
+    ap.add_argument("--lib", default=None,
+                    help="engine library to load instead of the in-tree build (comparisons between builds)")
     ap.add_argument("--serial-only", action="store_true",
                     help="only run --steps steps with every stage alone on one stream (for per-kernel counters)")
     ap.add_argument("--sur-steps", type=int, default=200, help="timed surrogate steps (0 skips the surrogate leg)")
@@ -315,7 +317,7 @@ def main():
         sys.exit(2)
 
     N, P, rs, theta, xi, nsum, dt = args.N, args.P, 2.0, 0.5, 1.0, 8, 0.05
-    lib = pyxmd.Lib()
+    lib = pyxmd.Lib(args.lib) if args.lib else pyxmd.Lib()
     lib.md_set_nhc_length_x(4)
     if args.only_batch:
         print(json.dumps({"batched_replicas": batch_bench(lib, args.batch_steps, one_by_one=False)}))
