@@ -16,8 +16,8 @@
 //    from a piecewise-polynomial table in LDS; k_pair_tiles covers the R/NI
 //    splits, energies and the general cube-clipped case.
 //  * Ewald reciprocal space uses per-slice structure factors
-//    S(k) = sum_l exp(i k.x_l) over half of the k set (k_phase, k_sk,
-//    k_sk_fin, k_recip_force, k_recip_energy).
+//    S(k) = sum_l exp(i k.x_l) over half of the k set (k_phase, k_sk_mfma,
+//    k_recip_force_mfma, k_recip_force_energy, k_recip_energy).
 //  * When the reference's own Ewald parameters are converged, the same sum is
 //    evaluated with the engine's split (plan_ewald).
 #include <algorithm>
